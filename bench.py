@@ -1,0 +1,145 @@
+"""Throughput of the Manifold DMC pipeline on one MI355X.
+
+Workload (default): the 512^3 float32 gyroid sin(x)cos(y)+sin(y)cos(z)+sin(z)cos(x)
+with 8 periods per axis plus 0.1*N(0,1) seeded noise, iso 0, pad by one, all
+post-processes (P3X3Y colour + old, P3333, non-manifold split, non-manifold
+vertex detection).  One step = one full extraction (DualMarchingCubes::run +
+runPostProcesses) of the field, which is resident in HBM before timing starts.
+Metric: Mcells/s = cells of the processed (padded) grid / seconds per step.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--cpu-planes", type=int, default=20, help="z-planes of the field in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def make_field(n):
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from manifolddmc_amd import grids
+
+    out = np.empty((n, n, n), np.float32)
+    chunk = 16
+
+    def work(z0):
+        out[z0:min(n, z0 + chunk)] = grids.gyroid(n, z0=z0, z1=min(n, z0 + chunk))
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(work, range(0, n, chunk)))
+    return out
+
+
+def cpu_baseline(n, planes, flags):
+    """The CPU oracle (single thread) on a z-slab of the same field."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as oa
+
+    from manifolddmc_amd import grids
+
+    slab = grids.gyroid(n, z0=0, z1=planes)
+    t0 = time.perf_counter()
+    m = oa.extract(slab, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags=flags)
+    dt = time.perf_counter() - t0
+    cells = (n + 1) * (n + 1) * (planes + 1)
+    return {"value": cells / dt / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
+            "sample": "z-planes 0..%d of the same %d^3 gyroid (%dx%dx%d, padded to %d cells), same flags, %.1f s, %d faces"
+                      % (planes - 1, n, n, n, planes, cells, dt, len(m.face_he))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; --gpus %d is not supported (run with --gpus 1)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import manifolddmc_amd as mdmc
+
+    n = args.size
+    print("generating %d^3 gyroid" % n, file=sys.stderr, flush=True)
+    field = make_field(n)
+    dev = torch.device("cuda:0")
+    t_field = torch.from_numpy(field).to(dev)
+    torch.cuda.synchronize()
+    flags = mdmc.PAD1 | mdmc.SURF_ITER | mdmc.ALL_POST
+    d = mdmc.DualMarchingCubes(device=0)
+
+    def step():
+        return d.extract_raw(t_field.data_ptr(), (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags, 0, True)
+
+    for i in range(args.warmup):
+        st = step()
+        print("warmup %d: %.1f ms" % (i, st.ms_total), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step().as_dict())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / args.steps
+    s = stats[-1]
+    cells = s["cells"]
+    value = cells / (ms / 1e3) / 1e6
+    # dominant kernel: vertex + quad generation of the grid pass
+    P = (n + 2) ** 3
+    alg_bytes = 4 * P + 24 * s["n_dmc_vertices"] + 16 * s["n_dmc_quads"]
+    gen_ms = float(np.mean([x["ms_dmc_generate_kernel"] for x in stats]))
+    cnt_ms = float(np.mean([x["ms_dmc_count_kernel"] for x in stats]))
+    achieved = alg_bytes / (gen_ms / 1e3) / 1e9
+    stage_keys = ["ms_grid", "ms_dmc", "ms_halfedge", "ms_simplify", "ms_fix_twins", "ms_split", "ms_detect", "ms_total"]
+    out = {
+        "metric": "Mcells/s DMC+repair @512^3 fp32",
+        "value": round(value, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "gyroid %d^3, 8 periods, +0.1*N(0,1), iso 0, pad-by-1, all post-processes" % n,
+                   "cells": cells, "vertices": s["n_vertices"], "faces": s["n_faces"], "dmc_vertices": s["n_dmc_vertices"],
+                   "dmc_quads": s["n_dmc_quads"], "split_iterations": s["split_iterations"], "nm_edges4": s["nm_edges4"],
+                   "nm_vertices": s["nm_vertices"]},
+        "roofline": {"kernel": "kDmcGenerate", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": alg_bytes, "kernel_ms": round(gen_ms, 4),
+                     "grid_pass_ms": round(gen_ms + cnt_ms, 4),
+                     "grid_pass_frac": round(alg_bytes / ((gen_ms + cnt_ms) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "stages_ms": {k: round(float(np.mean([x[k] for x in stats])), 3) for k in stage_keys},
+    }
+    if not args.no_cpu:
+        print("cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_planes, flags)
+    print(json.dumps(out), flush=True)
+    d.close()
+
+
+if __name__ == "__main__":
+    main()

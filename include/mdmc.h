@@ -1,0 +1,134 @@
+/*
+ * mdmc.h -- C ABI of the MI355X (gfx950) Manifold Dual Marching Cubes library.
+ *
+ * Drop-in for the DMC plugin path of PhiliGuertler/ManifoldDMC:
+ *   DMC::MarchingCubes::run(float isoValue, Mesh&, const UniformGridHost<float>&)
+ *     (MarchingCubes/VisualDMC/src/DMC/DualMarchingCubes.h:58), implemented by
+ *   DMC::DualMarchingCubes::run (DualMarchingCubes.cu:220-289) with
+ *   runPostProcesses (DualMarchingCubes.cu:335-380).
+ * The grid flags of UniformGridHost::readFromFile(file, refinement, addZeroBoundary)
+ * (UniformGrid.h:388, UniformGrid.inl:573-608) and the boolean members of
+ * DualMarchingCubes (DualMarchingCubes.cu:110-133) become mdmc_options.
+ *
+ * Plain C types only.  Ids are int32 like the reference's IndexType.  All
+ * device pointers are HIP device pointers on the context's device.
+ */
+#ifndef MDMC_H
+#define MDMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference has none: it logs, throws or truncates) ---- */
+typedef enum {
+    MDMC_OK = 0,
+    MDMC_ERR_INVALID_ARG = 1,
+    MDMC_ERR_OOM = 2,
+    MDMC_ERR_CAPACITY = 3,   /* a count exceeds the int32 id range */
+    MDMC_ERR_HASH_FULL = 4,  /* reserved: the library sizes every table from exact counts */
+    MDMC_ERR_HIP = 5,
+    MDMC_ERR_NO_RESULT = 6,
+    MDMC_ERR_INTERNAL = 7    /* an id produced by a stage is out of range; the run was stopped */
+} mdmc_status;
+
+/* ---- option bits (DualMarchingCubes members, DualMarchingCubes.cu:110-133;
+ *      addZeroBoundary of UniformGridHost::readFromFile, UniformGrid.inl:575) ---- */
+#define MDMC_PAD1            0x01 /* pad by one clamped (duplicated) layer, UniformGrid.inl:290-295 */
+#define MDMC_SURF_ITER       0x02 /* m_enableSurfaceToVertexIteration (default on) */
+#define MDMC_P3X3Y_COLOR     0x04 /* m_simplifyP3x3yColor, HalfedgeMesh::simplify3X3Y */
+#define MDMC_P3X3Y_OLD       0x08 /* m_simplifyP3x3yOld, HalfedgeMesh::simplify3X3YOld */
+#define MDMC_P3333           0x10 /* m_simplifyP3333, HalfedgeMesh::simplify3333 */
+#define MDMC_SPLIT_NM        0x20 /* m_splitNonManifoldHalfedges, ManifoldSplitter::run */
+#define MDMC_DETECT_NM_VERTS 0x40 /* m_detectNonManifoldVertices */
+#define MDMC_DEFAULT_FLAGS   (MDMC_PAD1 | MDMC_SURF_ITER | MDMC_SPLIT_NM | MDMC_DETECT_NM_VERTS)
+#define MDMC_ALL_POST        (MDMC_P3X3Y_COLOR | MDMC_P3X3Y_OLD | MDMC_P3333 | MDMC_SPLIT_NM | MDMC_DETECT_NM_VERTS)
+
+/* SplittingStep, ManifoldSplitter.h:155-166 */
+#define MDMC_SPLIT_STEP_NONE 0
+#define MDMC_SPLIT_STEP_SUBDIVIDE 1
+#define MDMC_SPLIT_STEP_COLLAPSE 2
+#define MDMC_SPLIT_STEP_MERGE 3
+#define MDMC_SPLIT_STEP_TWINS 4
+#define MDMC_SPLIT_STEP_MOVE 5
+#define MDMC_SPLIT_STEP_ITERATIVE 6
+#define MDMC_SPLIT_STEP_ALL 100
+
+typedef struct {
+    int32_t flags;           /* MDMC_* bits */
+    int32_t refinement;      /* refinement cuts per cell (UniformGrid.inl:587), >= 0 */
+    int32_t split_step;      /* MDMC_SPLIT_STEP_* (reference default: ALL) */
+    int32_t max_split_iters; /* guard for the reference's unbounded split loop; <=0 means 64 */
+} mdmc_options;
+
+/* A uniform scalar grid, x fastest then y then z (UniformGrid.h:50-52).
+ * origin/delta as in UniformGridStats (UniformGrid.h:121-129). */
+typedef struct {
+    int32_t dims[3];
+    float origin[3];
+    float delta[3];
+    const float* data;
+    int32_t data_on_device; /* 1: data is a device pointer, 0: host pointer */
+} mdmc_grid;
+
+/* Counters of the last extraction (LayerCUDA.cpp:246-255 statistics; HalfedgeMesh.cu:1164-1239). */
+typedef struct {
+    int64_t n_vertices, n_halfedges, n_faces;
+    int64_t n_dmc_vertices, n_dmc_quads;
+    int64_t nm_edges4, nm_edges3, n_edges, nm_vertices; /* nm_vertices = -1 when detection is off */
+    int64_t quads_with_nm[5];                           /* faces with k edges of >2 faces, k = 0..4 */
+    int64_t split_iterations;
+    int64_t anomalies; /* reference-undefined situations met */
+    int64_t cells;     /* cells of the processed grid */
+    /* stage times (ms, HIP events on the context's stream): whole call, pad/refine, DMC
+     * (count + scan + generate), halfedge build, simplifications, twin repair, split, detection */
+    double ms_total, ms_grid, ms_dmc, ms_halfedge, ms_simplify, ms_fix_twins, ms_split, ms_detect;
+    /* the two grid-pass kernels alone: classification/count and vertex + quad generation */
+    double ms_dmc_count_kernel, ms_dmc_generate_kernel;
+} mdmc_stats;
+
+/* Sizes of the resident result. */
+typedef struct {
+    int64_t n_vertices, n_halfedges, n_faces;
+} mdmc_mesh_info;
+
+typedef struct mdmc_ctx mdmc_ctx;
+
+/* One context per host thread; it owns a HIP stream and reuses its buffers across
+ * calls (the reference keeps 20M-capacity buffers in the DualMarchingCubes object,
+ * DualMarchingCubes.cu:291-313). device < 0 means the current device. */
+mdmc_status mdmc_create(mdmc_ctx** out, int32_t device);
+void mdmc_destroy(mdmc_ctx* ctx);
+const char* mdmc_last_error(const mdmc_ctx* ctx);
+
+/* Run all following work on this stream (hipStream_t passed as void*); NULL = own stream. */
+mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* hip_stream);
+
+/* Replaces DualMarchingCubes::run(iso, mesh, grid) + runPostProcesses
+ * (DualMarchingCubes.cu:220-289, :335-380).  The mesh stays on the device in the
+ * context until the next call.  stats may be NULL.  Synchronises the stream. */
+mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const mdmc_options* opt, mdmc_stats* stats);
+
+mdmc_status mdmc_mesh_get_info(const mdmc_ctx* ctx, mdmc_mesh_info* info);
+
+/* Copies the resident mesh into caller buffers (any may be NULL):
+ *   positions/normals float[3*nV], halfedges int32[4*nH] = {origin, face, next, twin}
+ *   (Halfedge.h:120-210), face_halfedge int32[nF], face_attr uint16[nF]
+ *   (QuadrilateralAttribute bits, DeviceVectors.h:334-450), vertex_flags uint8[nV]
+ *   (Flags, Halfedge.h:14-111), vertex_halfedge int32[nV].
+ * dst_on_device: 1 = device buffers, 0 = host buffers. */
+mdmc_status mdmc_mesh_copy(const mdmc_ctx* ctx, float* positions, float* normals, int32_t* halfedges,
+                           int32_t* face_halfedge, uint16_t* face_attr, uint8_t* vertex_flags,
+                           int32_t* vertex_halfedge, int32_t dst_on_device);
+
+/* Triangle index buffer, two triangles per face (x,y,z),(z,w,x): HalfedgeMesh.cu:690-714.
+ * indices uint32[6*nF], device or host. */
+mdmc_status mdmc_mesh_triangles(const mdmc_ctx* ctx, uint32_t* indices, int32_t dst_on_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDMC_H */

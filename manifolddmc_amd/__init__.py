@@ -1,0 +1,271 @@
+"""manifolddmc_amd -- Manifold Dual Marching Cubes on AMD MI355X (gfx950).
+
+Host-side mirror of the reference's DMC plugin interface
+(MarchingCubes/VisualDMC/src/DMC/DualMarchingCubes.h:68-186) over the C ABI in
+include/mdmc.h.  All computation happens in the HIP library libmdmc.so; this
+module only marshals arguments.  It fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import grids  # noqa: F401  (re-exported helpers)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdmc.so")
+
+# option bits, include/mdmc.h
+PAD1 = 0x01
+SURF_ITER = 0x02
+P3X3Y_COLOR = 0x04
+P3X3Y_OLD = 0x08
+P3333 = 0x10
+SPLIT_NM = 0x20
+DETECT_NM_VERTS = 0x40
+DEFAULT_FLAGS = PAD1 | SURF_ITER | SPLIT_NM | DETECT_NM_VERTS
+ALL_POST = P3X3Y_COLOR | P3X3Y_OLD | P3333 | SPLIT_NM | DETECT_NM_VERTS
+
+
+class SplittingStep:
+    """ManifoldSplitter.h:155-166."""
+    NONE = 0
+    SUBDIVIDE_QUADS = 1
+    COLLAPSE_NON_MANIFOLD_EDGES = 2
+    MERGE_VERTICES = 3
+    UPDATE_HALFEDGE_TWINS = 4
+    MOVE_VERTICES_TO_SURFACE = 5
+    ENABLE_ITERATIVE_REFINEMENT = 6
+    ALL = 100
+
+
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "CAPACITY", 4: "HASH_FULL", 5: "HIP", 6: "NO_RESULT", 7: "INTERNAL"}
+
+
+class MdmcError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("mdmc %s: %s" % (STATUS.get(status, status), message))
+        self.status = status
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_int32), ("refinement", ctypes.c_int32),
+                ("split_step", ctypes.c_int32), ("max_split_iters", ctypes.c_int32)]
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32 * 3), ("origin", ctypes.c_float * 3), ("delta", ctypes.c_float * 3),
+                ("data", ctypes.c_void_p), ("data_on_device", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_vertices", ctypes.c_int64), ("n_halfedges", ctypes.c_int64), ("n_faces", ctypes.c_int64),
+                ("n_dmc_vertices", ctypes.c_int64), ("n_dmc_quads", ctypes.c_int64),
+                ("nm_edges4", ctypes.c_int64), ("nm_edges3", ctypes.c_int64), ("n_edges", ctypes.c_int64),
+                ("nm_vertices", ctypes.c_int64), ("quads_with_nm", ctypes.c_int64 * 5),
+                ("split_iterations", ctypes.c_int64), ("anomalies", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("ms_total", ctypes.c_double), ("ms_grid", ctypes.c_double), ("ms_dmc", ctypes.c_double),
+                ("ms_halfedge", ctypes.c_double), ("ms_simplify", ctypes.c_double), ("ms_fix_twins", ctypes.c_double),
+                ("ms_split", ctypes.c_double), ("ms_detect", ctypes.c_double),
+                ("ms_dmc_count_kernel", ctypes.c_double), ("ms_dmc_generate_kernel", ctypes.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if k == "quads_with_nm" else v
+        return out
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("n_vertices", ctypes.c_int64), ("n_halfedges", ctypes.c_int64), ("n_faces", ctypes.c_int64)]
+
+
+_lib = None
+
+EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", "mdmc_extract",
+           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles"]
+
+
+def lib():
+    """The loaded libmdmc.so (built by manifolddmc_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmdmc.so is missing: build it with `python -m manifolddmc_amd.build` "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.mdmc_create.argtypes = [ctypes.POINTER(vp), i32]
+        L.mdmc_create.restype = i32
+        L.mdmc_destroy.argtypes = [vp]
+        L.mdmc_destroy.restype = None
+        L.mdmc_last_error.argtypes = [vp]
+        L.mdmc_last_error.restype = ctypes.c_char_p
+        L.mdmc_set_stream.argtypes = [vp, vp]
+        L.mdmc_set_stream.restype = i32
+        L.mdmc_extract.argtypes = [vp, ctypes.POINTER(_Grid), ctypes.c_float, ctypes.POINTER(_Options), ctypes.POINTER(Stats)]
+        L.mdmc_extract.restype = i32
+        L.mdmc_mesh_get_info.argtypes = [vp, ctypes.POINTER(_Info)]
+        L.mdmc_mesh_get_info.restype = i32
+        L.mdmc_mesh_copy.argtypes = [vp] + [vp] * 7 + [i32]
+        L.mdmc_mesh_copy.restype = i32
+        L.mdmc_mesh_triangles.argtypes = [vp, vp, i32]
+        L.mdmc_mesh_triangles.restype = i32
+        _lib = L
+    return _lib
+
+
+class HalfedgeMesh:
+    """The mesh returned by an extraction (Halfedge.h:120-210, HalfedgeVectors.h:225-282).
+
+    positions/normals float32[V,3]; halfedges int32[H,4] = (origin, face, next, twin);
+    face_halfedge int32[F]; face_attr uint16[F]; vertex_flags uint8[V]; vertex_halfedge int32[V]."""
+
+    def __init__(self, positions, normals, halfedges, face_halfedge, face_attr, vertex_flags, vertex_halfedge):
+        self.positions = positions
+        self.normals = normals
+        self.halfedges = halfedges
+        self.face_halfedge = face_halfedge
+        self.face_attr = face_attr
+        self.vertex_flags = vertex_flags
+        self.vertex_halfedge = vertex_halfedge
+
+    def faces(self):
+        """Vertex ids of each face, walked from its first halfedge (the order Mesh.cu:446-506 writes)."""
+        he = np.asarray(self.halfedges)
+        h = np.asarray(self.face_halfedge).astype(np.int64)
+        out = np.empty((len(h), 4), np.int32)
+        for i in range(4):
+            out[:, i] = he[h, 0]
+            h = he[h, 2]
+        return out
+
+    def write_obj(self, path):
+        """The reference's .obj layout (Mesh.cu:446-506): v, vn, then f a//a b//b c//c d//d (1-based)."""
+        with open(path, "w") as f:
+            for p in np.asarray(self.positions):
+                f.write("v %g %g %g\n" % tuple(p))
+            for n in np.asarray(self.normals):
+                f.write("vn %g %g %g\n" % tuple(n))
+            for q in self.faces() + 1:
+                f.write("f %d//%d %d//%d %d//%d %d//%d\n" % (q[0], q[0], q[1], q[1], q[2], q[2], q[3], q[3]))
+
+
+class DualMarchingCubes:
+    """MI355X counterpart of DMC::DualMarchingCubes (DualMarchingCubes.h:68-186).
+
+    The option attributes carry the reference's member names and defaults
+    (DualMarchingCubes.cu:110-133); the grid flags of UniformGridHost::readFromFile
+    (refinement, add_zero_boundary) are passed with the grid."""
+
+    def __init__(self, device=-1):
+        self.enable_surface_to_vertex_iteration = True
+        self.simplify_p3x3y_color = False
+        self.simplify_p3x3y_old = False
+        self.simplify_p3333 = False
+        self.split_non_manifold_halfedges = True
+        self.detect_non_manifold_vertices = True
+        self.splitting_step = SplittingStep.ALL
+        self.max_split_iterations = 64
+        self._ctx = ctypes.c_void_p()
+        st = lib().mdmc_create(ctypes.byref(self._ctx), device)
+        if st != 0:
+            raise MdmcError(st, "mdmc_create failed")
+        self.last_stats = None
+
+    def close(self):
+        if self._ctx:
+            lib().mdmc_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def flags(self, add_zero_boundary=True):
+        f = PAD1 if add_zero_boundary else 0
+        if self.enable_surface_to_vertex_iteration:
+            f |= SURF_ITER
+        if self.simplify_p3x3y_color:
+            f |= P3X3Y_COLOR
+        if self.simplify_p3x3y_old:
+            f |= P3X3Y_OLD
+        if self.simplify_p3333:
+            f |= P3333
+        if self.split_non_manifold_halfedges:
+            f |= SPLIT_NM
+        if self.detect_non_manifold_vertices:
+            f |= DETECT_NM_VERTS
+        return f
+
+    def set_stream(self, stream_ptr):
+        st = lib().mdmc_set_stream(self._ctx, ctypes.c_void_p(stream_ptr))
+        self._check(st)
+
+    def _check(self, st):
+        if st != 0:
+            raise MdmcError(st, lib().mdmc_last_error(self._ctx).decode())
+
+    def extract_raw(self, data_ptr, dims, origin, delta, iso, flags, refinement=0, on_device=False):
+        """Runs the pipeline on a grid given by pointer; the mesh stays on the device."""
+        g = _Grid()
+        g.dims[:] = [int(d) for d in dims]
+        g.origin[:] = [float(np.float32(o)) for o in origin]
+        g.delta[:] = [float(np.float32(d)) for d in delta]
+        g.data = ctypes.c_void_p(data_ptr)
+        g.data_on_device = 1 if on_device else 0
+        o = _Options(int(flags), int(refinement), int(self.splitting_step), int(self.max_split_iterations))
+        st = Stats()
+        self._check(lib().mdmc_extract(self._ctx, ctypes.byref(g), ctypes.c_float(iso), ctypes.byref(o), ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def info(self):
+        i = _Info()
+        self._check(lib().mdmc_mesh_get_info(self._ctx, ctypes.byref(i)))
+        return i.n_vertices, i.n_halfedges, i.n_faces
+
+    def mesh_numpy(self):
+        V, H, F = self.info()
+        pos = np.zeros((V, 3), np.float32)
+        nrm = np.zeros((V, 3), np.float32)
+        he = np.zeros((H, 4), np.int32)
+        fh = np.zeros(F, np.int32)
+        fa = np.zeros(F, np.uint16)
+        vf = np.zeros(V, np.uint8)
+        vh = np.zeros(V, np.int32)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self._check(lib().mdmc_mesh_copy(self._ctx, p(pos), p(nrm), p(he), p(fh), p(fa), p(vf), p(vh), 0))
+        return HalfedgeMesh(pos, nrm, he, fh, fa, vf, vh)
+
+    def run(self, iso, values, origin=(0.0, 0.0, 0.0), delta=(1.0, 1.0, 1.0), refinement=0, add_zero_boundary=True):
+        """DualMarchingCubes::run(iso, mesh, grid): values is a float32 array [z][y][x] on the host."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        nz, ny, nx = v.shape
+        self.extract_raw(v.ctypes.data, (nx, ny, nz), origin, delta, iso, self.flags(add_zero_boundary), refinement, False)
+        return self.mesh_numpy()
+
+    # statistics getters of DualMarchingCubes (DualMarchingCubes.h:98-118)
+    def getNumVertices(self):
+        return self.last_stats.n_dmc_vertices
+
+    def getNumQuads(self):
+        return self.last_stats.n_dmc_quads
+
+    def getNumNonManifoldEdges4(self):
+        return self.last_stats.nm_edges4
+
+    def getNumNonManifoldEdges3(self):
+        return self.last_stats.nm_edges3
+
+    def getNumEdges(self):
+        return self.last_stats.n_edges
+
+    def getNumNonManifoldVertices(self):
+        return self.last_stats.nm_vertices
+
+    def getNumQuadsWithNonManifoldEdges(self, k):
+        return self.last_stats.quads_with_nm[k]

@@ -1,0 +1,253 @@
+// mdmc_api.hip -- the C ABI (include/mdmc.h) over the stage driver.
+//
+// mdmc_extract follows DualMarchingCubes::run + runPostProcesses
+// (DualMarchingCubes.cu:220-289, :335-380) with the grid preparation of
+// UniformGridHost::readFromFile(file, refinement, addZeroBoundary)
+// (UniformGrid.inl:573-608):
+//   pad/refine -> DMC -> HalfedgeMesh::initialize -> [P3X3Y colour] -> [P3X3Y old]
+//   -> [P3333] -> fixManifoldTwins (always) -> [split] -> [detect NM vertices]
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/mdmc.h"
+#include "mdmc_pipeline.h"
+
+struct mdmc_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    mdmc::Pipeline p;
+    std::string err;
+    bool hasResult = false;
+    hipEvent_t ev[9] = {};
+};
+
+namespace {
+
+mdmc_status fail(mdmc_ctx* ctx, mdmc_status st, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return st;
+}
+
+mdmc_status fromException(mdmc_ctx* ctx, const std::exception& e) {
+    if (auto* h = dynamic_cast<const mdmc::HipError*>(&e)) {
+        if (h->code == -2) return fail(ctx, MDMC_ERR_OOM, e.what());
+        if (h->code == -3) return fail(ctx, MDMC_ERR_CAPACITY, e.what());
+        if (h->code == -4) return fail(ctx, MDMC_ERR_INTERNAL, e.what());
+        return fail(ctx, MDMC_ERR_HIP, e.what());
+    }
+    return fail(ctx, MDMC_ERR_HIP, e.what());
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        (void)hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+mdmc_status mdmc_create(mdmc_ctx** out, int32_t device) {
+    if (!out) return MDMC_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return MDMC_ERR_HIP;
+    int dev = device;
+    if (dev < 0) (void)hipGetDevice(&dev);
+    if (dev >= count) return MDMC_ERR_INVALID_ARG;
+    mdmc_ctx* ctx = new (std::nothrow) mdmc_ctx();
+    if (!ctx) return MDMC_ERR_OOM;
+    ctx->device = dev;
+    DeviceGuard g(dev);
+    if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MDMC_ERR_HIP;
+    }
+    for (auto& e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return MDMC_ERR_HIP;
+        }
+    ctx->p.stream = ctx->own;
+    *out = ctx;
+    return MDMC_OK;
+}
+
+void mdmc_destroy(mdmc_ctx* ctx) {
+    if (!ctx) return;
+    {
+        DeviceGuard g(ctx->device);
+        if (ctx->p.stream) (void)hipStreamSynchronize(ctx->p.stream);
+        ctx->p.arena.release();
+        for (auto& e : ctx->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    }
+    delete ctx;
+}
+
+const char* mdmc_last_error(const mdmc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* s) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    ctx->p.stream = s ? (hipStream_t)s : ctx->own;
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const mdmc_options* opt, mdmc_stats* stats) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    ctx->hasResult = false;
+    if (!grid || !grid->data || !opt) return fail(ctx, MDMC_ERR_INVALID_ARG, "null grid, data or options");
+    for (int i = 0; i < 3; ++i)
+        if (grid->dims[i] < 1) return fail(ctx, MDMC_ERR_INVALID_ARG, "grid dims must be >= 1");
+    if (opt->refinement < 0 || opt->refinement > 64) return fail(ctx, MDMC_ERR_INVALID_ARG, "refinement must be in [0, 64]");
+    const int flags = opt->flags;
+    const int pad = (flags & MDMC_PAD1) ? 1 : 0;
+    const int r = opt->refinement;
+    int od[3];
+    for (int i = 0; i < 3; ++i) od[i] = grid->dims[i] + (grid->dims[i] - 1) * r + 2 * pad;
+    if (od[0] < 2 || od[1] < 2 || od[2] < 2) return fail(ctx, MDMC_ERR_INVALID_ARG, "processed grid needs at least 2 points per axis");
+    const int64_t P = (int64_t)od[0] * od[1] * od[2];
+    if ((int64_t)3 * P >= ((int64_t)1 << 40)) return fail(ctx, MDMC_ERR_CAPACITY, "grid too large");
+    DeviceGuard dg(ctx->device);
+    mdmc::Pipeline& p = ctx->p;
+    try {
+        hipStream_t s = p.stream;
+        MDMC_HIP(hipEventRecord(ctx->ev[0], s));
+        p.resetCounters();
+        p.splitIterations = 0;
+        p.extraAnomalies = 0;
+        // ---- grid: upload if needed, then pad / refine (refineGrid)
+        const int64_t Pin = (int64_t)grid->dims[0] * grid->dims[1] * grid->dims[2];
+        const float* in = grid->data;
+        if (!grid->data_on_device) {
+            float* d = p.arena.get<float>("input", (size_t)Pin);
+            MDMC_HIP(hipMemcpyAsync(d, grid->data, (size_t)Pin * sizeof(float), hipMemcpyHostToDevice, s));
+            in = d;
+        }
+        float* gp = p.arena.get<float>("grid", (size_t)P);
+        mdmc::launchPrepareGrid(in, grid->dims, gp, od, r, pad, s);
+        mdmc::GridDev g;
+        g.v = gp;
+        g.nx = od[0];
+        g.ny = od[1];
+        g.nz = od[2];
+        const float f = 1.f / (static_cast<float>(r) + 1.f);
+        g.dx = grid->delta[0] * f;
+        g.dy = grid->delta[1] * f;
+        g.dz = grid->delta[2] * f;
+        g.ox = grid->origin[0];
+        g.oy = grid->origin[1];
+        g.oz = grid->origin[2];
+        if (pad) {
+            g.ox -= g.dx;
+            g.oy -= g.dy;
+            g.oz -= g.dz;
+        }
+        MDMC_HIP(hipEventRecord(ctx->ev[1], s));
+        // ---- DMC
+        p.dmc(g, iso, (flags & MDMC_SURF_ITER) != 0);
+        MDMC_HIP(hipEventRecord(ctx->ev[2], s));
+        int64_t st9[9] = {0};
+        if (p.V > 0) {
+            p.halfedgeInitialize();
+            MDMC_HIP(hipEventRecord(ctx->ev[3], s));
+            if (flags & MDMC_P3X3Y_COLOR) p.simplify3X3Y(true);
+            if (flags & MDMC_P3X3Y_OLD) p.simplify3X3Y(false);
+            if (flags & MDMC_P3333) p.simplify3333();
+            MDMC_HIP(hipEventRecord(ctx->ev[4], s));
+            p.fixManifoldTwins(g, iso);
+            MDMC_HIP(hipEventRecord(ctx->ev[5], s));
+            if (flags & MDMC_SPLIT_NM) p.split(g, iso, opt->split_step, opt->max_split_iters > 0 ? opt->max_split_iters : 64);
+            MDMC_HIP(hipEventRecord(ctx->ev[6], s));
+            if (flags & MDMC_DETECT_NM_VERTS) p.detectNonManifoldVertices();
+            MDMC_HIP(hipEventRecord(ctx->ev[7], s));
+            p.computeStats(st9);
+        } else {
+            // DualMarchingCubes::run returns early when no vertex was produced (DualMarchingCubes.cu:257-261)
+            p.Q = p.H = p.F = 0;
+            for (int i = 3; i < 8; ++i) MDMC_HIP(hipEventRecord(ctx->ev[i], s));
+        }
+        MDMC_HIP(hipEventRecord(ctx->ev[8], s));
+        MDMC_HIP(hipEventSynchronize(ctx->ev[8]));
+        ctx->hasResult = true;
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->n_vertices = p.V;
+            stats->n_halfedges = p.H;
+            stats->n_faces = p.F;
+            stats->n_dmc_vertices = p.dmcV;
+            stats->n_dmc_quads = p.dmcQ;
+            stats->nm_edges4 = st9[0];
+            stats->nm_edges3 = st9[1];
+            stats->n_edges = st9[2];
+            for (int k = 0; k < 5; ++k) stats->quads_with_nm[k] = st9[3 + k];
+            stats->nm_vertices = (flags & MDMC_DETECT_NM_VERTS) ? st9[8] : -1;
+            stats->split_iterations = p.splitIterations;
+            stats->anomalies = p.anomalies() + p.extraAnomalies;
+            stats->cells = (int64_t)(od[0] - 1) * (od[1] - 1) * (od[2] - 1);
+            float ms = 0.f;
+            double* outs[8] = {&stats->ms_total, &stats->ms_grid, &stats->ms_dmc, &stats->ms_halfedge,
+                               &stats->ms_simplify, &stats->ms_fix_twins, &stats->ms_split, &stats->ms_detect};
+            MDMC_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[8]));
+            *outs[0] = ms;
+            for (int i = 1; i < 8; ++i) {
+                MDMC_HIP(hipEventElapsedTime(&ms, ctx->ev[i - 1], ctx->ev[i]));
+                *outs[i] = ms;
+            }
+            p.kernelTimes(&stats->ms_dmc_count_kernel, &stats->ms_dmc_generate_kernel);
+        }
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        (void)hipStreamSynchronize(p.stream);
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_mesh_get_info(const mdmc_ctx* ctx, mdmc_mesh_info* info) {
+    if (!ctx || !info) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return MDMC_ERR_NO_RESULT;
+    info->n_vertices = ctx->p.V;
+    info->n_halfedges = ctx->p.H;
+    info->n_faces = ctx->p.F;
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_mesh_copy(const mdmc_ctx* cctx, float* positions, float* normals, int32_t* halfedges, int32_t* face_halfedge, uint16_t* face_attr,
+                           uint8_t* vertex_flags, int32_t* vertex_halfedge, int32_t dst_on_device) {
+    mdmc_ctx* ctx = const_cast<mdmc_ctx*>(cctx);
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return fail(ctx, MDMC_ERR_NO_RESULT, "no extraction result");
+    DeviceGuard dg(ctx->device);
+    try {
+        ctx->p.copyOut(positions, normals, halfedges, face_halfedge, face_attr, vertex_flags, vertex_halfedge, dst_on_device != 0);
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_mesh_triangles(const mdmc_ctx* cctx, uint32_t* indices, int32_t dst_on_device) {
+    mdmc_ctx* ctx = const_cast<mdmc_ctx*>(cctx);
+    if (!ctx || !indices) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return fail(ctx, MDMC_ERR_NO_RESULT, "no extraction result");
+    DeviceGuard dg(ctx->device);
+    try {
+        ctx->p.triangles(indices, dst_on_device != 0);
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
+}  // extern "C"

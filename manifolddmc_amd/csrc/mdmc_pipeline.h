@@ -1,0 +1,63 @@
+// mdmc_pipeline.h -- the stage driver behind mdmc_extract.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "mdmc_arena.h"
+#include "mdmc_internal.h"
+
+namespace mdmc {
+
+class Pipeline {
+   public:
+    Pipeline();
+    ~Pipeline();
+
+    hipStream_t stream = nullptr;
+    Arena arena;
+
+    // live sizes of the resident mesh
+    int64_t V = 0, Q = 0, H = 0, F = 0;
+    int64_t dmcV = 0, dmcQ = 0;
+    int64_t vertHeSize = 0;
+    int64_t splitIterations = 0;
+    int64_t extraAnomalies = 0;
+
+    void resetCounters();
+    int64_t anomalies();
+
+    void dmc(const GridDev& g, float iso, bool surfIter);
+    void halfedgeInitialize();
+    void simplify3X3Y(bool colour);
+    void simplify3333();
+    void fixManifoldTwins(const GridDev& g, float iso);
+    void split(const GridDev& g, float iso, int step, int maxIters);
+    void detectNonManifoldVertices();
+    void computeStats(int64_t out[9]);
+
+    void copyOut(float* positions, float* normals, int32_t* halfedges, int32_t* face_halfedge, uint16_t* face_attr, uint8_t* vertex_flags,
+                 int32_t* vertex_halfedge, bool dstDevice);
+    void triangles(uint32_t* idx, bool dstDevice);
+
+    int64_t readI64(const int64_t* dev);
+    // durations of the last grid-pass kernels, from events on the launch stream
+    void kernelTimes(double* msCount, double* msGenerate);
+    hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+   private:
+    int64_t* counters();
+    int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
+    void zero(void* p, size_t bytes);
+    void fillNeg(void* p, size_t bytes);
+    void buildBuckets(bool setTwins);
+    void refreshHashTable();
+    void checkValid(const char* what);
+    void simplifyInit(int32_t** valence, int32_t** count);
+    void simplifyCompactAndRebuild(int dropType, bool p3333);
+    void buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag);
+};
+
+}  // namespace mdmc

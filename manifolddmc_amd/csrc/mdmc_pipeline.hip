@@ -1,0 +1,1554 @@
+// mdmc_pipeline.hip -- halfedge mesh, simplifications, non-manifold repair and
+// the host-side stage driver.
+//
+// Restates (MarchingCubes/VisualDMC/src/DMC):
+//   HalfedgeMesh::initialize / collectHalfedgeElements / connectHalfedgeTwins /
+//     refresh*Flags                                   Halfedges/HalfedgeMesh.cu:14-291, :783-857
+//   simplify3X3Y, simplify3X3YOld, simplify3333       Halfedges/HalfedgeMesh.cu:296-683, :859-1048
+//   HalfedgeMesh::refreshHashTable                    Halfedges/HalfedgeMesh.cu:1063-1100
+//   ManifoldCreator::fixManifoldTwins                 PostProcesses/ManifoldCreator.cu:39-312, :471-486
+//   ManifoldSplitter::run and its kernels             PostProcesses/ManifoldSplitter.cu:25-1387
+//   ManifoldCreator::detectNonManifoldVertices        PostProcesses/ManifoldCreator.cu:319-536
+//
+// Where the reference appends with atomics or lets racing threads overwrite a
+// value, this code produces the result of one fixed legal schedule: threads of a
+// launch taking effect in ascending thread-id order.  Appends become prefix sums,
+// "last writer wins" becomes atomicMax, and per-bucket slot order becomes a sort
+// by (face, position in face).  The vertex-pair hash table becomes a CSR list of
+// outgoing halfedges per vertex: a bucket is the set of halfedges joining the
+// same two vertices, found by scanning two short lists.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "mdmc_arena.h"
+#include "mdmc_device.h"
+#include "mdmc_internal.h"
+#include "mdmc_pipeline.h"
+
+namespace mdmc {
+
+namespace {
+
+constexpr int kMaxBucket = 16;
+constexpr int kFanGuard = 4096;
+
+// ---------------------------------------------------------------- counters
+enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_NCOUNTERS = 8 };
+
+__device__ __forceinline__ void anomaly(int64_t* ctr) { atomicAdd((unsigned long long*)&ctr[A_ANOMALY], 1ULL); }
+
+__device__ __forceinline__ int heO(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h]; }
+__device__ __forceinline__ int heF(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h + 1]; }
+__device__ __forceinline__ int heN(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h + 2]; }
+__device__ __forceinline__ int heT(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h + 3]; }
+__device__ __forceinline__ void setT(int4* he, int h, int t) { ((int*)he)[4 * (int64_t)h + 3] = t; }
+
+// ---------------------------------------------------------------- halfedge construction
+__global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ vert_he, int32_t* __restrict__ hkey) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v = quads[q];
+    const int b = (int)(4 * q);
+    he[b + 0] = make_int4(v.x, (int)q, b + 1, -1);
+    he[b + 1] = make_int4(v.y, (int)q, b + 2, -1);
+    he[b + 2] = make_int4(v.z, (int)q, b + 3, -1);
+    he[b + 3] = make_int4(v.w, (int)q, b + 0, -1);
+    face_he[q] = b;
+    fattr[q] = qattr[q];
+    atomicMax(&vert_he[v.x], b + 0);  // "last writer wins" in ascending thread order
+    atomicMax(&vert_he[v.y], b + 1);
+    atomicMax(&vert_he[v.z], b + 2);
+    atomicMax(&vert_he[v.w], b + 3);
+    hkey[b + 0] = b + 0;
+    hkey[b + 1] = b + 1;
+    hkey[b + 2] = b + 2;
+    hkey[b + 3] = b + 3;
+}
+
+// bucket order after refreshHashTable: (face, position along the face from its first halfedge)
+__global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, int32_t* __restrict__ hkey) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    int h = face_he[f];
+    for (int i = 0; i < 4; ++i) {
+        hkey[h] = (int32_t)(4 * f + i);
+        h = heN(he, h);
+    }
+}
+
+__global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ deg, int64_t* ctr) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) { anomaly(ctr); return; }
+    atomicAdd(&deg[o], 1);
+}
+
+__global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const int64_t* __restrict__ off, int32_t* __restrict__ cursor,
+                           int32_t* __restrict__ list) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) return;
+    const int slot = atomicAdd(&cursor[o], 1);
+    list[off[o] + slot] = (int32_t)h;
+}
+
+// the halfedges joining the two vertices of h, sorted by bucket key; returns the true count
+__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
+                            const int32_t* __restrict__ hkey, int mem[kMaxBucket]) {
+    const int a = heO(he, h), b = heO(he, heN(he, h));
+    int n = 0;
+    for (int64_t i = off[a]; i < off[a + 1]; ++i) {
+        const int g = list[i];
+        if (heO(he, heN(he, g)) == b) {
+            if (n < kMaxBucket) mem[n] = g;
+            ++n;
+        }
+    }
+    if (a != b) {
+        for (int64_t i = off[b]; i < off[b + 1]; ++i) {
+            const int g = list[i];
+            if (heO(he, heN(he, g)) == a) {
+                if (n < kMaxBucket) mem[n] = g;
+                ++n;
+            }
+        }
+    }
+    const int m = n < kMaxBucket ? n : kMaxBucket;
+    for (int i = 1; i < m; ++i) {  // insertion sort by key
+        const int x = mem[i];
+        const int kx = hkey[x];
+        int j = i - 1;
+        while (j >= 0 && hkey[mem[j]] > kx) {
+            mem[j + 1] = mem[j];
+            --j;
+        }
+        mem[j + 1] = x;
+    }
+    return n;
+}
+
+// the four slots of a bucket: entries beyond the fourth land in slot 0 (glm::ivec4::operator[] default)
+__device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, int s[4]) {
+    for (int i = 0; i < 4; ++i) s[i] = (i < n) ? mem[i] : -1;
+    for (int k = 4; k < n && k < kMaxBucket; ++k) s[0] = mem[k];
+}
+
+// connectHalfedgeTwins (HalfedgeMesh.cu:173-291) for setTwins, bucket sizes for the flag kernels
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
+                         const int32_t* __restrict__ hkey, int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, int64_t* ctr) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int h = (int)hh;
+    int mem[kMaxBucket];
+    const int n = gatherBucket(h, he, off, list, hkey, mem);
+    if (n > kMaxBucket) anomaly(ctr);
+    int r = 0;
+    for (int i = 0; i < n && i < kMaxBucket; ++i)
+        if (mem[i] == h) r = i;
+    bsize[h] = n;
+    brank[h] = r;
+    if (!setTwins) return;
+    int t = -1;
+    if (n == 2) {
+        t = mem[1 - r];
+    } else if (n == 3) {
+        t = mem[(r + 1) % 3];
+    } else if (n == 4) {
+        const int v0 = heO(he, mem[0]), v1 = heO(he, mem[1]), v2 = heO(he, mem[2]);
+        int pair[4];
+        if (v0 != v1) { pair[0] = 1; pair[1] = 0; pair[2] = 3; pair[3] = 2; }
+        else if (v0 != v2) { pair[0] = 2; pair[2] = 0; pair[1] = 3; pair[3] = 1; }
+        else { pair[0] = 3; pair[3] = 0; pair[1] = 2; pair[2] = 1; }
+        t = mem[pair[r]];
+    }
+    setT(he, h, t);
+}
+
+// face NM flags from 4-face edges (connectHalfedgeTwins case 4 / refreshFaceFlagsGPU)
+__global__ void kFaceFlags(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
+                           uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int refresh) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    int h = face_he[f];
+    bool nm = false;
+    for (int i = 0; i < 4; ++i) {
+        nm = nm || bsize[h] == 4;
+        h = heN(he, h);
+    }
+    uint32_t a = fattr[f];
+    if (refresh) a &= ~(QA_PATTERN | QA_REMOVE | QA_NM);  // ResetFaceFlags
+    if (nm) a |= QA_NM;
+    fattr[f] = a;
+    if (qattr && nm) qattr[f] |= QA_NM;
+}
+
+// refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
+__global__ void kVertexFlags(const int64_t* __restrict__ off, const int32_t* __restrict__ list, int64_t V, const int32_t* __restrict__ bsize,
+                             uint32_t* __restrict__ vflags) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint32_t f = 0;
+    for (int64_t i = off[v]; i < off[v + 1]; ++i) {
+        const int n = bsize[list[i]];
+        if (n == 4) f |= VF_NM;
+        if (n == 3) f |= VF_WEIRD;
+        if (n == 1) f |= VF_BOUNDARY;
+    }
+    vflags[v] = f;
+}
+
+// ---------------------------------------------------------------- simplifications
+__global__ void kValence(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ valence) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    atomicAdd(&valence[heO(he, (int)h)], 1);
+    if (heT(he, (int)h) == -1) atomicAdd(&valence[heO(he, heN(he, (int)h))], 1);
+}
+
+__device__ __forceinline__ bool pattern1(const int v[4]) { return v[0] == 3 && v[1] >= 5 && v[2] == 3 && v[3] >= 5; }
+__device__ __forceinline__ bool pattern2(const int v[4]) { return v[0] >= 5 && v[1] == 3 && v[2] >= 5 && v[3] == 3; }
+
+__global__ void kMark3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ valence) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    uint32_t a = qattr[q];
+    if ((a & QA_NM) || (a & QA_BOUNDARY)) return;
+    a &= 0x3F;
+    const int4 v = quads[q];
+    const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
+    if (pattern1(val) || pattern2(val)) a |= QA_PATTERN;
+    qattr[q] = a;
+}
+
+__global__ void kCount3x3yOld(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ valence,
+                              int32_t* __restrict__ count) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    uint32_t a = qattr[q] & 0x3F;
+    const int4 v = quads[q];
+    const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
+    if (pattern1(val)) {
+        atomicAdd(&count[v.x], 1);
+        atomicAdd(&count[v.z], 1);
+        a |= QA_PATTERN;
+    } else if (pattern2(val)) {
+        atomicAdd(&count[v.y], 1);
+        atomicAdd(&count[v.w], 1);
+        a |= QA_PATTERN;
+    }
+    qattr[q] = a;
+}
+
+// HalfedgeMesh.cu:368-388
+__device__ __forceinline__ void handleMerge(float* pos, float* nrm, int32_t* vtype, int32_t* vtarget, int x, int y, int z, int w) {
+    f3 p = mk3(pos[3 * y], pos[3 * y + 1], pos[3 * y + 2]);
+    const f3 pw = mk3(pos[3 * w], pos[3 * w + 1], pos[3 * w + 2]);
+    p = p + scl(0.5f, pw - p);
+    f3 n = mk3(nrm[3 * y], nrm[3 * y + 1], nrm[3 * y + 2]);
+    const f3 nw = mk3(nrm[3 * w], nrm[3 * w + 1], nrm[3 * w + 2]);
+    n = n + scl(0.5f, nw - n);
+    n = normalize3(n);
+    pos[3 * x] = p.x; pos[3 * x + 1] = p.y; pos[3 * x + 2] = p.z;
+    nrm[3 * x] = n.x; nrm[3 * x + 1] = n.y; nrm[3 * x + 2] = n.z;
+    vtype[z] = 3;  // VertexType::PatternRemovable
+    vtarget[z] = x;
+}
+
+// the four edge-neighbour faces; false when an edge has no twin (the reference
+// then reads halfedges[-1], which is undefined; here the quad is left alone)
+__device__ __forceinline__ bool neighborFaces(const int4* __restrict__ he, int first, int nb[4]) {
+    int h = first;
+    for (int i = 0; i < 4; ++i) {
+        const int t = heT(he, h);
+        if (t < 0) return false;
+        nb[i] = heF(he, t);
+        h = heN(he, h);
+    }
+    return true;
+}
+
+// mergeVerticesP3X3YColor (colour = 1) and mergeVerticesP3X3Y (colour = 0), HalfedgeMesh.cu:391-498
+__global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int4* __restrict__ he,
+                           const int32_t* __restrict__ face_he, const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
+                           float* __restrict__ pos, float* __restrict__ nrm, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const uint32_t a = qattr[q];
+    if (!colour && ((a & QA_NM) || (a & QA_BOUNDARY))) return;
+    if (!(a & QA_PATTERN)) return;
+    int nb[4];
+    if (!neighborFaces(he, face_he[q], nb)) return;
+    for (int i = 0; i < 4; ++i)
+        if ((qattr[nb[i]] & QA_NM) || (a & QA_BOUNDARY)) return;
+    if (colour) {
+        const uint32_t c = a & QA_COLOR;
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t na = qattr[nb[i]];
+            if ((na & QA_PATTERN) && ((na & QA_COLOR) <= c)) return;
+        }
+    }
+    const int4 v = quads[q];
+    const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
+    if (pattern1(val) && (colour || (count[v.x] == 1 && count[v.z] == 1))) {
+        handleMerge(pos, nrm, vtype, vtarget, v.x, v.y, v.z, v.w);
+        atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
+    } else if (pattern2(val) && (colour || (count[v.y] == 1 && count[v.w] == 1))) {
+        handleMerge(pos, nrm, vtype, vtarget, v.y, v.x, v.w, v.z);
+        atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
+    }
+}
+
+// markElementsP3333 first half: every eligible face clears its pattern bits before any face marks
+__global__ void kClear3333(uint32_t* __restrict__ fattr, int64_t F) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint32_t a = fattr[f];
+    if ((a & QA_NM) || (a & QA_BOUNDARY)) return;
+    fattr[f] = a & 0x3F;
+}
+
+// markElementsP3333 second half, HalfedgeMesh.cu:547-628
+__global__ void kMark3333(const int4* __restrict__ he, const int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int64_t F,
+                          const int32_t* __restrict__ valence, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint32_t a0 = fattr[f];
+    if ((a0 & QA_NM) || (a0 & QA_BOUNDARY)) return;
+    int qh[4];
+    qh[0] = face_he[f];
+    for (int i = 0; i < 4; ++i) {
+        if (i < 3) qh[i + 1] = heN(he, qh[i]);
+        if (valence[heO(he, qh[i])] != 3) return;
+    }
+    int nbf[4], nbv[4], nbval[4];
+    for (int i = 0; i < 4; ++i) {
+        const int tw = heT(he, qh[i]);
+        if (tw < 0) return;  // no neighbour across a boundary edge
+        nbf[i] = heF(he, tw);
+        const uint32_t na = fattr[nbf[i]];
+        if ((na & QA_NM) || (na & QA_BOUNDARY)) return;
+        nbv[i] = heO(he, heN(he, heN(he, tw)));
+        nbval[i] = valence[nbv[i]];
+        if (nbval[i] == 3) return;
+    }
+    bool all4 = true;
+    for (int i = 0; i < 4; ++i) all4 = all4 && nbval[i] == 4;
+    if (all4) return;
+    atomicOr(&fattr[f], (uint32_t)QA_PATTERN);
+    for (int i = 0; i < 4; ++i) {
+        atomicOr(&fattr[nbf[i]], (uint32_t)QA_REMOVE);
+        const int v = heO(he, qh[i]);
+        vtype[v] = 1;  // VertexType::Pattern3333
+        vtarget[v] = nbv[i];
+    }
+}
+
+__global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int dropType, int32_t* __restrict__ keep) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    keep[v] = vtype[v] == dropType ? 0 : 1;
+}
+
+__global__ void kCompactVertices(const float* __restrict__ pos, const float* __restrict__ nrm, const int32_t* __restrict__ keep,
+                                 const int64_t* __restrict__ addr, int64_t V, float* __restrict__ npos, float* __restrict__ nnrm,
+                                 int32_t* __restrict__ address) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    if (!keep[v]) return;
+    const int64_t o = addr[v];
+    address[v] = (int32_t)o;
+    for (int k = 0; k < 3; ++k) {
+        npos[3 * o + k] = pos[3 * v + k];
+        nnrm[3 * o + k] = nrm[3 * v + k];
+    }
+}
+
+__global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t* __restrict__ keep) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    keep[q] = (attr[q] & QA_REMOVE) ? 0 : 1;
+}
+
+// removeQuadrilateralsP3X3Y (p3333 = 0) / removeQuadsP3333 (p3333 = 1), HalfedgeMesh.cu:513-538, :657-683
+__global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __restrict__ attr, int64_t Q, const int32_t* __restrict__ keep,
+                              const int64_t* __restrict__ qaddr, const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
+                              const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    if (!keep[q]) return;
+    const uint32_t a = attr[q];
+    const int4 v4 = quads[q];
+    const int v[4] = {v4.x, v4.y, v4.z, v4.w};
+    int o[4];
+    for (int i = 0; i < 4; ++i) {
+        if (p3333) o[i] = (a & QA_PATTERN) ? address[vtarget[v[i]]] : address[v[i]];
+        else o[i] = (vtype[v[i]] == 3) ? address[vtarget[v[i]]] : address[v[i]];
+    }
+    const int64_t d = qaddr[q];
+    nquads[d] = make_int4(o[0], o[1], o[2], o[3]);
+    uint32_t na = QA_COLOR;
+    if (a & QA_NM) na |= QA_NM;
+    if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
+    nattr[d] = na;
+}
+
+// ---------------------------------------------------------------- fixManifoldTwins
+__device__ __forceinline__ void cellOf(const GridDev& g, f3 p, int c[3]) {
+    const f3 q = (p - mk3(g.ox, g.oy, g.oz)) / mk3(g.dx, g.dy, g.dz);
+    c[0] = (int)floorf(q.x);
+    c[1] = (int)floorf(q.y);
+    c[2] = (int)floorf(q.z);
+}
+__device__ __forceinline__ f3 nudge(int it) {
+    float x = (it & 1) ? 1.f : 0.f, y = (it & 2) ? 1.f : 0.f, z = (it & 4) ? 1.f : 0.f;
+    if (it & 8) x = -x;
+    if (it & 16) y = -y;
+    if (it & 32) z = -z;
+    return mk3(x * 0.0001f, y * 0.0001f, z * 0.0001f);
+}
+__device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
+
+// reconnectHalfedges, ManifoldCreator.cu:257-312
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
+                          const int32_t* __restrict__ hkey, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
+                          const float* __restrict__ pos, GridDev g, float iso, int64_t* ctr) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int h = (int)hh;
+    if (bsize[h] < 4 || brank[h] != 0) return;
+    int mem[kMaxBucket];
+    const int n = gatherBucket(h, he, off, list, hkey, mem);
+    int s[4];
+    bucketSlots(mem, n, s);
+    const int h0 = s[0];
+    f3 p0 = ld3(pos, heO(he, h0));
+    f3 p1 = ld3(pos, heO(he, heN(he, h0)));
+    int c0[3], c1[3];
+    cellOf(g, p0, c0);
+    cellOf(g, p1, c1);
+    bool found = false;
+    for (int i = 0; i < 64 && !found; ++i) {
+        const f3 o0 = nudge(i);
+        int a[3];
+        cellOf(g, p0 + o0, a);
+        for (int k = 0; k < 64; ++k) {
+            const f3 o1 = nudge(k);
+            int b[3];
+            cellOf(g, p1 + o1, b);
+            const int sum = abs(a[0] - b[0]) + abs(a[1] - b[1]) + abs(a[2] - b[2]);
+            if (sum == 1) {
+                p0 = p0 + o0;
+                cellOf(g, p0, c0);
+                p1 = p1 + o1;
+                cellOf(g, p1, c1);
+                found = true;
+                break;
+            }
+        }
+    }
+    const int64_t g0 = (int64_t)c0[2] * g.nx * g.ny + (int64_t)c0[1] * g.nx + c0[0];
+    const int64_t g1 = (int64_t)c1[2] * g.nx * g.ny + (int64_t)c1[1] * g.nx + c1[0];
+    int ca[3], cb[3];
+    for (int i = 0; i < 3; ++i) {
+        ca[i] = g0 < g1 ? c0[i] : c1[i];
+        cb[i] = g0 < g1 ? c1[i] : c0[i];
+    }
+    int wn[3], xo[3] = {0, 0, 0}, yo[3] = {0, 0, 0}, mi[3];
+    for (int i = 0; i < 3; ++i) {
+        wn[i] = max(ca[i], cb[i]) - min(ca[i], cb[i]);
+        mi[i] = max(ca[i], cb[i]);
+    }
+    if (wn[0] == 0) {
+        xo[0] = 1;
+        if (wn[1] == 0) yo[1] = 1;
+        else yo[2] = 1;
+    } else {
+        xo[1] = 1;
+        yo[2] = 1;
+    }
+    if (mi[0] < 0 || mi[1] < 0 || mi[2] < 0 || mi[0] + 1 >= g.nx || mi[1] + 1 >= g.ny || mi[2] + 1 >= g.nz) {
+        anomaly(ctr);
+        return;
+    }
+    auto gv = [&](int dx, int dy, int dz) { return g.v[(int64_t)(mi[2] + dz) * g.nx * g.ny + (int64_t)(mi[1] + dy) * g.nx + (mi[0] + dx)]; };
+    const float w0 = gv(0, 0, 0), w1 = gv(xo[0], xo[1], xo[2]), w2 = gv(xo[0] + yo[0], xo[1] + yo[1], xo[2] + yo[2]), w3 = gv(yo[0], yo[1], yo[2]);
+    const f3 o = mk3(g.ox, g.oy, g.oz), d = mk3(g.dx, g.dy, g.dz);
+    const f3 pc0 = o + d * mk3((float)mi[0], (float)mi[1], (float)mi[2]);
+    const f3 pc2 = o + d * mk3((float)(mi[0] + xo[0] + yo[0]), (float)(mi[1] + xo[1] + yo[1]), (float)(mi[2] + xo[2] + yo[2]));
+    const bool omitX = xo[0] == 0 && yo[0] == 0;
+    const bool omitY = xo[1] == 0 && yo[1] == 0;
+    auto down = [&](f3 p, float& u, float& v) {
+        if (omitX) { u = p.y; v = p.z; }
+        else if (omitY) { u = p.x; v = p.z; }
+        else { u = p.x; v = p.y; }
+    };
+    float minU, minV, maxU, maxV;
+    down(pc0, minU, minV);
+    down(pc2, maxU, maxV);
+    int sorted[4] = {-1, -1, -1, -1};
+    for (int i = 0; i < 4; ++i) {
+        const int hid = s[i];
+        const f3 vp = ld3(pos, heO(he, heN(he, heN(he, hid))));
+        float u, v;
+        down(vp, u, v);
+        if (v - minV < 0.f) sorted[0] = hid;
+        if (u - maxU > 0.f) sorted[1] = hid;
+        if (v - maxV > 0.f) sorted[2] = hid;
+        if (u - minU < 0.f) sorted[3] = hid;
+    }
+    for (int i = 0; i < 4; ++i)
+        if (sorted[i] == -1) {
+            anomaly(ctr);
+            return;
+        }
+    // AsymptoticDecider2D(w0, w1, w2, w3).quadrant0BelongsTogether(iso), AsymptoticDecider.h:107-164
+    const float C = w0 - w1 - w3 + w2;
+    const float delta = (w0 * w2 - w1 * w3) / C;
+    const bool q0 = (w0 < iso && delta >= iso) || (w0 >= iso && delta < iso);
+    if (!q0) {
+        setT(he, sorted[0], sorted[1]); setT(he, sorted[1], sorted[0]);
+        setT(he, sorted[2], sorted[3]); setT(he, sorted[3], sorted[2]);
+    } else {
+        setT(he, sorted[0], sorted[3]); setT(he, sorted[3], sorted[0]);
+        setT(he, sorted[1], sorted[2]); setT(he, sorted[2], sorted[1]);
+    }
+}
+
+// ---------------------------------------------------------------- ManifoldSplitter
+__global__ void kNMFacePred(const uint32_t* __restrict__ fattr, int64_t F, int32_t* __restrict__ pred) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    pred[f] = (fattr[f] & QA_NM) ? 1 : 0;
+}
+
+__global__ void kScatterIndex(const int32_t* __restrict__ pred, const int64_t* __restrict__ addr, int64_t n, int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (pred[i]) out[addr[i]] = (int32_t)i;
+}
+
+struct WC {
+    // vertices (capacity 8n)
+    float *pos, *nrm, *pos0, *nrm0;
+    uint32_t* vflags;
+    int32_t *v_c2o, *v_o2c;
+    // halfedges (capacity 20n)
+    int4* he;
+    uint32_t* hflags;
+    int32_t *h_c2o, *h_o2c;
+    // faces (capacity 5n)
+    int32_t* f_first;
+    uint32_t* f_attr;
+    uint32_t* fflags;
+    int32_t *f_c2o, *f_o2c;
+    int n;
+};
+
+// copyVerticesToWorkingCopy / copyHalfedgesToWorkingCopy / copyFacesToWorkingCopy (ManifoldSplitter.cu:307-457)
+__global__ void kWcCopy(const int32_t* __restrict__ nm, int n, const int4* __restrict__ he, const int32_t* __restrict__ face_he,
+                        const uint32_t* __restrict__ fattr, const float* __restrict__ pos, const float* __restrict__ nrm, WC w) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int f = nm[r];
+    int h = face_he[f];
+    for (int i = 0; i < 4; ++i) {
+        const int4 e = he[h];
+        const int c = 4 * r + i;
+        const int v = e.x;
+        for (int k = 0; k < 3; ++k) {
+            w.pos[3 * c + k] = pos[3 * (int64_t)v + k];
+            w.nrm[3 * c + k] = nrm[3 * (int64_t)v + k];
+        }
+        w.v_c2o[c] = v;
+        atomicMax(&w.v_o2c[v], c);  // racy "last writer" in ascending order
+        w.he[c] = e;
+        w.h_c2o[c] = h;
+        w.h_o2c[h] = c;
+        h = e.z;
+    }
+    w.f_first[r] = 4 * r;
+    w.f_attr[r] = fattr[f];
+    w.f_c2o[r] = f;
+    w.f_o2c[f] = r;
+}
+
+// updateWorkingCopyHalfedges, ManifoldSplitter.cu:463-505
+__global__ void kWcUpdate(WC w, int count) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    int4 e = w.he[c];
+    e.x = w.v_o2c[e.x];
+    e.y = w.f_o2c[e.y];
+    e.z = w.h_o2c[e.z];
+    if (e.w >= 0) {
+        const int m = w.h_o2c[e.w];
+        e.w = (m == -1) ? (-e.w - 2) : m;
+    }
+    w.he[c] = e;
+}
+
+// markHalfedgesAsNonManifold, ManifoldSplitter.cu:527-555: the four slotted halfedges of every full bucket
+__global__ void kWcNMFlags(int64_t H, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, WC w, int64_t* ctr) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int n = bsize[hh];
+    if (n < 4) return;
+    const int r = brank[hh];
+    const bool slotted = (n == 4) ? true : (r == 1 || r == 2 || r == 3 || r == n - 1);
+    if (!slotted) return;
+    const int c = w.h_o2c[hh];
+    if (c < 0) { anomaly(ctr); return; }
+    w.hflags[c] |= VF_NM;
+    atomicOr(&w.vflags[w.he[c].x], (uint32_t)VF_NM);
+}
+
+// performSubdivideQuads, ManifoldSplitter.cu:570-731.  Face r appends 4 vertices at 4n+4r,
+// 16 halfedges at 4n+16r and 4 faces at n+4r, which is where the ascending schedule puts them.
+__global__ void kSubdivide(const int32_t* __restrict__ nm, WC w) {
+    const int n = w.n;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int faceID = w.f_o2c[nm[r]];
+    int fh[4], vid[4];
+    int cur = w.f_first[faceID];
+    for (int i = 0; i < 4; ++i) {
+        fh[i] = cur;
+        const int4 e = w.he[cur];
+        vid[i] = e.x;
+        cur = e.z;
+    }
+    f3 vp[4], vn[4];
+    for (int i = 0; i < 4; ++i) {
+        vp[i] = ld3(w.pos, vid[i]);
+        vn[i] = ld3(w.nrm, vid[i]);
+    }
+    const float us[4] = {0.33f, 0.67f, 0.67f, 0.33f}, vs[4] = {0.33f, 0.33f, 0.67f, 0.67f};
+    int inner[4];
+    for (int k = 0; k < 4; ++k) {
+        const f3 ap = mixv(vp[0], vp[1], us[k]), bp = mixv(vp[3], vp[2], us[k]);
+        const f3 an = mixv(vn[0], vn[1], us[k]), bn = mixv(vn[3], vn[2], us[k]);
+        const f3 p = mixv(ap, bp, vs[k]), q = mixv(an, bn, vs[k]);
+        const int c = 4 * n + 4 * r + k;
+        w.pos[3 * c] = p.x; w.pos[3 * c + 1] = p.y; w.pos[3 * c + 2] = p.z;
+        w.nrm[3 * c] = q.x; w.nrm[3 * c + 1] = q.y; w.nrm[3 * c + 2] = q.z;
+        w.vflags[c] &= ~(uint32_t)VF_REMOVE;
+        inner[k] = c;
+    }
+    const int hb = 4 * n + 16 * r;
+    int ih[4];
+    for (int i = 0; i < 4; ++i) ih[i] = hb + i;
+    for (int i = 0; i < 4; ++i) w.he[ih[i]] = make_int4(inner[i], faceID, ih[(i + 1) & 3], -1);
+    w.f_first[faceID] = ih[0];
+    int of[4][4];
+    int prev = 3;
+    for (int i = 0; i < 4; ++i) {
+        for (int k = 0; k < 3; ++k) of[i][k] = hb + 4 + 3 * i + k;
+        of[i][3] = fh[prev];
+        const int nf = n + 4 * r + i;
+        const int qv[3] = {vid[i], inner[i], inner[prev]};
+        for (int k = 0; k < 3; ++k) w.he[of[i][k]] = make_int4(qv[k], nf, of[i][k + 1], -1);
+        int4 e3 = w.he[fh[prev]];
+        e3.y = nf;
+        e3.z = of[i][0];
+        w.he[fh[prev]] = e3;
+        w.f_first[nf] = of[i][0];
+        w.f_attr[nf] = QA_COLOR;  // HalfedgeFace()
+        prev = i;
+    }
+    prev = 3;
+    for (int i = 0; i < 4; ++i) {
+        setT(w.he, of[i][2], of[prev][0]);
+        setT(w.he, of[prev][0], of[i][2]);
+        setT(w.he, of[i][1], ih[prev]);
+        setT(w.he, ih[prev], of[i][1]);
+        prev = i;
+    }
+    for (int i = 0; i < 4; ++i) {  // swapHalfedgeMappings(outer, inner)
+        const int a = fh[i], b = ih[i];
+        const int ao = w.h_c2o[a], bo = w.h_c2o[b];
+        w.h_c2o[a] = bo;
+        w.h_c2o[b] = ao;
+        if (ao != -1 && bo == -1) w.h_o2c[ao] = b;
+    }
+}
+
+// performCollapseNonManifoldHalfedges: which working-copy halfedges collapse
+__global__ void kCollapsePred(const WC w, int count, int32_t* __restrict__ pred, int64_t* ctr) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    int p = 0;
+    if (w.hflags[h] & VF_NM) {
+        const int t = heT(w.he, h);
+        if (t < 0) anomaly(ctr);
+        else if (t >= h) p = 1;
+    }
+    pred[h] = p;
+}
+
+// collapseHalfedge, ManifoldSplitter.cu:749-835: remove flags plus the merge marks, recorded in call order
+__global__ void kCollapse(WC w, int count, const int32_t* __restrict__ pred, const int64_t* __restrict__ rank, int2* __restrict__ vrec,
+                          int2* __restrict__ hrec) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count || !pred[h]) return;
+    const int64_t k = rank[h];
+    int cf[2][4];
+    int4 cur = w.he[h];
+    cf[0][0] = h;
+    cf[1][0] = cur.w;
+    int4 ct = w.he[cur.w];
+    for (int i = 1; i < 4; ++i) {
+        cf[0][i] = cur.z;
+        cf[1][4 - i] = ct.z;
+        cur = w.he[cur.z];
+        ct = w.he[ct.z];
+    }
+    for (int fc = 0; fc < 2; ++fc) {
+        for (int i = 0; i < 4; ++i) atomicOr(&w.hflags[cf[fc][i]], (uint32_t)VF_REMOVE);
+        atomicOr(&w.fflags[heF(w.he, cf[fc][0])], (uint32_t)VF_REMOVE);
+    }
+    const int a = heO(w.he, cf[0][2]), b = heO(w.he, cf[1][1]);
+    const int c = heO(w.he, cf[0][3]), d = heO(w.he, cf[1][2]);
+    vrec[4 * k + 0] = make_int2(a, b);
+    vrec[4 * k + 1] = make_int2(b, a);
+    vrec[4 * k + 2] = make_int2(c, d);
+    vrec[4 * k + 3] = make_int2(d, c);
+    for (int i = 1; i < 4; ++i) {
+        const int ta = heT(w.he, cf[0][i]), tb = heT(w.he, cf[1][i]);
+        int2* o = hrec + 12 * k + 4 * (i - 1);
+        o[0] = make_int2(cf[0][i], tb);
+        o[1] = make_int2(tb, cf[0][i]);
+        o[2] = make_int2(cf[1][i], ta);
+        o[3] = make_int2(ta, cf[1][i]);
+    }
+}
+
+__global__ void kRecCount(const int2* __restrict__ rec, int64_t R, int count, int32_t* __restrict__ cnt, int64_t* ctr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    const int t = rec[i].x;
+    if (t < 0 || t >= count) { anomaly(ctr); return; }
+    atomicAdd(&cnt[t], 1);
+}
+__global__ void kRecScatter(const int2* __restrict__ rec, int64_t R, int count, const int64_t* __restrict__ roff, int32_t* __restrict__ cursor,
+                            int32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    const int t = rec[i].x;
+    if (t < 0 || t >= count) return;
+    idx[roff[t] + atomicAdd(&cursor[t], 1)] = (int32_t)i;
+}
+// markVerticesAsMergable / markHalfedgesAsMergeable applied in record order to glm::ivec2
+// storage: the counter always advances; the second mark goes to .y, marks 1, 3, 4, 5 to .x,
+// later marks are dropped (ManifoldSplitter.cu:749-765)
+__global__ void kRecApply(const int2* __restrict__ rec, int count, const int64_t* __restrict__ roff, int32_t* __restrict__ idx,
+                          const int32_t* __restrict__ cnt, int32_t* __restrict__ num, int32_t* __restrict__ ex, int32_t* __restrict__ ey) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int m = cnt[t];
+    num[t] = 1 + m;
+    ex[t] = -1;
+    ey[t] = -1;
+    if (m == 0) return;
+    int32_t* s = idx + roff[t];
+    for (int i = 1; i < m; ++i) {  // records in call order
+        const int x = s[i];
+        int j = i - 1;
+        while (j >= 0 && s[j] > x) {
+            s[j + 1] = s[j];
+            --j;
+        }
+        s[j + 1] = x;
+    }
+    for (int j = 0; j < m && j <= 4; ++j) {
+        const int p = rec[s[j]].y;
+        if (j == 1) ey[t] = p;
+        else ex[t] = p;
+    }
+}
+
+// DeviceMergeList::getAllPartnersOf, ManifoldSplitter.cu:25-62
+__device__ int partnersOf(int v, int size, const int32_t* __restrict__ num, const int32_t* __restrict__ ex, const int32_t* __restrict__ ey,
+                          int out[12], int64_t* ctr) {
+    for (int i = 0; i < 12; ++i) out[i] = -1;
+    if (v < 0 || v >= size) { anomaly(ctr); return 1; }
+    int n = num[v];
+    if (n <= 1) return n;
+    for (int i = 0; i < n - 1; ++i) {
+        if (i >= 12) { anomaly(ctr); break; }
+        out[i] = (i == 1) ? ey[v] : ex[v];
+    }
+    for (int i = 0; i < n - 1 && i < 12; ++i) {
+        const int p = out[i];
+        if (p < 0 || p >= size) { anomaly(ctr); continue; }
+        const int np = num[p];
+        for (int k = 0; k < np - 1; ++k) {
+            const int pp = (k == 1) ? ey[p] : ex[p];
+            bool contained = v == pp;
+            for (int z = 0; z < n - 1 && z < 12; ++z) contained = contained || out[z] == pp;
+            if (!contained) {
+                if (n - 1 >= 12) { anomaly(ctr); continue; }
+                out[n - 1] = pp;
+                ++n;
+            }
+        }
+    }
+    return n;
+}
+
+// mergeVertices, ManifoldSplitter.cu:936-990.  A vertex that survives reads partners
+// with larger ids only, which no earlier thread writes, so reading the snapshot is the
+// ascending schedule's result.
+__global__ void kMergeVertices(WC w, int count, const int32_t* __restrict__ num, const int32_t* __restrict__ ex, const int32_t* __restrict__ ey,
+                               int64_t* ctr) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    int P[12];
+    const int np = partnersOf(v, count, num, ex, ey, P, ctr);
+    if (np == 1) return;
+    if (np < 1 || np > 12) { anomaly(ctr); return; }
+    int smallest = v;
+    f3 p = ld3(w.pos0, v), nn = ld3(w.nrm0, v);
+    bool obsolete = false;
+    for (int i = 0; i < np - 1; ++i) {
+        const int q = P[i];
+        if (q < 0 || q >= count) { anomaly(ctr); continue; }
+        p = p + ld3(w.pos0, q);
+        nn = nn + ld3(w.nrm0, q);
+        if (q < smallest) { obsolete = true; smallest = q; }
+    }
+    if (obsolete) {
+        atomicOr(&w.vflags[v], (uint32_t)VF_REMOVE);
+        return;
+    }
+    p = divs(p, (float)np);
+    w.pos[3 * smallest] = p.x; w.pos[3 * smallest + 1] = p.y; w.pos[3 * smallest + 2] = p.z;
+    w.nrm[3 * smallest] = p.x; w.nrm[3 * smallest + 1] = p.y; w.nrm[3 * smallest + 2] = p.z;  // position stored as normal, :989
+}
+
+// updateHalfedgeVertexIDs, ManifoldSplitter.cu:992-1019
+__global__ void kUpdateVertexIDs(WC w, int count, int vcount, const int32_t* __restrict__ num, const int32_t* __restrict__ ex,
+                                 const int32_t* __restrict__ ey, int64_t* ctr) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    const int v = heO(w.he, h);
+    int P[12];
+    const int np = partnersOf(v, vcount, num, ex, ey, P, ctr);
+    if (np == 1) return;
+    int smallest = v;
+    for (int i = 0; i < np - 1; ++i)
+        if (P[i] >= 0 && P[i] < smallest) smallest = P[i];
+    ((int*)w.he)[4 * (int64_t)h] = smallest;
+}
+
+// mergeHalfedges, ManifoldSplitter.cu:1021-1067, planned in parallel on the pre-launch twins.
+// valid[h] = new twin or -1.  Flags a conflict when a partner whose twin this launch
+// writes is read, or a halfedge would be written twice; then the ascending-order
+// single-thread kernel below runs instead.
+__device__ int planMerge(const WC& w, int h, int count, const int32_t* __restrict__ num, const int32_t* __restrict__ ex, const int32_t* __restrict__ ey,
+                         int P[12], int& np, int64_t* ctr) {
+    np = partnersOf(h, count, num, ex, ey, P, ctr);
+    if (np == 1) return -1;
+    for (int i = 0; i < np - 1; ++i) {
+        if (P[i] < 0 || P[i] >= count) continue;
+        const int t = heT(w.he, P[i]);
+        if (t < 0 || t >= count) { anomaly(ctr); continue; }
+        if (!(w.hflags[t] & VF_REMOVE)) return t;
+    }
+    return -1;
+}
+
+__global__ void kMergePlan(WC w, int count, const int32_t* __restrict__ num, const int32_t* __restrict__ ex, const int32_t* __restrict__ ey,
+                           int32_t* __restrict__ valid, int32_t* __restrict__ firstWriter, int64_t* ctr) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    valid[h] = -1;
+    if (w.hflags[h] & VF_REMOVE) return;
+    int P[12], np;
+    const int t = planMerge(w, h, count, num, ex, ey, P, np, ctr);
+    if (t == -1 || t < h) return;
+    valid[h] = t;
+    atomicMin(&firstWriter[h], h);
+    atomicMin(&firstWriter[t], h);
+    if (atomicAdd(&firstWriter[count + h], 1) > 0) atomicAdd((unsigned long long*)&ctr[A_CONFLICT], 1ULL);
+    if (atomicAdd(&firstWriter[count + t], 1) > 0) atomicAdd((unsigned long long*)&ctr[A_CONFLICT], 1ULL);
+}
+
+__global__ void kMergeCheck(WC w, int count, const int32_t* __restrict__ num, const int32_t* __restrict__ ex, const int32_t* __restrict__ ey,
+                            const int32_t* __restrict__ firstWriter, int64_t* ctr) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    if (w.hflags[h] & VF_REMOVE) return;
+    int P[12];
+    const int np = partnersOf(h, count, num, ex, ey, P, ctr);
+    if (np == 1) return;
+    for (int i = 0; i < np - 1; ++i) {
+        if (P[i] < 0 || P[i] >= count) continue;
+        if (firstWriter[P[i]] < h) {  // an earlier thread rewrites a twin this thread reads
+            atomicAdd((unsigned long long*)&ctr[A_CONFLICT], 1ULL);
+            return;
+        }
+        const int t = heT(w.he, P[i]);
+        if (t < 0 || t >= count) continue;
+        if (!(w.hflags[t] & VF_REMOVE)) return;
+    }
+}
+
+__global__ void kMergeApply(WC w, int count, const int32_t* __restrict__ valid, const int64_t* ctr) {
+    if (ctr[A_CONFLICT] != 0) return;
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    const int t = valid[h];
+    if (t < 0) return;
+    setT(w.he, h, t);
+    setT(w.he, t, h);
+}
+
+__global__ void kMergeSequential(WC w, int count, const int32_t* __restrict__ num, const int32_t* __restrict__ ex, const int32_t* __restrict__ ey,
+                                 int64_t* ctr) {
+    if (ctr[A_CONFLICT] == 0) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int h = 0; h < count; ++h) {
+        if (w.hflags[h] & VF_REMOVE) continue;
+        int P[12], np;
+        const int t = planMerge(w, h, count, num, ex, ey, P, np, ctr);
+        if (t == -1 || t < h) continue;
+        setT(w.he, h, t);
+        setT(w.he, t, h);
+    }
+}
+
+// performMoveVerticesToSurface, ManifoldSplitter.cu:1130-1160
+__global__ void kMoveToSurface(WC w, int count, GridDev g, float iso) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    if (w.vflags[v] & VF_REMOVE) return;
+    f3 point = ld3(w.pos, v);
+    int c[3];
+    cellOf(g, point, c);
+    float s[8];
+    for (int k = 0; k < 8; ++k) {
+        const int x = min(max(c[0] + (k & 1), 0), g.nx - 1);
+        const int y = min(max(c[1] + ((k >> 1) & 1), 0), g.ny - 1);
+        const int z = min(max(c[2] + ((k >> 2) & 1), 0), g.nz - 1);
+        s[k] = g.v[(int64_t)z * g.nx * g.ny + (int64_t)y * g.nx + x];
+    }
+    const f3 o = mk3(g.ox, g.oy, g.oz), d = mk3(g.dx, g.dy, g.dz);
+    const f3 iv = mk3((float)c[0], (float)c[1], (float)c[2]);
+    const f3 iv1 = mk3((float)(c[0] + 1), (float)(c[1] + 1), (float)(c[2] + 1));
+    const f3 mn = o + iv * d, mx = o + iv1 * d;
+    point = (point - mn) / (mx - mn);
+    movePointToSurface(s, iso, point);
+    point = o + (iv + point) * d;
+    w.pos[3 * v] = point.x;
+    w.pos[3 * v + 1] = point.y;
+    w.pos[3 * v + 2] = point.z;
+}
+
+// copyWorkingCopiesBack, ManifoldSplitter.cu:1176-1295
+__global__ void kNewPred(const uint32_t* __restrict__ flags, const int32_t* __restrict__ c2o, int count, int32_t* __restrict__ pred) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    pred[i] = (!(flags[i] & VF_REMOVE) && c2o[i] == -1) ? 1 : 0;
+}
+
+// several working-copy vertices can copy one original; the last one in ascending order writes it
+__global__ void kLastCopy(WC w, int count, int32_t* __restrict__ last) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    if (w.vflags[v] & VF_REMOVE) return;
+    const int o = w.v_c2o[v];
+    if (o >= 0) atomicMax(&last[o], v);
+}
+
+__global__ void kCopyBackVertices(WC w, int count, const int64_t* __restrict__ addr, int64_t V0, float* __restrict__ pos, float* __restrict__ nrm,
+                                  const int32_t* __restrict__ last) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    if (w.vflags[v] & VF_REMOVE) return;
+    int64_t o = w.v_c2o[v];
+    if (o == -1) {
+        o = V0 + addr[v];
+        w.v_c2o[v] = (int32_t)o;
+    } else if (last[o] != v) {
+        return;
+    }
+    for (int k = 0; k < 3; ++k) {
+        pos[3 * o + k] = w.pos[3 * v + k];
+        nrm[3 * o + k] = w.nrm[3 * v + k];
+    }
+}
+
+__global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ addr, int64_t H0, int4* __restrict__ he) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    if (w.hflags[h] & VF_REMOVE) return;
+    int64_t o = w.h_c2o[h];
+    if (o == -1) {
+        o = H0 + addr[h];
+        w.h_c2o[h] = (int32_t)o;
+    }
+    he[o] = w.he[h];
+}
+
+__global__ void kCopyBackFaces(WC w, int count, const int64_t* __restrict__ addr, int64_t F0, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= count) return;
+    if (w.fflags[f] & VF_REMOVE) return;
+    int64_t o = w.f_c2o[f];
+    if (o == -1) {
+        o = F0 + addr[f];
+        w.f_c2o[f] = (int32_t)o;
+    }
+    face_he[o] = w.h_c2o[w.f_first[f]];
+    fattr[o] = w.f_attr[f];
+}
+
+__global__ void kUpdateBack(WC w, int count, int4* __restrict__ he) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= count) return;
+    if (w.hflags[h] & VF_REMOVE) return;
+    const int64_t me = w.h_c2o[h];
+    int4 e = he[me];
+    e.z = w.h_c2o[e.z];
+    e.y = w.f_c2o[e.y];
+    e.x = w.v_c2o[e.x];
+    int t = e.w;
+    if (t < 0) {
+        t = -t - 2;
+        e.w = t;
+        if (t != -1) setT(he, t, (int)me);
+    } else {
+        e.w = w.h_c2o[t];
+    }
+    he[me] = e;
+}
+
+// ---------------------------------------------------------------- detectNonManifoldVertices
+__device__ int fanWalk(const int4* __restrict__ he, int start, bool cw, int64_t* ctr) {
+    auto step = [&](int h) -> int {
+        if (cw) {
+            const int t = heT(he, h);
+            return (t != -1) ? heN(he, t) : -1;
+        }
+        const int s = heN(he, heN(he, heN(he, h)));
+        const int t = heT(he, s);
+        return (t != -1) ? t : -1;
+    };
+    const int origin = heO(he, start);
+    int result = start;
+    int cur = step(start);
+    int guard = 0;
+    while (cur != -1) {
+        if (++guard > kFanGuard) { anomaly(ctr); break; }
+        if (cur == result) break;
+        if (origin != heO(he, cur)) break;
+        if (cur < result) result = cur;
+        cur = step(cur);
+    }
+    return result;
+}
+
+__global__ void kFanKeys(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ key, int64_t* ctr) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int h = (int)hh;
+    const int a = fanWalk(he, h, true, ctr);
+    const int b = fanWalk(he, h, false, ctr);
+    key[h] = min(h, min(a, b));
+}
+
+__global__ void kFanCount(const int64_t* __restrict__ off, const int32_t* __restrict__ list, int64_t V, const int32_t* __restrict__ key,
+                          uint32_t* __restrict__ vflags) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int distinct = 0;
+    for (int64_t i = off[v]; i < off[v + 1]; ++i) {
+        const int k = key[list[i]];
+        bool seen = false;
+        for (int64_t j = off[v]; j < i; ++j) seen = seen || key[list[j]] == k;
+        if (!seen) ++distinct;
+    }
+    if (distinct > 1) vflags[v] |= VF_FANS;
+    else vflags[v] &= ~(uint32_t)VF_FANS;
+}
+
+// ---------------------------------------------------------------- consistency checks
+// Every id the following kernels chase is checked first; a bad one stops the
+// pipeline with an error instead of an out-of-range access.
+__global__ void kValidateQuads(const int4* __restrict__ quads, int64_t Q, int64_t V, int64_t* ctr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v = quads[q];
+    if (v.x < 0 || v.x >= V || v.y < 0 || v.y >= V || v.z < 0 || v.z >= V || v.w < 0 || v.w >= V)
+        atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+}
+__global__ void kValidateMesh(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, int64_t V, int64_t* ctr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < H) {
+        const int4 e = he[i];
+        bad = bad || e.x < 0 || e.x >= V || e.y < 0 || e.y >= F || e.z < 0 || e.z >= H || e.w < -1 || e.w >= H;
+    }
+    if (i < F) {
+        const int h = face_he[i];
+        bad = bad || h < 0 || h >= H;
+    }
+    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+}
+
+// ---------------------------------------------------------------- statistics
+// counts over the vertex-pair buckets (HalfedgeMesh.cu:1164-1239, ManifoldCreator.cu:545-549)
+__global__ void kStats(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
+                       const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags, int64_t V, unsigned long long* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < H && brank[i] == 0) {
+        const int n = bsize[i];
+        if (n == 4) atomicAdd(&out[0], 1ULL);
+        if (n == 3) atomicAdd(&out[1], 1ULL);
+        if (n > 0) atomicAdd(&out[2], 1ULL);
+    }
+    if (i < F) {
+        int h = face_he[i];
+        int k = 0;
+        for (int j = 0; j < 4; ++j) {
+            if (bsize[h] > 2) ++k;
+            h = heN(he, h);
+        }
+        atomicAdd(&out[3 + k], 1ULL);
+    }
+    if (i < V && (vflags[i] & VF_FANS)) atomicAdd(&out[8], 1ULL);
+}
+
+__global__ void kNarrow16(const uint32_t* __restrict__ in, int64_t n, uint16_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint16_t)in[i];
+}
+__global__ void kNarrow8(const uint32_t* __restrict__ in, int64_t n, uint8_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint8_t)in[i];
+}
+
+}  // namespace
+
+// ============================================================================ host driver
+
+Pipeline::Pipeline() {}
+Pipeline::~Pipeline() {
+    for (auto& e : kev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+void Pipeline::kernelTimes(double* msCount, double* msGenerate) {
+    float a = 0.f, b = 0.f;
+    MDMC_HIP(hipEventElapsedTime(&a, kev[0], kev[1]));
+    MDMC_HIP(hipEventElapsedTime(&b, kev[2], kev[3]));
+    *msCount = a;
+    *msGenerate = b;
+}
+
+int64_t Pipeline::readI64(const int64_t* dev) {
+    int64_t v = 0;
+    MDMC_HIP(hipMemcpyAsync(&v, dev, sizeof v, hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    return v;
+}
+
+int64_t* Pipeline::counters() { return arena.get<int64_t>("counters", A_NCOUNTERS); }
+
+int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out) {
+    *out = arena.get<int64_t>(outName, (size_t)n + 1);
+    void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(n + 1));
+    int64_t* tot = arena.get<int64_t>("scan_total", 1);
+    scanExclusiveI32(in, *out, n, tot, scratch, stream);
+    return readI64(tot);
+}
+
+void Pipeline::zero(void* p, size_t bytes) { MDMC_HIP(hipMemsetAsync(p, 0, bytes, stream)); }
+void Pipeline::fillNeg(void* p, size_t bytes) { MDMC_HIP(hipMemsetAsync(p, 0xFF, bytes, stream)); }
+
+void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t)); }
+
+int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
+
+// ---- grid pass
+void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
+    const int64_t C = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+    const int64_t nb = (C + kBlock - 1) / kBlock;
+    uint16_t* lp16 = arena.get<uint16_t>("lp16", (size_t)C);
+    int32_t* bv = arena.get<int32_t>("blockV", (size_t)nb);
+    int32_t* bq = arena.get<int32_t>("blockQ", (size_t)nb);
+    if (!kev[0])
+        for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
+    MDMC_HIP(hipEventRecord(kev[0], stream));
+    launchDmcCount(g, iso, lp16, bv, bq, stream);
+    MDMC_HIP(hipEventRecord(kev[1], stream));
+    int64_t *bvo = nullptr, *bqo = nullptr;
+    V = scan32(bv, nb, "blockVOff", &bvo);
+    Q = scan32(bq, nb, "blockQOff", &bqo);
+    if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
+    float* pos = arena.get<float>("pos", 3 * (size_t)V);
+    float* nrm = arena.get<float>("nrm", 3 * (size_t)V);
+    int4* quads = arena.get<int4>("quads", (size_t)Q);
+    uint32_t* qattr = arena.get<uint32_t>("qattr", (size_t)Q);
+    MDMC_HIP(hipEventRecord(kev[2], stream));
+    if (C > 0) launchDmcGenerate(g, iso, surfIter ? 1 : 0, lp16, bvo, bqo, pos, nrm, quads, qattr, stream);
+    MDMC_HIP(hipEventRecord(kev[3], stream));
+    dmcV = V;
+    dmcQ = Q;
+}
+
+// ---- HalfedgeMesh::initialize / refreshHashTable: CSR of outgoing halfedges and the buckets
+void Pipeline::buildBuckets(bool setTwins) {
+    int4* he = arena.ptr<int4>("he");
+    int32_t* hkey = arena.ptr<int32_t>("hkey");
+    int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
+    zero(deg, ((size_t)V + 1) * sizeof(int32_t));
+    if (H > 0) kCountOrigin<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, deg, counters());
+    int64_t* off = nullptr;
+    scan32(deg, V + 1, "off", &off);
+    int32_t* cursor = arena.get<int32_t>("cursor", (size_t)V + 1);
+    zero(cursor, ((size_t)V + 1) * sizeof(int32_t));
+    int32_t* list = arena.get<int32_t>("list", (size_t)H);
+    int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
+    int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
+    if (H > 0) {
+        kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, off, cursor, list);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, off, list, hkey, setTwins ? 1 : 0, bsize, brank, counters());
+    }
+    uint32_t* vflags = arena.ptr<uint32_t>("vflags");
+    if (V > 0) kVertexFlags<<<gridFor(V), kBlock, 0, stream>>>(off, list, V, bsize, vflags);
+}
+
+void Pipeline::halfedgeInitialize() {
+    H = 4 * Q;
+    F = Q;
+    int4* he = arena.get<int4>("he", (size_t)H);
+    int32_t* face_he = arena.get<int32_t>("face_he", (size_t)F);
+    uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
+    int32_t* vert_he = arena.get<int32_t>("vert_he", (size_t)V);
+    uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)V);
+    int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
+    if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
+    checkValid("quad vertex ids");
+    fillNeg(vert_he, (size_t)V * sizeof(int32_t));
+    vertHeSize = V;
+    zero(vflags, (size_t)V * sizeof(uint32_t));
+    if (Q > 0) kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, vert_he, hkey);
+    buildBuckets(true);
+    if (F > 0) kFaceFlags<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, arena.ptr<int32_t>("bsize"), fattr, arena.ptr<uint32_t>("qattr"), 0);
+}
+
+void Pipeline::checkValid(const char* what) {
+    const int64_t bad = readI64(counters() + A_INVALID);
+    if (bad) throw HipError(std::string("inconsistent mesh after ") + what + " (" + std::to_string(bad) + " bad ids)", -4);
+}
+
+void Pipeline::refreshHashTable() {
+    int4* he = arena.ptr<int4>("he");
+    int32_t* face_he = arena.ptr<int32_t>("face_he");
+    const int64_t n = std::max(H, F);
+    if (n > 0) kValidateMesh<<<gridFor(n), kBlock, 0, stream>>>(he, H, face_he, F, V, counters());
+    checkValid("splitting");
+    int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
+    if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey);
+    buildBuckets(false);
+    if (F > 0) kFaceFlags<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, arena.ptr<int32_t>("bsize"), arena.ptr<uint32_t>("fattr"), nullptr, 1);
+}
+
+// ---- simplifications
+void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
+    int32_t* vtype = arena.ptr<int32_t>("vtype");
+    int32_t* vtarget = arena.ptr<int32_t>("vtarget");
+    int32_t* keep = arena.get<int32_t>("keepV", (size_t)V);
+    if (V > 0) kKeepVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, V, dropType, keep);
+    int64_t* vaddr = nullptr;
+    const int64_t nV = scan32(keep, V, "vaddr", &vaddr);
+    int32_t* address = arena.get<int32_t>("vaddress", (size_t)V);
+    fillNeg(address, (size_t)V * sizeof(int32_t));
+    float* npos = arena.get<float>("pos2", 3 * (size_t)nV);
+    float* nnrm = arena.get<float>("nrm2", 3 * (size_t)nV);
+    if (V > 0) kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), keep, vaddr, V, npos, nnrm, address);
+    const uint32_t* attr = p3333 ? arena.ptr<uint32_t>("fattr") : arena.ptr<uint32_t>("qattr");
+    int32_t* keepq = arena.get<int32_t>("keepQ", (size_t)Q);
+    if (Q > 0) kKeepQuads<<<gridFor(Q), kBlock, 0, stream>>>(attr, Q, keepq);
+    int64_t* qaddr = nullptr;
+    const int64_t nQ = scan32(keepq, Q, "qaddr", &qaddr);
+    int4* nquads = arena.get<int4>("quads2", (size_t)nQ);
+    uint32_t* nattr = arena.get<uint32_t>("qattr2", (size_t)nQ);
+    if (Q > 0)
+        kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr);
+    arena.swap("pos", "pos2");
+    arena.swap("nrm", "nrm2");
+    arena.swap("quads", "quads2");
+    arena.swap("qattr", "qattr2");
+    V = nV;
+    Q = nQ;
+    halfedgeInitialize();
+}
+
+void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
+    *valence = arena.get<int32_t>("valence", (size_t)V);
+    *count = arena.get<int32_t>("vcount", (size_t)V);
+    int32_t* vtype = arena.get<int32_t>("vtype", (size_t)V);
+    int32_t* vtarget = arena.get<int32_t>("vtarget", (size_t)V);
+    zero(*valence, (size_t)V * sizeof(int32_t));
+    zero(*count, (size_t)V * sizeof(int32_t));
+    zero(vtype, (size_t)V * sizeof(int32_t));
+    fillNeg(vtarget, (size_t)V * sizeof(int32_t));
+    if (H > 0) kValence<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, *valence);
+}
+
+void Pipeline::simplify3X3Y(bool colour) {
+    if (V == 0) return;
+    int32_t *valence, *count;
+    simplifyInit(&valence, &count);
+    int4* quads = arena.ptr<int4>("quads");
+    uint32_t* qattr = arena.ptr<uint32_t>("qattr");
+    if (Q > 0) {
+        if (colour) kMark3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence);
+        else kCount3x3yOld<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence, count);
+        kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), valence, count,
+                                                      arena.ptr<float>("pos"), arena.ptr<float>("nrm"), arena.ptr<int32_t>("vtype"),
+                                                      arena.ptr<int32_t>("vtarget"), colour ? 1 : 0);
+    }
+    simplifyCompactAndRebuild(3, false);
+}
+
+void Pipeline::simplify3333() {
+    if (V == 0) return;
+    int32_t *valence, *count;
+    simplifyInit(&valence, &count);
+    if (F > 0) {
+        kClear3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F);
+        kMark3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"), F, valence,
+                                                     arena.ptr<int32_t>("vtype"), arena.ptr<int32_t>("vtarget"));
+    }
+    simplifyCompactAndRebuild(1, true);
+}
+
+void Pipeline::fixManifoldTwins(const GridDev& g, float iso) {
+    if (H == 0) return;
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("off"), arena.ptr<int32_t>("list"), arena.ptr<int32_t>("hkey"),
+                                                 arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), g, iso, counters());
+}
+
+// ---- ManifoldSplitter::run
+void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
+    const bool copyDataBack = true;
+    int iter = 0;
+    while (true) {
+        if (iter >= maxIters) {
+            extraAnomalies += 1;
+            break;
+        }
+        // countNonManifoldFaces
+        int32_t* pred = arena.get<int32_t>("nmpred", (size_t)F);
+        if (F > 0) kNMFacePred<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, pred);
+        int64_t* faddr = nullptr;
+        const int64_t n64 = scan32(pred, F, "nmaddr", &faddr);
+        if (n64 == 0) break;
+        if (20 * n64 >= (int64_t)INT32_MAX / 2) throw HipError("working copy exceeds the int32 id range", -3);
+        const int n = (int)n64;
+        int32_t* nm = arena.get<int32_t>("nmlist", (size_t)n);
+        kScatterIndex<<<gridFor(F), kBlock, 0, stream>>>(pred, faddr, F, nm);
+
+        WC w;
+        w.n = n;
+        const int nv = 8 * n, nh = 20 * n, nf = 5 * n;
+        w.pos = arena.get<float>("wc_pos", 3 * (size_t)nv);
+        w.nrm = arena.get<float>("wc_nrm", 3 * (size_t)nv);
+        w.pos0 = arena.get<float>("wc_pos0", 3 * (size_t)nv);
+        w.nrm0 = arena.get<float>("wc_nrm0", 3 * (size_t)nv);
+        w.vflags = arena.get<uint32_t>("wc_vflags", (size_t)nv);
+        w.v_c2o = arena.get<int32_t>("wc_v_c2o", (size_t)nv);
+        w.v_o2c = arena.get<int32_t>("wc_v_o2c", (size_t)(V + nv));
+        w.he = arena.get<int4>("wc_he", (size_t)nh);
+        w.hflags = arena.get<uint32_t>("wc_hflags", (size_t)nh);
+        w.h_c2o = arena.get<int32_t>("wc_h_c2o", (size_t)nh);
+        w.h_o2c = arena.get<int32_t>("wc_h_o2c", (size_t)(H + nh));
+        w.f_first = arena.get<int32_t>("wc_f_first", (size_t)nf);
+        w.f_attr = arena.get<uint32_t>("wc_f_attr", (size_t)nf);
+        w.fflags = arena.get<uint32_t>("wc_fflags", (size_t)nf);
+        w.f_c2o = arena.get<int32_t>("wc_f_c2o", (size_t)nf);
+        w.f_o2c = arena.get<int32_t>("wc_f_o2c", (size_t)(F + nf));
+        zero(w.pos, 3 * (size_t)nv * sizeof(float));
+        zero(w.nrm, 3 * (size_t)nv * sizeof(float));
+        zero(w.vflags, (size_t)nv * sizeof(uint32_t));  // WorkingCopy ignores its initialFlags argument: flags start at 0
+        fillNeg(w.v_c2o, (size_t)nv * sizeof(int32_t));
+        fillNeg(w.v_o2c, (size_t)(V + nv) * sizeof(int32_t));
+        fillNeg(w.he, (size_t)nh * sizeof(int4));
+        zero(w.hflags, (size_t)nh * sizeof(uint32_t));
+        fillNeg(w.h_c2o, (size_t)nh * sizeof(int32_t));
+        fillNeg(w.h_o2c, (size_t)(H + nh) * sizeof(int32_t));
+        fillNeg(w.f_first, (size_t)nf * sizeof(int32_t));
+        zero(w.f_attr, (size_t)nf * sizeof(uint32_t));
+        zero(w.fflags, (size_t)nf * sizeof(uint32_t));
+        fillNeg(w.f_c2o, (size_t)nf * sizeof(int32_t));
+        fillNeg(w.f_o2c, (size_t)(F + nf) * sizeof(int32_t));
+
+        kWcCopy<<<gridFor(n), kBlock, 0, stream>>>(nm, n, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"),
+                                                   arena.ptr<float>("pos"), arena.ptr<float>("nrm"), w);
+        int vsize = 4 * n, hsize = 4 * n, fsize = n;
+        kWcUpdate<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize);
+        if (H > 0) kWcNMFlags<<<gridFor(H), kBlock, 0, stream>>>(H, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, counters());
+
+        if (step >= 1) {
+            kSubdivide<<<gridFor(n), kBlock, 0, stream>>>(nm, w);
+            vsize = 8 * n;
+            hsize = 20 * n;
+            fsize = 5 * n;
+        }
+        if (step >= 2) {
+            int32_t* cpred = arena.get<int32_t>("cpred", (size_t)hsize);
+            kCollapsePred<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, cpred, counters());
+            int64_t* crank = nullptr;
+            const int64_t ncol = scan32(cpred, hsize, "crank", &crank);
+            int2* vrec = arena.get<int2>("vrec", 4 * (size_t)ncol);
+            int2* hrec = arena.get<int2>("hrec", 12 * (size_t)ncol);
+            if (ncol > 0) kCollapse<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, cpred, crank, vrec, hrec);
+            int32_t* mlvN = arena.get<int32_t>("mlv_num", (size_t)vsize);
+            int32_t* mlvX = arena.get<int32_t>("mlv_x", (size_t)vsize);
+            int32_t* mlvY = arena.get<int32_t>("mlv_y", (size_t)vsize);
+            int32_t* mlhN = arena.get<int32_t>("mlh_num", (size_t)hsize);
+            int32_t* mlhX = arena.get<int32_t>("mlh_x", (size_t)hsize);
+            int32_t* mlhY = arena.get<int32_t>("mlh_y", (size_t)hsize);
+            buildMergeList(vrec, 4 * ncol, vsize, mlvN, mlvX, mlvY, "mlv");
+            buildMergeList(hrec, 12 * ncol, hsize, mlhN, mlhX, mlhY, "mlh");
+            if (step >= 3) {
+                MDMC_HIP(hipMemcpyAsync(w.pos0, w.pos, 3 * (size_t)vsize * sizeof(float), hipMemcpyDeviceToDevice, stream));
+                MDMC_HIP(hipMemcpyAsync(w.nrm0, w.nrm, 3 * (size_t)vsize * sizeof(float), hipMemcpyDeviceToDevice, stream));
+                kMergeVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, mlvN, mlvX, mlvY, counters());
+                kUpdateVertexIDs<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, vsize, mlvN, mlvX, mlvY, counters());
+            }
+            if (step >= 4) {
+                int32_t* valid = arena.get<int32_t>("mvalid", (size_t)hsize);
+                int32_t* fw = arena.get<int32_t>("mfirst", 2 * (size_t)hsize);
+                std::vector<int32_t> init(2 * (size_t)hsize);
+                for (int i = 0; i < hsize; ++i) init[i] = INT32_MAX;
+                for (int i = hsize; i < 2 * hsize; ++i) init[i] = 0;
+                MDMC_HIP(hipMemcpyAsync(fw, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+                zero(counters() + A_CONFLICT, sizeof(int64_t));
+                kMergePlan<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, valid, fw, counters());
+                kMergeCheck<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, fw, counters());
+                kMergeApply<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, valid, counters());
+                kMergeSequential<<<1, 1, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, counters());
+                MDMC_HIP(hipStreamSynchronize(stream));  // init vector lifetime
+            }
+        }
+        if (step >= 5) kMoveToSurface<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, g, iso);
+
+        if (copyDataBack || step >= 6) {
+            // vertices
+            int32_t* p = arena.get<int32_t>("cb_pred", (size_t)std::max(vsize, std::max(hsize, fsize)));
+            kNewPred<<<gridFor(vsize), kBlock, 0, stream>>>(w.vflags, w.v_c2o, vsize, p);
+            int64_t* a = nullptr;
+            const int64_t addV = scan32(p, vsize, "cb_addr", &a);
+            const int64_t V0 = V;
+            float* pos = arena.grow<float>("pos", 3 * (size_t)(V0 + addV), 3 * (size_t)V0, stream);
+            float* nrm = arena.grow<float>("nrm", 3 * (size_t)(V0 + addV), 3 * (size_t)V0, stream);
+            int32_t* last = arena.get<int32_t>("cb_last", (size_t)V0 + 1);
+            fillNeg(last, ((size_t)V0 + 1) * sizeof(int32_t));
+            kLastCopy<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, last);
+            kCopyBackVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, a, V0, pos, nrm, last);
+            // halfedges
+            kNewPred<<<gridFor(hsize), kBlock, 0, stream>>>(w.hflags, w.h_c2o, hsize, p);
+            const int64_t addH = scan32(p, hsize, "cb_addr", &a);
+            const int64_t H0 = H;
+            int4* he = arena.grow<int4>("he", (size_t)(H0 + addH), (size_t)H0, stream);
+            kCopyBackHalfedges<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, a, H0, he);
+            // faces
+            kNewPred<<<gridFor(fsize), kBlock, 0, stream>>>(w.fflags, w.f_c2o, fsize, p);
+            const int64_t addF = scan32(p, fsize, "cb_addr", &a);
+            const int64_t F0 = F;
+            int32_t* face_he = arena.grow<int32_t>("face_he", (size_t)(F0 + addF), (size_t)F0, stream);
+            uint32_t* fattr = arena.grow<uint32_t>("fattr", (size_t)(F0 + addF), (size_t)F0, stream);
+            kCopyBackFaces<<<gridFor(fsize), kBlock, 0, stream>>>(w, fsize, a, F0, face_he, fattr);
+            kUpdateBack<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, he);
+            // the halfedge-vertex arrays are resized to V + working-copy size (thrust resize:
+            // kept prefix, new slots value-initialised to 0)
+            const int64_t Vres = V0 + vsize;
+            int32_t* vert_he = arena.grow<int32_t>("vert_he", (size_t)Vres, (size_t)std::min(vertHeSize, Vres), stream);
+            if (Vres > vertHeSize) zero(vert_he + vertHeSize, (size_t)(Vres - vertHeSize) * sizeof(int32_t));
+            vertHeSize = Vres;
+            arena.grow<uint32_t>("vflags", (size_t)Vres, 0, stream);
+            V = V0 + addV;
+            H = H0 + addH;
+            F = F0 + addF;
+        }
+        refreshHashTable();
+        ++iter;
+        splitIterations = iter;
+        if (!(step >= 6) || !copyDataBack) break;
+    }
+}
+
+void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag) {
+    int32_t* cnt = arena.get<int32_t>(tag + "_cnt", (size_t)count + 1);
+    zero(cnt, ((size_t)count + 1) * sizeof(int32_t));
+    if (R > 0) kRecCount<<<gridFor(R), kBlock, 0, stream>>>(rec, R, count, cnt, counters());
+    int64_t* roff = nullptr;
+    scan32(cnt, count + 1, tag + "_off", &roff);
+    int32_t* cursor = arena.get<int32_t>(tag + "_cur", (size_t)count + 1);
+    zero(cursor, ((size_t)count + 1) * sizeof(int32_t));
+    int32_t* idx = arena.get<int32_t>(tag + "_idx", (size_t)R);
+    if (R > 0) kRecScatter<<<gridFor(R), kBlock, 0, stream>>>(rec, R, count, roff, cursor, idx);
+    if (count > 0) kRecApply<<<gridFor(count), kBlock, 0, stream>>>(rec, count, roff, idx, cnt, num, ex, ey);
+}
+
+void Pipeline::detectNonManifoldVertices() {
+    if (H == 0) return;
+    int32_t* key = arena.get<int32_t>("fankey", (size_t)H);
+    kFanKeys<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, key, counters());
+    kFanCount<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<int64_t>("off"), arena.ptr<int32_t>("list"), V, key, arena.ptr<uint32_t>("vflags"));
+}
+
+void Pipeline::computeStats(int64_t out[9]) {
+    unsigned long long* d = arena.get<unsigned long long>("stats", 9);
+    zero(d, 9 * sizeof(unsigned long long));
+    const int64_t n = std::max(H, std::max(F, V));
+    if (n > 0 && H > 0)
+        kStats<<<gridFor(n), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int32_t>("face_he"), F, arena.ptr<int32_t>("bsize"),
+                                                  arena.ptr<int32_t>("brank"), arena.ptr<uint32_t>("vflags"), V, d);
+    MDMC_HIP(hipMemcpyAsync(out, d, 9 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+void Pipeline::copyOut(float* positions, float* normals, int32_t* halfedges, int32_t* face_halfedge, uint16_t* face_attr, uint8_t* vertex_flags,
+                       int32_t* vertex_halfedge, bool dstDevice) {
+    const hipMemcpyKind k = dstDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (positions && V) MDMC_HIP(hipMemcpyAsync(positions, arena.ptr<float>("pos"), 3 * (size_t)V * sizeof(float), k, stream));
+    if (normals && V) MDMC_HIP(hipMemcpyAsync(normals, arena.ptr<float>("nrm"), 3 * (size_t)V * sizeof(float), k, stream));
+    if (halfedges && H) MDMC_HIP(hipMemcpyAsync(halfedges, arena.ptr<int4>("he"), (size_t)H * sizeof(int4), k, stream));
+    if (face_halfedge && F) MDMC_HIP(hipMemcpyAsync(face_halfedge, arena.ptr<int32_t>("face_he"), (size_t)F * sizeof(int32_t), k, stream));
+    if (vertex_halfedge && V) MDMC_HIP(hipMemcpyAsync(vertex_halfedge, arena.ptr<int32_t>("vert_he"), (size_t)V * sizeof(int32_t), k, stream));
+    if (face_attr && F) {
+        uint16_t* tmp = arena.get<uint16_t>("out_fattr", (size_t)F);
+        kNarrow16<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, tmp);
+        MDMC_HIP(hipMemcpyAsync(face_attr, tmp, (size_t)F * sizeof(uint16_t), k, stream));
+    }
+    if (vertex_flags && V) {
+        uint8_t* tmp = arena.get<uint8_t>("out_vflags", (size_t)V);
+        kNarrow8<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<uint32_t>("vflags"), V, tmp);
+        MDMC_HIP(hipMemcpyAsync(vertex_flags, tmp, (size_t)V, k, stream));
+    }
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+void Pipeline::triangles(uint32_t* idx, bool dstDevice) {
+    if (F == 0) return;
+    if (dstDevice) {
+        launchDmcTrianglesFromFaces(arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), F, idx, stream);
+    } else {
+        uint32_t* tmp = arena.get<uint32_t>("out_tri", 6 * (size_t)F);
+        launchDmcTrianglesFromFaces(arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), F, tmp, stream);
+        MDMC_HIP(hipMemcpyAsync(idx, tmp, 6 * (size_t)F * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    }
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace mdmc

@@ -236,7 +236,7 @@ static Polygon polygonAsymptotic(const R s[8], R iso) {
         int cnt = 0;
         p.edge[pos + cnt++] = e;
         int guard = 0;
-        while (out != e && guard++ < 12) {
+        while (out != e && guard++ < 12 && out < 12 && pos + cnt < 12) {
             p.edge[pos + cnt++] = out;
             int in = out;
             out = seg[in] & 0xF;
@@ -884,7 +884,8 @@ struct MergeList {
     std::vector<int> ex, ey;  // glm::ivec2 storage
     explicit MergeList(int n) : num(n, 1), ex(n, -1), ey(n, -1) {}
     int elem(int a, int i) const { return i == 1 ? ey[a] : ex[a]; }  // glm operator[] default -> x
-    void mark(int a, int b) {
+    void mark(int a, int b, int64_t& anomalies) {
+        if (a < 0 || a >= (int)num.size()) { anomalies++; return; }
         const int idx = num[a]++;
         if (idx > 5) return;
         if (idx - 1 == 1) ey[a] = b;
@@ -892,8 +893,9 @@ struct MergeList {
     }
     // DeviceMergeList::getAllPartnersOf
     int partners(int v, int out[12], int64_t& anomalies) const {
-        int n = num[v];
         for (int i = 0; i < 12; ++i) out[i] = -1;
+        if (v < 0 || v >= (int)num.size()) { anomalies++; return 1; }
+        int n = num[v];
         if (n <= 1) return n;
         for (int i = 0; i < n - 1; ++i) {
             if (i >= 12) { anomalies++; break; }
@@ -901,6 +903,7 @@ struct MergeList {
         }
         for (int i = 0; i < n - 1 && i < 12; ++i) {
             const int p = out[i];
+            if (p < 0 || p >= (int)num.size()) { anomalies++; continue; }
             const int np = num[p];
             for (int k = 0; k < np - 1; ++k) {
                 const int pp = elem(p, k);
@@ -1122,20 +1125,20 @@ static void splitNonManifold(Mesh<R>& m, const Grid& g, R iso, int step, int max
                 }
                 {
                     const int a = wh.he[cf[0][2]].o, b = wh.he[cf[1][1]].o;
-                    mlv.mark(a, b);
-                    mlv.mark(b, a);
+                    mlv.mark(a, b, m.anomalies);
+                    mlv.mark(b, a, m.anomalies);
                 }
                 {
                     const int a = wh.he[cf[0][3]].o, b = wh.he[cf[1][2]].o;
-                    mlv.mark(a, b);
-                    mlv.mark(b, a);
+                    mlv.mark(a, b, m.anomalies);
+                    mlv.mark(b, a, m.anomalies);
                 }
                 for (int i = 1; i < 4; ++i) {
                     const int ta = wh.he[cf[0][i]].t, tb = wh.he[cf[1][i]].t;
-                    mlh.mark(cf[0][i], tb);
-                    mlh.mark(tb, cf[0][i]);
-                    mlh.mark(cf[1][i], ta);
-                    mlh.mark(ta, cf[1][i]);
+                    mlh.mark(cf[0][i], tb, m.anomalies);
+                    mlh.mark(tb, cf[0][i], m.anomalies);
+                    mlh.mark(cf[1][i], ta, m.anomalies);
+                    mlh.mark(ta, cf[1][i], m.anomalies);
                 }
             }
             if (step >= S_MERGE) {
@@ -1150,6 +1153,7 @@ static void splitNonManifold(Mesh<R>& m, const Grid& g, R iso, int step, int max
                     bool obsolete = false;
                     for (int i = 0; i < np - 1; ++i) {
                         const int q = P[i];
+                        if (q < 0 || q >= wv.size) { m.anomalies++; continue; }
                         p = p + wv.pos[q];
                         nn = nn + wv.nrm[q];
                         if (q < smallest) { obsolete = true; smallest = q; }
@@ -1168,7 +1172,7 @@ static void splitNonManifold(Mesh<R>& m, const Grid& g, R iso, int step, int max
                     if (np == 1) continue;
                     int smallest = v;
                     for (int i = 0; i < np - 1; ++i)
-                        if (P[i] < smallest) smallest = P[i];
+                        if (P[i] >= 0 && P[i] < smallest) smallest = P[i];
                     wh.he[h].o = smallest;
                 }
             }
@@ -1181,8 +1185,9 @@ static void splitNonManifold(Mesh<R>& m, const Grid& g, R iso, int step, int max
                     if (np == 1) continue;
                     int valid = -1;
                     for (int i = 0; i < np - 1; ++i) {
+                        if (P[i] < 0 || P[i] >= wh.size) continue;
                         const int tw = wh.he[P[i]].t;
-                        if (tw < 0) { m.anomalies++; continue; }
+                        if (tw < 0 || tw >= wh.size) { m.anomalies++; continue; }
                         if (!(wh.flags[tw] & VF_REMOVE)) { valid = tw; break; }
                     }
                     if (valid == -1) continue;
@@ -1266,7 +1271,6 @@ static void splitNonManifold(Mesh<R>& m, const Grid& g, R iso, int step, int max
 // ManifoldCreator::detectNonManifoldVertices, ManifoldCreator.cu:319-536
 template <class R>
 static int fanWalk(const Mesh<R>& m, int start, bool cw, int64_t& anomalies) {
-    const int H = (int)m.he.size();
     auto step = [&](int h) -> int {
         const HE& e = m.he[h];
         if (cw) {
@@ -1282,7 +1286,7 @@ static int fanWalk(const Mesh<R>& m, int start, bool cw, int64_t& anomalies) {
     int cur = step(start);
     int guard = 0;
     while (cur != -1) {
-        if (++guard > H + 4) { anomalies++; break; }
+        if (++guard > 4096) { anomalies++; break; }
         const HE& e = m.he[cur];
         if (cur == result) break;
         if (origin != e.o) break;
