@@ -32,7 +32,7 @@ __device__ __forceinline__ f3 vclamp(f3 v, float lo, float hi) { return f3{gclam
 // glm::normalize = v * inversesqrt(dot(v,v)), dot = (x*x + y*y) + z*z, inversesqrt = 1/sqrt
 __device__ __forceinline__ f3 normalize3(f3 v) {
     const float d = v.x * v.x + v.y * v.y + v.z * v.z;
-    const float inv = 1.0f / __fsqrt_rn(d);
+    const float inv = 1.0f / sqrtf(d);  // sqrtf lowers to the correctly rounded sequence (__fsqrt_rn does not here)
     return f3{v.x * inv, v.y * inv, v.z * inv};
 }
 // glm::mix(x, y, a) = x*(1-a) + y*a

@@ -1103,25 +1103,38 @@ __global__ void kValidateMesh(const int4* __restrict__ he, int64_t H, const int3
 
 // ---------------------------------------------------------------- statistics
 // counts over the vertex-pair buckets (HalfedgeMesh.cu:1164-1239, ManifoldCreator.cu:545-549)
-__global__ void kStats(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
-                       const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags, int64_t V, unsigned long long* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < H && brank[i] == 0) {
-        const int n = bsize[i];
-        if (n == 4) atomicAdd(&out[0], 1ULL);
-        if (n == 3) atomicAdd(&out[1], 1ULL);
-        if (n > 0) atomicAdd(&out[2], 1ULL);
-    }
-    if (i < F) {
-        int h = face_he[i];
-        int k = 0;
-        for (int j = 0; j < 4; ++j) {
-            if (bsize[h] > 2) ++k;
-            h = heN(he, h);
+__global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F,
+                                                 const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags,
+                                                 int64_t V, unsigned long long* __restrict__ out) {
+    __shared__ unsigned int acc[9];
+    if (threadIdx.x < 9) acc[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned int loc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < H || i < F || i < V; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < H && brank[i] == 0) {
+            const int n = bsize[i];
+            loc[0] += (n == 4);
+            loc[1] += (n == 3);
+            loc[2] += (n > 0);
         }
-        atomicAdd(&out[3 + k], 1ULL);
+        if (i < F) {
+            int h = face_he[i];
+            int k = 0;
+            for (int j = 0; j < 4; ++j) {
+                if (bsize[h] > 2) ++k;
+                h = heN(he, h);
+            }
+            loc[3 + k] += 1;
+        }
+        if (i < V) loc[8] += (vflags[i] & VF_FANS) ? 1u : 0u;
     }
-    if (i < V && (vflags[i] & VF_FANS)) atomicAdd(&out[8], 1ULL);
+    for (int k = 0; k < 9; ++k) {
+        unsigned int v = loc[k];
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&acc[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < 9 && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)acc[threadIdx.x]);
 }
 
 __global__ void kNarrow16(const uint32_t* __restrict__ in, int64_t n, uint16_t* __restrict__ out) {
@@ -1512,7 +1525,7 @@ void Pipeline::computeStats(int64_t out[9]) {
     zero(d, 9 * sizeof(unsigned long long));
     const int64_t n = std::max(H, std::max(F, V));
     if (n > 0 && H > 0)
-        kStats<<<gridFor(n), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int32_t>("face_he"), F, arena.ptr<int32_t>("bsize"),
+        kStats<<<(unsigned)std::min<int64_t>(gridFor(n), 4096), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int32_t>("face_he"), F, arena.ptr<int32_t>("bsize"),
                                                   arena.ptr<int32_t>("brank"), arena.ptr<uint32_t>("vflags"), V, d);
     MDMC_HIP(hipMemcpyAsync(out, d, 9 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));

