@@ -85,8 +85,8 @@ typedef struct {
     /* stage times (ms, HIP events on the context's stream): whole call, pad/refine, DMC
      * (count + scan + generate), halfedge build, simplifications, twin repair, split, detection */
     double ms_total, ms_grid, ms_dmc, ms_halfedge, ms_simplify, ms_fix_twins, ms_split, ms_detect;
-    /* the two grid-pass kernels alone: classification/count and vertex + quad generation */
-    double ms_dmc_count_kernel, ms_dmc_generate_kernel;
+    /* the grid-pass kernel alone (classification, dual vertices, quads) */
+    double ms_grid_pass_kernel;
 } mdmc_stats;
 
 /* Sizes of the resident result. */

@@ -67,7 +67,7 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_grid", ctypes.c_double), ("ms_dmc", ctypes.c_double),
                 ("ms_halfedge", ctypes.c_double), ("ms_simplify", ctypes.c_double), ("ms_fix_twins", ctypes.c_double),
                 ("ms_split", ctypes.c_double), ("ms_detect", ctypes.c_double),
-                ("ms_dmc_count_kernel", ctypes.c_double), ("ms_dmc_generate_kernel", ctypes.c_double)]
+                ("ms_grid_pass_kernel", ctypes.c_double)]
 
     def as_dict(self):
         out = {}

@@ -205,7 +205,7 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
                 MDMC_HIP(hipEventElapsedTime(&ms, ctx->ev[i - 1], ctx->ev[i]));
                 *outs[i] = ms;
             }
-            p.kernelTimes(&stats->ms_dmc_count_kernel, &stats->ms_dmc_generate_kernel);
+            p.kernelTimes(&stats->ms_grid_pass_kernel);
         }
         return MDMC_OK;
     } catch (const std::exception& e) {

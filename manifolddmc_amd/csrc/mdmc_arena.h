@@ -65,6 +65,11 @@ class Arena {
         return (T*)b.p;
     }
     template <class T>
+    size_t capacity(const std::string& name) const {
+        auto it = bufs_.find(name);
+        return it == bufs_.end() ? 0 : it->second.cap / sizeof(T);
+    }
+    template <class T>
     T* ptr(const std::string& name) {
         auto it = bufs_.find(name);
         return it == bufs_.end() ? nullptr : (T*)it->second.p;

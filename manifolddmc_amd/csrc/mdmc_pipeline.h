@@ -43,8 +43,9 @@ class Pipeline {
     void triangles(uint32_t* idx, bool dstDevice);
 
     int64_t readI64(const int64_t* dev);
-    // durations of the last grid-pass kernels, from events on the launch stream
-    void kernelTimes(double* msCount, double* msGenerate);
+    // duration of the last grid-pass kernel, from events on the launch stream
+    void kernelTimes(double* msGridPass);
+    int64_t lastGridV = 0, lastGridQ = 0;
     hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};
 
    private:

@@ -24,6 +24,7 @@
 
 #include "mdmc_arena.h"
 #include "mdmc_device.h"
+#include "mdmc_gridpass.h"
 #include "mdmc_internal.h"
 #include "mdmc_pipeline.h"
 
@@ -1156,12 +1157,10 @@ Pipeline::~Pipeline() {
         if (e) (void)hipEventDestroy(e);
 }
 
-void Pipeline::kernelTimes(double* msCount, double* msGenerate) {
-    float a = 0.f, b = 0.f;
+void Pipeline::kernelTimes(double* msGridPass) {
+    float a = 0.f;
     MDMC_HIP(hipEventElapsedTime(&a, kev[0], kev[1]));
-    MDMC_HIP(hipEventElapsedTime(&b, kev[2], kev[3]));
-    *msCount = a;
-    *msGenerate = b;
+    *msGridPass = a;
 }
 
 int64_t Pipeline::readI64(const int64_t* dev) {
@@ -1190,29 +1189,66 @@ int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
 
 // ---- grid pass
 void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
-    const int64_t C = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
-    const int64_t nb = (C + kBlock - 1) / kBlock;
-    uint16_t* lp16 = arena.get<uint16_t>("lp16", (size_t)C);
-    int32_t* bv = arena.get<int32_t>("blockV", (size_t)nb);
-    int32_t* bq = arena.get<int32_t>("blockQ", (size_t)nb);
+    const int CX = g.nx - 1, CY = g.ny - 1, CZ = g.nz - 1;
+    const int64_t C = (int64_t)CX * CY * CZ;
+    const int T = gridPassTile();
+    const int64_t numTiles = (C + T - 1) / T;
+    GridPassArgs a;
+    a.g = g;
+    a.iso = iso;
+    a.surfIter = surfIter ? 1 : 0;
+    a.C = C;
+    a.CX = CX;
+    a.CY = CY;
+    a.numTiles = numTiles;
+    // flags: status (u64) | ready | done | ticket | err, zeroed together
+    const size_t flagBytes = (size_t)numTiles * (8 + 4 + 4) + 64;
+    char* flags = arena.get<char>("gp_flags", flagBytes);
+    a.status = (unsigned long long*)flags;
+    a.ready = (int*)(flags + 8 * (size_t)numTiles);
+    a.done = a.ready + numTiles;
+    a.ticket = (unsigned int*)(a.done + numTiles);
+    a.err = (int*)(a.ticket + 4);
+    a.R = std::max<int64_t>(8 * ((int64_t)CX * CY + CX + T), 1 << 16);
+    a.ring = arena.get<int2>("gp_ring", (size_t)a.R);
+    int64_t* tot = arena.get<int64_t>("gp_totals", 4);
+    // output capacity: what the buffers hold, at least an estimate; exact sizes on overflow
+    int64_t capV = std::max<int64_t>(lastGridV + lastGridV / 16 + 1024, C / 6 + 4096);
+    int64_t capQ = std::max<int64_t>(lastGridQ + lastGridQ / 16 + 1024, C / 6 + 4096);
     if (!kev[0])
         for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
-    MDMC_HIP(hipEventRecord(kev[0], stream));
-    launchDmcCount(g, iso, lp16, bv, bq, stream);
-    MDMC_HIP(hipEventRecord(kev[1], stream));
-    int64_t *bvo = nullptr, *bqo = nullptr;
-    V = scan32(bv, nb, "blockVOff", &bvo);
-    Q = scan32(bq, nb, "blockQOff", &bqo);
-    if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
-    float* pos = arena.get<float>("pos", 3 * (size_t)V);
-    float* nrm = arena.get<float>("nrm", 3 * (size_t)V);
-    int4* quads = arena.get<int4>("quads", (size_t)Q);
-    uint32_t* qattr = arena.get<uint32_t>("qattr", (size_t)Q);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        a.pos = arena.get<float>("pos", 3 * (size_t)capV);
+        a.nrm = arena.get<float>("nrm", 3 * (size_t)capV);
+        a.quads = arena.get<int4>("quads", (size_t)capQ);
+        a.qattr = arena.get<uint32_t>("qattr", (size_t)capQ);
+        a.capV = capV;
+        a.capQ = capQ;
+        zero(flags, flagBytes);
+        MDMC_HIP(hipEventRecord(kev[0], stream));
+        launchGridPass(a, stream);
+        MDMC_HIP(hipEventRecord(kev[1], stream));
+        launchGridPassTotals(a.status, numTiles, tot, stream);
+        int64_t h[4] = {0, 0, 0, 0};
+        int err = 0;
+        MDMC_HIP(hipMemcpyAsync(h, tot, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+        MDMC_HIP(hipMemcpyAsync(&err, a.err, sizeof(int), hipMemcpyDeviceToHost, stream));
+        MDMC_HIP(hipStreamSynchronize(stream));
+        if (err & GP_ERR_TIMEOUT) throw HipError("grid pass: a tile wait timed out", -4);
+        V = h[0];
+        Q = h[1];
+        if (!(err & GP_ERR_OVERFLOW)) break;
+        if (attempt == 1) throw HipError("grid pass: output overflow after resize", -4);
+        capV = V + 1;
+        capQ = Q + 1;
+    }
     MDMC_HIP(hipEventRecord(kev[2], stream));
-    if (C > 0) launchDmcGenerate(g, iso, surfIter ? 1 : 0, lp16, bvo, bqo, pos, nrm, quads, qattr, stream);
     MDMC_HIP(hipEventRecord(kev[3], stream));
+    if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
     dmcV = V;
     dmcQ = Q;
+    lastGridV = V;
+    lastGridQ = Q;
 }
 
 // ---- HalfedgeMesh::initialize / refreshHashTable: CSR of outgoing halfedges and the buckets
@@ -1295,10 +1331,11 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     uint32_t* nattr = arena.get<uint32_t>("qattr2", (size_t)nQ);
     if (Q > 0)
         kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr);
-    arena.swap("pos", "pos2");
-    arena.swap("nrm", "nrm2");
-    arena.swap("quads", "quads2");
-    arena.swap("qattr", "qattr2");
+    // back into the primary buffers (they keep the grid pass's capacity)
+    MDMC_HIP(hipMemcpyAsync(arena.ptr<float>("pos"), npos, 3 * (size_t)nV * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    MDMC_HIP(hipMemcpyAsync(arena.ptr<float>("nrm"), nnrm, 3 * (size_t)nV * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    MDMC_HIP(hipMemcpyAsync(arena.ptr<int4>("quads"), nquads, (size_t)nQ * sizeof(int4), hipMemcpyDeviceToDevice, stream));
+    MDMC_HIP(hipMemcpyAsync(arena.ptr<uint32_t>("qattr"), nattr, (size_t)nQ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
     V = nV;
     Q = nQ;
     halfedgeInitialize();
