@@ -8,7 +8,7 @@
 
 namespace mdmc {
 
-enum : int { GP_ERR_OVERFLOW = 1, GP_ERR_TIMEOUT = 2 };
+enum : int { GP_ERR_OVERFLOW = 1, GP_ERR_TIMEOUT = 2, GP_ERR_RANGE = 4 };
 
 struct GridPassArgs {
     GridDev g;
@@ -18,11 +18,11 @@ struct GridPassArgs {
     int CX, CY;
     int64_t numTiles;
     unsigned long long* status;  // per tile: flag | quads | vertices (look-back)
-    int* ready;                  // per tile: ring entries published
     int* done;                   // per tile: finished reading the ring
     unsigned int* ticket;        // tile claim counter
-    int2* ring;                  // (first vertex id, edge->contour map) of cell c at c % R
-    int64_t R;
+    unsigned long long* ring;    // tag | edge->contour map | first vertex id of cell c at c & (R-1)
+    int64_t R;                   // power of two, 1 << ringShift
+    int ringShift;
     float* pos;
     float* nrm;
     int64_t capV;
@@ -33,6 +33,8 @@ struct GridPassArgs {
 };
 
 int gridPassTile();
+// ring size (power of two) for a grid of C cells, CX x CY per z-slice
+void gridPassRing(int64_t C, int CX, int CY, int64_t* R, int* shift);
 void launchGridPass(const GridPassArgs& a, hipStream_t s);
 void launchGridPassTotals(const unsigned long long* status, int64_t numTiles, int64_t* out, hipStream_t s);
 

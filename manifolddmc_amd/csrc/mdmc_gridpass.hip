@@ -9,8 +9,10 @@
 // The reference appends vertices with one global atomic counter and scatters
 // their ids into a 64-bit keyed quad hash that a second kernel compacts.  Here
 // one kernel walks the cells in linear order in tiles of 2048:
-//   1. classify the tile's cells (MC case, polygon, owned edges) into LDS;
-//   2. scan the per-cell vertex and quad counts inside the tile;
+//   1. find the tile's active cells (MC case neither 0 nor 255) and compact
+//      them, in cell order, into LDS; then compute polygon and owned edges of
+//      the active cells only, with every lane busy;
+//   2. scan the per-cell vertex and quad counts over the active cells;
 //   3. get the tile's global vertex / quad offsets by decoupled look-back over
 //      the tiles before it (tiles are claimed with an atomic ticket, so every
 //      predecessor is already running and the wait always ends);
@@ -20,8 +22,16 @@
 //   6. write the quads whose edge starts at one of the tile's cells, one per
 //      thread, taking the other three vertex ids from LDS or from the ring.
 // Vertices come out ordered by (cell, contour) and quads by edge key
-// 3*point+axis, with no hash table and no global append counter.  Every wait is
-// bounded; a timeout raises an error flag instead of hanging.
+// 3*point+axis, with no hash table and no global append counter.
+//
+// Cross-tile communication uses relaxed device-scope atomics only: every word
+// another tile reads carries its own validity (the look-back status packs flag
+// and both counts; a ring entry carries a tag naming the cell it holds), so no
+// release/acquire fence -- which on this chip writes back or invalidates whole
+// caches -- is needed.  Every wait is bounded; a timeout raises an error flag
+// instead of hanging.
+#include <algorithm>
+
 #include "mdmc_device.h"
 #include "mdmc_gridpass.h"
 
@@ -32,6 +42,8 @@ namespace {
 constexpr int GP_THREADS = 256;
 constexpr int GP_CPT = 8;
 constexpr int GP_TILE = GP_THREADS * GP_CPT;
+constexpr int GP_WAVES = GP_THREADS / 64;
+constexpr int GP_RUNS = GP_TILE / 64;  // runs of 64 consecutive cells, one ballot each
 constexpr int kSpinLimit = 1 << 22;
 
 constexpr uint64_t FLAG_AGG = 1ULL, FLAG_INC = 2ULL;
@@ -40,25 +52,42 @@ __device__ __forceinline__ uint64_t stFlag(uint64_t s) { return s >> 62; }
 __device__ __forceinline__ uint64_t stV(uint64_t s) { return s & 0x7FFFFFFFULL; }
 __device__ __forceinline__ uint64_t stQ(uint64_t s) { return (s >> 31) & 0x7FFFFFFFULL; }
 
-// Polling uses relaxed agent-scope loads (they bypass the CU's L1 without invalidating
-// it); one acquire fence per tile then orders the reads that depend on the flags.
 __device__ __forceinline__ uint64_t loadRelaxed(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int loadRelaxedI(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void fenceAcquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
-__device__ __forceinline__ void storeRelease(unsigned long long* p, uint64_t v) {
-    __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void storeRelaxed(unsigned long long* p, uint64_t v) {
+    __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void storeReleaseI(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int loadRelaxedI(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void storeRelaxedI(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// bounded wait for a flag set by an earlier tile (relaxed polling; the caller fences)
+// bounded wait for a flag set by an earlier tile
 __device__ __forceinline__ void waitFlag(const int* flag, int* err) {
     for (int it = 0; it < kSpinLimit; ++it) {
         if (loadRelaxedI(flag)) return;
         __builtin_amdgcn_s_sleep(1);
     }
     atomicOr(err, GP_ERR_TIMEOUT);
+}
+
+// Ring entry of cell c: bits 0-30 first vertex id, 31-54 the contour (2 bits) of each of
+// the 12 edges, 55-63 tag = (c >> ringShift) + 1.  The ring is cleared before the pass and
+// holds fewer than 511 generations, so a tag never matches an older or missing entry.
+__device__ __forceinline__ uint64_t ringTag(int64_t c, int shift) { return (uint64_t)((c >> shift) + 1) & 0x1FFULL; }
+
+__device__ __forceinline__ int ringVertex(const GridPassArgs& a, int64_t n, int e) {
+    const uint64_t tag = ringTag(n, a.ringShift);
+    const unsigned long long* p = a.ring + (n & (a.R - 1));
+    uint64_t v = loadRelaxed(p);
+    for (int it = 0; (v >> 55) != tag; ++it) {
+        if (it >= kSpinLimit) {
+            atomicOr(a.err, GP_ERR_TIMEOUT);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = loadRelaxed(p);
+    }
+    return (int)(v & 0x7FFFFFFFULL) + (int)((v >> (31 + 2 * e)) & 3ULL);
 }
 
 __device__ __forceinline__ void loadCorners(const GridDev& g, int x, int y, int z, float s[8]) {
@@ -147,17 +176,38 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
     wp = mk3(g.ox, g.oy, g.oz) + (mk3((float)x, (float)y, (float)z) + p) * mk3(g.dx, g.dy, g.dz);
 }
 
+// coordinates of cell c0 + l from those of c0 (x0, y0, z0) and l < GP_TILE, in 32-bit arithmetic
+struct TileBase {
+    int x0, y0, z0;
+};
+__device__ __forceinline__ void cellXYZ(const TileBase& b, int l, int CX, int CY, int& x, int& y, int& z) {
+    const unsigned xs = (unsigned)(b.x0 + l);
+    const unsigned r = xs / (unsigned)CX;
+    x = (int)(xs - r * (unsigned)CX);
+    const unsigned ys = (unsigned)b.y0 + r;
+    const unsigned zz = ys / (unsigned)CY;
+    y = (int)(ys - zz * (unsigned)CY);
+    z = b.z0 + (int)zz;
+}
+
 __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
+    // per active cell (index in cell order): polygon, owned-edge bits, tile cell, scans
     __shared__ uint64_t s_poly[GP_TILE];
+    __shared__ uint16_t s_cell[GP_TILE];
     __shared__ uint16_t s_vpre[GP_TILE];
-    __shared__ uint16_t s_qpre[GP_TILE];
+    __shared__ uint16_t s_qpre[GP_TILE];  // also holds the MC case of each tile cell in phase 1
     __shared__ uint8_t s_own[GP_TILE];
-    __shared__ int s_wave[GP_THREADS / 64];
+    // per tile cell: its active index (valid for active cells)
+    __shared__ uint16_t s_aidx[GP_TILE];
+    __shared__ unsigned long long s_mask[GP_RUNS];
+    __shared__ int s_mpre[GP_RUNS + 1];
+    __shared__ int s_wave[GP_WAVES];
     __shared__ long long s_tile;
     __shared__ long long s_voff, s_qoff;
-    __shared__ int s_VT, s_QT;
+    uint8_t* const s_mc = reinterpret_cast<uint8_t*>(s_qpre);
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
     if (tid == 0) s_tile = (long long)atomicAdd(a.ticket, 1u);
     __syncthreads();
     const int64_t t = s_tile;
@@ -168,42 +218,81 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
     const int64_t C = a.C;
     const int64_t c0 = t * GP_TILE;
 
-    // ---- 1. classify
+    TileBase tb;
+    {
+        const int64_t r0 = c0 / CX;
+        tb.x0 = (int)(c0 - r0 * CX);
+        tb.y0 = (int)(r0 % CY);
+        tb.z0 = (int)(r0 / CY);
+    }
+
+    // ---- 1a. MC case of every cell; a ballot per run of 64 cells marks the active ones
+#pragma unroll
     for (int k = 0; k < GP_CPT; ++k) {
         const int l = k * GP_THREADS + tid;
-        const int64_t c = c0 + l;
-        uint64_t w = 0;
-        int own = 0;
-        if (c < C) {
-            const int x = (int)(c % CX), y = (int)((c / CX) % CY), z = (int)(c / ((int64_t)CX * CY));
+        int mc = 0;
+        if (c0 + l < C) {
+            int x, y, z;
+            cellXYZ(tb, l, CX, CY, x, y, z);
             float s[8];
             loadCorners(g, x, y, z, s);
-            int mc = 0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) mc |= (s[q] >= iso ? 1 : 0) << q;
-            if (mc != 0 && mc != 255) {
-                w = cellPolygon(mc, s, iso);
-                own = ownedBits(s, iso, x, y, z);
-            }
         }
-        s_poly[l] = w;
-        s_own[l] = (uint8_t)own;
+        const uint64_t m = __ballot(mc != 0 && mc != 255);
+        if (lane == 0) s_mask[k * GP_WAVES + wid] = m;
+        s_mc[l] = (uint8_t)mc;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int v = tid < GP_RUNS ? __popcll(s_mask[tid]) : 0;
+        int x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (tid >= d) x += y;
+        }
+        if (tid < GP_RUNS) s_mpre[tid] = x - v;
+        if (tid == GP_RUNS - 1) s_mpre[GP_RUNS] = x;
+    }
+    __syncthreads();
+    const int NA = s_mpre[GP_RUNS];
+#pragma unroll
+    for (int k = 0; k < GP_CPT; ++k) {
+        const int run = k * GP_WAVES + wid;
+        const uint64_t m = s_mask[run];
+        if ((m >> lane) & 1ULL) {
+            const int ai = s_mpre[run] + __popcll(m & ((1ULL << lane) - 1ULL));
+            const int l = k * GP_THREADS + tid;
+            s_cell[ai] = (uint16_t)l;
+            s_aidx[l] = (uint16_t)ai;
+        }
     }
     __syncthreads();
 
-    // ---- 2. prefix of vertex / quad counts in cell order (thread tid owns cells 8*tid .. 8*tid+7)
-    int cnt = 0;
+    // ---- 1b. polygon and owned edges of the active cells
+    for (int ai = tid; ai < NA; ai += GP_THREADS) {
+        const int l = s_cell[ai];
+        int x, y, z;
+        cellXYZ(tb, l, CX, CY, x, y, z);
+        float s[8];
+        loadCorners(g, x, y, z, s);
+        s_poly[ai] = cellPolygon(s_mc[l], s, iso);
+        s_own[ai] = (uint8_t)ownedBits(s, iso, x, y, z);
+    }
+    __syncthreads();
+
+    // ---- 2. prefix of vertex / quad counts over the active cells (thread tid owns 8*tid .. 8*tid+7)
+    int cnt = 0, VT = 0, QT = 0;
     int nvq[GP_CPT];
 #pragma unroll
     for (int j = 0; j < GP_CPT; ++j) {
-        const int l = tid * GP_CPT + j;
-        const int nv = polyCount(s_poly[l]);
-        const int nq = __popc(s_own[l] & 7);
-        nvq[j] = nv | (nq << 16);
+        const int ai = tid * GP_CPT + j;
+        nvq[j] = 0;
+        if (ai < NA) nvq[j] = polyCount(s_poly[ai]) | (__popc(s_own[ai] & 7) << 16);
         cnt += nvq[j];
     }
     {
-        const int lane = tid & 63, wid = tid >> 6;
         int x = cnt;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -214,7 +303,7 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
         __syncthreads();
         int add = 0, tot = 0;
 #pragma unroll
-        for (int q = 0; q < GP_THREADS / 64; ++q) {
+        for (int q = 0; q < GP_WAVES; ++q) {
             const int v = s_wave[q];
             if (q < wid) add += v;
             tot += v;
@@ -222,30 +311,27 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
         int run = x - cnt + add;
 #pragma unroll
         for (int j = 0; j < GP_CPT; ++j) {
-            const int l = tid * GP_CPT + j;
-            s_vpre[l] = (uint16_t)(run & 0xFFFF);
-            s_qpre[l] = (uint16_t)(run >> 16);
+            const int ai = tid * GP_CPT + j;
+            if (ai < NA) {
+                s_vpre[ai] = (uint16_t)(run & 0xFFFF);
+                s_qpre[ai] = (uint16_t)(run >> 16);
+            }
             run += nvq[j];
         }
-        if (tid == 0) {
-            s_VT = tot & 0xFFFF;
-            s_QT = tot >> 16;
-        }
+        VT = tot & 0xFFFF;
+        QT = tot >> 16;
     }
-    __syncthreads();
-    const int VT = s_VT, QT = s_QT;
 
     // ---- 3. decoupled look-back (wave 0)
     if (tid < 64) {
-        const int lane = tid;
         if (t == 0) {
             if (lane == 0) {
-                storeRelease(&a.status[0], packStatus(FLAG_INC, (uint64_t)VT, (uint64_t)QT));
+                storeRelaxed(&a.status[0], packStatus(FLAG_INC, (uint64_t)VT, (uint64_t)QT));
                 s_voff = 0;
                 s_qoff = 0;
             }
         } else {
-            if (lane == 0) storeRelease(&a.status[t], packStatus(FLAG_AGG, (uint64_t)VT, (uint64_t)QT));
+            if (lane == 0) storeRelaxed(&a.status[t], packStatus(FLAG_AGG, (uint64_t)VT, (uint64_t)QT));
             int64_t accV = 0, accQ = 0;
             int64_t i = t - 1;
             int spins = 0;
@@ -281,7 +367,8 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
                 else i -= 64;
             }
             if (lane == 0) {
-                storeRelease(&a.status[t], packStatus(FLAG_INC, (uint64_t)(accV + VT) & 0x7FFFFFFFULL, (uint64_t)(accQ + QT) & 0x7FFFFFFFULL));
+                if (accV + VT > 0x7FFFFFFFLL || accQ + QT > 0x7FFFFFFFLL) atomicOr(a.err, GP_ERR_RANGE);
+                storeRelaxed(&a.status[t], packStatus(FLAG_INC, (uint64_t)(accV + VT) & 0x7FFFFFFFULL, (uint64_t)(accQ + QT) & 0x7FFFFFFFULL));
                 s_voff = accV;
                 s_qoff = accQ;
             }
@@ -299,32 +386,25 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
         }
     }
     __syncthreads();
-    fenceAcquire();
     const int64_t voff = s_voff, qoff = s_qoff;
 
     // ---- 4. publish first vertex id + contour map of the tile's active cells
-    for (int k = 0; k < GP_CPT; ++k) {
-        const int l = k * GP_THREADS + tid;
-        const uint64_t w = s_poly[l];
-        if (w) {
-            const int64_t c = c0 + l;
-            a.ring[c % a.R] = make_int2((int)(voff + s_vpre[l]), (int)contourMap(w));
-        }
+    for (int ai = tid; ai < NA; ai += GP_THREADS) {
+        const int64_t c = c0 + s_cell[ai];
+        const uint64_t ent = (ringTag(c, a.ringShift) << 55) | ((uint64_t)contourMap(s_poly[ai]) << 31) | (uint64_t)((voff + s_vpre[ai]) & 0x7FFFFFFF);
+        storeRelaxed(a.ring + (c & (a.R - 1)), ent);
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) storeReleaseI(&a.ready[t], 1);
 
     // ---- 5. dual vertices, one per thread
     for (int i = tid; i < VT; i += GP_THREADS) {
-        const int l = upperIndex(s_vpre, GP_TILE, i);
-        const int tt = i - s_vpre[l];
-        const int64_t c = c0 + l;
-        const int x = (int)(c % CX), y = (int)((c / CX) % CY), z = (int)(c / ((int64_t)CX * CY));
+        const int ai = upperIndex(s_vpre, NA, i);
+        const int tt = i - s_vpre[ai];
+        int x, y, z;
+        cellXYZ(tb, s_cell[ai], CX, CY, x, y, z);
         float s[8];
         loadCorners(g, x, y, z, s);
         f3 wp, nn;
-        dualVertex(g, iso, a.surfIter, x, y, z, s, s_poly[l], tt, wp, nn);
+        dualVertex(g, iso, a.surfIter, x, y, z, s, s_poly[ai], tt, wp, nn);
         const int64_t v = voff + i;
         if (v < a.capV) {
             a.pos[3 * v + 0] = wp.x;
@@ -339,48 +419,38 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
     }
 
     // ---- 6. quads whose edge starts at a cell of this tile.  Their other cells lie
-    // 1, CX, CX+1, CX*CY, CX*CY+1 or CX*CY+CX cells back; wait once for those tiles.
-    if (QT > 0) {
-        if (tid < 12) {
-            const int64_t CXY = (int64_t)CX * CY;
-            const int64_t offs[6] = {1, CX, (int64_t)CX + 1, CXY, CXY + 1, CXY + CX};
-            const int64_t off = offs[tid >> 1];
-            const int64_t cell = (tid & 1) ? c0 + GP_TILE - 1 - off : c0 - off;
-            if (cell >= 0) {
-                const int64_t tn = cell / GP_TILE;
-                if (tn < t) waitFlag(&a.ready[tn], a.err);
-            }
-        }
-        __syncthreads();
-        fenceAcquire();
-    }
+    // 1, CX, CX+1, CX*CY, CX*CY+1 or CX*CY+CX cells back: in LDS or in the ring.
     for (int jq = tid; jq < QT; jq += GP_THREADS) {
-        const int l = upperIndex(s_qpre, GP_TILE, jq);
-        int r = jq - s_qpre[l];
-        const int own = s_own[l];
+        const int ai = upperIndex(s_qpre, NA, jq);
+        int r = jq - s_qpre[ai];
+        const int own = s_own[ai];
         int axis = 0;
         for (int b = 0; b < 3; ++b) {
             if ((own >> b) & 1) {
-                if (r == 0) { axis = b; break; }
+                if (r == 0) {
+                    axis = b;
+                    break;
+                }
                 --r;
             }
         }
         const int edgy = (own >> 3) & 1;
-        const int64_t c = c0 + l;
-        const int x = (int)(c % CX), y = (int)((c / CX) % CY), z = (int)(c / ((int64_t)CX * CY));
+        int x, y, z;
+        cellXYZ(tb, s_cell[ai], CX, CY, x, y, z);
         // own edge, then (dx, dy, dz, edge) of the three other cells around it, packed
         // 4 bits each (dx | dy<<1 | dz<<2, edge) per cell: x: (y-1: e2) (z-1: e4) (y-1,z-1: e6);
         // y: (x-1: e1) (z-1: e7) (x-1,z-1: e5); z: (x-1: e9) (y-1: e11) (x-1,y-1: e10)
         const int ownE = axis == 0 ? 0 : (axis == 1 ? 3 : 8);
         const uint32_t nbTable = axis == 0 ? 0x664422u : (axis == 1 ? 0x557411u : 0xA3B291u);
         int s0 = -1, s1 = -1, s2 = -1, s3 = -1;
-        auto put = [&](int sl, int v) {
-            if (sl == 0) s0 = v;
-            else if (sl == 1) s1 = v;
-            else if (sl == 2) s2 = v;
-            else s3 = v;
-        };
-        put(edgeSlot(ownE, edgy), (int)(voff + s_vpre[l] + polyContourOf(s_poly[l], ownE)));
+        {
+            const int sl = edgeSlot(ownE, edgy);
+            const int vid = (int)(voff + s_vpre[ai] + polyContourOf(s_poly[ai], ownE));
+            s0 = sl == 0 ? vid : s0;
+            s1 = sl == 1 ? vid : s1;
+            s2 = sl == 2 ? vid : s2;
+            s3 = sl == 3 ? vid : s3;
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const uint32_t ent = (nbTable >> (8 * k)) & 0xFFu;
@@ -388,13 +458,16 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
             const int64_t n = (int64_t)(z - ((d >> 2) & 1)) * CX * CY + (int64_t)(y - ((d >> 1) & 1)) * CX + (x - (d & 1));
             int vid;
             if (n >= c0) {
-                const int ln = (int)(n - c0);
-                vid = (int)(voff + s_vpre[ln] + polyContourOf(s_poly[ln], e));
+                const int an = s_aidx[n - c0];
+                vid = (int)(voff + s_vpre[an] + polyContourOf(s_poly[an], e));
             } else {
-                const int2 rent = a.ring[n % a.R];
-                vid = rent.x + (int)((((uint32_t)rent.y) >> (2 * e)) & 3u);
+                vid = ringVertex(a, n, e);
             }
-            put(edgeSlot(e, edgy), vid);
+            const int sl = edgeSlot(e, edgy);
+            s0 = sl == 0 ? vid : s0;
+            s1 = sl == 1 ? vid : s1;
+            s2 = sl == 2 ? vid : s2;
+            s3 = sl == 3 ? vid : s3;
         }
         const int64_t q = qoff + jq;
         if (q < a.capQ) {
@@ -404,11 +477,9 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
             atomicOr(a.err, GP_ERR_OVERFLOW);
         }
     }
+    // every ring value read above has been consumed by a store, so the reads are complete
     __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        storeReleaseI(&a.done[t], 1);
-    }
+    if (tid == 0) storeRelaxedI(&a.done[t], 1);
 }
 
 __global__ void kGridPassTotals(const unsigned long long* status, int64_t numTiles, int64_t* out) {
@@ -423,6 +494,16 @@ __global__ void kGridPassTotals(const unsigned long long* status, int64_t numTil
 }  // namespace
 
 int gridPassTile() { return GP_TILE; }
+
+void gridPassRing(int64_t C, int CX, int CY, int64_t* R, int* shift) {
+    // at least 8 z-slices of cells (the quads of a tile reach one slice back), and few
+    // enough generations that the 9-bit tag of a ring entry never wraps
+    const int64_t want = std::max<int64_t>({8 * ((int64_t)CX * CY + CX + GP_TILE), (C + 509) / 510, (int64_t)1 << 16});
+    int s = 16;
+    while (((int64_t)1 << s) < want) ++s;
+    *R = (int64_t)1 << s;
+    *shift = s;
+}
 
 void launchGridPass(const GridPassArgs& a, hipStream_t s) {
     if (a.numTiles <= 0) return;

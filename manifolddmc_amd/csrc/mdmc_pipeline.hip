@@ -1201,16 +1201,15 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
     a.CX = CX;
     a.CY = CY;
     a.numTiles = numTiles;
-    // flags: status (u64) | ready | done | ticket | err, zeroed together
-    const size_t flagBytes = (size_t)numTiles * (8 + 4 + 4) + 64;
+    // flags: status (u64) | done | ticket | err, zeroed together
+    const size_t flagBytes = (size_t)numTiles * (8 + 4) + 64;
     char* flags = arena.get<char>("gp_flags", flagBytes);
     a.status = (unsigned long long*)flags;
-    a.ready = (int*)(flags + 8 * (size_t)numTiles);
-    a.done = a.ready + numTiles;
+    a.done = (int*)(flags + 8 * (size_t)numTiles);
     a.ticket = (unsigned int*)(a.done + numTiles);
     a.err = (int*)(a.ticket + 4);
-    a.R = std::max<int64_t>(8 * ((int64_t)CX * CY + CX + T), 1 << 16);
-    a.ring = arena.get<int2>("gp_ring", (size_t)a.R);
+    gridPassRing(C, CX, CY, &a.R, &a.ringShift);
+    a.ring = arena.get<unsigned long long>("gp_ring", (size_t)a.R);
     int64_t* tot = arena.get<int64_t>("gp_totals", 4);
     // output capacity: what the buffers hold, at least an estimate; exact sizes on overflow
     int64_t capV = std::max<int64_t>(lastGridV + lastGridV / 16 + 1024, C / 6 + 4096);
@@ -1225,6 +1224,7 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
         a.capV = capV;
         a.capQ = capQ;
         zero(flags, flagBytes);
+        zero(a.ring, (size_t)a.R * sizeof(unsigned long long));
         MDMC_HIP(hipEventRecord(kev[0], stream));
         launchGridPass(a, stream);
         MDMC_HIP(hipEventRecord(kev[1], stream));
@@ -1235,6 +1235,7 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
         MDMC_HIP(hipMemcpyAsync(&err, a.err, sizeof(int), hipMemcpyDeviceToHost, stream));
         MDMC_HIP(hipStreamSynchronize(stream));
         if (err & GP_ERR_TIMEOUT) throw HipError("grid pass: a tile wait timed out", -4);
+        if (err & GP_ERR_RANGE) throw HipError("mesh exceeds the int32 id range", -3);
         V = h[0];
         Q = h[1];
         if (!(err & GP_ERR_OVERFLOW)) break;
