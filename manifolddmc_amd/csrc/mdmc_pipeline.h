@@ -53,7 +53,7 @@ class Pipeline {
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
-    void buildBuckets(bool setTwins);
+    void buildBuckets(bool fromQuads);
     void refreshHashTable();
     void checkValid(const char* what);
     void simplifyInit(int32_t** valence, int32_t** count);

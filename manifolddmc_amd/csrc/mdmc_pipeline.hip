@@ -47,8 +47,33 @@ __device__ __forceinline__ int heT(const int4* he, int h) { return ((const int*)
 __device__ __forceinline__ void setT(int4* he, int h, int t) { ((int*)he)[4 * (int64_t)h + 3] = t; }
 
 // ---------------------------------------------------------------- halfedge construction
+// bucket order after refreshHashTable: (face, position along the face from its first halfedge)
+// and ResetFaceFlags (refreshFaceFlagsGPU) ahead of the NM bits the bucket pass sets again
+__global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, int32_t* __restrict__ hkey,
+                          uint32_t* __restrict__ fattr) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    int h = face_he[f];
+    for (int i = 0; i < 4; ++i) {
+        hkey[h] = (int32_t)(4 * f + i);
+        h = heN(he, h);
+    }
+    fattr[f] &= ~(uint32_t)(QA_PATTERN | QA_REMOVE | QA_NM);
+}
+
+// ---- buckets: the halfedges joining one vertex pair.  They are listed under the
+// smaller endpoint m: entry (h, other endpoint | 0x80000000 when h starts at the
+// other endpoint), so one short contiguous list holds every bucket of m.
+__device__ __forceinline__ int2 minEntry(int h, int a, int b) {
+    return a <= b ? make_int2(h, b) : make_int2(h, (int)(0x80000000u | (uint32_t)a));
+}
+__device__ __forceinline__ int entryOther(int2 e) { return e.y & 0x7FFFFFFF; }
+
+// HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
+// bucket list count of every halfedge
 __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ vert_he, int32_t* __restrict__ hkey) {
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ vert_he, int32_t* __restrict__ hkey,
+                         int32_t* __restrict__ degM) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -63,64 +88,63 @@ __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restr
     atomicMax(&vert_he[v.y], b + 1);
     atomicMax(&vert_he[v.z], b + 2);
     atomicMax(&vert_he[v.w], b + 3);
-    hkey[b + 0] = b + 0;
-    hkey[b + 1] = b + 1;
-    hkey[b + 2] = b + 2;
-    hkey[b + 3] = b + 3;
+    *reinterpret_cast<int4*>(hkey + b) = make_int4(b, b + 1, b + 2, b + 3);
+    atomicAdd(&degM[min(v.x, v.y)], 1);
+    atomicAdd(&degM[min(v.y, v.z)], 1);
+    atomicAdd(&degM[min(v.z, v.w)], 1);
+    atomicAdd(&degM[min(v.w, v.x)], 1);
 }
 
-// bucket order after refreshHashTable: (face, position along the face from its first halfedge)
-__global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, int32_t* __restrict__ hkey) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    int h = face_he[f];
+// bucket lists straight from the quads (halfedge 4q+i joins corner i and corner i+1);
+// degM counts down to zero while it hands out the slots
+__global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const int64_t* __restrict__ offM, int32_t* __restrict__ degM,
+                              int2* __restrict__ listM) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v = quads[q];
+    const int c[5] = {v.x, v.y, v.z, v.w, v.x};
+    const int b = (int)(4 * q);
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
-        hkey[h] = (int32_t)(4 * f + i);
-        h = heN(he, h);
+        const int m = min(c[i], c[i + 1]);
+        const int slot = atomicSub(&degM[m], 1) - 1;
+        listM[offM[m] + slot] = minEntry(b + i, c[i], c[i + 1]);
     }
 }
 
-__global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ deg, int64_t* ctr) {
+__global__ void kCountMin(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ degM) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
-    const int o = heO(he, (int)h);
-    if (o < 0 || o >= V) { anomaly(ctr); return; }
-    atomicAdd(&deg[o], 1);
+    const int4 e = he[h];
+    atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
 }
 
-__global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const int64_t* __restrict__ off, int32_t* __restrict__ cursor,
-                           int32_t* __restrict__ list) {
+__global__ void kFillMin(const int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, int32_t* __restrict__ degM,
+                         int2* __restrict__ listM) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
-    const int o = heO(he, (int)h);
-    if (o < 0 || o >= V) return;
-    const int slot = atomicAdd(&cursor[o], 1);
-    list[off[o] + slot] = (int32_t)h;
+    const int4 e = he[h];
+    const int a = e.x, b = heO(he, e.z);
+    const int m = min(a, b);
+    const int slot = atomicSub(&degM[m], 1) - 1;
+    listM[offM[m] + slot] = minEntry((int)h, a, b);
 }
 
 // the halfedges joining the two vertices of h, sorted by bucket key; returns the true count
-__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
+__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int2* __restrict__ listM,
                             const int32_t* __restrict__ hkey, int mem[kMaxBucket]) {
     const int a = heO(he, h), b = heO(he, heN(he, h));
+    const int m = min(a, b), o = max(a, b);
     int n = 0;
-    for (int64_t i = off[a]; i < off[a + 1]; ++i) {
-        const int g = list[i];
-        if (heO(he, heN(he, g)) == b) {
-            if (n < kMaxBucket) mem[n] = g;
+    for (int64_t i = offM[m]; i < offM[m + 1]; ++i) {
+        const int2 e = listM[i];
+        if (entryOther(e) == o) {
+            if (n < kMaxBucket) mem[n] = e.x;
             ++n;
         }
     }
-    if (a != b) {
-        for (int64_t i = off[b]; i < off[b + 1]; ++i) {
-            const int g = list[i];
-            if (heO(he, heN(he, g)) == a) {
-                if (n < kMaxBucket) mem[n] = g;
-                ++n;
-            }
-        }
-    }
-    const int m = n < kMaxBucket ? n : kMaxBucket;
-    for (int i = 1; i < m; ++i) {  // insertion sort by key
+    const int cnt = n < kMaxBucket ? n : kMaxBucket;
+    for (int i = 1; i < cnt; ++i) {  // insertion sort by key
         const int x = mem[i];
         const int kx = hkey[x];
         int j = i - 1;
@@ -139,68 +163,117 @@ __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, in
     for (int k = 4; k < n && k < kMaxBucket; ++k) s[0] = mem[k];
 }
 
-// connectHalfedgeTwins (HalfedgeMesh.cu:173-291) for setTwins, bucket sizes for the flag kernels
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
-                         const int32_t* __restrict__ hkey, int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, int64_t* ctr) {
-    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H) return;
-    const int h = (int)hh;
-    int mem[kMaxBucket];
-    const int n = gatherBucket(h, he, off, list, hkey, mem);
-    if (n > kMaxBucket) anomaly(ctr);
-    int r = 0;
-    for (int i = 0; i < n && i < kMaxBucket; ++i)
-        if (mem[i] == h) r = i;
-    bsize[h] = n;
-    brank[h] = r;
-    if (!setTwins) return;
-    int t = -1;
-    if (n == 2) {
-        t = mem[1 - r];
-    } else if (n == 3) {
-        t = mem[(r + 1) % 3];
-    } else if (n == 4) {
-        const int v0 = heO(he, mem[0]), v1 = heO(he, mem[1]), v2 = heO(he, mem[2]);
-        int pair[4];
-        if (v0 != v1) { pair[0] = 1; pair[1] = 0; pair[2] = 3; pair[3] = 2; }
-        else if (v0 != v2) { pair[0] = 2; pair[2] = 0; pair[1] = 3; pair[3] = 1; }
-        else { pair[0] = 3; pair[3] = 0; pair[1] = 2; pair[2] = 1; }
-        t = mem[pair[r]];
+__device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF_NM : (n == 3 ? VF_WEIRD : (n == 1 ? VF_BOUNDARY : 0u)); }
+
+// One thread per vertex m handles every bucket listed under m:
+//   bucket size and rank of each halfedge (rank = position in key order),
+//   connectHalfedgeTwins (HalfedgeMesh.cu:173-291) when setTwins,
+//   refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
+//     as OR-ed bits on the origins (vflags cleared beforehand),
+//   the NM bit of faces on 4-halfedge buckets (fattr, and qattr when given).
+__global__ void kBucketsMin(int4* __restrict__ he, int64_t V, const int64_t* __restrict__ offM, const int2* __restrict__ listM,
+                            const int32_t* __restrict__ hkey, int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank,
+                            uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int64_t* ctr) {
+    const int64_t mm = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (mm >= V) return;
+    const int m = (int)mm;
+    const int64_t b0 = offM[m], b1 = offM[m + 1];
+    for (int64_t i = b0; i < b1; ++i) {
+        const int2 ei = listM[i];
+        const int o = entryOther(ei);
+        bool leader = true;
+        for (int64_t j = b0; j < i && leader; ++j) leader = entryOther(listM[j]) != o;
+        if (!leader) continue;
+        // members of bucket {m, o} in list order, then sorted by key
+        int mem[kMaxBucket];
+        int n = 0;
+        bool fromM = false, fromO = false;
+        for (int64_t j = i; j < b1; ++j) {
+            const int2 e = listM[j];
+            if (entryOther(e) != o) continue;
+            if (n < kMaxBucket) mem[n] = e.x;
+            ++n;
+            if (e.y < 0) fromO = true;
+            else fromM = true;
+        }
+        const int cnt = n < kMaxBucket ? n : kMaxBucket;
+        int key[kMaxBucket];
+        for (int k = 0; k < cnt; ++k) key[k] = hkey[mem[k]];
+        for (int k = 1; k < cnt; ++k) {
+            const int x = mem[k], kx = key[k];
+            int j = k - 1;
+            while (j >= 0 && key[j] > kx) {
+                mem[j + 1] = mem[j];
+                key[j + 1] = key[j];
+                --j;
+            }
+            mem[j + 1] = x;
+            key[j + 1] = kx;
+        }
+        if (n > kMaxBucket) {
+            // out of the supported range: sizes only, ranks 0, no twins
+            atomicAdd((unsigned long long*)&ctr[A_ANOMALY], (unsigned long long)n);
+            for (int64_t j = i; j < b1; ++j) {
+                const int2 e = listM[j];
+                if (entryOther(e) != o) continue;
+                bsize[e.x] = n;
+                brank[e.x] = 0;
+                if (setTwins) setT(he, e.x, -1);
+            }
+            for (int k = 0; k < cnt; ++k) brank[mem[k]] = k;
+            continue;
+        }
+        int pair[4] = {0, 0, 0, 0};
+        if (setTwins && n == 4) {
+            const int v0 = heO(he, mem[0]), v1 = heO(he, mem[1]), v2 = heO(he, mem[2]);
+            if (v0 != v1) { pair[0] = 1; pair[1] = 0; pair[2] = 3; pair[3] = 2; }
+            else if (v0 != v2) { pair[0] = 2; pair[2] = 0; pair[1] = 3; pair[3] = 1; }
+            else { pair[0] = 3; pair[3] = 0; pair[1] = 2; pair[2] = 1; }
+        }
+        for (int r = 0; r < n; ++r) {
+            const int h = mem[r];
+            bsize[h] = n;
+            brank[h] = r;
+            if (setTwins) {
+                int t = -1;
+                if (n == 2) t = mem[1 - r];
+                else if (n == 3) t = mem[(r + 1) % 3];
+                else if (n == 4) t = mem[pair[r]];
+                setT(he, h, t);
+            }
+        }
+        const uint32_t vf = bucketVertexFlag(n);
+        if (vf) {
+            if (fromM) atomicOr(&vflags[m], vf);
+            if (fromO) atomicOr(&vflags[o], vf);
+        }
+        if (n == 4) {
+            for (int r = 0; r < 4; ++r) {
+                const int f = heF(he, mem[r]);
+                atomicOr(&fattr[f], (uint32_t)QA_NM);
+                if (qattr) atomicOr(&qattr[f], (uint32_t)QA_NM);
+            }
+        }
     }
-    setT(he, h, t);
 }
 
-// face NM flags from 4-face edges (connectHalfedgeTwins case 4 / refreshFaceFlagsGPU)
-__global__ void kFaceFlags(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
-                           uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int refresh) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    int h = face_he[f];
-    bool nm = false;
-    for (int i = 0; i < 4; ++i) {
-        nm = nm || bsize[h] == 4;
-        h = heN(he, h);
-    }
-    uint32_t a = fattr[f];
-    if (refresh) a &= ~(QA_PATTERN | QA_REMOVE | QA_NM);  // ResetFaceFlags
-    if (nm) a |= QA_NM;
-    fattr[f] = a;
-    if (qattr && nm) qattr[f] |= QA_NM;
+// outgoing-halfedge lists (by origin), for the fan detection
+__global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ deg, int64_t* ctr) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) { anomaly(ctr); return; }
+    atomicAdd(&deg[o], 1);
 }
 
-// refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
-__global__ void kVertexFlags(const int64_t* __restrict__ off, const int32_t* __restrict__ list, int64_t V, const int32_t* __restrict__ bsize,
-                             uint32_t* __restrict__ vflags) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    uint32_t f = 0;
-    for (int64_t i = off[v]; i < off[v + 1]; ++i) {
-        const int n = bsize[list[i]];
-        if (n == 4) f |= VF_NM;
-        if (n == 3) f |= VF_WEIRD;
-        if (n == 1) f |= VF_BOUNDARY;
-    }
-    vflags[v] = f;
+__global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const int64_t* __restrict__ off, int32_t* __restrict__ deg,
+                           int32_t* __restrict__ list) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) return;
+    const int slot = atomicSub(&deg[o], 1) - 1;
+    list[off[o] + slot] = (int32_t)h;
 }
 
 // ---------------------------------------------------------------- simplifications
@@ -415,7 +488,7 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listM,
                           const int32_t* __restrict__ hkey, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                           const float* __restrict__ pos, GridDev g, float iso, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -423,7 +496,7 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __res
     const int h = (int)hh;
     if (bsize[h] < 4 || brank[h] != 0) return;
     int mem[kMaxBucket];
-    const int n = gatherBucket(h, he, off, list, hkey, mem);
+    const int n = gatherBucket(h, he, offM, listM, hkey, mem);
     int s[4];
     bucketSlots(mem, n, s);
     const int h0 = s[0];
@@ -899,6 +972,13 @@ __global__ void kMergeCheck(WC w, int count, const int32_t* __restrict__ num, co
     }
 }
 
+// firstWriter: [0, count) = INT32_MAX (atomicMin targets), [count, 2 count) = 0 (write counts)
+__global__ void kFillFirstWriter(int32_t* __restrict__ fw, int count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * (int64_t)count) return;
+    fw[i] = i < count ? INT32_MAX : 0;
+}
+
 __global__ void kMergeApply(WC w, int count, const int32_t* __restrict__ valid, const int64_t* ctr) {
     if (ctr[A_CONFLICT] != 0) return;
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1054,28 +1134,76 @@ __device__ int fanWalk(const int4* __restrict__ he, int start, bool cw, int64_t*
     return result;
 }
 
-__global__ void kFanKeys(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ key, int64_t* ctr) {
-    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H) return;
-    const int h = (int)hh;
-    const int a = fanWalk(he, h, true, ctr);
-    const int b = fanWalk(he, h, false, ctr);
-    key[h] = min(h, min(a, b));
-}
-
-__global__ void kFanCount(const int64_t* __restrict__ off, const int32_t* __restrict__ list, int64_t V, const int32_t* __restrict__ key,
-                          uint32_t* __restrict__ vflags) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
+// One thread per vertex v over its outgoing halfedges: the fan key of each (the smallest
+// id reached by fanWalk clockwise and counter-clockwise) and the number of distinct keys.
+// Up to kFanLocal halfedges the walks run on successor indices in LDS: a step to a
+// halfedge that does not start at v ends the walk, as the origin test of fanWalk does.
+constexpr int kFanLocal = 16;
+__global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ he, int64_t V, const int64_t* __restrict__ off,
+                                                      const int32_t* __restrict__ list, uint32_t* __restrict__ vflags,
+                                                      int32_t* __restrict__ keyScratch, int64_t* ctr) {
+    __shared__ int s_h[kFanLocal * kBlock];
+    __shared__ int8_t s_cw[kFanLocal * kBlock];
+    __shared__ int8_t s_ccw[kFanLocal * kBlock];
+    const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vv >= V) return;
+    const int tid = threadIdx.x;
+    const int64_t b0 = off[vv];
+    const int d = (int)(off[vv + 1] - b0);
     int distinct = 0;
-    for (int64_t i = off[v]; i < off[v + 1]; ++i) {
-        const int k = key[list[i]];
-        bool seen = false;
-        for (int64_t j = off[v]; j < i; ++j) seen = seen || key[list[j]] == k;
-        if (!seen) ++distinct;
+    if (d <= kFanLocal) {
+        auto H_ = [&](int k) -> int& { return s_h[k * kBlock + tid]; };
+        for (int k = 0; k < d; ++k) H_(k) = list[b0 + k];
+        // successor codes: index in the list, -1 = no successor, -2 = a halfedge of another origin
+        auto local = [&](int g) -> int {
+            if (g == -1) return -1;
+            for (int j = 0; j < d; ++j)
+                if (H_(j) == g) return j;
+            return -2;
+        };
+        for (int k = 0; k < d; ++k) {
+            const int h = H_(k);
+            const int t = heT(he, h);
+            s_cw[k * kBlock + tid] = (int8_t)local(t != -1 ? heN(he, t) : -1);
+            s_ccw[k * kBlock + tid] = (int8_t)local(heT(he, heN(he, heN(he, heN(he, h)))));
+        }
+        auto walk = [&](int start, const int8_t* succ) -> int {
+            int result = H_(start);
+            int cur = succ[start * kBlock + tid];
+            int guard = 0;
+            while (cur != -1) {
+                if (++guard > kFanGuard) {
+                    anomaly(ctr);
+                    break;
+                }
+                if (cur == -2) break;
+                const int hc = H_(cur);
+                if (hc == result) break;
+                if (hc < result) result = hc;
+                cur = succ[cur * kBlock + tid];
+            }
+            return result;
+        };
+        int keys[kFanLocal];
+        for (int k = 0; k < d; ++k) {
+            const int kk = min(walk(k, s_cw), walk(k, s_ccw));
+            bool seen = false;
+            for (int j = 0; j < k; ++j) seen = seen || keys[j] == kk;
+            keys[k] = kk;
+            if (!seen) ++distinct;
+        }
+    } else {
+        for (int k = 0; k < d; ++k) {
+            const int h = list[b0 + k];
+            const int kk = min(h, min(fanWalk(he, h, true, ctr), fanWalk(he, h, false, ctr)));
+            bool seen = false;
+            for (int j = 0; j < k; ++j) seen = seen || keyScratch[b0 + j] == kk;
+            keyScratch[b0 + k] = kk;
+            if (!seen) ++distinct;
+        }
     }
-    if (distinct > 1) vflags[v] |= VF_FANS;
-    else vflags[v] &= ~(uint32_t)VF_FANS;
+    if (distinct > 1) vflags[vv] |= VF_FANS;
+    else vflags[vv] &= ~(uint32_t)VF_FANS;
 }
 
 // ---------------------------------------------------------------- consistency checks
@@ -1253,25 +1381,26 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
 }
 
 // ---- HalfedgeMesh::initialize / refreshHashTable: CSR of outgoing halfedges and the buckets
-void Pipeline::buildBuckets(bool setTwins) {
+// bucket lists by smaller endpoint (degM holds the counts on entry and is zero on exit),
+// then the per-vertex bucket pass
+void Pipeline::buildBuckets(bool fromQuads) {
     int4* he = arena.ptr<int4>("he");
-    int32_t* hkey = arena.ptr<int32_t>("hkey");
-    int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
-    zero(deg, ((size_t)V + 1) * sizeof(int32_t));
-    if (H > 0) kCountOrigin<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, deg, counters());
-    int64_t* off = nullptr;
-    scan32(deg, V + 1, "off", &off);
-    int32_t* cursor = arena.get<int32_t>("cursor", (size_t)V + 1);
-    zero(cursor, ((size_t)V + 1) * sizeof(int32_t));
-    int32_t* list = arena.get<int32_t>("list", (size_t)H);
+    int32_t* degM = arena.ptr<int32_t>("degM");
+    int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
+    void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
+    scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
+    int2* listM = arena.get<int2>("listM", (size_t)H);
     int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
     int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
-    if (H > 0) {
-        kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, off, cursor, list);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, off, list, hkey, setTwins ? 1 : 0, bsize, brank, counters());
-    }
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
-    if (V > 0) kVertexFlags<<<gridFor(V), kBlock, 0, stream>>>(off, list, V, bsize, vflags);
+    zero(vflags, (size_t)V * sizeof(uint32_t));
+    if (H > 0) {
+        if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, degM, listM);
+        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, degM, listM);
+    }
+    if (V > 0)
+        kBucketsMin<<<gridFor(V), kBlock, 0, stream>>>(he, V, offM, listM, arena.ptr<int32_t>("hkey"), fromQuads ? 1 : 0, bsize, brank, vflags,
+                                                       arena.ptr<uint32_t>("fattr"), fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, counters());
 }
 
 void Pipeline::halfedgeInitialize() {
@@ -1281,16 +1410,17 @@ void Pipeline::halfedgeInitialize() {
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)F);
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     int32_t* vert_he = arena.get<int32_t>("vert_he", (size_t)V);
-    uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)V);
+    arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
+    int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
     checkValid("quad vertex ids");
     fillNeg(vert_he, (size_t)V * sizeof(int32_t));
     vertHeSize = V;
-    zero(vflags, (size_t)V * sizeof(uint32_t));
-    if (Q > 0) kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, vert_he, hkey);
+    zero(degM, ((size_t)V + 1) * sizeof(int32_t));
+    if (Q > 0)
+        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, vert_he, hkey, degM);
     buildBuckets(true);
-    if (F > 0) kFaceFlags<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, arena.ptr<int32_t>("bsize"), fattr, arena.ptr<uint32_t>("qattr"), 0);
 }
 
 void Pipeline::checkValid(const char* what) {
@@ -1305,9 +1435,11 @@ void Pipeline::refreshHashTable() {
     if (n > 0) kValidateMesh<<<gridFor(n), kBlock, 0, stream>>>(he, H, face_he, F, V, counters());
     checkValid("splitting");
     int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
-    if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey);
+    if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey, arena.ptr<uint32_t>("fattr"));
+    int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
+    zero(degM, ((size_t)V + 1) * sizeof(int32_t));
+    if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, degM);
     buildBuckets(false);
-    if (F > 0) kFaceFlags<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, arena.ptr<int32_t>("bsize"), arena.ptr<uint32_t>("fattr"), nullptr, 1);
 }
 
 // ---- simplifications
@@ -1384,7 +1516,7 @@ void Pipeline::simplify3333() {
 
 void Pipeline::fixManifoldTwins(const GridDev& g, float iso) {
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("off"), arena.ptr<int32_t>("list"), arena.ptr<int32_t>("hkey"),
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listM"), arena.ptr<int32_t>("hkey"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), g, iso, counters());
 }
 
@@ -1479,16 +1611,12 @@ void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
             if (step >= 4) {
                 int32_t* valid = arena.get<int32_t>("mvalid", (size_t)hsize);
                 int32_t* fw = arena.get<int32_t>("mfirst", 2 * (size_t)hsize);
-                std::vector<int32_t> init(2 * (size_t)hsize);
-                for (int i = 0; i < hsize; ++i) init[i] = INT32_MAX;
-                for (int i = hsize; i < 2 * hsize; ++i) init[i] = 0;
-                MDMC_HIP(hipMemcpyAsync(fw, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+                kFillFirstWriter<<<gridFor(2 * (int64_t)hsize), kBlock, 0, stream>>>(fw, hsize);
                 zero(counters() + A_CONFLICT, sizeof(int64_t));
                 kMergePlan<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, valid, fw, counters());
                 kMergeCheck<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, fw, counters());
                 kMergeApply<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, valid, counters());
                 kMergeSequential<<<1, 1, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, counters());
-                MDMC_HIP(hipStreamSynchronize(stream));  // init vector lifetime
             }
         }
         if (step >= 5) kMoveToSurface<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, g, iso);
@@ -1552,10 +1680,18 @@ void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* nu
 }
 
 void Pipeline::detectNonManifoldVertices() {
-    if (H == 0) return;
+    if (H == 0 || V == 0) return;
+    int4* he = arena.ptr<int4>("he");
+    int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
+    zero(deg, ((size_t)V + 1) * sizeof(int32_t));
+    kCountOrigin<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, deg, counters());
+    int64_t* off = arena.get<int64_t>("off", (size_t)V + 1);
+    void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
+    scanExclusiveI32(deg, off, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
+    int32_t* list = arena.get<int32_t>("list", (size_t)H);
+    kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, off, deg, list);
     int32_t* key = arena.get<int32_t>("fankey", (size_t)H);
-    kFanKeys<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, key, counters());
-    kFanCount<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<int64_t>("off"), arena.ptr<int32_t>("list"), V, key, arena.ptr<uint32_t>("vflags"));
+    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, off, list, arena.ptr<uint32_t>("vflags"), key, counters());
 }
 
 void Pipeline::computeStats(int64_t out[9]) {

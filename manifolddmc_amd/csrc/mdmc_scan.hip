@@ -62,11 +62,13 @@ __global__ void kScanTotal(const int64_t* sums, const int64_t* offs, int64_t nb,
 
 int64_t tiles(int64_t n) { return (n + kTile - 1) / kTile; }
 
+__global__ void kZero64(int64_t* p) { *p = 0; }
+
 template <class T>
 void scanImpl(const T* in, int64_t* out, int64_t n, int64_t* total, char* scratch, hipStream_t s) {
     const int64_t nb = tiles(n);
     if (n <= 0) {
-        hipMemsetAsync(total, 0, sizeof(int64_t), s);
+        kZero64<<<1, 1, 0, s>>>(total);
         return;
     }
     int64_t* sums = (int64_t*)scratch;
@@ -78,7 +80,7 @@ void scanImpl(const T* in, int64_t* out, int64_t n, int64_t* total, char* scratc
         scanImpl<int64_t>(sums, offs, nb, t2, rest + 64, s);
         kScanAdd<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, offs);
     } else {
-        hipMemsetAsync(offs, 0, sizeof(int64_t), s);
+        kZero64<<<1, 1, 0, s>>>(offs);
     }
     kScanTotal<<<1, 1, 0, s>>>(sums, offs, nb, total);
 }
