@@ -62,18 +62,19 @@ __global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict
 }
 
 // ---- buckets: the halfedges joining one vertex pair.  They are listed under the
-// smaller endpoint m: entry (h, other endpoint | 0x80000000 when h starts at the
-// other endpoint), so one short contiguous list holds every bucket of m.
-__device__ __forceinline__ int2 minEntry(int h, int a, int b) {
-    return a <= b ? make_int2(h, b) : make_int2(h, (int)(0x80000000u | (uint32_t)a));
+// smaller endpoint m as entries (h, other endpoint | 0x80000000 when h starts at the
+// other endpoint, bucket key of h, m), so one short contiguous list holds every
+// bucket of m and each entry carries what its bucket pass needs.
+__device__ __forceinline__ int4 minEntry(int h, int a, int b, int key) {
+    return a <= b ? make_int4(h, b, key, a) : make_int4(h, (int)(0x80000000u | (uint32_t)a), key, b);
 }
-__device__ __forceinline__ int entryOther(int2 e) { return e.y & 0x7FFFFFFF; }
+__device__ __forceinline__ int entryOther(int4 e) { return e.y & 0x7FFFFFFF; }
+__device__ __forceinline__ int entryOrigin(int4 e) { return e.y < 0 ? (e.y & 0x7FFFFFFF) : e.w; }
 
 // HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
 // bucket list count of every halfedge
 __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ vert_he, int32_t* __restrict__ hkey,
-                         int32_t* __restrict__ degM) {
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ vert_he, int32_t* __restrict__ degM) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -88,17 +89,16 @@ __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restr
     atomicMax(&vert_he[v.y], b + 1);
     atomicMax(&vert_he[v.z], b + 2);
     atomicMax(&vert_he[v.w], b + 3);
-    *reinterpret_cast<int4*>(hkey + b) = make_int4(b, b + 1, b + 2, b + 3);
     atomicAdd(&degM[min(v.x, v.y)], 1);
     atomicAdd(&degM[min(v.y, v.z)], 1);
     atomicAdd(&degM[min(v.z, v.w)], 1);
     atomicAdd(&degM[min(v.w, v.x)], 1);
 }
 
-// bucket lists straight from the quads (halfedge 4q+i joins corner i and corner i+1);
-// degM counts down to zero while it hands out the slots
+// bucket lists straight from the quads (halfedge 4q+i joins corner i and corner i+1,
+// its key is its id); degM counts down to zero while it hands out the slots
 __global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const int64_t* __restrict__ offM, int32_t* __restrict__ degM,
-                              int2* __restrict__ listM) {
+                              int4* __restrict__ listM) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -108,7 +108,7 @@ __global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const i
     for (int i = 0; i < 4; ++i) {
         const int m = min(c[i], c[i + 1]);
         const int slot = atomicSub(&degM[m], 1) - 1;
-        listM[offM[m] + slot] = minEntry(b + i, c[i], c[i + 1]);
+        listM[offM[m] + slot] = minEntry(b + i, c[i], c[i + 1], b + i);
     }
 }
 
@@ -119,40 +119,45 @@ __global__ void kCountMin(const int4* __restrict__ he, int64_t H, int32_t* __res
     atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
 }
 
-__global__ void kFillMin(const int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, int32_t* __restrict__ degM,
-                         int2* __restrict__ listM) {
+__global__ void kFillMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ hkey, const int64_t* __restrict__ offM,
+                         int32_t* __restrict__ degM, int4* __restrict__ listM) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int4 e = he[h];
     const int a = e.x, b = heO(he, e.z);
     const int m = min(a, b);
     const int slot = atomicSub(&degM[m], 1) - 1;
-    listM[offM[m] + slot] = minEntry((int)h, a, b);
+    listM[offM[m] + slot] = minEntry((int)h, a, b, hkey[h]);
 }
 
 // the halfedges joining the two vertices of h, sorted by bucket key; returns the true count
-__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int2* __restrict__ listM,
-                            const int32_t* __restrict__ hkey, int mem[kMaxBucket]) {
+__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int4* __restrict__ listM,
+                            int mem[kMaxBucket]) {
     const int a = heO(he, h), b = heO(he, heN(he, h));
     const int m = min(a, b), o = max(a, b);
+    int key[kMaxBucket];
     int n = 0;
     for (int64_t i = offM[m]; i < offM[m + 1]; ++i) {
-        const int2 e = listM[i];
+        const int4 e = listM[i];
         if (entryOther(e) == o) {
-            if (n < kMaxBucket) mem[n] = e.x;
+            if (n < kMaxBucket) {
+                mem[n] = e.x;
+                key[n] = e.z;
+            }
             ++n;
         }
     }
     const int cnt = n < kMaxBucket ? n : kMaxBucket;
     for (int i = 1; i < cnt; ++i) {  // insertion sort by key
-        const int x = mem[i];
-        const int kx = hkey[x];
+        const int x = mem[i], kx = key[i];
         int j = i - 1;
-        while (j >= 0 && hkey[mem[j]] > kx) {
+        while (j >= 0 && key[j] > kx) {
             mem[j + 1] = mem[j];
+            key[j + 1] = key[j];
             --j;
         }
         mem[j + 1] = x;
+        key[j + 1] = kx;
     }
     return n;
 }
@@ -165,95 +170,69 @@ __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, in
 
 __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF_NM : (n == 3 ? VF_WEIRD : (n == 1 ? VF_BOUNDARY : 0u)); }
 
-// One thread per vertex m handles every bucket listed under m:
-//   bucket size and rank of each halfedge (rank = position in key order),
+// One thread per list entry (= per halfedge h), reading only the short list of its
+// smaller endpoint:
+//   bucket size and rank of h (rank = position in key order),
 //   connectHalfedgeTwins (HalfedgeMesh.cu:173-291) when setTwins,
 //   refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
-//     as OR-ed bits on the origins (vflags cleared beforehand),
-//   the NM bit of faces on 4-halfedge buckets (fattr, and qattr when given).
-__global__ void kBucketsMin(int4* __restrict__ he, int64_t V, const int64_t* __restrict__ offM, const int2* __restrict__ listM,
-                            const int32_t* __restrict__ hkey, int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank,
-                            uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int64_t* ctr) {
-    const int64_t mm = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (mm >= V) return;
-    const int m = (int)mm;
-    const int64_t b0 = offM[m], b1 = offM[m + 1];
-    for (int64_t i = b0; i < b1; ++i) {
-        const int2 ei = listM[i];
-        const int o = entryOther(ei);
-        bool leader = true;
-        for (int64_t j = b0; j < i && leader; ++j) leader = entryOther(listM[j]) != o;
-        if (!leader) continue;
-        // members of bucket {m, o} in list order, then sorted by key
-        int mem[kMaxBucket];
-        int n = 0;
-        bool fromM = false, fromO = false;
-        for (int64_t j = i; j < b1; ++j) {
-            const int2 e = listM[j];
-            if (entryOther(e) != o) continue;
-            if (n < kMaxBucket) mem[n] = e.x;
-            ++n;
-            if (e.y < 0) fromO = true;
-            else fromM = true;
-        }
-        const int cnt = n < kMaxBucket ? n : kMaxBucket;
-        int key[kMaxBucket];
-        for (int k = 0; k < cnt; ++k) key[k] = hkey[mem[k]];
-        for (int k = 1; k < cnt; ++k) {
-            const int x = mem[k], kx = key[k];
-            int j = k - 1;
-            while (j >= 0 && key[j] > kx) {
-                mem[j + 1] = mem[j];
-                key[j + 1] = key[j];
-                --j;
+//     as OR-ed bits on the origin of h (vflags cleared beforehand),
+//   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given).
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int4* __restrict__ listM, int setTwins,
+                         int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
+                         uint32_t* __restrict__ qattr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H) return;
+    const int4 e = listM[i];
+    const int o = entryOther(e), key = e.z;
+    const int64_t lo = offM[e.w], hi = offM[e.w + 1];
+    int n = 0, r = 0;
+    for (int64_t j = lo; j < hi; ++j) {
+        const int4 f = listM[j];
+        if (entryOther(f) != o) continue;
+        ++n;
+        if (f.z < key) ++r;
+    }
+    const int h = e.x;
+    bsize[h] = n;
+    brank[h] = r;
+    if (setTwins) {
+        int t = -1;
+        if (n >= 2 && n <= 4) {
+            // members by rank: the one after h (n == 3), the other one (n == 2), or all four
+            int byRank[4] = {-1, -1, -1, -1};
+            int orgRank[4] = {-1, -1, -1, -1};
+            for (int64_t j = lo; j < hi; ++j) {
+                const int4 f = listM[j];
+                if (entryOther(f) != o) continue;
+                int rf = 0;
+                for (int64_t k = lo; k < hi; ++k) {
+                    const int4 g = listM[k];
+                    if (entryOther(g) == o && g.z < f.z) ++rf;
+                }
+                byRank[rf] = f.x;
+                orgRank[rf] = entryOrigin(f);
             }
-            mem[j + 1] = x;
-            key[j + 1] = kx;
-        }
-        if (n > kMaxBucket) {
-            // out of the supported range: sizes only, ranks 0, no twins
-            atomicAdd((unsigned long long*)&ctr[A_ANOMALY], (unsigned long long)n);
-            for (int64_t j = i; j < b1; ++j) {
-                const int2 e = listM[j];
-                if (entryOther(e) != o) continue;
-                bsize[e.x] = n;
-                brank[e.x] = 0;
-                if (setTwins) setT(he, e.x, -1);
-            }
-            for (int k = 0; k < cnt; ++k) brank[mem[k]] = k;
-            continue;
-        }
-        int pair[4] = {0, 0, 0, 0};
-        if (setTwins && n == 4) {
-            const int v0 = heO(he, mem[0]), v1 = heO(he, mem[1]), v2 = heO(he, mem[2]);
-            if (v0 != v1) { pair[0] = 1; pair[1] = 0; pair[2] = 3; pair[3] = 2; }
-            else if (v0 != v2) { pair[0] = 2; pair[2] = 0; pair[1] = 3; pair[3] = 1; }
-            else { pair[0] = 3; pair[3] = 0; pair[1] = 2; pair[2] = 1; }
-        }
-        for (int r = 0; r < n; ++r) {
-            const int h = mem[r];
-            bsize[h] = n;
-            brank[h] = r;
-            if (setTwins) {
-                int t = -1;
-                if (n == 2) t = mem[1 - r];
-                else if (n == 3) t = mem[(r + 1) % 3];
-                else if (n == 4) t = mem[pair[r]];
-                setT(he, h, t);
+            if (n == 2) {
+                t = byRank[1 - r];
+            } else if (n == 3) {
+                t = byRank[(r + 1) % 3];
+            } else {
+                const int v0 = orgRank[0], v1 = orgRank[1], v2 = orgRank[2];
+                int pr;
+                if (v0 != v1) pr = r ^ 1;                          // (0,1) (2,3)
+                else if (v0 != v2) pr = r ^ 2;                     // (0,2) (1,3)
+                else pr = 3 - r;                                   // (0,3) (1,2)
+                t = byRank[pr];
             }
         }
-        const uint32_t vf = bucketVertexFlag(n);
-        if (vf) {
-            if (fromM) atomicOr(&vflags[m], vf);
-            if (fromO) atomicOr(&vflags[o], vf);
-        }
-        if (n == 4) {
-            for (int r = 0; r < 4; ++r) {
-                const int f = heF(he, mem[r]);
-                atomicOr(&fattr[f], (uint32_t)QA_NM);
-                if (qattr) atomicOr(&qattr[f], (uint32_t)QA_NM);
-            }
-        }
+        setT(he, h, t);
+    }
+    const uint32_t vf = bucketVertexFlag(n);
+    if (vf) atomicOr(&vflags[entryOrigin(e)], vf);
+    if (n == 4) {
+        const int f = heF(he, h);
+        atomicOr(&fattr[f], (uint32_t)QA_NM);
+        if (qattr) atomicOr(&qattr[f], (uint32_t)QA_NM);
     }
 }
 
@@ -488,15 +467,16 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listM,
-                          const int32_t* __restrict__ hkey, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int4* __restrict__ listM,
+                          const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                           const float* __restrict__ pos, GridDev g, float iso, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
     const int h = (int)hh;
     if (bsize[h] < 4 || brank[h] != 0) return;
     int mem[kMaxBucket];
-    const int n = gatherBucket(h, he, offM, listM, hkey, mem);
+    const int n = gatherBucket(h, he, offM, listM, mem);
+    if (n > kMaxBucket) anomaly(ctr);
     int s[4];
     bucketSlots(mem, n, s);
     const int h0 = s[0];
@@ -1389,18 +1369,17 @@ void Pipeline::buildBuckets(bool fromQuads) {
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
-    int2* listM = arena.get<int2>("listM", (size_t)H);
+    int4* listM = arena.get<int4>("listM", (size_t)H);
     int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
     int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
     zero(vflags, (size_t)V * sizeof(uint32_t));
     if (H > 0) {
         if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, degM, listM);
-        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, degM, listM);
+        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, arena.ptr<int32_t>("hkey"), offM, degM, listM);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, listM, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
+                                                    fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr);
     }
-    if (V > 0)
-        kBucketsMin<<<gridFor(V), kBlock, 0, stream>>>(he, V, offM, listM, arena.ptr<int32_t>("hkey"), fromQuads ? 1 : 0, bsize, brank, vflags,
-                                                       arena.ptr<uint32_t>("fattr"), fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, counters());
 }
 
 void Pipeline::halfedgeInitialize() {
@@ -1411,7 +1390,6 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     int32_t* vert_he = arena.get<int32_t>("vert_he", (size_t)V);
     arena.get<uint32_t>("vflags", (size_t)V);
-    int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
     checkValid("quad vertex ids");
@@ -1419,7 +1397,7 @@ void Pipeline::halfedgeInitialize() {
     vertHeSize = V;
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
-        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, vert_he, hkey, degM);
+        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, vert_he, degM);
     buildBuckets(true);
 }
 
@@ -1516,7 +1494,7 @@ void Pipeline::simplify3333() {
 
 void Pipeline::fixManifoldTwins(const GridDev& g, float iso) {
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listM"), arena.ptr<int32_t>("hkey"),
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int4>("listM"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), g, iso, counters());
 }
 
