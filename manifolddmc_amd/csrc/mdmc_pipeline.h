@@ -22,7 +22,8 @@ class Pipeline {
     // live sizes of the resident mesh
     int64_t V = 0, Q = 0, H = 0, F = 0;
     int64_t dmcV = 0, dmcQ = 0;
-    int64_t vertHeSize = 0;
+    // vertices and quads of the last halfedge initialize (the vertex -> halfedge array)
+    int64_t vertHeV = 0, vertHeQ = 0;
     int64_t splitIterations = 0;
     int64_t extraAnomalies = 0;
 

@@ -62,19 +62,25 @@ __global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict
 }
 
 // ---- buckets: the halfedges joining one vertex pair.  They are listed under the
-// smaller endpoint m as entries (h, other endpoint | 0x80000000 when h starts at the
-// other endpoint, bucket key of h, m), so one short contiguous list holds every
-// bucket of m and each entry carries what its bucket pass needs.
-__device__ __forceinline__ int4 minEntry(int h, int a, int b, int key) {
-    return a <= b ? make_int4(h, b, key, a) : make_int4(h, (int)(0x80000000u | (uint32_t)a), key, b);
+// smaller endpoint m, so one short contiguous list holds every bucket of m.  An entry
+// is split in two arrays: A = (other endpoint | 0x80000000 when h starts at the other
+// endpoint, bucket key of h), read by every scan of the list, and B = (h, m).
+__device__ __forceinline__ void minEntry(int2* __restrict__ A, int2* __restrict__ B, int64_t i, int h, int a, int b, int key) {
+    if (a <= b) {
+        A[i] = make_int2(b, key);
+        B[i] = make_int2(h, a);
+    } else {
+        A[i] = make_int2((int)(0x80000000u | (uint32_t)a), key);
+        B[i] = make_int2(h, b);
+    }
 }
-__device__ __forceinline__ int entryOther(int4 e) { return e.y & 0x7FFFFFFF; }
-__device__ __forceinline__ int entryOrigin(int4 e) { return e.y < 0 ? (e.y & 0x7FFFFFFF) : e.w; }
+__device__ __forceinline__ int entryOther(int2 a) { return a.x & 0x7FFFFFFF; }
+__device__ __forceinline__ int entryOrigin(int2 a, int2 b) { return a.x < 0 ? (a.x & 0x7FFFFFFF) : b.y; }
 
 // HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
 // bucket list count of every halfedge
 __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ vert_he, int32_t* __restrict__ degM) {
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -85,20 +91,36 @@ __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restr
     he[b + 3] = make_int4(v.w, (int)q, b + 0, -1);
     face_he[q] = b;
     fattr[q] = qattr[q];
-    atomicMax(&vert_he[v.x], b + 0);  // "last writer wins" in ascending thread order
-    atomicMax(&vert_he[v.y], b + 1);
-    atomicMax(&vert_he[v.z], b + 2);
-    atomicMax(&vert_he[v.w], b + 3);
     atomicAdd(&degM[min(v.x, v.y)], 1);
     atomicAdd(&degM[min(v.y, v.z)], 1);
     atomicAdd(&degM[min(v.z, v.w)], 1);
     atomicAdd(&degM[min(v.w, v.x)], 1);
 }
 
+// HalfedgeMesh's vertex -> halfedge array as collectHalfedgeElements leaves it: the
+// highest halfedge starting at each vertex ("last writer wins" in ascending thread order)
+__global__ void kVertexHalfedge(const int4* __restrict__ quads, int64_t Q, int32_t* __restrict__ vert_he) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v = quads[q];
+    const int b = (int)(4 * q);
+    atomicMax(&vert_he[v.x], b + 0);
+    atomicMax(&vert_he[v.y], b + 1);
+    atomicMax(&vert_he[v.z], b + 2);
+    atomicMax(&vert_he[v.w], b + 3);
+}
+
+// entries [0, n0) = -1 (vertices of the last initialize), [n0, n) = 0 (slots added by resizes)
+__global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vert_he[i] = i < n0 ? -1 : 0;
+}
+
 // bucket lists straight from the quads (halfedge 4q+i joins corner i and corner i+1,
 // its key is its id); degM counts down to zero while it hands out the slots
 __global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const int64_t* __restrict__ offM, int32_t* __restrict__ degM,
-                              int4* __restrict__ listM) {
+                              int2* __restrict__ listA, int2* __restrict__ listB) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -108,7 +130,7 @@ __global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const i
     for (int i = 0; i < 4; ++i) {
         const int m = min(c[i], c[i + 1]);
         const int slot = atomicSub(&degM[m], 1) - 1;
-        listM[offM[m] + slot] = minEntry(b + i, c[i], c[i + 1], b + i);
+        minEntry(listA, listB, offM[m] + slot, b + i, c[i], c[i + 1], b + i);
     }
 }
 
@@ -120,29 +142,29 @@ __global__ void kCountMin(const int4* __restrict__ he, int64_t H, int32_t* __res
 }
 
 __global__ void kFillMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ hkey, const int64_t* __restrict__ offM,
-                         int32_t* __restrict__ degM, int4* __restrict__ listM) {
+                         int32_t* __restrict__ degM, int2* __restrict__ listA, int2* __restrict__ listB) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int4 e = he[h];
     const int a = e.x, b = heO(he, e.z);
     const int m = min(a, b);
     const int slot = atomicSub(&degM[m], 1) - 1;
-    listM[offM[m] + slot] = minEntry((int)h, a, b, hkey[h]);
+    minEntry(listA, listB, offM[m] + slot, (int)h, a, b, hkey[h]);
 }
 
 // the halfedges joining the two vertices of h, sorted by bucket key; returns the true count
-__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int4* __restrict__ listM,
-                            int mem[kMaxBucket]) {
+__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
+                            const int2* __restrict__ listB, int mem[kMaxBucket]) {
     const int a = heO(he, h), b = heO(he, heN(he, h));
     const int m = min(a, b), o = max(a, b);
     int key[kMaxBucket];
     int n = 0;
     for (int64_t i = offM[m]; i < offM[m + 1]; ++i) {
-        const int4 e = listM[i];
+        const int2 e = listA[i];
         if (entryOther(e) == o) {
             if (n < kMaxBucket) {
-                mem[n] = e.x;
-                key[n] = e.z;
+                mem[n] = listB[i].x;
+                key[n] = e.y;
             }
             ++n;
         }
@@ -177,58 +199,61 @@ __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF
 //   refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
 //     as OR-ed bits on the origin of h (vflags cleared beforehand),
 //   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given).
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int4* __restrict__ listM, int setTwins,
-                         int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
-                         uint32_t* __restrict__ qattr) {
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
+                         const int2* __restrict__ listB, int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank,
+                         uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= H) return;
-    const int4 e = listM[i];
-    const int o = entryOther(e), key = e.z;
-    const int64_t lo = offM[e.w], hi = offM[e.w + 1];
+    const int2 ea = listA[i], eb = listB[i];
+    const int o = entryOther(ea), key = ea.y;
+    const int64_t lo = offM[eb.y], hi = offM[eb.y + 1];
     int n = 0, r = 0;
+    int64_t other = -1;  // a member other than h (the twin when n == 2)
     for (int64_t j = lo; j < hi; ++j) {
-        const int4 f = listM[j];
+        const int2 f = listA[j];
         if (entryOther(f) != o) continue;
         ++n;
-        if (f.z < key) ++r;
+        if (f.y < key) ++r;
+        if (j != i) other = j;
     }
-    const int h = e.x;
+    const int h = eb.x;
     bsize[h] = n;
     brank[h] = r;
     if (setTwins) {
         int t = -1;
-        if (n >= 2 && n <= 4) {
-            // members by rank: the one after h (n == 3), the other one (n == 2), or all four
+        if (n == 2) {
+            t = listB[other].x;
+        } else if (n == 3 || n == 4) {
+            // members by rank: the one after h (n == 3) or the pairing of all four
             int byRank[4] = {-1, -1, -1, -1};
             int orgRank[4] = {-1, -1, -1, -1};
             for (int64_t j = lo; j < hi; ++j) {
-                const int4 f = listM[j];
+                const int2 f = listA[j];
                 if (entryOther(f) != o) continue;
                 int rf = 0;
                 for (int64_t k = lo; k < hi; ++k) {
-                    const int4 g = listM[k];
-                    if (entryOther(g) == o && g.z < f.z) ++rf;
+                    const int2 g = listA[k];
+                    if (entryOther(g) == o && g.y < f.y) ++rf;
                 }
-                byRank[rf] = f.x;
-                orgRank[rf] = entryOrigin(f);
+                const int2 fb = listB[j];
+                byRank[rf] = fb.x;
+                orgRank[rf] = entryOrigin(f, fb);
             }
-            if (n == 2) {
-                t = byRank[1 - r];
-            } else if (n == 3) {
+            if (n == 3) {
                 t = byRank[(r + 1) % 3];
             } else {
                 const int v0 = orgRank[0], v1 = orgRank[1], v2 = orgRank[2];
                 int pr;
-                if (v0 != v1) pr = r ^ 1;                          // (0,1) (2,3)
-                else if (v0 != v2) pr = r ^ 2;                     // (0,2) (1,3)
-                else pr = 3 - r;                                   // (0,3) (1,2)
+                if (v0 != v1) pr = r ^ 1;       // (0,1) (2,3)
+                else if (v0 != v2) pr = r ^ 2;  // (0,2) (1,3)
+                else pr = 3 - r;                // (0,3) (1,2)
                 t = byRank[pr];
             }
         }
         setT(he, h, t);
     }
     const uint32_t vf = bucketVertexFlag(n);
-    if (vf) atomicOr(&vflags[entryOrigin(e)], vf);
+    if (vf) atomicOr(&vflags[entryOrigin(ea, eb)], vf);
     if (n == 4) {
         const int f = heF(he, h);
         atomicOr(&fattr[f], (uint32_t)QA_NM);
@@ -467,7 +492,8 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int4* __restrict__ listM,
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
+                          const int2* __restrict__ listB,
                           const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                           const float* __restrict__ pos, GridDev g, float iso, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,7 +501,7 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __res
     const int h = (int)hh;
     if (bsize[h] < 4 || brank[h] != 0) return;
     int mem[kMaxBucket];
-    const int n = gatherBucket(h, he, offM, listM, mem);
+    const int n = gatherBucket(h, he, offM, listA, listB, mem);
     if (n > kMaxBucket) anomaly(ctr);
     int s[4];
     bucketSlots(mem, n, s);
@@ -1300,6 +1326,7 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
     const int CX = g.nx - 1, CY = g.ny - 1, CZ = g.nz - 1;
     const int64_t C = (int64_t)CX * CY * CZ;
     const int T = gridPassTile();
+    vertHeV = vertHeQ = 0;
     const int64_t numTiles = (C + T - 1) / T;
     GridPassArgs a;
     a.g = g;
@@ -1369,15 +1396,16 @@ void Pipeline::buildBuckets(bool fromQuads) {
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
-    int4* listM = arena.get<int4>("listM", (size_t)H);
+    int2* listA = arena.get<int2>("listA", (size_t)H);
+    int2* listB = arena.get<int2>("listB", (size_t)H);
     int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
     int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
     zero(vflags, (size_t)V * sizeof(uint32_t));
     if (H > 0) {
-        if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, degM, listM);
-        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, arena.ptr<int32_t>("hkey"), offM, degM, listM);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, listM, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
+        if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, degM, listA, listB);
+        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, arena.ptr<int32_t>("hkey"), offM, degM, listA, listB);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, listA, listB, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
                                                     fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr);
     }
 }
@@ -1388,16 +1416,16 @@ void Pipeline::halfedgeInitialize() {
     int4* he = arena.get<int4>("he", (size_t)H);
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)F);
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
-    int32_t* vert_he = arena.get<int32_t>("vert_he", (size_t)V);
     arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
     checkValid("quad vertex ids");
-    fillNeg(vert_he, (size_t)V * sizeof(int32_t));
-    vertHeSize = V;
+    // the vertex -> halfedge array is derived from these quads when it is copied out
+    vertHeV = V;
+    vertHeQ = Q;
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
-        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, vert_he, degM);
+        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM);
     buildBuckets(true);
 }
 
@@ -1494,7 +1522,7 @@ void Pipeline::simplify3333() {
 
 void Pipeline::fixManifoldTwins(const GridDev& g, float iso) {
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int4>("listM"),
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listA"), arena.ptr<int2>("listB"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), g, iso, counters());
 }
 
@@ -1626,12 +1654,10 @@ void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
             uint32_t* fattr = arena.grow<uint32_t>("fattr", (size_t)(F0 + addF), (size_t)F0, stream);
             kCopyBackFaces<<<gridFor(fsize), kBlock, 0, stream>>>(w, fsize, a, F0, face_he, fattr);
             kUpdateBack<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, he);
-            // the halfedge-vertex arrays are resized to V + working-copy size (thrust resize:
-            // kept prefix, new slots value-initialised to 0)
+            // the vertex arrays are resized to V + working-copy size (thrust resize: kept
+            // prefix, new slots value-initialised to 0); for the vertex -> halfedge array that
+            // leaves the initialize result on [0, vertHeV) and zeros after it
             const int64_t Vres = V0 + vsize;
-            int32_t* vert_he = arena.grow<int32_t>("vert_he", (size_t)Vres, (size_t)std::min(vertHeSize, Vres), stream);
-            if (Vres > vertHeSize) zero(vert_he + vertHeSize, (size_t)(Vres - vertHeSize) * sizeof(int32_t));
-            vertHeSize = Vres;
             arena.grow<uint32_t>("vflags", (size_t)Vres, 0, stream);
             V = V0 + addV;
             H = H0 + addH;
@@ -1690,7 +1716,12 @@ void Pipeline::copyOut(float* positions, float* normals, int32_t* halfedges, int
     if (normals && V) MDMC_HIP(hipMemcpyAsync(normals, arena.ptr<float>("nrm"), 3 * (size_t)V * sizeof(float), k, stream));
     if (halfedges && H) MDMC_HIP(hipMemcpyAsync(halfedges, arena.ptr<int4>("he"), (size_t)H * sizeof(int4), k, stream));
     if (face_halfedge && F) MDMC_HIP(hipMemcpyAsync(face_halfedge, arena.ptr<int32_t>("face_he"), (size_t)F * sizeof(int32_t), k, stream));
-    if (vertex_halfedge && V) MDMC_HIP(hipMemcpyAsync(vertex_halfedge, arena.ptr<int32_t>("vert_he"), (size_t)V * sizeof(int32_t), k, stream));
+    if (vertex_halfedge && V) {
+        int32_t* vh = arena.get<int32_t>("vert_he", (size_t)V);
+        kVertexHalfedgeInit<<<gridFor(V), kBlock, 0, stream>>>(vh, std::min(vertHeV, V), V);
+        if (vertHeQ > 0) kVertexHalfedge<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vh);
+        MDMC_HIP(hipMemcpyAsync(vertex_halfedge, vh, (size_t)V * sizeof(int32_t), k, stream));
+    }
     if (face_attr && F) {
         uint16_t* tmp = arena.get<uint16_t>("out_fattr", (size_t)F);
         kNarrow16<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, tmp);
