@@ -160,6 +160,7 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
         MDMC_HIP(hipEventRecord(ctx->ev[2], s));
         int64_t st9[9] = {0};
         if (p.V > 0) {
+            p.simplifyQueue = ((flags & MDMC_P3X3Y_COLOR) ? 1 : 0) + ((flags & MDMC_P3X3Y_OLD) ? 1 : 0) + ((flags & MDMC_P3333) ? 1 : 0);
             p.halfedgeInitialize();
             MDMC_HIP(hipEventRecord(ctx->ev[3], s));
             if (flags & MDMC_P3X3Y_COLOR) p.simplify3X3Y(true);

@@ -18,18 +18,16 @@ struct GridDev {
 };
 
 // --- scans (mdmc_scan.hip): exclusive prefix of n int32 values into int64,
-// returns the total through *total_dev (device int64).  scratch is sized by scanScratchBytes.
+// returns the total through *total_dev (device int64).  scratch is sized by scanScratchBytes;
+// after the scan, *scanTimeoutFlag(scratch) is nonzero if a look-back wait timed out.
 size_t scanScratchBytes(int64_t n);
 void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s);
-void scanExclusiveI64(const int64_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s);
+const int64_t* scanTimeoutFlag(const void* scratch);
 
 inline unsigned gridFor(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
 // --- DMC grid pass (mdmc_dmc.hip)
 void launchPrepareGrid(const float* in, const int inDims[3], float* out, const int outDims[3], int refinement, int pad, hipStream_t s);
-void launchDmcCount(const GridDev& g, float iso, uint16_t* lp16, int32_t* blockV, int32_t* blockQ, hipStream_t s);
-void launchDmcGenerate(const GridDev& g, float iso, int surfIter, const uint16_t* lp16, const int64_t* blockVOff,
-                       const int64_t* blockQOff, float* pos, float* nrm, int4* quads, uint32_t* qattr, hipStream_t s);
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s);
 
 }  // namespace mdmc

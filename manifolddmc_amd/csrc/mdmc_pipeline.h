@@ -27,6 +27,11 @@ class Pipeline {
     int64_t splitIterations = 0;
     int64_t extraAnomalies = 0;
 
+    // simplifications still to run: while positive, the halfedge initialize computes the
+    // vertex valence they start from
+    int simplifyQueue = 0;
+    bool valenceReady = false;
+
     void resetCounters();
     int64_t anomalies();
 

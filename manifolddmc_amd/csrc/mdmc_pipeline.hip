@@ -78,9 +78,9 @@ __device__ __forceinline__ int entryOther(int2 a) { return a.x & 0x7FFFFFFF; }
 __device__ __forceinline__ int entryOrigin(int2 a, int2 b) { return a.x < 0 ? (a.x & 0x7FFFFFFF) : b.y; }
 
 // HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
-// bucket list count of every halfedge
+// bucket list count of every halfedge; pos gets the halfedge's rank in its list
 __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM) {
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM, int32_t* __restrict__ pos) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -91,10 +91,12 @@ __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restr
     he[b + 3] = make_int4(v.w, (int)q, b + 0, -1);
     face_he[q] = b;
     fattr[q] = qattr[q];
-    atomicAdd(&degM[min(v.x, v.y)], 1);
-    atomicAdd(&degM[min(v.y, v.z)], 1);
-    atomicAdd(&degM[min(v.z, v.w)], 1);
-    atomicAdd(&degM[min(v.w, v.x)], 1);
+    int4 r;
+    r.x = atomicAdd(&degM[min(v.x, v.y)], 1);
+    r.y = atomicAdd(&degM[min(v.y, v.z)], 1);
+    r.z = atomicAdd(&degM[min(v.z, v.w)], 1);
+    r.w = atomicAdd(&degM[min(v.w, v.x)], 1);
+    *reinterpret_cast<int4*>(pos + b) = r;
 }
 
 // HalfedgeMesh's vertex -> halfedge array as collectHalfedgeElements leaves it: the
@@ -118,38 +120,42 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
 }
 
 // bucket lists straight from the quads (halfedge 4q+i joins corner i and corner i+1,
-// its key is its id); degM counts down to zero while it hands out the slots
-__global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const int64_t* __restrict__ offM, int32_t* __restrict__ degM,
+// its key is its id) at offset + rank; pos turns from rank into list index
+__global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const int64_t* __restrict__ offM, int32_t* __restrict__ pos,
                               int2* __restrict__ listA, int2* __restrict__ listB) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
     const int c[5] = {v.x, v.y, v.z, v.w, v.x};
     const int b = (int)(4 * q);
+    const int4 r = *reinterpret_cast<const int4*>(pos + b);
+    const int rk[4] = {r.x, r.y, r.z, r.w};
+    int p[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int m = min(c[i], c[i + 1]);
-        const int slot = atomicSub(&degM[m], 1) - 1;
-        minEntry(listA, listB, offM[m] + slot, b + i, c[i], c[i + 1], b + i);
+        p[i] = (int)(offM[m] + rk[i]);
+        minEntry(listA, listB, p[i], b + i, c[i], c[i + 1], b + i);
     }
+    *reinterpret_cast<int4*>(pos + b) = make_int4(p[0], p[1], p[2], p[3]);
 }
 
-__global__ void kCountMin(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ degM) {
+__global__ void kCountMin(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ degM, int32_t* __restrict__ pos) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int4 e = he[h];
-    atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
+    pos[h] = atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
 }
 
 __global__ void kFillMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ hkey, const int64_t* __restrict__ offM,
-                         int32_t* __restrict__ degM, int2* __restrict__ listA, int2* __restrict__ listB) {
+                         int32_t* __restrict__ pos, int2* __restrict__ listA, int2* __restrict__ listB) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int4 e = he[h];
     const int a = e.x, b = heO(he, e.z);
-    const int m = min(a, b);
-    const int slot = atomicSub(&degM[m], 1) - 1;
-    minEntry(listA, listB, offM[m] + slot, (int)h, a, b, hkey[h]);
+    const int p = (int)(offM[min(a, b)] + pos[h]);
+    minEntry(listA, listB, p, (int)h, a, b, hkey[h]);
+    pos[h] = p;
 }
 
 // the halfedges joining the two vertices of h, sorted by bucket key; returns the true count
@@ -192,18 +198,20 @@ __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, in
 
 __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF_NM : (n == 3 ? VF_WEIRD : (n == 1 ? VF_BOUNDARY : 0u)); }
 
-// One thread per list entry (= per halfedge h), reading only the short list of its
-// smaller endpoint:
+// One thread per halfedge h (at its list index pos[h]), reading only the short list of
+// its smaller endpoint:
 //   bucket size and rank of h (rank = position in key order),
 //   connectHalfedgeTwins (HalfedgeMesh.cu:173-291) when setTwins,
 //   refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
 //     as OR-ed bits on the origin of h (vflags cleared beforehand),
 //   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given).
 __global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
-                         const int2* __restrict__ listB, int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank,
-                         uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= H) return;
+                         const int2* __restrict__ listB, const int32_t* __restrict__ pos, int setTwins, int32_t* __restrict__ bsize,
+                         int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr,
+                         int32_t* __restrict__ valence) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int64_t i = pos[hh];
     const int2 ea = listA[i], eb = listB[i];
     const int o = entryOther(ea), key = ea.y;
     const int64_t lo = offM[eb.y], hi = offM[eb.y + 1];
@@ -251,6 +259,12 @@ __global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __rest
             }
         }
         setT(he, h, t);
+        if (valence) {
+            // the simplifications' vertex valence: outgoing halfedges, plus incoming boundary ones
+            const int org = entryOrigin(ea, eb);
+            atomicAdd(&valence[org], 1);
+            if (t == -1) atomicAdd(&valence[org == eb.y ? o : eb.y], 1);
+        }
     }
     const uint32_t vf = bucketVertexFlag(n);
     if (vf) atomicOr(&vflags[entryOrigin(ea, eb)], vf);
@@ -1311,7 +1325,12 @@ int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outNam
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(n + 1));
     int64_t* tot = arena.get<int64_t>("scan_total", 1);
     scanExclusiveI32(in, *out, n, tot, scratch, stream);
-    return readI64(tot);
+    int64_t h[2] = {0, 0};
+    MDMC_HIP(hipMemcpyAsync(&h[0], tot, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipMemcpyAsync(&h[1], scanTimeoutFlag(scratch), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    if (h[1]) throw HipError("prefix sum: a look-back wait timed out", -4);
+    return h[0];
 }
 
 void Pipeline::zero(void* p, size_t bytes) { MDMC_HIP(hipMemsetAsync(p, 0, bytes, stream)); }
@@ -1390,9 +1409,12 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
 // ---- HalfedgeMesh::initialize / refreshHashTable: CSR of outgoing halfedges and the buckets
 // bucket lists by smaller endpoint (degM holds the counts on entry and is zero on exit),
 // then the per-vertex bucket pass
+// bucket lists by smaller endpoint from the counts in degM and the ranks in pos, then the
+// per-halfedge bucket pass
 void Pipeline::buildBuckets(bool fromQuads) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
+    int32_t* pos = arena.ptr<int32_t>("bl_pos");
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
@@ -1402,11 +1424,18 @@ void Pipeline::buildBuckets(bool fromQuads) {
     int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
     zero(vflags, (size_t)V * sizeof(uint32_t));
+    int32_t* valence = nullptr;
+    valenceReady = false;
+    if (fromQuads && simplifyQueue > 0) {
+        valence = arena.get<int32_t>("valence", (size_t)V);
+        zero(valence, (size_t)V * sizeof(int32_t));
+        valenceReady = true;
+    }
     if (H > 0) {
-        if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, degM, listA, listB);
-        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, arena.ptr<int32_t>("hkey"), offM, degM, listA, listB);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, listA, listB, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
-                                                    fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr);
+        if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, pos, listA, listB);
+        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, arena.ptr<int32_t>("hkey"), offM, pos, listA, listB);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, listA, listB, pos, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
+                                                    fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
     }
 }
 
@@ -1418,6 +1447,7 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
+    int32_t* pos = arena.get<int32_t>("bl_pos", (size_t)H);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
     checkValid("quad vertex ids");
     // the vertex -> halfedge array is derived from these quads when it is copied out
@@ -1425,7 +1455,7 @@ void Pipeline::halfedgeInitialize() {
     vertHeQ = Q;
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
-        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM);
+        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, pos);
     buildBuckets(true);
 }
 
@@ -1443,8 +1473,9 @@ void Pipeline::refreshHashTable() {
     int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
     if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey, arena.ptr<uint32_t>("fattr"));
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
+    int32_t* pos = arena.get<int32_t>("bl_pos", (size_t)H);
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
-    if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, degM);
+    if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, degM, pos);
     buildBuckets(false);
 }
 
@@ -1481,15 +1512,19 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
 }
 
 void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
+    if (simplifyQueue > 0) --simplifyQueue;
     *valence = arena.get<int32_t>("valence", (size_t)V);
     *count = arena.get<int32_t>("vcount", (size_t)V);
     int32_t* vtype = arena.get<int32_t>("vtype", (size_t)V);
     int32_t* vtarget = arena.get<int32_t>("vtarget", (size_t)V);
-    zero(*valence, (size_t)V * sizeof(int32_t));
     zero(*count, (size_t)V * sizeof(int32_t));
     zero(vtype, (size_t)V * sizeof(int32_t));
     fillNeg(vtarget, (size_t)V * sizeof(int32_t));
-    if (H > 0) kValence<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, *valence);
+    if (!valenceReady) {  // not computed by the last bucket pass
+        zero(*valence, (size_t)V * sizeof(int32_t));
+        if (H > 0) kValence<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, *valence);
+    }
+    valenceReady = false;
 }
 
 void Pipeline::simplify3X3Y(bool colour) {

@@ -1,107 +1,163 @@
-// mdmc_scan.hip -- device-wide exclusive prefix sums (reduce, scan the block sums, add back).
+// mdmc_scan.hip -- device-wide exclusive prefix sum of int32 counts into int64 offsets,
+// in one pass: tiles of 4096 counts are claimed in order with an atomic ticket and get
+// their offset by decoupled look-back over the tiles before them.
+//
+// Look-back words pack flag (2 bits) and value (62 bits), so relaxed device-scope
+// atomics suffice: a reader needs nothing beyond the word it reads.  Every wait is
+// bounded; a timeout sets scratch word 1, which scanTimedOut reports.
 #include "mdmc_internal.h"
 
 namespace mdmc {
 
 namespace {
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 16;
 constexpr int kTile = kScanBlock * kScanItems;
+constexpr int kSpinLimit = 1 << 22;
 
-// inclusive scan of one int64 per thread across a 256-thread block (4 waves of 64)
-__device__ __forceinline__ int64_t blockInclusive(int64_t v, int64_t* lds) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int64_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    if (lane == 63) lds[wid] = v;
-    __syncthreads();
-    int64_t add = 0;
-    for (int w = 0; w < wid; ++w) add += lds[w];
-    __syncthreads();
-    return v + add;
+constexpr uint64_t FLAG_AGG = 1ULL, FLAG_INC = 2ULL;
+constexpr uint64_t VAL_MASK = (1ULL << 62) - 1;
+
+__device__ __forceinline__ uint64_t loadRelaxed(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void storeRelaxed(unsigned long long* p, uint64_t v) {
+    __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <class T>
-__global__ void __launch_bounds__(kScanBlock) kScanLocal(const T* __restrict__ in, int64_t* __restrict__ out, int64_t n, int64_t* __restrict__ sums) {
-    __shared__ int64_t lds[4];
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
-    int64_t v[kScanItems];
+// scratch layout: [0] ticket, [1] timeout flag, [2 ..] one look-back word per tile
+__global__ void kScanReset(unsigned long long* scratch, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) scratch[i] = 0;
+}
+
+__global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ in, int64_t* __restrict__ out, int64_t n, int64_t numTiles,
+                                                    unsigned long long* __restrict__ scratch, int64_t* __restrict__ total) {
+    __shared__ int32_t s_in[kTile];
+    __shared__ int64_t s_wave[kScanBlock / 64];
+    __shared__ long long s_tile, s_off;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) s_tile = (long long)atomicAdd(&scratch[0], 1ULL);
+    __syncthreads();
+    const int64_t t = s_tile;
+    if (t >= numTiles) return;
+    unsigned long long* status = scratch + 2;
+    const int64_t base = t * kTile;
+
+    // coalesced load into LDS, then 16 consecutive counts per thread
+    for (int i = tid; i < kTile; i += kScanBlock) {
+        const int64_t j = base + i;
+        s_in[i] = j < n ? in[j] : 0;
+    }
+    __syncthreads();
+    int32_t v[kScanItems];
     int64_t acc = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        const int64_t j = base + i;
-        v[i] = (j < n) ? (int64_t)in[j] : 0;
-        acc += v[i];
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = s_in[tid * kScanItems + k];
+        acc += v[k];
     }
-    const int64_t incl = blockInclusive(acc, lds);
-    int64_t run = incl - acc;
+    int64_t x = acc;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        const int64_t j = base + i;
-        if (j < n) out[j] = run;
-        run += v[i];
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
-    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = incl;
-}
-
-__global__ void __launch_bounds__(kScanBlock) kScanAdd(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ offs) {
-    const int64_t base = (int64_t)blockIdx.x * kTile;
-    const int64_t o = offs[blockIdx.x];
-    for (int i = threadIdx.x; i < kTile; i += kScanBlock) {
-        const int64_t j = base + i;
-        if (j < n) out[j] += o;
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    int64_t add = 0, tileSum = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        const int64_t s = s_wave[w];
+        if (w < wid) add += s;
+        tileSum += s;
     }
-}
+    int64_t run = x - acc + add;  // exclusive prefix of this thread inside the tile
 
-__global__ void kScanTotal(const int64_t* sums, const int64_t* offs, int64_t nb, int64_t* total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = (nb > 0) ? offs[nb - 1] + sums[nb - 1] : 0;
-}
-
-int64_t tiles(int64_t n) { return (n + kTile - 1) / kTile; }
-
-__global__ void kZero64(int64_t* p) { *p = 0; }
-
-template <class T>
-void scanImpl(const T* in, int64_t* out, int64_t n, int64_t* total, char* scratch, hipStream_t s) {
-    const int64_t nb = tiles(n);
-    if (n <= 0) {
-        kZero64<<<1, 1, 0, s>>>(total);
-        return;
+    // look-back (wave 0)
+    if (tid < 64) {
+        if (t == 0) {
+            if (lane == 0) {
+                storeRelaxed(&status[0], (FLAG_INC << 62) | ((uint64_t)tileSum & VAL_MASK));
+                s_off = 0;
+            }
+        } else {
+            if (lane == 0) storeRelaxed(&status[t], (FLAG_AGG << 62) | ((uint64_t)tileSum & VAL_MASK));
+            int64_t accum = 0;
+            int64_t i = t - 1;
+            int spins = 0;
+            bool done = false;
+            while (!done) {
+                const int64_t j = i - lane;
+                const uint64_t st = j >= 0 ? loadRelaxed(&status[j]) : (FLAG_INC << 62);
+                const uint64_t f = st >> 62;
+                const uint64_t incMask = __ballot(f == FLAG_INC);
+                const uint64_t zeroMask = __ballot(f == 0);
+                const int firstInc = incMask ? __ffsll((unsigned long long)incMask) - 1 : 64;
+                const uint64_t upto = firstInc >= 63 ? ~0ULL : ((1ULL << (firstInc + 1)) - 1ULL);
+                if (zeroMask & upto) {
+                    if (++spins > kSpinLimit) {
+                        if (lane == 0) storeRelaxed(&scratch[1], 1ULL);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                int64_t val = lane <= firstInc ? (int64_t)(st & VAL_MASK) : 0;
+#pragma unroll
+                for (int d = 32; d > 0; d >>= 1) val += __shfl_down(val, d, 64);
+                accum += __shfl(val, 0, 64);
+                if (firstInc < 64) done = true;
+                else i -= 64;
+            }
+            if (lane == 0) {
+                storeRelaxed(&status[t], (FLAG_INC << 62) | ((uint64_t)(accum + tileSum) & VAL_MASK));
+                s_off = accum;
+            }
+        }
     }
-    int64_t* sums = (int64_t*)scratch;
-    int64_t* offs = sums + nb;
-    char* rest = (char*)(offs + nb);
-    kScanLocal<T><<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums);
-    if (nb > 1) {
-        int64_t* t2 = (int64_t*)rest;
-        scanImpl<int64_t>(sums, offs, nb, t2, rest + 64, s);
-        kScanAdd<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, offs);
-    } else {
-        kZero64<<<1, 1, 0, s>>>(offs);
+    __syncthreads();
+    run += s_off;
+
+    // exclusive offsets back through LDS, half a tile at a time, for coalesced int64 stores
+    int64_t* s_out = reinterpret_cast<int64_t*>(s_in);  // 4096 int32 slots hold 2048 int64
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        if ((tid >> 7) == half) {
+            int64_t r = run;
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k) {
+                s_out[(tid & 127) * kScanItems + k] = r;
+                r += v[k];
+            }
+        }
+        __syncthreads();
+        const int64_t o = base + (int64_t)half * (kTile / 2);
+        for (int i = tid; i < kTile / 2; i += kScanBlock)
+            if (o + i < n) out[o + i] = s_out[i];
     }
-    kScanTotal<<<1, 1, 0, s>>>(sums, offs, nb, total);
+    if (t == numTiles - 1 && tid == 0) *total = s_off + tileSum;
 }
 }  // namespace
 
 size_t scanScratchBytes(int64_t n) {
-    size_t bytes = 0;
-    int64_t m = n;
-    do {
-        const int64_t nb = tiles(m);
-        bytes += (size_t)(2 * nb) * sizeof(int64_t) + 64;
-        m = nb;
-    } while (m > 1);
-    return bytes + 256;
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    return (size_t)(tiles + 2) * sizeof(unsigned long long) + 256;
 }
 
 void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total, void* scratch, hipStream_t s) {
-    scanImpl<int32_t>(in, out, n, total, (char*)scratch, s);
+    unsigned long long* w = (unsigned long long*)scratch;
+    const int64_t tiles = n > 0 ? (n + kTile - 1) / kTile : 0;
+    const int64_t words = tiles + 2;
+    kScanReset<<<(unsigned)((words + 255) / 256), 256, 0, s>>>(w, words);
+    if (tiles == 0) {
+        kScanReset<<<1, 1, 0, s>>>((unsigned long long*)total, 1);
+        return;
+    }
+    kScan<<<(unsigned)tiles, kScanBlock, 0, s>>>(in, out, n, tiles, w, total);
 }
-void scanExclusiveI64(const int64_t* in, int64_t* out, int64_t n, int64_t* total, void* scratch, hipStream_t s) {
-    scanImpl<int64_t>(in, out, n, total, (char*)scratch, s);
-}
+
+const int64_t* scanTimeoutFlag(const void* scratch) { return (const int64_t*)scratch + 1; }
 
 }  // namespace mdmc
