@@ -80,10 +80,11 @@ typedef struct {
     int64_t nm_edges4, nm_edges3, n_edges, nm_vertices; /* nm_vertices = -1 when detection is off */
     int64_t quads_with_nm[5];                           /* faces with k edges of >2 faces, k = 0..4 */
     int64_t split_iterations;
-    int64_t anomalies; /* reference-undefined situations met */
+    int64_t anomalies; /* situations the reference leaves undefined (out-of-range ids it would
+                          read, fan walks past 4096 steps, buckets past 16 halfedges); 0 normally */
     int64_t cells;     /* cells of the processed grid */
-    /* stage times (ms, HIP events on the context's stream): whole call, pad/refine, DMC
-     * (count + scan + generate), halfedge build, simplifications, twin repair, split, detection */
+    /* stage times (ms, HIP events on the context's stream): whole call, pad/refine, DMC grid
+     * pass, halfedge build, simplifications, twin repair, split, detection */
     double ms_total, ms_grid, ms_dmc, ms_halfedge, ms_simplify, ms_fix_twins, ms_split, ms_detect;
     /* the grid-pass kernel alone (classification, dual vertices, quads) */
     double ms_grid_pass_kernel;

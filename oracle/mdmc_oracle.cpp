@@ -23,10 +23,10 @@
 // (normalize = v * (1/sqrt(dot)), min/max/clamp as (y<x)?y:x, mix = x(1-a)+ya).
 // The same code instantiated with double gives the FP64 oracle.
 //
-// Parity status of the functions below is recorded in DESIGN.md; every public
-// stage is pinned by the reference's Data-and-Results fixtures, closed forms or
-// invariants in tests/, except the P3X3Y/P3333 simplifications, which no
-// reference fixture covers ("parity unpinned" beyond invariants).
+// Parity status: every public stage is pinned by the reference's Data-and-Results
+// fixtures, closed forms or invariants in tests/, except the P3X3Y/P3333
+// simplifications, which no reference fixture covers: those are "parity
+// unpinned" beyond the invariants and the Euler-characteristic checks.
 
 #include <algorithm>
 #include <cmath>
