@@ -43,6 +43,8 @@ typedef enum {
 #define MDMC_P3333           0x10 /* m_simplifyP3333, HalfedgeMesh::simplify3333 */
 #define MDMC_SPLIT_NM        0x20 /* m_splitNonManifoldHalfedges, ManifoldSplitter::run */
 #define MDMC_DETECT_NM_VERTS 0x40 /* m_detectNonManifoldVertices */
+#define MDMC_RECORD_ADJOINT  0x80 /* no reference counterpart: keep the DMC topology for mdmc_backward
+                                     (needs MDMC_SURF_ITER, P3X3Y*, P3333 and SPLIT_NM off) */
 #define MDMC_DEFAULT_FLAGS   (MDMC_PAD1 | MDMC_SURF_ITER | MDMC_SPLIT_NM | MDMC_DETECT_NM_VERTS)
 #define MDMC_ALL_POST        (MDMC_P3X3Y_COLOR | MDMC_P3X3Y_OLD | MDMC_P3333 | MDMC_SPLIT_NM | MDMC_DETECT_NM_VERTS)
 
@@ -123,6 +125,15 @@ mdmc_status mdmc_mesh_get_info(const mdmc_ctx* ctx, mdmc_mesh_info* info);
 mdmc_status mdmc_mesh_copy(const mdmc_ctx* ctx, float* positions, float* normals, int32_t* halfedges,
                            int32_t* face_halfedge, uint16_t* face_attr, uint8_t* vertex_flags,
                            int32_t* vertex_halfedge, int32_t dst_on_device);
+
+/* Adjoint of the last mdmc_extract (no reference counterpart; the reference has no backward
+ * pass).  That extract must have had MDMC_RECORD_ADJOINT.  With the topology frozen, maps
+ * dL_dpos (device float[3 * n_vertices], d loss / d vertex position, the mesh's vertex order)
+ * to dL_dfield (device float[dims[0]*dims[1]*dims[2]] of the grid passed to mdmc_extract),
+ * through the dual-vertex formula of CellIntersection::sliceP (CellIntersection.inl:368-439)
+ * and the padding / refinement of refineGrid (UniformGrid.inl:279-319).  accumulate = 1 adds
+ * to dL_dfield instead of overwriting it.  Asynchronous on the context's stream; deterministic. */
+mdmc_status mdmc_backward(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield, int32_t accumulate);
 
 /* Triangle index buffer, two triangles per face (x,y,z),(z,w,x): HalfedgeMesh.cu:690-714.
  * indices uint32[6*nF], device or host. */

@@ -23,6 +23,7 @@ P3X3Y_OLD = 0x08
 P3333 = 0x10
 SPLIT_NM = 0x20
 DETECT_NM_VERTS = 0x40
+RECORD_ADJOINT = 0x80  # keep the DMC topology for backward() (no reference counterpart)
 DEFAULT_FLAGS = PAD1 | SURF_ITER | SPLIT_NM | DETECT_NM_VERTS
 ALL_POST = P3X3Y_COLOR | P3X3Y_OLD | P3333 | SPLIT_NM | DETECT_NM_VERTS
 
@@ -84,7 +85,7 @@ class _Info(ctypes.Structure):
 _lib = None
 
 EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", "mdmc_extract",
-           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles"]
+           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_backward"]
 
 
 def lib():
@@ -112,6 +113,8 @@ def lib():
         L.mdmc_mesh_copy.restype = i32
         L.mdmc_mesh_triangles.argtypes = [vp, vp, i32]
         L.mdmc_mesh_triangles.restype = i32
+        L.mdmc_backward.argtypes = [vp, vp, vp, i32]
+        L.mdmc_backward.restype = i32
         _lib = L
     return _lib
 
@@ -173,6 +176,7 @@ class DualMarchingCubes:
         if st != 0:
             raise MdmcError(st, "mdmc_create failed")
         self.last_stats = None
+        self.generation = 0  # counts extractions; backward() belongs to the latest one
 
     def close(self):
         if self._ctx:
@@ -219,9 +223,20 @@ class DualMarchingCubes:
         g.data_on_device = 1 if on_device else 0
         o = _Options(int(flags), int(refinement), int(self.splitting_step), int(self.max_split_iterations))
         st = Stats()
+        self.generation += 1
         self._check(lib().mdmc_extract(self._ctx, ctypes.byref(g), ctypes.c_float(iso), ctypes.byref(o), ctypes.byref(st)))
         self.last_stats = st
         return st
+
+    def backward(self, dpos_ptr, dfield_ptr, accumulate=False):
+        """mdmc_backward: device dL/d(vertex position) [V,3] -> device dL/d(field) of the
+        last extraction, which must have used RECORD_ADJOINT."""
+        self._check(lib().mdmc_backward(self._ctx, ctypes.c_void_p(dpos_ptr), ctypes.c_void_p(dfield_ptr), 1 if accumulate else 0))
+
+    def copy_device(self, positions=0, normals=0, halfedges=0, face_halfedge=0):
+        """mdmc_mesh_copy into device buffers given by pointer (0 = skip)."""
+        p = lambda a: ctypes.c_void_p(a) if a else None
+        self._check(lib().mdmc_mesh_copy(self._ctx, p(positions), p(normals), p(halfedges), p(face_halfedge), None, None, None, 1))
 
     def info(self):
         i = _Info()

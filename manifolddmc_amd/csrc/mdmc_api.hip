@@ -12,7 +12,18 @@
 #include <string>
 
 #include "../../include/mdmc.h"
+#include "mdmc_adjoint.h"
 #include "mdmc_pipeline.h"
+
+// what mdmc_backward needs from the last mdmc_extract
+struct AdjointState {
+    bool valid = false;
+    mdmc::GridDev g{};  // the processed grid (arena buffer "grid")
+    float iso = 0.f;
+    int inDims[3] = {0, 0, 0}, outDims[3] = {0, 0, 0};
+    int refinement = 0, pad = 0;
+    int64_t V = 0;
+};
 
 struct mdmc_ctx {
     int device = 0;
@@ -21,6 +32,7 @@ struct mdmc_ctx {
     std::string err;
     bool hasResult = false;
     hipEvent_t ev[9] = {};
+    AdjointState adj;
 };
 
 namespace {
@@ -117,6 +129,11 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
     int od[3];
     for (int i = 0; i < 3; ++i) od[i] = grid->dims[i] + (grid->dims[i] - 1) * r + 2 * pad;
     if (od[0] < 2 || od[1] < 2 || od[2] < 2) return fail(ctx, MDMC_ERR_INVALID_ARG, "processed grid needs at least 2 points per axis");
+    const bool record = (flags & MDMC_RECORD_ADJOINT) != 0;
+    if (record && (flags & (MDMC_SURF_ITER | MDMC_P3X3Y_COLOR | MDMC_P3X3Y_OLD | MDMC_P3333 | MDMC_SPLIT_NM)))
+        return fail(ctx, MDMC_ERR_INVALID_ARG,
+                    "MDMC_RECORD_ADJOINT needs MDMC_SURF_ITER and the vertex-changing post-processes (P3X3Y, P3333, SPLIT_NM) off");
+    ctx->adj.valid = false;
     const int64_t P = (int64_t)od[0] * od[1] * od[2];
     if ((int64_t)3 * P >= ((int64_t)1 << 40)) return fail(ctx, MDMC_ERR_CAPACITY, "grid too large");
     DeviceGuard dg(ctx->device);
@@ -156,7 +173,7 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
         }
         MDMC_HIP(hipEventRecord(ctx->ev[1], s));
         // ---- DMC
-        p.dmc(g, iso, (flags & MDMC_SURF_ITER) != 0);
+        p.dmc(g, iso, (flags & MDMC_SURF_ITER) != 0, record);
         MDMC_HIP(hipEventRecord(ctx->ev[2], s));
         int64_t st9[9] = {0};
         if (p.V > 0) {
@@ -182,6 +199,19 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
         MDMC_HIP(hipEventRecord(ctx->ev[8], s));
         MDMC_HIP(hipEventSynchronize(ctx->ev[8]));
         ctx->hasResult = true;
+        if (record) {
+            AdjointState& a = ctx->adj;
+            a.valid = true;
+            a.g = g;
+            a.iso = iso;
+            for (int i = 0; i < 3; ++i) {
+                a.inDims[i] = grid->dims[i];
+                a.outDims[i] = od[i];
+            }
+            a.refinement = r;
+            a.pad = pad;
+            a.V = p.dmcV;
+        }
         if (stats) {
             std::memset(stats, 0, sizeof *stats);
             stats->n_vertices = p.V;
@@ -208,6 +238,37 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
             }
             p.kernelTimes(&stats->ms_grid_pass_kernel);
         }
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        (void)hipStreamSynchronize(p.stream);
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_backward(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield, int32_t accumulate) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult || !ctx->adj.valid)
+        return fail(ctx, MDMC_ERR_NO_RESULT, "mdmc_backward needs a preceding mdmc_extract with MDMC_RECORD_ADJOINT");
+    const AdjointState& a = ctx->adj;
+    if (a.V > 0 && !dL_dpos) return fail(ctx, MDMC_ERR_INVALID_ARG, "null dL_dpos");
+    if (!dL_dfield) return fail(ctx, MDMC_ERR_INVALID_ARG, "null dL_dfield");
+    DeviceGuard dg(ctx->device);
+    mdmc::Pipeline& p = ctx->p;
+    try {
+        hipStream_t s = p.stream;
+        const int32_t* map = p.arena.ptr<int32_t>("adj_map");
+        const int32_t* cell = p.arena.ptr<int32_t>("adj_cell");
+        float* contrib = p.arena.get<float>("adj_contrib", 8 * (size_t)std::max<int64_t>(a.V, 1));
+        mdmc::launchAdjointVertex(a.g, a.iso, map, cell, a.V, dL_dpos, contrib, s);
+        if (a.pad == 0 && a.refinement == 0) {
+            mdmc::launchAdjointGather(a.g, map, contrib, dL_dfield, accumulate ? 1 : 0, s);
+        } else {
+            const int64_t P = (int64_t)a.outDims[0] * a.outDims[1] * a.outDims[2];
+            float* dproc = p.arena.get<float>("adj_dproc", (size_t)P);
+            mdmc::launchAdjointGather(a.g, map, contrib, dproc, 0, s);
+            mdmc::launchPrepareAdjoint(a.inDims, a.outDims, a.refinement, a.pad, dproc, dL_dfield, accumulate ? 1 : 0, s);
+        }
+        MDMC_HIP(hipGetLastError());
         return MDMC_OK;
     } catch (const std::exception& e) {
         (void)hipStreamSynchronize(p.stream);

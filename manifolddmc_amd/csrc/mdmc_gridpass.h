@@ -30,6 +30,10 @@ struct GridPassArgs {
     uint32_t* qattr;
     int64_t capQ;
     int* err;
+    // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1,
+    // per vertex its cell
+    int32_t* adjMap;
+    int32_t* adjCell;
 };
 
 int gridPassTile();

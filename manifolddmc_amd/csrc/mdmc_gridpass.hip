@@ -394,6 +394,22 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
         const uint64_t ent = (ringTag(c, a.ringShift) << 55) | ((uint64_t)contourMap(s_poly[ai]) << 31) | (uint64_t)((voff + s_vpre[ai]) & 0x7FFFFFFF);
         storeRelaxed(a.ring + (c & (a.R - 1)), ent);
     }
+    // for the adjoint: cell -> (first vertex << 2 | contours - 1), -1 for empty cells
+    if (a.adjMap) {
+#pragma unroll
+        for (int k = 0; k < GP_CPT; ++k) {
+            const int l = k * GP_THREADS + tid;
+            const int64_t c = c0 + l;
+            if (c < C) {
+                int val = -1;
+                if ((s_mask[k * GP_WAVES + wid] >> lane) & 1ULL) {
+                    const int ai = s_aidx[l];
+                    val = (int)(((voff + s_vpre[ai]) << 2) | (int64_t)(polyCount(s_poly[ai]) - 1));
+                }
+                a.adjMap[c] = val;
+            }
+        }
+    }
 
     // ---- 5. dual vertices, one per thread
     for (int i = tid; i < VT; i += GP_THREADS) {
@@ -413,6 +429,7 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
             a.nrm[3 * v + 0] = nn.x;
             a.nrm[3 * v + 1] = nn.y;
             a.nrm[3 * v + 2] = nn.z;
+            if (a.adjCell) a.adjCell[v] = (int)(c0 + s_cell[ai]);
         } else {
             atomicOr(a.err, GP_ERR_OVERFLOW);
         }

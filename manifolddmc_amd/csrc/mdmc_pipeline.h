@@ -35,7 +35,9 @@ class Pipeline {
     void resetCounters();
     int64_t anomalies();
 
-    void dmc(const GridDev& g, float iso, bool surfIter);
+    // the grid pass; recordAdjoint keeps the cell -> vertex map and vertex -> cell
+    // array ("adj_map", "adj_cell") that the adjoint reads
+    void dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoint = false);
     void halfedgeInitialize();
     void simplify3X3Y(bool colour);
     void simplify3333();

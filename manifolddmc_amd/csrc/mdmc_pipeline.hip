@@ -1341,7 +1341,7 @@ void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t))
 int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
 
 // ---- grid pass
-void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
+void Pipeline::dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoint) {
     const int CX = g.nx - 1, CY = g.ny - 1, CZ = g.nz - 1;
     const int64_t C = (int64_t)CX * CY * CZ;
     const int T = gridPassTile();
@@ -1377,6 +1377,8 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
         a.qattr = arena.get<uint32_t>("qattr", (size_t)capQ);
         a.capV = capV;
         a.capQ = capQ;
+        a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)C) : nullptr;
+        a.adjCell = recordAdjoint ? arena.get<int32_t>("adj_cell", (size_t)capV) : nullptr;
         zero(flags, flagBytes);
         zero(a.ring, (size_t)a.R * sizeof(unsigned long long));
         MDMC_HIP(hipEventRecord(kev[0], stream));
@@ -1400,6 +1402,8 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter) {
     MDMC_HIP(hipEventRecord(kev[2], stream));
     MDMC_HIP(hipEventRecord(kev[3], stream));
     if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
+    if (recordAdjoint && (V >= ((int64_t)1 << 29) || C >= (int64_t)INT32_MAX))
+        throw HipError("adjoint recording needs fewer than 2^29 vertices and 2^31 cells", -3);
     dmcV = V;
     dmcQ = Q;
     lastGridV = V;
