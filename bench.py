@@ -29,6 +29,7 @@ def parse():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--cpu-planes", type=int, default=20, help="z-planes of the field in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     return ap.parse_args()
 
 
@@ -65,6 +66,64 @@ def cpu_baseline(n, planes, flags):
     return {"value": cells / dt / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
             "sample": "z-planes 0..%d of the same %d^3 gyroid (%dx%dx%d, padded to %d cells), same flags, %.1f s, %d faces"
                       % (planes - 1, n, n, n, planes, cells, dt, len(m.face_he))}
+
+
+def adjoint_bench(steps=100):
+    """Differentiable form: 32-sphere SDF 256^3, iso 0, pad-by-1, dual vertices as edge means;
+    loss = mean squared distance of the vertices to the sphere of radius 0.3 at the centre.
+    Times the forward (mdmc_extract recording the topology + vertex copy) and forward +
+    loss + backward (mdmc_backward), then runs `steps` descent steps with re-extraction."""
+    import torch
+
+    from manifolddmc_amd import grids
+    from manifolddmc_amd.autograd import dual_marching_cubes
+
+    field_np, origin, delta, iso = grids.workload("spheres256")
+    field = torch.from_numpy(field_np).cuda()
+    centre = torch.tensor([0.5, 0.5, 0.5], device="cuda")
+
+    def loss_of(f):
+        pos, _ = dual_marching_cubes(f, iso, origin, delta)
+        r = torch.linalg.norm(pos - centre, dim=1)
+        return ((r - 0.3) ** 2).mean()
+
+    def fwd():
+        with torch.no_grad():
+            return loss_of(field)
+
+    def fwd_bwd():
+        f = field.detach().requires_grad_(True)
+        l = loss_of(f)
+        l.backward()
+        return f.grad
+
+    for _ in range(3):
+        fwd()
+        fwd_bwd()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) * 1e3 / reps
+    f = field.detach().clone()
+    losses = []
+    for _ in range(steps):
+        f.requires_grad_(True)
+        l = loss_of(f)
+        l.backward()
+        losses.append(float(l.detach()))
+        with torch.no_grad():  # move field values by at most a quarter cell
+            f = f - (0.25 * delta[0]) * f.grad / f.grad.abs().max()
+    return {"workload": "32-sphere SDF 256^3, iso 0, pad-by-1, no move-to-surface; loss = mean (|p - c| - 0.3)^2",
+            "fwd_ms": round(t_f, 3), "fwd_bwd_ms": round(t_fb, 3), "ratio": round(t_fb / t_f, 3),
+            "descent_steps": steps, "loss_first": losses[0], "loss_last": losses[-1]}
 
 
 def main():
@@ -131,11 +190,14 @@ def main():
                      "algorithmic_bytes": alg_bytes, "kernel_ms": round(gen_ms, 4)},
         "stages_ms": {k: round(float(np.mean([x[k] for x in stats])), 3) for k in stage_keys},
     }
+    d.close()
+    if not args.no_adjoint:
+        print("adjoint", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         print("cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_planes, flags)
     print(json.dumps(out), flush=True)
-    d.close()
 
 
 if __name__ == "__main__":
