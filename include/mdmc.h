@@ -126,6 +126,24 @@ mdmc_status mdmc_mesh_copy(const mdmc_ctx* ctx, float* positions, float* normals
                            int32_t* face_halfedge, uint16_t* face_attr, uint8_t* vertex_flags,
                            int32_t* vertex_halfedge, int32_t dst_on_device);
 
+/* Batched form (no reference counterpart: the reference extracts one grid per call).
+ * n_grids grids, each at isos_per_grid iso-values (isos[g * isos_per_grid + k]), form
+ * n_grids * isos_per_grid problems, problem b = g * isos_per_grid + k, all with the same
+ * options.  Every problem gets exactly the mesh mdmc_extract gives it; the meshes are
+ * resident CSR-packed: problem b owns vertices [vertex_offsets[b], vertex_offsets[b+1]),
+ * halfedges and faces likewise, and the ids in halfedges / face_halfedge are indices into
+ * the concatenation (subtract the problem's offsets for its own ids).  vertex_halfedge
+ * holds each problem's own values.  All problems run through one grid-pass launch and
+ * one pipeline.  stats (may be NULL) holds totals.  Synchronises the stream. */
+mdmc_status mdmc_extract_batch(mdmc_ctx* ctx, const mdmc_grid* grids, int32_t n_grids, const float* isos, int32_t isos_per_grid,
+                               const mdmc_options* opt, mdmc_stats* stats);
+/* Per-problem offsets of the resident result (host arrays of n_problems + 1; any may be NULL).
+ * After mdmc_extract (one problem) they are {0, n}. */
+mdmc_status mdmc_batch_offsets(const mdmc_ctx* ctx, int64_t* vertex_offsets, int64_t* halfedge_offsets, int64_t* face_offsets);
+/* Per-problem counts of the last mdmc_extract_batch (host array of n_problems; times are 0,
+ * anomalies and cells -1). */
+mdmc_status mdmc_batch_stats(const mdmc_ctx* ctx, mdmc_stats* per_problem);
+
 /* Adjoint of the last mdmc_extract (no reference counterpart; the reference has no backward
  * pass).  That extract must have had MDMC_RECORD_ADJOINT.  With the topology frozen, maps
  * dL_dpos (device float[3 * n_vertices], d loss / d vertex position, the mesh's vertex order)

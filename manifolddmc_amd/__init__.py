@@ -85,7 +85,8 @@ class _Info(ctypes.Structure):
 _lib = None
 
 EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", "mdmc_extract",
-           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_backward"]
+           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_backward",
+           "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats"]
 
 
 def lib():
@@ -115,6 +116,12 @@ def lib():
         L.mdmc_mesh_triangles.restype = i32
         L.mdmc_backward.argtypes = [vp, vp, vp, i32]
         L.mdmc_backward.restype = i32
+        L.mdmc_extract_batch.argtypes = [vp, ctypes.POINTER(_Grid), i32, vp, i32, ctypes.POINTER(_Options), ctypes.POINTER(Stats)]
+        L.mdmc_extract_batch.restype = i32
+        L.mdmc_batch_offsets.argtypes = [vp, vp, vp, vp]
+        L.mdmc_batch_offsets.restype = i32
+        L.mdmc_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.mdmc_batch_stats.restype = i32
         _lib = L
     return _lib
 
@@ -226,7 +233,64 @@ class DualMarchingCubes:
         self.generation += 1
         self._check(lib().mdmc_extract(self._ctx, ctypes.byref(g), ctypes.c_float(iso), ctypes.byref(o), ctypes.byref(st)))
         self.last_stats = st
+        self.n_problems = 1
         return st
+
+    def extract_batch_raw(self, grids, isos_per_grid, flags, refinement=0):
+        """mdmc_extract_batch.  grids: list of (data_ptr, dims (nx, ny, nz), origin, delta, on_device);
+        isos_per_grid: list (one per grid) of equally long iso-value lists.  Returns total Stats."""
+        G = len(grids)
+        I = len(isos_per_grid[0])
+        arr = (_Grid * G)()
+        for i, (ptr, dims, origin, delta, on_device) in enumerate(grids):
+            arr[i].dims[:] = [int(d) for d in dims]
+            arr[i].origin[:] = [float(np.float32(o)) for o in origin]
+            arr[i].delta[:] = [float(np.float32(d)) for d in delta]
+            arr[i].data = ctypes.c_void_p(ptr)
+            arr[i].data_on_device = 1 if on_device else 0
+        isos = np.ascontiguousarray(np.asarray(isos_per_grid, np.float32).reshape(G * I))
+        o = _Options(int(flags), int(refinement), int(self.splitting_step), int(self.max_split_iterations))
+        st = Stats()
+        self.generation += 1
+        self._check(lib().mdmc_extract_batch(self._ctx, arr, G, isos.ctypes.data_as(ctypes.c_void_p), I, ctypes.byref(o), ctypes.byref(st)))
+        self.last_stats = st
+        self.n_problems = G * I
+        return st
+
+    def extract_batch(self, fields, isos_per_grid, origins, deltas, flags, refinement=0):
+        """Host float32 fields [z][y][x]; returns the total Stats (meshes stay resident)."""
+        keep = [np.ascontiguousarray(f, np.float32) for f in fields]
+        grids = [(f.ctypes.data, (f.shape[2], f.shape[1], f.shape[0]), o, d, False) for f, o, d in zip(keep, origins, deltas)]
+        return self.extract_batch_raw(grids, isos_per_grid, flags, refinement)
+
+    def batch_offsets(self):
+        """(vertex, halfedge, face) offsets, each n_problems + 1 long."""
+        n = getattr(self, "n_problems", 1) + 1
+        v, h, f = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self._check(lib().mdmc_batch_offsets(self._ctx, p(v), p(h), p(f)))
+        return v, h, f
+
+    def batch_stats(self):
+        arr = (Stats * self.n_problems)()
+        self._check(lib().mdmc_batch_stats(self._ctx, arr))
+        return [arr[i] for i in range(self.n_problems)]
+
+    def batch_meshes(self):
+        """The resident batch split into per-problem HalfedgeMesh objects with their own ids."""
+        m = self.mesh_numpy()
+        vo, ho, fo = self.batch_offsets()
+        out = []
+        for b in range(len(vo) - 1):
+            he = m.halfedges[ho[b]:ho[b + 1]].copy()
+            if len(he):
+                he[:, 0] -= vo[b]
+                he[:, 1] -= fo[b]
+                he[:, 2] -= ho[b]
+                he[:, 3] = np.where(he[:, 3] >= 0, he[:, 3] - ho[b], he[:, 3])
+            out.append(HalfedgeMesh(m.positions[vo[b]:vo[b + 1]], m.normals[vo[b]:vo[b + 1]], he, m.face_halfedge[fo[b]:fo[b + 1]] - ho[b],
+                                    m.face_attr[fo[b]:fo[b + 1]], m.vertex_flags[vo[b]:vo[b + 1]], m.vertex_halfedge[vo[b]:vo[b + 1]]))
+        return out
 
     def backward(self, dpos_ptr, dfield_ptr, accumulate=False):
         """mdmc_backward: device dL/d(vertex position) [V,3] -> device dL/d(field) of the

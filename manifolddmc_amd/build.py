@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdmc.so")
-SOURCES = ["mdmc_scan.hip", "mdmc_dmc.hip", "mdmc_gridpass.hip", "mdmc_pipeline.hip", "mdmc_adjoint.hip", "mdmc_api.hip"]
+SOURCES = ["mdmc_scan.hip", "mdmc_dmc.hip", "mdmc_gridpass.hip", "mdmc_pipeline.hip", "mdmc_adjoint.hip", "mdmc_batch.hip", "mdmc_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MDMC_ARCH", "gfx950")

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/mdmc.h"
 #include "mdmc_adjoint.h"
@@ -33,6 +34,8 @@ struct mdmc_ctx {
     bool hasResult = false;
     hipEvent_t ev[9] = {};
     AdjointState adj;
+    int nProblems = 0;  // problems of the resident result
+    int flags = 0;
 };
 
 namespace {
@@ -116,64 +119,105 @@ mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* s) {
     return MDMC_OK;
 }
 
-mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const mdmc_options* opt, mdmc_stats* stats) {
-    if (!ctx) return MDMC_ERR_INVALID_ARG;
+}  // extern "C"
+
+namespace {
+
+// processed-grid size of one input grid
+void processedDims(const mdmc_grid* g, int r, int pad, int od[3]) {
+    for (int i = 0; i < 3; ++i) od[i] = g->dims[i] + (g->dims[i] - 1) * r + 2 * pad;
+}
+
+// mdmc_extract (G = I = 1) and mdmc_extract_batch: pad / refine every grid, then one
+// pipeline over all (grid, iso) problems
+mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const float* isos, int I, const mdmc_options* opt, mdmc_stats* stats) {
     ctx->hasResult = false;
-    if (!grid || !grid->data || !opt) return fail(ctx, MDMC_ERR_INVALID_ARG, "null grid, data or options");
-    for (int i = 0; i < 3; ++i)
-        if (grid->dims[i] < 1) return fail(ctx, MDMC_ERR_INVALID_ARG, "grid dims must be >= 1");
+    ctx->adj.valid = false;
+    ctx->nProblems = 0;
+    if (!grids || !isos || !opt || G < 1 || I < 1) return fail(ctx, MDMC_ERR_INVALID_ARG, "null grids, iso-values or options, or an empty batch");
+    if ((int64_t)G * I > (int64_t)1 << 20) return fail(ctx, MDMC_ERR_INVALID_ARG, "more than 2^20 problems");
     if (opt->refinement < 0 || opt->refinement > 64) return fail(ctx, MDMC_ERR_INVALID_ARG, "refinement must be in [0, 64]");
     const int flags = opt->flags;
     const int pad = (flags & MDMC_PAD1) ? 1 : 0;
     const int r = opt->refinement;
-    int od[3];
-    for (int i = 0; i < 3; ++i) od[i] = grid->dims[i] + (grid->dims[i] - 1) * r + 2 * pad;
-    if (od[0] < 2 || od[1] < 2 || od[2] < 2) return fail(ctx, MDMC_ERR_INVALID_ARG, "processed grid needs at least 2 points per axis");
     const bool record = (flags & MDMC_RECORD_ADJOINT) != 0;
     if (record && (flags & (MDMC_SURF_ITER | MDMC_P3X3Y_COLOR | MDMC_P3X3Y_OLD | MDMC_P3333 | MDMC_SPLIT_NM)))
         return fail(ctx, MDMC_ERR_INVALID_ARG,
                     "MDMC_RECORD_ADJOINT needs MDMC_SURF_ITER and the vertex-changing post-processes (P3X3Y, P3333, SPLIT_NM) off");
-    ctx->adj.valid = false;
-    const int64_t P = (int64_t)od[0] * od[1] * od[2];
-    if ((int64_t)3 * P >= ((int64_t)1 << 40)) return fail(ctx, MDMC_ERR_CAPACITY, "grid too large");
+    if (record && G * I != 1) return fail(ctx, MDMC_ERR_INVALID_ARG, "MDMC_RECORD_ADJOINT is for single extractions");
+    int64_t Ptot = 0, PinTot = 0, cells = 0;
+    std::vector<int64_t> pOff((size_t)G + 1, 0), inOff((size_t)G + 1, 0);
+    for (int gi = 0; gi < G; ++gi) {
+        const mdmc_grid* grid = &grids[gi];
+        if (!grid->data) return fail(ctx, MDMC_ERR_INVALID_ARG, "null grid data");
+        for (int i = 0; i < 3; ++i)
+            if (grid->dims[i] < 1) return fail(ctx, MDMC_ERR_INVALID_ARG, "grid dims must be >= 1");
+        int od[3];
+        processedDims(grid, r, pad, od);
+        if (od[0] < 2 || od[1] < 2 || od[2] < 2) return fail(ctx, MDMC_ERR_INVALID_ARG, "processed grid needs at least 2 points per axis");
+        const int64_t P = (int64_t)od[0] * od[1] * od[2];
+        Ptot += P;
+        if ((int64_t)3 * Ptot >= ((int64_t)1 << 40)) return fail(ctx, MDMC_ERR_CAPACITY, "grid too large");
+        pOff[gi + 1] = Ptot;
+        PinTot += grid->data_on_device ? 0 : (int64_t)grid->dims[0] * grid->dims[1] * grid->dims[2];
+        inOff[gi + 1] = PinTot;
+        cells += (int64_t)I * (od[0] - 1) * (od[1] - 1) * (od[2] - 1);
+    }
     DeviceGuard dg(ctx->device);
     mdmc::Pipeline& p = ctx->p;
+    const int B = G * I;
     try {
         hipStream_t s = p.stream;
         MDMC_HIP(hipEventRecord(ctx->ev[0], s));
         p.resetCounters();
         p.splitIterations = 0;
         p.extraAnomalies = 0;
-        // ---- grid: upload if needed, then pad / refine (refineGrid)
-        const int64_t Pin = (int64_t)grid->dims[0] * grid->dims[1] * grid->dims[2];
-        const float* in = grid->data;
-        if (!grid->data_on_device) {
-            float* d = p.arena.get<float>("input", (size_t)Pin);
-            MDMC_HIP(hipMemcpyAsync(d, grid->data, (size_t)Pin * sizeof(float), hipMemcpyHostToDevice, s));
-            in = d;
-        }
-        float* gp = p.arena.get<float>("grid", (size_t)P);
-        mdmc::launchPrepareGrid(in, grid->dims, gp, od, r, pad, s);
-        mdmc::GridDev g;
-        g.v = gp;
-        g.nx = od[0];
-        g.ny = od[1];
-        g.nz = od[2];
-        const float f = 1.f / (static_cast<float>(r) + 1.f);
-        g.dx = grid->delta[0] * f;
-        g.dy = grid->delta[1] * f;
-        g.dz = grid->delta[2] * f;
-        g.ox = grid->origin[0];
-        g.oy = grid->origin[1];
-        g.oz = grid->origin[2];
-        if (pad) {
-            g.ox -= g.dx;
-            g.oy -= g.dy;
-            g.oz -= g.dz;
+        // ---- grids: upload if needed, then pad / refine (refineGrid)
+        float* gp = p.arena.get<float>("grid", (size_t)Ptot);
+        float* up = PinTot ? p.arena.get<float>("input", (size_t)PinTot) : nullptr;
+        std::vector<mdmc::ProblemDev> problems;
+        problems.reserve((size_t)B);
+        mdmc::GridDev g0{};
+        for (int gi = 0; gi < G; ++gi) {
+            const mdmc_grid* grid = &grids[gi];
+            const float* in = grid->data;
+            if (!grid->data_on_device) {
+                const int64_t Pin = (int64_t)grid->dims[0] * grid->dims[1] * grid->dims[2];
+                MDMC_HIP(hipMemcpyAsync(up + inOff[gi], grid->data, (size_t)Pin * sizeof(float), hipMemcpyHostToDevice, s));
+                in = up + inOff[gi];
+            }
+            int od[3];
+            processedDims(grid, r, pad, od);
+            mdmc::launchPrepareGrid(in, grid->dims, gp + pOff[gi], od, r, pad, s);
+            mdmc::GridDev g;
+            g.v = gp + pOff[gi];
+            g.nx = od[0];
+            g.ny = od[1];
+            g.nz = od[2];
+            const float f = 1.f / (static_cast<float>(r) + 1.f);
+            g.dx = grid->delta[0] * f;
+            g.dy = grid->delta[1] * f;
+            g.dz = grid->delta[2] * f;
+            g.ox = grid->origin[0];
+            g.oy = grid->origin[1];
+            g.oz = grid->origin[2];
+            if (pad) {
+                g.ox -= g.dx;
+                g.oy -= g.dy;
+                g.oz -= g.dz;
+            }
+            if (gi == 0) g0 = g;
+            for (int k = 0; k < I; ++k) {
+                mdmc::ProblemDev pr{};
+                pr.g = g;
+                pr.iso = isos[(int64_t)gi * I + k];
+                problems.push_back(pr);
+            }
         }
         MDMC_HIP(hipEventRecord(ctx->ev[1], s));
         // ---- DMC
-        p.dmc(g, iso, (flags & MDMC_SURF_ITER) != 0, record);
+        p.dmc(problems, (flags & MDMC_SURF_ITER) != 0, record);
+        if (B > 1) p.problemDmcCounts(B);
         MDMC_HIP(hipEventRecord(ctx->ev[2], s));
         int64_t st9[9] = {0};
         if (p.V > 0) {
@@ -184,28 +228,39 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
             if (flags & MDMC_P3X3Y_OLD) p.simplify3X3Y(false);
             if (flags & MDMC_P3333) p.simplify3333();
             MDMC_HIP(hipEventRecord(ctx->ev[4], s));
-            p.fixManifoldTwins(g, iso);
+            p.fixManifoldTwins();
             MDMC_HIP(hipEventRecord(ctx->ev[5], s));
-            if (flags & MDMC_SPLIT_NM) p.split(g, iso, opt->split_step, opt->max_split_iters > 0 ? opt->max_split_iters : 64);
+            if (flags & MDMC_SPLIT_NM) p.split(opt->split_step, opt->max_split_iters > 0 ? opt->max_split_iters : 64);
             MDMC_HIP(hipEventRecord(ctx->ev[6], s));
             if (flags & MDMC_DETECT_NM_VERTS) p.detectNonManifoldVertices();
             MDMC_HIP(hipEventRecord(ctx->ev[7], s));
             p.computeStats(st9);
+            if (B > 1) p.regroupBatch(B);
         } else {
             // DualMarchingCubes::run returns early when no vertex was produced (DualMarchingCubes.cu:257-261)
             p.Q = p.H = p.F = 0;
+            if (B > 1) {
+                p.batchVOff.assign((size_t)B + 1, 0);
+                p.batchHOff.assign((size_t)B + 1, 0);
+                p.batchFOff.assign((size_t)B + 1, 0);
+                p.batchStats.assign(9 * (size_t)B, 0);
+            }
             for (int i = 3; i < 8; ++i) MDMC_HIP(hipEventRecord(ctx->ev[i], s));
         }
         MDMC_HIP(hipEventRecord(ctx->ev[8], s));
         MDMC_HIP(hipEventSynchronize(ctx->ev[8]));
         ctx->hasResult = true;
+        ctx->nProblems = B;
+        ctx->flags = flags;
         if (record) {
             AdjointState& a = ctx->adj;
             a.valid = true;
-            a.g = g;
-            a.iso = iso;
+            a.g = g0;
+            a.iso = isos[0];
+            int od[3];
+            processedDims(&grids[0], r, pad, od);
             for (int i = 0; i < 3; ++i) {
-                a.inDims[i] = grid->dims[i];
+                a.inDims[i] = grids[0].dims[i];
                 a.outDims[i] = od[i];
             }
             a.refinement = r;
@@ -226,7 +281,7 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
             stats->nm_vertices = (flags & MDMC_DETECT_NM_VERTS) ? st9[8] : -1;
             stats->split_iterations = p.splitIterations;
             stats->anomalies = p.anomalies() + p.extraAnomalies;
-            stats->cells = (int64_t)(od[0] - 1) * (od[1] - 1) * (od[2] - 1);
+            stats->cells = cells;
             float ms = 0.f;
             double* outs[8] = {&stats->ms_total, &stats->ms_grid, &stats->ms_dmc, &stats->ms_halfedge,
                                &stats->ms_simplify, &stats->ms_fix_twins, &stats->ms_split, &stats->ms_detect};
@@ -243,6 +298,67 @@ mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const 
         (void)hipStreamSynchronize(p.stream);
         return fromException(ctx, e);
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+mdmc_status mdmc_extract(mdmc_ctx* ctx, const mdmc_grid* grid, float iso, const mdmc_options* opt, mdmc_stats* stats) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    return runExtraction(ctx, grid, 1, &iso, 1, opt, stats);
+}
+
+mdmc_status mdmc_extract_batch(mdmc_ctx* ctx, const mdmc_grid* grids, int32_t n_grids, const float* isos, int32_t isos_per_grid,
+                               const mdmc_options* opt, mdmc_stats* stats) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    return runExtraction(ctx, grids, n_grids, isos, isos_per_grid, opt, stats);
+}
+
+mdmc_status mdmc_batch_offsets(const mdmc_ctx* ctx, int64_t* vertex_offsets, int64_t* halfedge_offsets, int64_t* face_offsets) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return MDMC_ERR_NO_RESULT;
+    const mdmc::Pipeline& p = ctx->p;
+    const int B = ctx->nProblems;
+    for (int b = 0; b <= B; ++b) {
+        if (B == 1) {
+            if (vertex_offsets) vertex_offsets[b] = b ? p.V : 0;
+            if (halfedge_offsets) halfedge_offsets[b] = b ? p.H : 0;
+            if (face_offsets) face_offsets[b] = b ? p.F : 0;
+        } else {
+            if (vertex_offsets) vertex_offsets[b] = p.batchVOff[b];
+            if (halfedge_offsets) halfedge_offsets[b] = p.batchHOff[b];
+            if (face_offsets) face_offsets[b] = p.batchFOff[b];
+        }
+    }
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_batch_stats(const mdmc_ctx* ctx, mdmc_stats* per_problem) {
+    if (!ctx || !per_problem) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return MDMC_ERR_NO_RESULT;
+    const mdmc::Pipeline& p = ctx->p;
+    const int B = ctx->nProblems;
+    if (B == 1) return MDMC_ERR_INVALID_ARG;
+    for (int b = 0; b < B; ++b) {
+        mdmc_stats& st = per_problem[b];
+        std::memset(&st, 0, sizeof st);
+        st.n_vertices = p.batchVOff[b + 1] - p.batchVOff[b];
+        st.n_halfedges = p.batchHOff[b + 1] - p.batchHOff[b];
+        st.n_faces = p.batchFOff[b + 1] - p.batchFOff[b];
+        st.n_dmc_vertices = p.batchDmc[2 * b];
+        st.n_dmc_quads = p.batchDmc[2 * b + 1];
+        const int64_t* s9 = &p.batchStats[9 * (size_t)b];
+        st.nm_edges4 = s9[0];
+        st.nm_edges3 = s9[1];
+        st.n_edges = s9[2];
+        for (int k = 0; k < 5; ++k) st.quads_with_nm[k] = s9[3 + k];
+        st.nm_vertices = (ctx->flags & MDMC_DETECT_NM_VERTS) ? s9[8] : -1;
+        st.split_iterations = p.splitIterations;
+        st.anomalies = -1;
+        st.cells = -1;
+    }
+    return MDMC_OK;
 }
 
 mdmc_status mdmc_backward(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield, int32_t accumulate) {

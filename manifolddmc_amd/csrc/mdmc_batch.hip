@@ -1,0 +1,299 @@
+// mdmc_batch.hip -- the batched form: many (grid, iso) problems in one extraction.
+//
+// No reference counterpart (the reference extracts one grid per call).  The
+// problems' meshes are disjoint, so every stage of the pipeline runs once over
+// their union: the grid pass walks all problems' tiles in one launch, and the
+// halfedge, simplification, twin-repair, split and detection kernels work on
+// global ids as they do for one mesh.  Element ids stay grouped by problem
+// except where the splitter appends; at the end a stable sort by problem makes
+// every problem's vertices, halfedges and faces contiguous (CSR-packed), in the
+// order a single extraction of that problem produces.
+//
+// Stable sort: least-significant-bit-first binary splits of a permutation, each a
+// prefix sum of the "bit is 0" flags and a scatter (ceil(log2 B) passes).
+#include <algorithm>
+#include <vector>
+
+#include "mdmc_device.h"
+#include "mdmc_internal.h"
+#include "mdmc_pipeline.h"
+
+namespace mdmc {
+
+namespace {
+
+__device__ __forceinline__ int heO(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h]; }
+
+__global__ void kIota(int32_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+// problem of each halfedge (its origin's) and of each face (its first halfedge's origin's)
+__global__ void kHalfedgeKeys(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ pid, int32_t* __restrict__ key) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < H) key[h] = pid[heO(he, (int)h)];
+}
+__global__ void kFaceKeys2(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ pid,
+                           int32_t* __restrict__ key) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < F) key[f] = pid[heO(he, face_he[f])];
+}
+
+__global__ void kSplitFlags(const int32_t* __restrict__ perm, const int32_t* __restrict__ key, int64_t n, int bit, int32_t* __restrict__ zero) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) zero[i] = ((key[perm[i]] >> bit) & 1) ? 0 : 1;
+}
+__global__ void kSplitScatter(const int32_t* __restrict__ perm, const int32_t* __restrict__ zero, const int64_t* __restrict__ pre, int64_t n,
+                              const int64_t* __restrict__ zeros, int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = pre[i];
+    out[zero[i] ? p : *zeros + (i - p)] = perm[i];
+}
+
+__global__ void kInverse(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+// per-key counts; keys arrive sorted, so a wave usually holds one key and adds once
+__global__ void kCountSorted(const int32_t* __restrict__ key, const int32_t* __restrict__ perm, int64_t n, unsigned long long* __restrict__ count,
+                             int stride, int slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const int k = live ? key[perm ? perm[i] : i] : -1;
+    const int k0 = __shfl(k, 0, 64);
+    const uint64_t same = __ballot(live && k == k0);
+    const uint64_t alive = __ballot(live);
+    if (same == alive) {
+        if ((threadIdx.x & 63) == 0 && k0 >= 0) atomicAdd(&count[(int64_t)k0 * stride + slot], (unsigned long long)__popcll(alive));
+    } else if (live) {
+        atomicAdd(&count[(int64_t)k * stride + slot], 1ULL);
+    }
+}
+
+__global__ void kPermuteVertices(const int32_t* __restrict__ perm, int64_t V, const float* __restrict__ pos, const float* __restrict__ nrm,
+                                 const uint32_t* __restrict__ vflags, const int32_t* __restrict__ pid, float* __restrict__ npos,
+                                 float* __restrict__ nnrm, uint32_t* __restrict__ nvflags, int32_t* __restrict__ npid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int64_t o = perm[i];
+    for (int k = 0; k < 3; ++k) {
+        npos[3 * i + k] = pos[3 * o + k];
+        nnrm[3 * i + k] = nrm[3 * o + k];
+    }
+    nvflags[i] = vflags[o];
+    npid[i] = pid[o];
+}
+
+__global__ void kPermuteHalfedges(const int32_t* __restrict__ perm, int64_t H, const int4* __restrict__ he, const int32_t* __restrict__ bsize,
+                                  const int32_t* __restrict__ brank, const int32_t* __restrict__ vNew, const int32_t* __restrict__ hNew,
+                                  const int32_t* __restrict__ fNew, int4* __restrict__ nhe, int32_t* __restrict__ nbsize,
+                                  int32_t* __restrict__ nbrank) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H) return;
+    const int o = perm[i];
+    const int4 e = he[o];
+    nhe[i] = make_int4(vNew[e.x], fNew[e.y], hNew[e.z], e.w >= 0 ? hNew[e.w] : e.w);
+    nbsize[i] = bsize[o];
+    nbrank[i] = brank[o];
+}
+
+__global__ void kPermuteFaces(const int32_t* __restrict__ perm, int64_t F, const int32_t* __restrict__ face_he, const uint32_t* __restrict__ fattr,
+                              const int32_t* __restrict__ hNew, int32_t* __restrict__ nface_he, uint32_t* __restrict__ nfattr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F) return;
+    const int o = perm[i];
+    nface_he[i] = hNew[face_he[o]];
+    nfattr[i] = fattr[o];
+}
+
+// the vertex -> halfedge array of each problem as its own extraction leaves it: highest
+// initialize-time halfedge 4q+i starting at the vertex, q counted inside the problem;
+// -1 where none; 0 for vertices the splitter added
+__global__ void kVertexHalfedgeInitBatch(int32_t* __restrict__ vh, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) vh[i] = -1;
+}
+__global__ void kVertexHalfedgeMax(const int4* __restrict__ quads, int64_t Q, int32_t* __restrict__ vh) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v = quads[q];
+    const int b = (int)(4 * q);
+    atomicMax(&vh[v.x], b + 0);
+    atomicMax(&vh[v.y], b + 1);
+    atomicMax(&vh[v.z], b + 2);
+    atomicMax(&vh[v.w], b + 3);
+}
+__global__ void kVertexHalfedgeLocal(const int32_t* __restrict__ vhInit, int64_t Vinit, const int32_t* __restrict__ vNew, const int32_t* __restrict__ pidNew,
+                                     const int64_t* __restrict__ qBase, int64_t V, int32_t* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    if (v >= Vinit) {
+        out[vNew[v]] = 0;
+        return;
+    }
+    const int h = vhInit[v];
+    const int64_t nv = vNew[v];
+    out[nv] = h < 0 ? -1 : (int32_t)(h - 4 * qBase[pidNew[nv]]);
+}
+
+// bucket / face / vertex counters of HalfedgeMesh.cu:1164-1239 per problem (elements sorted by problem)
+__device__ __forceinline__ void addPer(unsigned long long* out, int p, int slot, unsigned v) {
+    if (v) atomicAdd(&out[(int64_t)p * 9 + slot], (unsigned long long)v);
+}
+__global__ void kStatsBatch(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
+                            const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags, const int32_t* __restrict__ pid, int64_t V,
+                            unsigned long long* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < H && brank[i] == 0) {
+        const int p = pid[heO(he, (int)i)];
+        const int n = bsize[i];
+        addPer(out, p, 0, n == 4);
+        addPer(out, p, 1, n == 3);
+        addPer(out, p, 2, n > 0);
+    }
+    if (i < F) {
+        int h = face_he[i];
+        const int p = pid[heO(he, h)];
+        int k = 0;
+        for (int j = 0; j < 4; ++j) {
+            if (bsize[h] > 2) ++k;
+            h = ((const int*)he)[4 * (int64_t)h + 2];
+        }
+        addPer(out, p, 3 + k, 1);
+    }
+    if (i < V) addPer(out, pid[i], 8, (vflags[i] & VF_FANS) ? 1u : 0u);
+}
+
+}  // namespace
+
+// stable permutation of [0, n) sorting key[] in [0, B)
+int32_t* Pipeline::stableSortByKey(const int32_t* key, int64_t n, int B, const std::string& tag) {
+    int32_t* a = arena.get<int32_t>(tag + "_perm", (size_t)std::max<int64_t>(n, 1));
+    int32_t* b = arena.get<int32_t>(tag + "_perm2", (size_t)std::max<int64_t>(n, 1));
+    int32_t* flag = arena.get<int32_t>("sort_flag", (size_t)std::max<int64_t>(n, 1));
+    int64_t* pre = arena.get<int64_t>("sort_pre", (size_t)n + 1);
+    int64_t* zeros = arena.get<int64_t>("sort_zeros", 1);
+    void* scratch = arena.get<char>("sort_scan", scanScratchBytes(n + 1));
+    if (n == 0) return a;
+    kIota<<<gridFor(n), kBlock, 0, stream>>>(a, n);
+    int bits = 0;
+    while ((1 << bits) < B) ++bits;
+    for (int bit = 0; bit < bits; ++bit) {
+        kSplitFlags<<<gridFor(n), kBlock, 0, stream>>>(a, key, n, bit, flag);
+        scanExclusiveI32(flag, pre, n, zeros, scratch, stream);
+        kSplitScatter<<<gridFor(n), kBlock, 0, stream>>>(a, flag, pre, n, zeros, b);
+        std::swap(a, b);
+    }
+    return a;
+}
+
+void Pipeline::problemDmcCounts(int B) {
+    batchDmc.assign(2 * (size_t)B, 0);
+    unsigned long long* cnt = arena.get<unsigned long long>("batch_dmc", 2 * (size_t)B);
+    zero(cnt, 2 * (size_t)B * sizeof(unsigned long long));
+    if (V > 0) kCountSorted<<<gridFor(V), kBlock, 0, stream>>>(pid(), nullptr, V, cnt, 2, 0);
+    if (Q > 0) {
+        int32_t* qk = arena.get<int32_t>("batch_qkey", (size_t)Q);
+        // a quad's problem is its first vertex's: reuse the halfedge-key kernel on the quads (x = first vertex)
+        kHalfedgeKeys<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, pid(), qk);
+        kCountSorted<<<gridFor(Q), kBlock, 0, stream>>>(qk, nullptr, Q, cnt, 2, 1);
+    }
+    MDMC_HIP(hipMemcpyAsync(batchDmc.data(), cnt, 2 * (size_t)B * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+void Pipeline::regroupBatch(int B) {
+    int32_t* vp = pid();
+    int4* he = arena.ptr<int4>("he");
+    int32_t* face_he = arena.ptr<int32_t>("face_he");
+    // quads per problem at the last initialize (for the vertex -> halfedge values)
+    unsigned long long* cnt = arena.get<unsigned long long>("batch_cnt", 4 * (size_t)B);
+    zero(cnt, 4 * (size_t)B * sizeof(unsigned long long));
+    if (vertHeQ > 0) {
+        int32_t* qk = arena.get<int32_t>("batch_qkey", (size_t)vertHeQ);
+        kHalfedgeKeys<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vp, qk);
+        kCountSorted<<<gridFor(vertHeQ), kBlock, 0, stream>>>(qk, nullptr, vertHeQ, cnt, 4, 3);
+    }
+    int32_t* vhInit = arena.get<int32_t>("batch_vh_init", (size_t)std::max<int64_t>(vertHeV, 1));
+    if (vertHeV > 0) kVertexHalfedgeInitBatch<<<gridFor(vertHeV), kBlock, 0, stream>>>(vhInit, vertHeV);
+    if (vertHeQ > 0) kVertexHalfedgeMax<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vhInit);
+    // sort keys and permutations
+    int32_t* hk = arena.get<int32_t>("batch_hkey", (size_t)std::max<int64_t>(H, 1));
+    int32_t* fk = arena.get<int32_t>("batch_fkey", (size_t)std::max<int64_t>(F, 1));
+    if (H > 0) kHalfedgeKeys<<<gridFor(H), kBlock, 0, stream>>>(he, H, vp, hk);
+    if (F > 0) kFaceKeys2<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, vp, fk);
+    const int32_t* pv = stableSortByKey(vp, V, B, "sv");
+    const int32_t* ph = stableSortByKey(hk, H, B, "sh");
+    const int32_t* pf = stableSortByKey(fk, F, B, "sf");
+    if (V > 0) kCountSorted<<<gridFor(V), kBlock, 0, stream>>>(vp, pv, V, cnt, 4, 0);
+    if (H > 0) kCountSorted<<<gridFor(H), kBlock, 0, stream>>>(hk, ph, H, cnt, 4, 1);
+    if (F > 0) kCountSorted<<<gridFor(F), kBlock, 0, stream>>>(fk, pf, F, cnt, 4, 2);
+    std::vector<unsigned long long> c(4 * (size_t)B);
+    MDMC_HIP(hipMemcpyAsync(c.data(), cnt, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    batchVOff.assign((size_t)B + 1, 0);
+    batchHOff.assign((size_t)B + 1, 0);
+    batchFOff.assign((size_t)B + 1, 0);
+    std::vector<int64_t> qBase((size_t)B + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        batchVOff[b + 1] = batchVOff[b] + (int64_t)c[4 * b + 0];
+        batchHOff[b + 1] = batchHOff[b] + (int64_t)c[4 * b + 1];
+        batchFOff[b + 1] = batchFOff[b] + (int64_t)c[4 * b + 2];
+        qBase[b + 1] = qBase[b] + (int64_t)c[4 * b + 3];
+    }
+    int64_t* qBaseDev = arena.get<int64_t>("batch_qbase", (size_t)B + 1);
+    MDMC_HIP(hipMemcpyAsync(qBaseDev, qBase.data(), qBase.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    // inverse maps and the permuted arrays
+    int32_t* vNew = arena.get<int32_t>("batch_vnew", (size_t)std::max<int64_t>(V, 1));
+    int32_t* hNew = arena.get<int32_t>("batch_hnew", (size_t)std::max<int64_t>(H, 1));
+    int32_t* fNew = arena.get<int32_t>("batch_fnew", (size_t)std::max<int64_t>(F, 1));
+    if (V > 0) kInverse<<<gridFor(V), kBlock, 0, stream>>>(pv, V, vNew);
+    if (H > 0) kInverse<<<gridFor(H), kBlock, 0, stream>>>(ph, H, hNew);
+    if (F > 0) kInverse<<<gridFor(F), kBlock, 0, stream>>>(pf, F, fNew);
+    float* npos = arena.get<float>("pos2", 3 * (size_t)std::max<int64_t>(V, 1));
+    float* nnrm = arena.get<float>("nrm2", 3 * (size_t)std::max<int64_t>(V, 1));
+    uint32_t* nvf = arena.get<uint32_t>("vflags2", (size_t)std::max<int64_t>(V, 1));
+    int32_t* npid = arena.get<int32_t>("pid2", (size_t)std::max<int64_t>(V, 1));
+    int4* nhe = arena.get<int4>("he2", (size_t)std::max<int64_t>(H, 1));
+    int32_t* nbs = arena.get<int32_t>("bsize2", (size_t)std::max<int64_t>(H, 1));
+    int32_t* nbr = arena.get<int32_t>("brank2", (size_t)std::max<int64_t>(H, 1));
+    int32_t* nfh = arena.get<int32_t>("face_he2", (size_t)std::max<int64_t>(F, 1));
+    uint32_t* nfa = arena.get<uint32_t>("fattr2", (size_t)std::max<int64_t>(F, 1));
+    if (V > 0)
+        kPermuteVertices<<<gridFor(V), kBlock, 0, stream>>>(pv, V, arena.ptr<float>("pos"), arena.ptr<float>("nrm"), arena.ptr<uint32_t>("vflags"), vp, npos,
+                                                            nnrm, nvf, npid);
+    if (H > 0)
+        kPermuteHalfedges<<<gridFor(H), kBlock, 0, stream>>>(ph, H, he, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), vNew, hNew, fNew, nhe,
+                                                             nbs, nbr);
+    if (F > 0) kPermuteFaces<<<gridFor(F), kBlock, 0, stream>>>(pf, F, face_he, arena.ptr<uint32_t>("fattr"), hNew, nfh, nfa);
+    int32_t* vh = arena.get<int32_t>("batch_vert_he", (size_t)std::max<int64_t>(V, 1));
+    if (V > 0) kVertexHalfedgeLocal<<<gridFor(V), kBlock, 0, stream>>>(vhInit, vertHeV, vNew, npid, qBaseDev, V, vh);
+    auto back = [&](const char* name, const void* src, size_t bytes) {
+        if (bytes) MDMC_HIP(hipMemcpyAsync(arena.ptr<char>(name), src, bytes, hipMemcpyDeviceToDevice, stream));
+    };
+    back("pos", npos, 3 * (size_t)V * sizeof(float));
+    back("nrm", nnrm, 3 * (size_t)V * sizeof(float));
+    back("vflags", nvf, (size_t)V * sizeof(uint32_t));
+    back("pid", npid, (size_t)V * sizeof(int32_t));
+    back("he", nhe, (size_t)H * sizeof(int4));
+    back("bsize", nbs, (size_t)H * sizeof(int32_t));
+    back("brank", nbr, (size_t)H * sizeof(int32_t));
+    back("face_he", nfh, (size_t)F * sizeof(int32_t));
+    back("fattr", nfa, (size_t)F * sizeof(uint32_t));
+    batchVertHe = true;
+    // per-problem statistics
+    unsigned long long* st = arena.get<unsigned long long>("batch_stats", 9 * (size_t)B);
+    zero(st, 9 * (size_t)B * sizeof(unsigned long long));
+    const int64_t n = std::max(H, std::max(F, V));
+    if (n > 0)
+        kStatsBatch<<<gridFor(n), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int32_t>("face_he"), F, arena.ptr<int32_t>("bsize"),
+                                                       arena.ptr<int32_t>("brank"), arena.ptr<uint32_t>("vflags"), pid(), V, st);
+    batchStats.assign(9 * (size_t)B, 0);
+    MDMC_HIP(hipMemcpyAsync(batchStats.data(), st, batchStats.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace mdmc

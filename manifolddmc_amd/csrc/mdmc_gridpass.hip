@@ -75,6 +75,28 @@ __device__ __forceinline__ void waitFlag(const int* flag, int* err) {
 // holds fewer than 511 generations, so a tag never matches an older or missing entry.
 __device__ __forceinline__ uint64_t ringTag(int64_t c, int shift) { return (uint64_t)((c >> shift) + 1) & 0x1FFULL; }
 
+// the problem that holds global tile t, or global cell c (binary search over the bases)
+__device__ __forceinline__ int problemOfTile(const GridPassArgs& a, int64_t t) {
+    int lo = 0, hi = a.nProblems;  // probs[lo].tileBase <= t
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.probs[mid].tileBase <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ int64_t tileOfCell(const GridPassArgs& a, int64_t c) {
+    int lo = 0, hi = a.nProblems;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.probs[mid].cellBase <= c) lo = mid;
+        else hi = mid;
+    }
+    const ProblemDev& p = a.probs[lo];
+    const int64_t local = c - p.cellBase;
+    return p.tileBase + (local < p.cells ? local : p.cells - 1) / GP_TILE;
+}
+
 __device__ __forceinline__ int ringVertex(const GridPassArgs& a, int64_t n, int e) {
     const uint64_t tag = ringTag(n, a.ringShift);
     const unsigned long long* p = a.ring + (n & (a.R - 1));
@@ -203,20 +225,28 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
     __shared__ int s_mpre[GP_RUNS + 1];
     __shared__ int s_wave[GP_WAVES];
     __shared__ long long s_tile;
+    __shared__ int s_prob;
     __shared__ long long s_voff, s_qoff;
     uint8_t* const s_mc = reinterpret_cast<uint8_t*>(s_qpre);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) s_tile = (long long)atomicAdd(a.ticket, 1u);
+    if (tid == 0) {
+        const long long tk = (long long)atomicAdd(a.ticket, 1u);
+        s_tile = tk;
+        s_prob = tk < a.numTiles ? problemOfTile(a, tk) : 0;
+    }
     __syncthreads();
     const int64_t t = s_tile;
     if (t >= a.numTiles) return;  // uniform across the block
-    const GridDev& g = a.g;
-    const float iso = a.iso;
-    const int CX = a.CX, CY = a.CY;
-    const int64_t C = a.C;
-    const int64_t c0 = t * GP_TILE;
+    const int prob = s_prob;
+    const ProblemDev& P = a.probs[prob];
+    const GridDev g = P.g;
+    const float iso = P.iso;
+    const int CX = P.CX, CY = P.CY;
+    const int64_t C = P.cells;
+    const int64_t c0 = (t - P.tileBase) * GP_TILE;  // first cell of the tile inside its problem
+    const int64_t gc0 = P.cellBase + c0;             // ... and in the concatenation
 
     TileBase tb;
     {
@@ -374,13 +404,15 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
             }
         }
     }
-    // ring reuse: the slots this tile overwrites held cells [c0 - R, c0 + T - R); every tile
-    // that reads them (cells up to CX*CY + CX further on) must have finished
+    // ring reuse: the slots this tile overwrites held cells [gc0 - R, gc0 + T - R) of the
+    // concatenation; every tile that reads them (cells up to `reach` further on) must have finished
     {
-        const int64_t lo = c0 - a.R + 1;
-        if (lo + GP_TILE - 1 >= 0) {
-            const int64_t firstReader = (lo < 0 ? 0 : lo) / GP_TILE;
-            int64_t lastReader = (c0 + GP_TILE - a.R + (int64_t)CX * CY + CX) / GP_TILE;
+        const int64_t oldHi = gc0 + GP_TILE - 1 - a.R;
+        if (oldHi >= 0) {
+            const int64_t lo = gc0 - a.R + 1;
+            const int64_t firstReader = tileOfCell(a, lo < 0 ? 0 : lo);
+            const int64_t hiCell = oldHi + a.reach;
+            int64_t lastReader = tileOfCell(a, hiCell < a.totalCells ? hiCell : a.totalCells - 1);
             if (lastReader > t - 1) lastReader = t - 1;
             for (int64_t tr = firstReader + tid; tr <= lastReader; tr += GP_THREADS) waitFlag(&a.done[tr], a.err);
         }
@@ -390,7 +422,7 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
 
     // ---- 4. publish first vertex id + contour map of the tile's active cells
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
-        const int64_t c = c0 + s_cell[ai];
+        const int64_t c = gc0 + s_cell[ai];
         const uint64_t ent = (ringTag(c, a.ringShift) << 55) | ((uint64_t)contourMap(s_poly[ai]) << 31) | (uint64_t)((voff + s_vpre[ai]) & 0x7FFFFFFF);
         storeRelaxed(a.ring + (c & (a.R - 1)), ent);
     }
@@ -430,6 +462,7 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
             a.nrm[3 * v + 1] = nn.y;
             a.nrm[3 * v + 2] = nn.z;
             if (a.adjCell) a.adjCell[v] = (int)(c0 + s_cell[ai]);
+            if (a.vpid) a.vpid[v] = prob;
         } else {
             atomicOr(a.err, GP_ERR_OVERFLOW);
         }
@@ -478,7 +511,7 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
                 const int an = s_aidx[n - c0];
                 vid = (int)(voff + s_vpre[an] + polyContourOf(s_poly[an], e));
             } else {
-                vid = ringVertex(a, n, e);
+                vid = ringVertex(a, P.cellBase + n, e);
             }
             const int sl = edgeSlot(e, edgy);
             s0 = sl == 0 ? vid : s0;
@@ -512,10 +545,10 @@ __global__ void kGridPassTotals(const unsigned long long* status, int64_t numTil
 
 int gridPassTile() { return GP_TILE; }
 
-void gridPassRing(int64_t C, int CX, int CY, int64_t* R, int* shift) {
+void gridPassRing(int64_t totalCells, int64_t reach, int64_t* R, int* shift) {
     // at least 8 z-slices of cells (the quads of a tile reach one slice back), and few
     // enough generations that the 9-bit tag of a ring entry never wraps
-    const int64_t want = std::max<int64_t>({8 * ((int64_t)CX * CY + CX + GP_TILE), (C + 509) / 510, (int64_t)1 << 16});
+    const int64_t want = std::max<int64_t>({8 * (reach + GP_TILE), (totalCells + 509) / 510, (int64_t)1 << 16});
     int s = 16;
     while (((int64_t)1 << s) < want) ++s;
     *R = (int64_t)1 << s;

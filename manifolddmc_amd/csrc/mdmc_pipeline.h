@@ -6,7 +6,10 @@
 
 #include <string>
 
+#include <vector>
+
 #include "mdmc_arena.h"
+#include "mdmc_gridpass.h"
 #include "mdmc_internal.h"
 
 namespace mdmc {
@@ -35,14 +38,29 @@ class Pipeline {
     void resetCounters();
     int64_t anomalies();
 
-    // the grid pass; recordAdjoint keeps the cell -> vertex map and vertex -> cell
-    // array ("adj_map", "adj_cell") that the adjoint reads
-    void dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoint = false);
+    // the grid pass over one or more (grid, iso) problems (their cellBase / tileBase and
+    // cell counts are filled in); recordAdjoint keeps the cell -> vertex map and vertex ->
+    // cell array ("adj_map", "adj_cell") that the adjoint reads (one problem only)
+    void dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint = false);
+    int nProblems = 1;
+    ProblemDev* probsDev = nullptr;
+    // per vertex its problem (several problems), else nullptr
+    int32_t* pid();
+
+    // batched form (mdmc_batch.hip): per-problem DMC vertex / quad counts right after the
+    // grid pass, then the final regroup into contiguous per-problem ranges with offsets,
+    // per-problem vertex -> halfedge values and statistics
+    void problemDmcCounts(int B);
+    void regroupBatch(int B);
+    std::vector<int64_t> batchDmc;                     // [2B]: vertices, quads
+    std::vector<int64_t> batchVOff, batchHOff, batchFOff;  // [B + 1]
+    std::vector<int64_t> batchStats;                   // [9B], as computeStats
+    bool batchVertHe = false;
     void halfedgeInitialize();
     void simplify3X3Y(bool colour);
     void simplify3333();
-    void fixManifoldTwins(const GridDev& g, float iso);
-    void split(const GridDev& g, float iso, int step, int maxIters);
+    void fixManifoldTwins();
+    void split(int step, int maxIters);
     void detectNonManifoldVertices();
     void computeStats(int64_t out[9]);
 
@@ -67,6 +85,7 @@ class Pipeline {
     void simplifyInit(int32_t** valence, int32_t** count);
     void simplifyCompactAndRebuild(int dropType, bool p3333);
     void buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag);
+    int32_t* stableSortByKey(const int32_t* key, int64_t n, int B, const std::string& tag);
 };
 
 }  // namespace mdmc

@@ -446,9 +446,9 @@ __global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int 
     keep[v] = vtype[v] == dropType ? 0 : 1;
 }
 
-__global__ void kCompactVertices(const float* __restrict__ pos, const float* __restrict__ nrm, const int32_t* __restrict__ keep,
-                                 const int64_t* __restrict__ addr, int64_t V, float* __restrict__ npos, float* __restrict__ nnrm,
-                                 int32_t* __restrict__ address) {
+__global__ void kCompactVertices(const float* __restrict__ pos, const float* __restrict__ nrm, const int32_t* __restrict__ pid,
+                                 const int32_t* __restrict__ keep, const int64_t* __restrict__ addr, int64_t V, float* __restrict__ npos,
+                                 float* __restrict__ nnrm, int32_t* __restrict__ npid, int32_t* __restrict__ address) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     if (!keep[v]) return;
@@ -458,6 +458,7 @@ __global__ void kCompactVertices(const float* __restrict__ pos, const float* __r
         npos[3 * o + k] = pos[3 * v + k];
         nnrm[3 * o + k] = nrm[3 * v + k];
     }
+    if (pid) npid[o] = pid[v];
 }
 
 __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t* __restrict__ keep) {
@@ -509,7 +510,7 @@ __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int
 __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
                           const int2* __restrict__ listB,
                           const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
-                          const float* __restrict__ pos, GridDev g, float iso, int64_t* ctr) {
+                          const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
     const int h = (int)hh;
@@ -520,6 +521,9 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __res
     int s[4];
     bucketSlots(mem, n, s);
     const int h0 = s[0];
+    const ProblemDev& P = probs[pid ? pid[heO(he, h)] : 0];
+    const GridDev g = P.g;
+    const float iso = P.iso;
     f3 p0 = ld3(pos, heO(he, h0));
     f3 p1 = ld3(pos, heO(he, heN(he, h0)));
     int c0[3], c1[3];
@@ -640,12 +644,14 @@ struct WC {
     uint32_t* f_attr;
     uint32_t* fflags;
     int32_t *f_c2o, *f_o2c;
+    int32_t* pid;  // per vertex its problem (several problems), else nullptr
     int n;
 };
 
 // copyVerticesToWorkingCopy / copyHalfedgesToWorkingCopy / copyFacesToWorkingCopy (ManifoldSplitter.cu:307-457)
 __global__ void kWcCopy(const int32_t* __restrict__ nm, int n, const int4* __restrict__ he, const int32_t* __restrict__ face_he,
-                        const uint32_t* __restrict__ fattr, const float* __restrict__ pos, const float* __restrict__ nrm, WC w) {
+                        const uint32_t* __restrict__ fattr, const float* __restrict__ pos, const float* __restrict__ nrm,
+                        const int32_t* __restrict__ pid, WC w) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int f = nm[r];
@@ -659,6 +665,7 @@ __global__ void kWcCopy(const int32_t* __restrict__ nm, int n, const int4* __res
             w.nrm[3 * c + k] = nrm[3 * (int64_t)v + k];
         }
         w.v_c2o[c] = v;
+        if (w.pid) w.pid[c] = pid[v];
         atomicMax(&w.v_o2c[v], c);  // racy "last writer" in ascending order
         w.he[c] = e;
         w.h_c2o[c] = h;
@@ -731,6 +738,7 @@ __global__ void kSubdivide(const int32_t* __restrict__ nm, WC w) {
         w.pos[3 * c] = p.x; w.pos[3 * c + 1] = p.y; w.pos[3 * c + 2] = p.z;
         w.nrm[3 * c] = q.x; w.nrm[3 * c + 1] = q.y; w.nrm[3 * c + 2] = q.z;
         w.vflags[c] &= ~(uint32_t)VF_REMOVE;
+        if (w.pid) w.pid[c] = w.pid[vid[0]];
         inner[k] = c;
     }
     const int hb = 4 * n + 16 * r;
@@ -1024,10 +1032,13 @@ __global__ void kMergeSequential(WC w, int count, const int32_t* __restrict__ nu
 }
 
 // performMoveVerticesToSurface, ManifoldSplitter.cu:1130-1160
-__global__ void kMoveToSurface(WC w, int count, GridDev g, float iso) {
+__global__ void kMoveToSurface(WC w, int count, const ProblemDev* __restrict__ probs) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= count) return;
     if (w.vflags[v] & VF_REMOVE) return;
+    const ProblemDev& P = probs[w.pid ? w.pid[v] : 0];
+    const GridDev g = P.g;
+    const float iso = P.iso;
     f3 point = ld3(w.pos, v);
     int c[3];
     cellOf(g, point, c);
@@ -1067,7 +1078,7 @@ __global__ void kLastCopy(WC w, int count, int32_t* __restrict__ last) {
 }
 
 __global__ void kCopyBackVertices(WC w, int count, const int64_t* __restrict__ addr, int64_t V0, float* __restrict__ pos, float* __restrict__ nrm,
-                                  const int32_t* __restrict__ last) {
+                                  int32_t* __restrict__ pid, const int32_t* __restrict__ last) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= count) return;
     if (w.vflags[v] & VF_REMOVE) return;
@@ -1082,6 +1093,7 @@ __global__ void kCopyBackVertices(WC w, int count, const int64_t* __restrict__ a
         pos[3 * o + k] = w.pos[3 * v + k];
         nrm[3 * o + k] = w.nrm[3 * v + k];
     }
+    if (pid) pid[o] = w.pid[v];
 }
 
 __global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ addr, int64_t H0, int4* __restrict__ he) {
@@ -1341,19 +1353,31 @@ void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t))
 int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
 
 // ---- grid pass
-void Pipeline::dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoint) {
-    const int CX = g.nx - 1, CY = g.ny - 1, CZ = g.nz - 1;
-    const int64_t C = (int64_t)CX * CY * CZ;
+void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint) {
     const int T = gridPassTile();
     vertHeV = vertHeQ = 0;
-    const int64_t numTiles = (C + T - 1) / T;
+    batchVertHe = false;
+    nProblems = (int)problems.size();
+    int64_t C = 0, numTiles = 0, reach = 0;
+    for (ProblemDev& pr : problems) {
+        pr.CX = pr.g.nx - 1;
+        pr.CY = pr.g.ny - 1;
+        pr.cells = (int64_t)pr.CX * pr.CY * (pr.g.nz - 1);
+        pr.cellBase = C;
+        pr.tileBase = numTiles;
+        C += pr.cells;
+        numTiles += (pr.cells + T - 1) / T;
+        reach = std::max<int64_t>(reach, (int64_t)pr.CX * pr.CY + pr.CX);
+    }
+    if (recordAdjoint && nProblems != 1) throw HipError("adjoint recording needs a single problem", -4);
+    probsDev = arena.get<ProblemDev>("gp_probs", problems.size());
+    MDMC_HIP(hipMemcpyAsync(probsDev, problems.data(), problems.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, stream));
     GridPassArgs a;
-    a.g = g;
-    a.iso = iso;
+    a.probs = probsDev;
+    a.nProblems = nProblems;
     a.surfIter = surfIter ? 1 : 0;
-    a.C = C;
-    a.CX = CX;
-    a.CY = CY;
+    a.totalCells = C;
+    a.reach = reach;
     a.numTiles = numTiles;
     // flags: status (u64) | done | ticket | err, zeroed together
     const size_t flagBytes = (size_t)numTiles * (8 + 4) + 64;
@@ -1362,7 +1386,7 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoin
     a.done = (int*)(flags + 8 * (size_t)numTiles);
     a.ticket = (unsigned int*)(a.done + numTiles);
     a.err = (int*)(a.ticket + 4);
-    gridPassRing(C, CX, CY, &a.R, &a.ringShift);
+    gridPassRing(C, reach, &a.R, &a.ringShift);
     a.ring = arena.get<unsigned long long>("gp_ring", (size_t)a.R);
     int64_t* tot = arena.get<int64_t>("gp_totals", 4);
     // output capacity: what the buffers hold, at least an estimate; exact sizes on overflow
@@ -1379,6 +1403,7 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoin
         a.capQ = capQ;
         a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)C) : nullptr;
         a.adjCell = recordAdjoint ? arena.get<int32_t>("adj_cell", (size_t)capV) : nullptr;
+        a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)capV) : nullptr;
         zero(flags, flagBytes);
         zero(a.ring, (size_t)a.R * sizeof(unsigned long long));
         MDMC_HIP(hipEventRecord(kev[0], stream));
@@ -1409,6 +1434,8 @@ void Pipeline::dmc(const GridDev& g, float iso, bool surfIter, bool recordAdjoin
     lastGridV = V;
     lastGridQ = Q;
 }
+
+int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nullptr; }
 
 // ---- HalfedgeMesh::initialize / refreshHashTable: CSR of outgoing halfedges and the buckets
 // bucket lists by smaller endpoint (degM holds the counts on entry and is zero on exit),
@@ -1495,7 +1522,10 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     fillNeg(address, (size_t)V * sizeof(int32_t));
     float* npos = arena.get<float>("pos2", 3 * (size_t)nV);
     float* nnrm = arena.get<float>("nrm2", 3 * (size_t)nV);
-    if (V > 0) kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), keep, vaddr, V, npos, nnrm, address);
+    int32_t* npid = pid() ? arena.get<int32_t>("pid2", (size_t)nV) : nullptr;
+    if (V > 0)
+        kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), keep, vaddr, V, npos, nnrm, npid,
+                                                            address);
     const uint32_t* attr = p3333 ? arena.ptr<uint32_t>("fattr") : arena.ptr<uint32_t>("qattr");
     int32_t* keepq = arena.get<int32_t>("keepQ", (size_t)Q);
     if (Q > 0) kKeepQuads<<<gridFor(Q), kBlock, 0, stream>>>(attr, Q, keepq);
@@ -1510,6 +1540,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     MDMC_HIP(hipMemcpyAsync(arena.ptr<float>("nrm"), nnrm, 3 * (size_t)nV * sizeof(float), hipMemcpyDeviceToDevice, stream));
     MDMC_HIP(hipMemcpyAsync(arena.ptr<int4>("quads"), nquads, (size_t)nQ * sizeof(int4), hipMemcpyDeviceToDevice, stream));
     MDMC_HIP(hipMemcpyAsync(arena.ptr<uint32_t>("qattr"), nattr, (size_t)nQ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    if (npid) MDMC_HIP(hipMemcpyAsync(pid(), npid, (size_t)nV * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
     V = nV;
     Q = nQ;
     halfedgeInitialize();
@@ -1559,14 +1590,15 @@ void Pipeline::simplify3333() {
     simplifyCompactAndRebuild(1, true);
 }
 
-void Pipeline::fixManifoldTwins(const GridDev& g, float iso) {
+void Pipeline::fixManifoldTwins() {
     if (H == 0) return;
     kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listA"), arena.ptr<int2>("listB"),
-                                                 arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), g, iso, counters());
+                                                 arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
+                                                 counters());
 }
 
 // ---- ManifoldSplitter::run
-void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
+void Pipeline::split(int step, int maxIters) {
     const bool copyDataBack = true;
     int iter = 0;
     while (true) {
@@ -1604,6 +1636,7 @@ void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
         w.fflags = arena.get<uint32_t>("wc_fflags", (size_t)nf);
         w.f_c2o = arena.get<int32_t>("wc_f_c2o", (size_t)nf);
         w.f_o2c = arena.get<int32_t>("wc_f_o2c", (size_t)(F + nf));
+        w.pid = pid() ? arena.get<int32_t>("wc_pid", (size_t)nv) : nullptr;
         zero(w.pos, 3 * (size_t)nv * sizeof(float));
         zero(w.nrm, 3 * (size_t)nv * sizeof(float));
         zero(w.vflags, (size_t)nv * sizeof(uint32_t));  // WorkingCopy ignores its initialFlags argument: flags start at 0
@@ -1620,7 +1653,7 @@ void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
         fillNeg(w.f_o2c, (size_t)(F + nf) * sizeof(int32_t));
 
         kWcCopy<<<gridFor(n), kBlock, 0, stream>>>(nm, n, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"),
-                                                   arena.ptr<float>("pos"), arena.ptr<float>("nrm"), w);
+                                                   arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), w);
         int vsize = 4 * n, hsize = 4 * n, fsize = n;
         kWcUpdate<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize);
         if (H > 0) kWcNMFlags<<<gridFor(H), kBlock, 0, stream>>>(H, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, counters());
@@ -1664,7 +1697,7 @@ void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
                 kMergeSequential<<<1, 1, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, counters());
             }
         }
-        if (step >= 5) kMoveToSurface<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, g, iso);
+        if (step >= 5) kMoveToSurface<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, probsDev);
 
         if (copyDataBack || step >= 6) {
             // vertices
@@ -1675,10 +1708,11 @@ void Pipeline::split(const GridDev& g, float iso, int step, int maxIters) {
             const int64_t V0 = V;
             float* pos = arena.grow<float>("pos", 3 * (size_t)(V0 + addV), 3 * (size_t)V0, stream);
             float* nrm = arena.grow<float>("nrm", 3 * (size_t)(V0 + addV), 3 * (size_t)V0, stream);
+            int32_t* vp = pid() ? arena.grow<int32_t>("pid", (size_t)(V0 + addV), (size_t)V0, stream) : nullptr;
             int32_t* last = arena.get<int32_t>("cb_last", (size_t)V0 + 1);
             fillNeg(last, ((size_t)V0 + 1) * sizeof(int32_t));
             kLastCopy<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, last);
-            kCopyBackVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, a, V0, pos, nrm, last);
+            kCopyBackVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, a, V0, pos, nrm, vp, last);
             // halfedges
             kNewPred<<<gridFor(hsize), kBlock, 0, stream>>>(w.hflags, w.h_c2o, hsize, p);
             const int64_t addH = scan32(p, hsize, "cb_addr", &a);
@@ -1755,7 +1789,9 @@ void Pipeline::copyOut(float* positions, float* normals, int32_t* halfedges, int
     if (normals && V) MDMC_HIP(hipMemcpyAsync(normals, arena.ptr<float>("nrm"), 3 * (size_t)V * sizeof(float), k, stream));
     if (halfedges && H) MDMC_HIP(hipMemcpyAsync(halfedges, arena.ptr<int4>("he"), (size_t)H * sizeof(int4), k, stream));
     if (face_halfedge && F) MDMC_HIP(hipMemcpyAsync(face_halfedge, arena.ptr<int32_t>("face_he"), (size_t)F * sizeof(int32_t), k, stream));
-    if (vertex_halfedge && V) {
+    if (vertex_halfedge && V && batchVertHe) {
+        MDMC_HIP(hipMemcpyAsync(vertex_halfedge, arena.ptr<int32_t>("batch_vert_he"), (size_t)V * sizeof(int32_t), k, stream));
+    } else if (vertex_halfedge && V) {
         int32_t* vh = arena.get<int32_t>("vert_he", (size_t)V);
         kVertexHalfedgeInit<<<gridFor(V), kBlock, 0, stream>>>(vh, std::min(vertHeV, V), V);
         if (vertHeQ > 0) kVertexHalfedge<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vh);
