@@ -234,8 +234,13 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             MDMC_HIP(hipEventRecord(ctx->ev[6], s));
             if (flags & MDMC_DETECT_NM_VERTS) p.detectNonManifoldVertices();
             MDMC_HIP(hipEventRecord(ctx->ev[7], s));
-            p.computeStats(st9);
-            if (B > 1) p.regroupBatch(B);
+            if (B == 1) {
+                p.computeStats(st9);
+            } else {
+                p.regroupBatch(B);
+                for (int b = 0; b < B; ++b)
+                    for (int k = 0; k < 9; ++k) st9[k] += p.batchStats[9 * (size_t)b + k];
+            }
         } else {
             // DualMarchingCubes::run returns early when no vertex was produced (DualMarchingCubes.cu:257-261)
             p.Q = p.H = p.F = 0;

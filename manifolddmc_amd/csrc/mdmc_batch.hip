@@ -9,8 +9,11 @@
 // every problem's vertices, halfedges and faces contiguous (CSR-packed), in the
 // order a single extraction of that problem produces.
 //
-// Stable sort: least-significant-bit-first binary splits of a permutation, each a
-// prefix sum of the "bit is 0" flags and a scatter (ceil(log2 B) passes).
+// Until the splitter runs, every array is ordered by problem (the grid pass emits the
+// problems in order and every compaction is stable).  The splitter only appends, so
+// each array is a sorted prefix of n0 elements plus a tail; the tail is sorted stably
+// by problem (least-significant-bit-first binary splits: per bit a prefix sum of the
+// "bit is 0" flags and a scatter) and each problem's tail goes after its prefix part.
 #include <algorithm>
 #include <vector>
 
@@ -55,6 +58,31 @@ __global__ void kSplitScatter(const int32_t* __restrict__ perm, const int32_t* _
 __global__ void kInverse(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+// final position of every element: prefix element i of problem b -> out[b] + i - pre[b];
+// tail element at sorted tail position j -> out[b] + preCount[b] + j - tail[b]
+// (offsets: [0, B] prefix starts, [B+1, 2B+1] tail starts, [2B+2, 3B+2] output starts)
+__global__ void kPlace(const int32_t* __restrict__ key, int64_t n0, int64_t n, const int32_t* __restrict__ tailPerm, const int64_t* __restrict__ offs,
+                       int B, int32_t* __restrict__ perm, int32_t* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t* pre = offs;
+    const int64_t* tail = offs + (B + 1);
+    const int64_t* out = offs + 2 * (B + 1);
+    int64_t old, dst;
+    if (i < n0) {
+        old = i;
+        const int b = key[i];
+        dst = out[b] + (i - pre[b]);
+    } else {
+        const int64_t j = i - n0;
+        old = n0 + tailPerm[j];
+        const int b = key[old];
+        dst = out[b] + (pre[b + 1] - pre[b]) + (j - tail[b]);
+    }
+    perm[dst] = (int32_t)old;
+    inv[old] = (int32_t)dst;
 }
 
 // per-key counts; keys arrive sorted, so a wave usually holds one key and adds once
@@ -139,32 +167,52 @@ __global__ void kVertexHalfedgeLocal(const int32_t* __restrict__ vhInit, int64_t
     out[nv] = h < 0 ? -1 : (int32_t)(h - 4 * qBase[pidNew[nv]]);
 }
 
-// bucket / face / vertex counters of HalfedgeMesh.cu:1164-1239 per problem (elements sorted by problem)
-__device__ __forceinline__ void addPer(unsigned long long* out, int p, int slot, unsigned v) {
-    if (v) atomicAdd(&out[(int64_t)p * 9 + slot], (unsigned long long)v);
+// bucket / face / vertex counters of HalfedgeMesh.cu:1164-1239 per problem.  Elements are
+// sorted by problem, so a wave usually holds one problem: it then adds its sums once.
+__device__ __forceinline__ void addPer(unsigned long long* out, bool live, int p, int slot, unsigned v) {
+    const int p0 = __shfl(p, 0, 64);
+    const uint64_t alive = __ballot(live);
+    const uint64_t same = __ballot(live && p == p0);
+    if (alive == 0) return;
+    if (same == alive) {
+        unsigned sum = live ? v : 0u;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
+        const int pk = __shfl(p, __ffsll((unsigned long long)alive) - 1, 64);
+        if ((threadIdx.x & 63) == 0 && sum) atomicAdd(&out[(int64_t)pk * 9 + slot], (unsigned long long)sum);
+    } else if (live && v) {
+        atomicAdd(&out[(int64_t)p * 9 + slot], (unsigned long long)v);
+    }
 }
 __global__ void kStatsBatch(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
                             const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags, const int32_t* __restrict__ pid, int64_t V,
                             unsigned long long* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < H && brank[i] == 0) {
-        const int p = pid[heO(he, (int)i)];
-        const int n = bsize[i];
-        addPer(out, p, 0, n == 4);
-        addPer(out, p, 1, n == 3);
-        addPer(out, p, 2, n > 0);
+    {
+        const bool live = i < H && brank[i] == 0;
+        const int p = live ? pid[heO(he, (int)i)] : 0;
+        const int n = live ? bsize[i] : 0;
+        addPer(out, live, p, 0, n == 4);
+        addPer(out, live, p, 1, n == 3);
+        addPer(out, live, p, 2, n > 0);
     }
-    if (i < F) {
-        int h = face_he[i];
-        const int p = pid[heO(he, h)];
-        int k = 0;
-        for (int j = 0; j < 4; ++j) {
-            if (bsize[h] > 2) ++k;
-            h = ((const int*)he)[4 * (int64_t)h + 2];
+    {
+        const bool live = i < F;
+        int p = 0, k = 0;
+        if (live) {
+            int h = face_he[i];
+            p = pid[heO(he, h)];
+            for (int j = 0; j < 4; ++j) {
+                if (bsize[h] > 2) ++k;
+                h = ((const int*)he)[4 * (int64_t)h + 2];
+            }
         }
-        addPer(out, p, 3 + k, 1);
+        for (int kk = 0; kk < 5; ++kk) addPer(out, live, p, 3 + kk, k == kk);
     }
-    if (i < V) addPer(out, pid[i], 8, (vflags[i] & VF_FANS) ? 1u : 0u);
+    {
+        const bool live = i < V;
+        addPer(out, live, live ? pid[i] : 0, 8, live && (vflags[i] & VF_FANS) ? 1u : 0u);
+    }
 }
 
 }  // namespace
@@ -205,54 +253,64 @@ void Pipeline::problemDmcCounts(int B) {
     MDMC_HIP(hipStreamSynchronize(stream));
 }
 
+// permutation (new -> old) and its inverse for one array of n elements whose first n0
+// are ordered by key; per-key counts go to cnt[4 * b + slot]
+void Pipeline::placeByProblem(const int32_t* key, int64_t n0, int64_t n, int B, const std::string& tag, int32_t** perm, int32_t** inv,
+                              std::vector<int64_t>& off) {
+    *perm = arena.get<int32_t>(tag + "_perm", (size_t)std::max<int64_t>(n, 1));
+    *inv = arena.get<int32_t>(tag + "_inv", (size_t)std::max<int64_t>(n, 1));
+    const int64_t nt = n - n0;
+    const int32_t* tp = stableSortByKey(key + n0, nt, B, tag + "_tail");
+    unsigned long long* cnt = arena.get<unsigned long long>(tag + "_cnt", 2 * (size_t)B);
+    zero(cnt, 2 * (size_t)B * sizeof(unsigned long long));
+    if (n0 > 0) kCountSorted<<<gridFor(n0), kBlock, 0, stream>>>(key, nullptr, n0, cnt, 2, 0);
+    if (nt > 0) kCountSorted<<<gridFor(nt), kBlock, 0, stream>>>(key + n0, tp, nt, cnt, 2, 1);
+    std::vector<unsigned long long> c(2 * (size_t)B);
+    MDMC_HIP(hipMemcpyAsync(c.data(), cnt, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    std::vector<int64_t> offs(3 * ((size_t)B + 1), 0);
+    for (int b = 0; b < B; ++b) {
+        offs[b + 1] = offs[b] + (int64_t)c[2 * b];
+        offs[(B + 1) + b + 1] = offs[(B + 1) + b] + (int64_t)c[2 * b + 1];
+        offs[2 * (B + 1) + b + 1] = offs[2 * (B + 1) + b] + (int64_t)(c[2 * b] + c[2 * b + 1]);
+    }
+    off.assign(offs.begin() + 2 * (B + 1), offs.end());
+    int64_t* od = arena.get<int64_t>(tag + "_offs", offs.size());
+    MDMC_HIP(hipMemcpyAsync(od, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    if (n > 0) kPlace<<<gridFor(n), kBlock, 0, stream>>>(key, n0, n, tp, od, B, *perm, *inv);
+    MDMC_HIP(hipStreamSynchronize(stream));  // offs lifetime
+}
+
 void Pipeline::regroupBatch(int B) {
     int32_t* vp = pid();
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
     // quads per problem at the last initialize (for the vertex -> halfedge values)
-    unsigned long long* cnt = arena.get<unsigned long long>("batch_cnt", 4 * (size_t)B);
-    zero(cnt, 4 * (size_t)B * sizeof(unsigned long long));
+    unsigned long long* qc = arena.get<unsigned long long>("batch_qcnt", (size_t)B);
+    zero(qc, (size_t)B * sizeof(unsigned long long));
     if (vertHeQ > 0) {
         int32_t* qk = arena.get<int32_t>("batch_qkey", (size_t)vertHeQ);
         kHalfedgeKeys<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vp, qk);
-        kCountSorted<<<gridFor(vertHeQ), kBlock, 0, stream>>>(qk, nullptr, vertHeQ, cnt, 4, 3);
+        kCountSorted<<<gridFor(vertHeQ), kBlock, 0, stream>>>(qk, nullptr, vertHeQ, qc, 1, 0);
     }
     int32_t* vhInit = arena.get<int32_t>("batch_vh_init", (size_t)std::max<int64_t>(vertHeV, 1));
     if (vertHeV > 0) kVertexHalfedgeInitBatch<<<gridFor(vertHeV), kBlock, 0, stream>>>(vhInit, vertHeV);
     if (vertHeQ > 0) kVertexHalfedgeMax<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vhInit);
-    // sort keys and permutations
+    std::vector<unsigned long long> qcount((size_t)B);
+    MDMC_HIP(hipMemcpyAsync(qcount.data(), qc, (size_t)B * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    // keys; the elements that existed at the last initialize come first, in problem order
     int32_t* hk = arena.get<int32_t>("batch_hkey", (size_t)std::max<int64_t>(H, 1));
     int32_t* fk = arena.get<int32_t>("batch_fkey", (size_t)std::max<int64_t>(F, 1));
     if (H > 0) kHalfedgeKeys<<<gridFor(H), kBlock, 0, stream>>>(he, H, vp, hk);
     if (F > 0) kFaceKeys2<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, vp, fk);
-    const int32_t* pv = stableSortByKey(vp, V, B, "sv");
-    const int32_t* ph = stableSortByKey(hk, H, B, "sh");
-    const int32_t* pf = stableSortByKey(fk, F, B, "sf");
-    if (V > 0) kCountSorted<<<gridFor(V), kBlock, 0, stream>>>(vp, pv, V, cnt, 4, 0);
-    if (H > 0) kCountSorted<<<gridFor(H), kBlock, 0, stream>>>(hk, ph, H, cnt, 4, 1);
-    if (F > 0) kCountSorted<<<gridFor(F), kBlock, 0, stream>>>(fk, pf, F, cnt, 4, 2);
-    std::vector<unsigned long long> c(4 * (size_t)B);
-    MDMC_HIP(hipMemcpyAsync(c.data(), cnt, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    MDMC_HIP(hipStreamSynchronize(stream));
-    batchVOff.assign((size_t)B + 1, 0);
-    batchHOff.assign((size_t)B + 1, 0);
-    batchFOff.assign((size_t)B + 1, 0);
+    int32_t *pv, *ph, *pf, *vNew, *hNew, *fNew;
+    placeByProblem(vp, std::min(vertHeV, V), V, B, "sv", &pv, &vNew, batchVOff);
+    placeByProblem(hk, std::min(4 * vertHeQ, H), H, B, "sh", &ph, &hNew, batchHOff);
+    placeByProblem(fk, std::min(vertHeQ, F), F, B, "sf", &pf, &fNew, batchFOff);
     std::vector<int64_t> qBase((size_t)B + 1, 0);
-    for (int b = 0; b < B; ++b) {
-        batchVOff[b + 1] = batchVOff[b] + (int64_t)c[4 * b + 0];
-        batchHOff[b + 1] = batchHOff[b] + (int64_t)c[4 * b + 1];
-        batchFOff[b + 1] = batchFOff[b] + (int64_t)c[4 * b + 2];
-        qBase[b + 1] = qBase[b] + (int64_t)c[4 * b + 3];
-    }
+    for (int b = 0; b < B; ++b) qBase[b + 1] = qBase[b] + (int64_t)qcount[b];
     int64_t* qBaseDev = arena.get<int64_t>("batch_qbase", (size_t)B + 1);
     MDMC_HIP(hipMemcpyAsync(qBaseDev, qBase.data(), qBase.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
-    // inverse maps and the permuted arrays
-    int32_t* vNew = arena.get<int32_t>("batch_vnew", (size_t)std::max<int64_t>(V, 1));
-    int32_t* hNew = arena.get<int32_t>("batch_hnew", (size_t)std::max<int64_t>(H, 1));
-    int32_t* fNew = arena.get<int32_t>("batch_fnew", (size_t)std::max<int64_t>(F, 1));
-    if (V > 0) kInverse<<<gridFor(V), kBlock, 0, stream>>>(pv, V, vNew);
-    if (H > 0) kInverse<<<gridFor(H), kBlock, 0, stream>>>(ph, H, hNew);
-    if (F > 0) kInverse<<<gridFor(F), kBlock, 0, stream>>>(pf, F, fNew);
     float* npos = arena.get<float>("pos2", 3 * (size_t)std::max<int64_t>(V, 1));
     float* nnrm = arena.get<float>("nrm2", 3 * (size_t)std::max<int64_t>(V, 1));
     uint32_t* nvf = arena.get<uint32_t>("vflags2", (size_t)std::max<int64_t>(V, 1));

@@ -86,6 +86,8 @@ class Pipeline {
     void simplifyCompactAndRebuild(int dropType, bool p3333);
     void buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag);
     int32_t* stableSortByKey(const int32_t* key, int64_t n, int B, const std::string& tag);
+    void placeByProblem(const int32_t* key, int64_t n0, int64_t n, int B, const std::string& tag, int32_t** perm, int32_t** inv,
+                        std::vector<int64_t>& off);
 };
 
 }  // namespace mdmc
