@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES between two device buffers 5 times (reference for the HBM traffic counters) and exit")
     return ap.parse_args()
 
 
@@ -200,6 +202,16 @@ def adjoint_bench(steps=100):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported (run with --gpus 1)" % args.gpus, file=sys.stderr)
         sys.exit(2)

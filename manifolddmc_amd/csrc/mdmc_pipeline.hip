@@ -62,25 +62,18 @@ __global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict
 }
 
 // ---- buckets: the halfedges joining one vertex pair.  They are listed under the
-// smaller endpoint m, so one short contiguous list holds every bucket of m.  An entry
-// is split in two arrays: A = (other endpoint | 0x80000000 when h starts at the other
-// endpoint, bucket key of h), read by every scan of the list, and B = (h, m).
-__device__ __forceinline__ void minEntry(int2* __restrict__ A, int2* __restrict__ B, int64_t i, int h, int a, int b, int key) {
-    if (a <= b) {
-        A[i] = make_int2(b, key);
-        B[i] = make_int2(h, a);
-    } else {
-        A[i] = make_int2((int)(0x80000000u | (uint32_t)a), key);
-        B[i] = make_int2(h, b);
-    }
-}
-__device__ __forceinline__ int entryOther(int2 a) { return a.x & 0x7FFFFFFF; }
-__device__ __forceinline__ int entryOrigin(int2 a, int2 b) { return a.x < 0 ? (a.x & 0x7FFFFFFF) : b.y; }
+// smaller endpoint m: one short contiguous list per vertex holds every bucket of m.
+// A list entry A[i] is (other endpoint | 0x80000000 when the halfedge starts at the
+// other endpoint, bucket key); after an initialize the key is the halfedge id.  Each
+// halfedge keeps a record R[h] = (list start, list length, own entry) so that the
+// bucket pass reads its list with no index lookups.
+__device__ __forceinline__ int pairEntry(int a, int b) { return a <= b ? b : (int)(0x80000000u | (uint32_t)a); }
+__device__ __forceinline__ int entryOther(int x) { return x & 0x7FFFFFFF; }
 
 // HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
-// bucket list count of every halfedge; pos gets the halfedge's rank in its list
+// first list pass: R[h] = (rank in m's list, m, own entry, key = h)
 __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM, int32_t* __restrict__ pos) {
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM, int4* __restrict__ R) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -91,12 +84,12 @@ __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restr
     he[b + 3] = make_int4(v.w, (int)q, b + 0, -1);
     face_he[q] = b;
     fattr[q] = qattr[q];
-    int4 r;
-    r.x = atomicAdd(&degM[min(v.x, v.y)], 1);
-    r.y = atomicAdd(&degM[min(v.y, v.z)], 1);
-    r.z = atomicAdd(&degM[min(v.z, v.w)], 1);
-    r.w = atomicAdd(&degM[min(v.w, v.x)], 1);
-    *reinterpret_cast<int4*>(pos + b) = r;
+    const int c[5] = {v.x, v.y, v.z, v.w, v.x};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = min(c[i], c[i + 1]);
+        R[b + i] = make_int4(atomicAdd(&degM[m], 1), m, pairEntry(c[i], c[i + 1]), b + i);
+    }
 }
 
 // HalfedgeMesh's vertex -> halfedge array as collectHalfedgeElements leaves it: the
@@ -119,73 +112,50 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
     vert_he[i] = i < n0 ? -1 : 0;
 }
 
-// bucket lists straight from the quads (halfedge 4q+i joins corner i and corner i+1,
-// its key is its id) at offset + rank; pos turns from rank into list index
-__global__ void kFillMinQuads(const int4* __restrict__ quads, int64_t Q, const int64_t* __restrict__ offM, int32_t* __restrict__ pos,
-                              int2* __restrict__ listA, int2* __restrict__ listB) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    const int4 v = quads[q];
-    const int c[5] = {v.x, v.y, v.z, v.w, v.x};
-    const int b = (int)(4 * q);
-    const int4 r = *reinterpret_cast<const int4*>(pos + b);
-    const int rk[4] = {r.x, r.y, r.z, r.w};
-    int p[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int m = min(c[i], c[i + 1]);
-        p[i] = (int)(offM[m] + rk[i]);
-        minEntry(listA, listB, p[i], b + i, c[i], c[i + 1], b + i);
-    }
-    *reinterpret_cast<int4*>(pos + b) = make_int4(p[0], p[1], p[2], p[3]);
-}
-
-__global__ void kCountMin(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ degM, int32_t* __restrict__ pos) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
-    const int4 e = he[h];
-    pos[h] = atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
-}
-
-__global__ void kFillMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ hkey, const int64_t* __restrict__ offM,
-                         int32_t* __restrict__ pos, int2* __restrict__ listA, int2* __restrict__ listB) {
+// first list pass of refreshHashTable: R[h] = (rank in m's list, m, own entry, key)
+__global__ void kCountMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ hkey, int32_t* __restrict__ degM, int4* __restrict__ R) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int4 e = he[h];
     const int a = e.x, b = heO(he, e.z);
-    const int p = (int)(offM[min(a, b)] + pos[h]);
-    minEntry(listA, listB, p, (int)h, a, b, hkey[h]);
-    pos[h] = p;
+    const int m = min(a, b);
+    R[h] = make_int4(atomicAdd(&degM[m], 1), m, pairEntry(a, b), hkey[h]);
 }
 
-// the halfedges joining the two vertices of h, sorted by bucket key; returns the true count
-__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
-                            const int2* __restrict__ listB, int mem[kMaxBucket]) {
-    const int a = heO(he, h), b = heO(he, heN(he, h));
-    const int m = min(a, b), o = max(a, b);
-    int key[kMaxBucket];
+// second list pass, after the scan of the counts: the entry goes to offM[m] + rank and
+// R[h] becomes (list start, list length, own entry, key)
+__global__ void kPlaceEntries(int64_t H, const int64_t* __restrict__ offM, int4* __restrict__ R, int2* __restrict__ A) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int4 r = R[h];
+    const int64_t lo = offM[r.y];
+    const int len = (int)(offM[r.y + 1] - lo);
+    A[lo + r.x] = make_int2(r.z, r.w);
+    R[h] = make_int4((int)lo, len, r.z, r.w);
+}
+
+// the halfedges joining the two vertices of h, sorted by bucket key; returns the true
+// count.  Member ids are the keys, so this holds between an initialize and the next refresh.
+__device__ int gatherBucket(int h, const int4* __restrict__ R, const int2* __restrict__ A, int mem[kMaxBucket]) {
+    const int4 r = R[h];
+    const int o = entryOther(r.z);
     int n = 0;
-    for (int64_t i = offM[m]; i < offM[m + 1]; ++i) {
-        const int2 e = listA[i];
-        if (entryOther(e) == o) {
-            if (n < kMaxBucket) {
-                mem[n] = listB[i].x;
-                key[n] = e.y;
-            }
+    for (int i = r.x; i < r.x + r.y; ++i) {
+        const int2 e = A[i];
+        if (entryOther(e.x) == o) {
+            if (n < kMaxBucket) mem[n] = e.y;
             ++n;
         }
     }
     const int cnt = n < kMaxBucket ? n : kMaxBucket;
-    for (int i = 1; i < cnt; ++i) {  // insertion sort by key
-        const int x = mem[i], kx = key[i];
+    for (int i = 1; i < cnt; ++i) {  // insertion sort by key (= id)
+        const int x = mem[i];
         int j = i - 1;
-        while (j >= 0 && key[j] > kx) {
+        while (j >= 0 && mem[j] > x) {
             mem[j + 1] = mem[j];
-            key[j + 1] = key[j];
             --j;
         }
         mem[j + 1] = x;
-        key[j + 1] = kx;
     }
     return n;
 }
@@ -198,80 +168,76 @@ __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, in
 
 __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF_NM : (n == 3 ? VF_WEIRD : (n == 1 ? VF_BOUNDARY : 0u)); }
 
-// One thread per halfedge h (at its list index pos[h]), reading only the short list of
-// its smaller endpoint:
+// One thread per halfedge h, reading only the short list of its smaller endpoint:
 //   bucket size and rank of h (rank = position in key order),
-//   connectHalfedgeTwins (HalfedgeMesh.cu:173-291) when setTwins,
+//   connectHalfedgeTwins (HalfedgeMesh.cu:173-291) when setTwins (right after an
+//     initialize, where keys are halfedge ids),
 //   refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
 //     as OR-ed bits on the origin of h (vflags cleared beforehand),
-//   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given).
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
-                         const int2* __restrict__ listB, const int32_t* __restrict__ pos, int setTwins, int32_t* __restrict__ bsize,
-                         int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr,
-                         int32_t* __restrict__ valence) {
+//   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given),
+//   the simplifications' vertex valence when valence is given.
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int4* __restrict__ R, const int2* __restrict__ A, int setTwins,
+                         int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
+                         uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
-    const int64_t i = pos[hh];
-    const int2 ea = listA[i], eb = listB[i];
-    const int o = entryOther(ea), key = ea.y;
-    const int64_t lo = offM[eb.y], hi = offM[eb.y + 1];
-    int n = 0, r = 0;
-    int64_t other = -1;  // a member other than h (the twin when n == 2)
-    for (int64_t j = lo; j < hi; ++j) {
-        const int2 f = listA[j];
-        if (entryOther(f) != o) continue;
+    const int h = (int)hh;
+    const int4 r = R[h];
+    const int o = entryOther(r.z), key = r.w;
+    int n = 0, rk = 0, partner = -1;
+    for (int i = r.x; i < r.x + r.y; ++i) {
+        const int2 f = A[i];
+        if (entryOther(f.x) != o) continue;
         ++n;
-        if (f.y < key) ++r;
-        if (j != i) other = j;
+        if (f.y < key) ++rk;
+        if (f.y != key) partner = f.y;
     }
-    const int h = eb.x;
     bsize[h] = n;
-    brank[h] = r;
+    brank[h] = rk;
+    const int4 e = he[h];
     if (setTwins) {
         int t = -1;
         if (n == 2) {
-            t = listB[other].x;
+            t = partner;
         } else if (n == 3 || n == 4) {
-            // members by rank: the one after h (n == 3) or the pairing of all four
+            // members by rank (their keys are their ids) and whether they start at the list
+            // vertex: the one after h (n == 3), or the pairing of all four
             int byRank[4] = {-1, -1, -1, -1};
-            int orgRank[4] = {-1, -1, -1, -1};
-            for (int64_t j = lo; j < hi; ++j) {
-                const int2 f = listA[j];
-                if (entryOther(f) != o) continue;
+            int dirRank[4] = {0, 0, 0, 0};
+            for (int i = r.x; i < r.x + r.y; ++i) {
+                const int2 f = A[i];
+                if (entryOther(f.x) != o) continue;
                 int rf = 0;
-                for (int64_t k = lo; k < hi; ++k) {
-                    const int2 g = listA[k];
-                    if (entryOther(g) == o && g.y < f.y) ++rf;
+                for (int k = r.x; k < r.x + r.y; ++k) {
+                    const int2 g = A[k];
+                    if (entryOther(g.x) == o && g.y < f.y) ++rf;
                 }
-                const int2 fb = listB[j];
-                byRank[rf] = fb.x;
-                orgRank[rf] = entryOrigin(f, fb);
+                byRank[rf] = f.y;
+                dirRank[rf] = f.x < 0 ? 1 : 0;
             }
             if (n == 3) {
-                t = byRank[(r + 1) % 3];
+                t = byRank[(rk + 1) % 3];
             } else {
-                const int v0 = orgRank[0], v1 = orgRank[1], v2 = orgRank[2];
+                // pair the first with the first of another origin (HalfedgeMesh.cu:253-280)
                 int pr;
-                if (v0 != v1) pr = r ^ 1;       // (0,1) (2,3)
-                else if (v0 != v2) pr = r ^ 2;  // (0,2) (1,3)
-                else pr = 3 - r;                // (0,3) (1,2)
+                if (dirRank[0] != dirRank[1]) pr = rk ^ 1;       // (0,1) (2,3)
+                else if (dirRank[0] != dirRank[2]) pr = rk ^ 2;  // (0,2) (1,3)
+                else pr = 3 - rk;                                // (0,3) (1,2)
                 t = byRank[pr];
             }
         }
         setT(he, h, t);
         if (valence) {
-            // the simplifications' vertex valence: outgoing halfedges, plus incoming boundary ones
-            const int org = entryOrigin(ea, eb);
-            atomicAdd(&valence[org], 1);
-            if (t == -1) atomicAdd(&valence[org == eb.y ? o : eb.y], 1);
+            // outgoing halfedges, plus incoming boundary ones
+            atomicAdd(&valence[e.x], 1);
+            if (t == -1) atomicAdd(&valence[heO(he, e.z)], 1);
         }
     }
     const uint32_t vf = bucketVertexFlag(n);
-    if (vf) atomicOr(&vflags[entryOrigin(ea, eb)], vf);
+    if (vf) atomicOr(&vflags[e.x], vf);
     if (n == 4) {
-        const int f = heF(he, h);
-        atomicOr(&fattr[f], (uint32_t)QA_NM);
-        if (qattr) atomicOr(&qattr[f], (uint32_t)QA_NM);
+        atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+        if (qattr) atomicOr(&qattr[e.y], (uint32_t)QA_NM);
     }
 }
 
@@ -507,8 +473,7 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ listA,
-                          const int2* __restrict__ listB,
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restrict__ R, const int2* __restrict__ A,
                           const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                           const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -516,7 +481,7 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __res
     const int h = (int)hh;
     if (bsize[h] < 4 || brank[h] != 0) return;
     int mem[kMaxBucket];
-    const int n = gatherBucket(h, he, offM, listA, listB, mem);
+    const int n = gatherBucket(h, R, A, mem);
     if (n > kMaxBucket) anomaly(ctr);
     int s[4];
     bucketSlots(mem, n, s);
@@ -1440,17 +1405,16 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 // ---- HalfedgeMesh::initialize / refreshHashTable: CSR of outgoing halfedges and the buckets
 // bucket lists by smaller endpoint (degM holds the counts on entry and is zero on exit),
 // then the per-vertex bucket pass
-// bucket lists by smaller endpoint from the counts in degM and the ranks in pos, then the
-// per-halfedge bucket pass
+// bucket lists by smaller endpoint from the counts in degM and the records of the first
+// list pass, then the per-halfedge bucket pass
 void Pipeline::buildBuckets(bool fromQuads) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
-    int32_t* pos = arena.ptr<int32_t>("bl_pos");
+    int4* R = arena.ptr<int4>("bl_rec");
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
-    int2* listA = arena.get<int2>("listA", (size_t)H);
-    int2* listB = arena.get<int2>("listB", (size_t)H);
+    int2* A = arena.get<int2>("listA", (size_t)H);
     int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
     int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
@@ -1463,9 +1427,8 @@ void Pipeline::buildBuckets(bool fromQuads) {
         valenceReady = true;
     }
     if (H > 0) {
-        if (fromQuads) kFillMinQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, offM, pos, listA, listB);
-        else kFillMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, arena.ptr<int32_t>("hkey"), offM, pos, listA, listB);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, offM, listA, listB, pos, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
+        kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(H, offM, R, A);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, R, A, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
                                                     fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
     }
 }
@@ -1478,7 +1441,7 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
-    int32_t* pos = arena.get<int32_t>("bl_pos", (size_t)H);
+    int4* R = arena.get<int4>("bl_rec", (size_t)H);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
     checkValid("quad vertex ids");
     // the vertex -> halfedge array is derived from these quads when it is copied out
@@ -1486,7 +1449,7 @@ void Pipeline::halfedgeInitialize() {
     vertHeQ = Q;
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
-        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, pos);
+        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, R);
     buildBuckets(true);
 }
 
@@ -1504,9 +1467,9 @@ void Pipeline::refreshHashTable() {
     int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
     if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey, arena.ptr<uint32_t>("fattr"));
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
-    int32_t* pos = arena.get<int32_t>("bl_pos", (size_t)H);
+    int4* R = arena.get<int4>("bl_rec", (size_t)H);
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
-    if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, degM, pos);
+    if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, hkey, degM, R);
     buildBuckets(false);
 }
 
@@ -1592,7 +1555,7 @@ void Pipeline::simplify3333() {
 
 void Pipeline::fixManifoldTwins() {
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listA"), arena.ptr<int2>("listB"),
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("bl_rec"), arena.ptr<int2>("listA"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
                                                  counters());
 }
