@@ -79,8 +79,8 @@ class Pipeline {
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
-    void buildBuckets(bool fromQuads);
-    void refreshHashTable();
+    void buildBuckets(bool fromQuads, const int32_t* list, int64_t nList);
+    void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);
     void checkValid(const char* what);
     void simplifyInit(int32_t** valence, int32_t** count);
     void simplifyCompactAndRebuild(int dropType, bool p3333);

@@ -112,10 +112,13 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
     vert_he[i] = i < n0 ? -1 : 0;
 }
 
-// first list pass of refreshHashTable: R[h] = (rank in m's list, m, own entry, key)
-__global__ void kCountMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ hkey, int32_t* __restrict__ degM, int4* __restrict__ R) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
+// first list pass of refreshHashTable: R[h] = (rank in m's list, m, own entry, key), for
+// every halfedge or for the ones in `list`
+__global__ void kCountMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ list, const int32_t* __restrict__ hkey,
+                          int32_t* __restrict__ degM, int4* __restrict__ R) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= H) return;
+    const int h = list ? list[t] : (int)t;
     const int4 e = he[h];
     const int a = e.x, b = heO(he, e.z);
     const int m = min(a, b);
@@ -124,9 +127,10 @@ __global__ void kCountMin(const int4* __restrict__ he, int64_t H, const int32_t*
 
 // second list pass, after the scan of the counts: the entry goes to offM[m] + rank and
 // R[h] becomes (list start, list length, own entry, key)
-__global__ void kPlaceEntries(int64_t H, const int64_t* __restrict__ offM, int4* __restrict__ R, int2* __restrict__ A) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
+__global__ void kPlaceEntries(int64_t H, const int32_t* __restrict__ list, const int64_t* __restrict__ offM, int4* __restrict__ R, int2* __restrict__ A) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= H) return;
+    const int h = list ? list[t] : (int)t;
     const int4 r = R[h];
     const int64_t lo = offM[r.y];
     const int len = (int)(offM[r.y + 1] - lo);
@@ -176,12 +180,12 @@ __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF
 //     as OR-ed bits on the origin of h (vflags cleared beforehand),
 //   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given),
 //   the simplifications' vertex valence when valence is given.
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int4* __restrict__ R, const int2* __restrict__ A, int setTwins,
-                         int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
-                         uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int32_t* __restrict__ list, const int4* __restrict__ R, const int2* __restrict__ A,
+                         int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
+                         uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
-    const int h = (int)hh;
+    const int h = list ? list[hh] : (int)hh;
     const int4 r = R[h];
     const int o = entryOther(r.z), key = r.w;
     int n = 0, rk = 0, partner = -1;
@@ -233,12 +237,40 @@ __global__ void kBuckets(int4* __restrict__ he, int64_t H, const int4* __restric
             if (t == -1) atomicAdd(&valence[heO(he, e.z)], 1);
         }
     }
+    if (!vflags) return;
     const uint32_t vf = bucketVertexFlag(n);
     if (vf) atomicOr(&vflags[e.x], vf);
     if (n == 4) {
         atomicOr(&fattr[e.y], (uint32_t)QA_NM);
         if (qattr) atomicOr(&qattr[e.y], (uint32_t)QA_NM);
     }
+}
+
+// vertex flags and face NM bits from the bucket sizes of all halfedges (vflags and the
+// faces' NM bits cleared beforehand)
+__global__ void kFlagsFromBuckets(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ bsize, uint32_t* __restrict__ vflags,
+                                  uint32_t* __restrict__ fattr) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int n = bsize[h];
+    const uint32_t vf = bucketVertexFlag(n);
+    if (!vf) return;
+    const int4 e = he[h];
+    atomicOr(&vflags[e.x], vf);
+    if (n == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+}
+
+// refreshHashTable after a split step: the halfedges whose bucket can have changed are
+// those with an endpoint the splitter touched, and the ones it appended
+__global__ void kDirtyHalfedges(const int4* __restrict__ he, int64_t H, int64_t oldH, const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ flag) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    int f = 1;
+    if (h < oldH) {
+        const int4 e = he[h];
+        f = (dirtyV[e.x] || dirtyV[heO(he, e.z)]) ? 1 : 0;
+    }
+    flag[h] = f;
 }
 
 // outgoing-halfedge lists (by origin), for the fan detection
@@ -1086,6 +1118,14 @@ __global__ void kCopyBackFaces(WC w, int count, const int64_t* __restrict__ addr
     fattr[o] = w.f_attr[f];
 }
 
+// the vertices a split step touched (its working copy's, by their ids after the copy back)
+__global__ void kMarkWcVertices(WC w, int count, uint8_t* __restrict__ dirtyV) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    const int o = w.v_c2o[c];
+    if (o >= 0) dirtyV[o] = 1;
+}
+
 __global__ void kUpdateBack(WC w, int count, int4* __restrict__ he) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= count) return;
@@ -1407,16 +1447,17 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 // then the per-vertex bucket pass
 // bucket lists by smaller endpoint from the counts in degM and the records of the first
 // list pass, then the per-halfedge bucket pass
-void Pipeline::buildBuckets(bool fromQuads) {
+void Pipeline::buildBuckets(bool fromQuads, const int32_t* list, int64_t nList) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
     int4* R = arena.ptr<int4>("bl_rec");
+    const int64_t n = list ? nList : H;
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
-    int2* A = arena.get<int2>("listA", (size_t)H);
-    int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
-    int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
+    int2* A = arena.get<int2>("listA", (size_t)std::max<int64_t>(n, 1));
+    int32_t* bsize = arena.ptr<int32_t>("bsize");
+    int32_t* brank = arena.ptr<int32_t>("brank");
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
     zero(vflags, (size_t)V * sizeof(uint32_t));
     int32_t* valence = nullptr;
@@ -1426,11 +1467,12 @@ void Pipeline::buildBuckets(bool fromQuads) {
         zero(valence, (size_t)V * sizeof(int32_t));
         valenceReady = true;
     }
-    if (H > 0) {
-        kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(H, offM, R, A);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, R, A, fromQuads ? 1 : 0, bsize, brank, vflags, arena.ptr<uint32_t>("fattr"),
-                                                    fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
+    if (n > 0) {
+        kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(n, list, offM, R, A);
+        kBuckets<<<gridFor(n), kBlock, 0, stream>>>(he, n, list, R, A, fromQuads ? 1 : 0, bsize, brank, list ? nullptr : vflags,
+                                                    arena.ptr<uint32_t>("fattr"), fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
     }
+    if (list && H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, arena.ptr<uint32_t>("fattr"));
 }
 
 void Pipeline::halfedgeInitialize() {
@@ -1442,6 +1484,8 @@ void Pipeline::halfedgeInitialize() {
     arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
     int4* R = arena.get<int4>("bl_rec", (size_t)H);
+    arena.get<int32_t>("bsize", (size_t)H);
+    arena.get<int32_t>("brank", (size_t)H);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
     checkValid("quad vertex ids");
     // the vertex -> halfedge array is derived from these quads when it is copied out
@@ -1450,7 +1494,7 @@ void Pipeline::halfedgeInitialize() {
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
         kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, R);
-    buildBuckets(true);
+    buildBuckets(true, nullptr, 0);
 }
 
 void Pipeline::checkValid(const char* what) {
@@ -1458,7 +1502,11 @@ void Pipeline::checkValid(const char* what) {
     if (bad) throw HipError(std::string("inconsistent mesh after ") + what + " (" + std::to_string(bad) + " bad ids)", -4);
 }
 
-void Pipeline::refreshHashTable() {
+// refreshHashTable (HalfedgeMesh.cu:1063-1100).  With dirtyV (vertices the splitter
+// touched) only the halfedges whose bucket can have changed are re-bucketed; the others
+// keep their sizes and ranks (same members, same keys), and the flags are rebuilt from
+// all sizes.
+void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
     const int64_t n = std::max(H, F);
@@ -1468,9 +1516,22 @@ void Pipeline::refreshHashTable() {
     if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey, arena.ptr<uint32_t>("fattr"));
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
     int4* R = arena.get<int4>("bl_rec", (size_t)H);
+    arena.grow<int32_t>("bsize", (size_t)H, (size_t)std::min(oldH, H), stream);
+    arena.grow<int32_t>("brank", (size_t)H, (size_t)std::min(oldH, H), stream);
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
-    if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, hkey, degM, R);
-    buildBuckets(false);
+    if (!dirtyV) {
+        if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, nullptr, hkey, degM, R);
+        buildBuckets(false, nullptr, 0);
+        return;
+    }
+    int32_t* flag = arena.get<int32_t>("dirty_flag", (size_t)std::max<int64_t>(H, 1));
+    if (H > 0) kDirtyHalfedges<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, dirtyV, flag);
+    int64_t* addr = nullptr;
+    const int64_t nd = scan32(flag, H, "dirty_addr", &addr);
+    int32_t* list = arena.get<int32_t>("dirty_list", (size_t)std::max<int64_t>(nd, 1));
+    if (H > 0) kScatterIndex<<<gridFor(H), kBlock, 0, stream>>>(flag, addr, H, list);
+    if (nd > 0) kCountMin<<<gridFor(nd), kBlock, 0, stream>>>(he, nd, list, hkey, degM, R);
+    buildBuckets(false, list, nd);
 }
 
 // ---- simplifications
@@ -1569,6 +1630,7 @@ void Pipeline::split(int step, int maxIters) {
             extraAnomalies += 1;
             break;
         }
+        const int64_t iterH = H;
         // countNonManifoldFaces
         int32_t* pred = arena.get<int32_t>("nmpred", (size_t)F);
         if (F > 0) kNMFacePred<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, pred);
@@ -1699,7 +1761,10 @@ void Pipeline::split(int step, int maxIters) {
             H = H0 + addH;
             F = F0 + addF;
         }
-        refreshHashTable();
+        uint8_t* dirtyV = arena.get<uint8_t>("dirty_v", (size_t)std::max<int64_t>(V, 1));
+        zero(dirtyV, (size_t)std::max<int64_t>(V, 1));
+        if (vsize > 0) kMarkWcVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, dirtyV);
+        refreshHashTable(dirtyV, iterH);
         ++iter;
         splitIterations = iter;
         if (!(step >= 6) || !copyDataBack) break;
