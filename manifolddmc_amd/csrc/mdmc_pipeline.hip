@@ -273,23 +273,64 @@ __global__ void kDirtyHalfedges(const int4* __restrict__ he, int64_t H, int64_t 
     flag[h] = f;
 }
 
-// outgoing-halfedge lists (by origin), for the fan detection
-__global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ deg, int64_t* ctr) {
+// outgoing-halfedge lists (by origin) for the fan detection, of the vertices in `only`
+// (all when null)
+__global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, const uint8_t* __restrict__ only, int32_t* __restrict__ deg,
+                             int64_t* ctr) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int o = heO(he, (int)h);
     if (o < 0 || o >= V) { anomaly(ctr); return; }
+    if (only && !only[o]) return;
     atomicAdd(&deg[o], 1);
 }
 
-__global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const int64_t* __restrict__ off, int32_t* __restrict__ deg,
-                           int32_t* __restrict__ list) {
+__global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const uint8_t* __restrict__ only, const int64_t* __restrict__ off,
+                           int32_t* __restrict__ deg, int32_t* __restrict__ list) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int o = heO(he, (int)h);
     if (o < 0 || o >= V) return;
+    if (only && !only[o]) return;
     const int slot = atomicSub(&deg[o], 1) - 1;
     list[off[o] + slot] = (int32_t)h;
+}
+
+// out-degree of every vertex and one of its outgoing halfedges
+__global__ void kOutDegree(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ outdeg, int32_t* __restrict__ rep) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) return;  // counted as an anomaly by kCountOrigin
+    if (atomicAdd(&outdeg[o], 1) == 0) rep[o] = (int32_t)h;
+}
+
+// A vertex whose outgoing halfedges form one closed clockwise cycle (next(twin(h)) from
+// any of them visits all and returns) has a single fan: every fan walk of fanWalk reaches
+// the same minimum.  The others are decided by kDetectFans over their outgoing lists.
+__global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
+                         uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow) {
+    const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vv >= V) return;
+    const int v = (int)vv;
+    const int d = outdeg[v];
+    bool single = d == 0;  // no outgoing halfedge: no fan, flag cleared
+    if (d > 0) {
+        const int start = rep[v];
+        int cur = start;
+        for (int k = 1; k <= d; ++k) {
+            const int t = heT(he, cur);
+            if (t < 0) break;
+            cur = heN(he, t);
+            if (heO(he, cur) != v) break;
+            if (cur == start) {
+                single = k == d;
+                break;
+            }
+        }
+    }
+    slow[v] = single ? 0 : 1;
+    if (single) vflags[v] &= ~(uint32_t)VF_FANS;
 }
 
 // ---------------------------------------------------------------- simplifications
@@ -1176,7 +1217,7 @@ __device__ int fanWalk(const int4* __restrict__ he, int start, bool cw, int64_t*
 // Up to kFanLocal halfedges the walks run on successor indices in LDS: a step to a
 // halfedge that does not start at v ends the walk, as the origin test of fanWalk does.
 constexpr int kFanLocal = 16;
-__global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ he, int64_t V, const int64_t* __restrict__ off,
+__global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ he, int64_t V, const uint8_t* __restrict__ only, const int64_t* __restrict__ off,
                                                       const int32_t* __restrict__ list, uint32_t* __restrict__ vflags,
                                                       int32_t* __restrict__ keyScratch, int64_t* ctr) {
     __shared__ int s_h[kFanLocal * kBlock];
@@ -1184,6 +1225,7 @@ __global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ h
     __shared__ int8_t s_ccw[kFanLocal * kBlock];
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vv >= V) return;
+    if (only && !only[vv]) return;
     const int tid = threadIdx.x;
     const int64_t b0 = off[vv];
     const int d = (int)(off[vv + 1] - b0);
@@ -1787,16 +1829,25 @@ void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* nu
 void Pipeline::detectNonManifoldVertices() {
     if (H == 0 || V == 0) return;
     int4* he = arena.ptr<int4>("he");
+    uint32_t* vflags = arena.ptr<uint32_t>("vflags");
+    // fast path: vertices with one closed fan
+    int32_t* outdeg = arena.get<int32_t>("outdeg", (size_t)V);
+    int32_t* rep = arena.get<int32_t>("outrep", (size_t)V);
+    uint8_t* slow = arena.get<uint8_t>("fan_slow", (size_t)V);
+    zero(outdeg, (size_t)V * sizeof(int32_t));
+    kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep);
+    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow);
+    // the others over their outgoing lists
     int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
     zero(deg, ((size_t)V + 1) * sizeof(int32_t));
-    kCountOrigin<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, deg, counters());
+    kCountOrigin<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, deg, counters());
     int64_t* off = arena.get<int64_t>("off", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(deg, off, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
     int32_t* list = arena.get<int32_t>("list", (size_t)H);
-    kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, off, deg, list);
+    kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, off, deg, list);
     int32_t* key = arena.get<int32_t>("fankey", (size_t)H);
-    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, off, list, arena.ptr<uint32_t>("vflags"), key, counters());
+    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, slow, off, list, vflags, key, counters());
 }
 
 void Pipeline::computeStats(int64_t out[9]) {
