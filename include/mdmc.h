@@ -109,6 +109,25 @@ const char* mdmc_last_error(const mdmc_ctx* ctx);
 /* Run all following work on this stream (hipStream_t passed as void*); NULL = own stream. */
 mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* hip_stream);
 
+/* Tile shape of the grid pass (no reference counterpart; the reference launches one thread
+ * per grid point, DualMarchingCubes.cu:38-67).  MDMC_TILE_AUTO (the default): on the first
+ * extraction of a workload class (cell count and problem count to the next power of two,
+ * move-to-surface on/off) the context times every shape on that input and keeps the
+ * fastest for the class.  Any other value forces that shape.  Every shape gives the same
+ * mesh.  There is no hash table to tune: edges and halfedges are matched through exact-size
+ * lists, not open addressing. */
+#define MDMC_TILE_AUTO 0
+#define MDMC_TILE_1024 1          /* 1024 cells per tile, tiles dealt to XCDs in contiguous runs */
+#define MDMC_TILE_2048 2
+#define MDMC_TILE_4096 3
+#define MDMC_TILE_1024_LINEAR 4   /* the same tiles in launch order */
+#define MDMC_TILE_2048_LINEAR 5
+#define MDMC_TILE_4096_LINEAR 6
+mdmc_status mdmc_set_tile_shape(mdmc_ctx* ctx, int32_t shape);
+/* The shape the last extraction used; tune_ms (may be NULL, double[6]) receives the grid-pass
+ * times of shapes 1..6 from this context's last tuning (0 where none ran). */
+mdmc_status mdmc_get_tile_shape(const mdmc_ctx* ctx, int32_t* last_shape, double* tune_ms);
+
 /* Replaces DualMarchingCubes::run(iso, mesh, grid) + runPostProcesses
  * (DualMarchingCubes.cu:220-289, :335-380).  The mesh stays on the device in the
  * context until the next call.  stats may be NULL.  Synchronises the stream. */

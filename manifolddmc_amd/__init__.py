@@ -40,6 +40,11 @@ class SplittingStep:
     ALL = 100
 
 
+# grid-pass tile shapes, include/mdmc.h (TILE_AUTO: autotuned per workload class)
+TILE_AUTO = 0
+TILE_SHAPES = {1: "1024 cells, XCD-chunked", 2: "2048 cells, XCD-chunked", 3: "4096 cells, XCD-chunked",
+               4: "1024 cells, launch order", 5: "2048 cells, launch order", 6: "4096 cells, launch order"}
+
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "CAPACITY", 4: "HASH_FULL", 5: "HIP", 6: "NO_RESULT", 7: "INTERNAL"}
 
 
@@ -86,7 +91,7 @@ _lib = None
 
 EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", "mdmc_extract",
            "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_backward",
-           "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats"]
+           "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape"]
 
 
 def lib():
@@ -122,6 +127,10 @@ def lib():
         L.mdmc_batch_offsets.restype = i32
         L.mdmc_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.mdmc_batch_stats.restype = i32
+        L.mdmc_set_tile_shape.argtypes = [vp, i32]
+        L.mdmc_set_tile_shape.restype = i32
+        L.mdmc_get_tile_shape.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
+        L.mdmc_get_tile_shape.restype = i32
         _lib = L
     return _lib
 
@@ -162,12 +171,28 @@ class HalfedgeMesh:
                 f.write("f %d//%d %d//%d %d//%d %d//%d\n" % (q[0], q[0], q[1], q[1], q[2], q[2], q[3], q[3]))
 
 
+def _member(name):
+    """Property under one of the reference's member names that reads and writes
+    the snake_case attribute `name`."""
+    return property(lambda self: getattr(self, name), lambda self, v: setattr(self, name, v))
+
+
 class DualMarchingCubes:
     """MI355X counterpart of DMC::DualMarchingCubes (DualMarchingCubes.h:68-186).
 
-    The option attributes carry the reference's member names and defaults
-    (DualMarchingCubes.cu:110-133); the grid flags of UniformGridHost::readFromFile
+    The options have the reference's defaults (DualMarchingCubes.cu:110-133) and
+    answer to both a snake_case name and the reference's member name
+    (m_simplifyP3x3yColor, ...); the grid flags of UniformGridHost::readFromFile
     (refinement, add_zero_boundary) are passed with the grid."""
+
+    # the reference's member names, DualMarchingCubes.h:168-185
+    m_enableSurfaceToVertexIteration = _member("enable_surface_to_vertex_iteration")
+    m_simplifyP3x3yColor = _member("simplify_p3x3y_color")
+    m_simplifyP3x3yOld = _member("simplify_p3x3y_old")
+    m_simplifyP3333 = _member("simplify_p3333")
+    m_splitNonManifoldHalfedges = _member("split_non_manifold_halfedges")
+    m_detectNonManifoldVertices = _member("detect_non_manifold_vertices")
+    m_programmaticSplittingStep = _member("splitting_step")
 
     def __init__(self, device=-1):
         self.enable_surface_to_vertex_iteration = True
@@ -211,6 +236,17 @@ class DualMarchingCubes:
         if self.detect_non_manifold_vertices:
             f |= DETECT_NM_VERTS
         return f
+
+    def set_tile_shape(self, shape):
+        """mdmc_set_tile_shape: TILE_AUTO or one of TILE_SHAPES."""
+        self._check(lib().mdmc_set_tile_shape(self._ctx, int(shape)))
+
+    def tile_shape(self):
+        """(shape of the last extraction, {shape: grid-pass ms of this context's last tuning})."""
+        sh = ctypes.c_int32()
+        ms = (ctypes.c_double * 6)()
+        self._check(lib().mdmc_get_tile_shape(self._ctx, ctypes.byref(sh), ms))
+        return sh.value, {k: ms[k - 1] for k in TILE_SHAPES}
 
     def set_stream(self, stream_ptr):
         st = lib().mdmc_set_stream(self._ctx, ctypes.c_void_p(stream_ptr))

@@ -119,6 +119,24 @@ mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* s) {
     return MDMC_OK;
 }
 
+static_assert(MDMC_TILE_AUTO == mdmc::GP_SHAPE_AUTO && MDMC_TILE_1024 == mdmc::GP_SHAPE_FIRST && MDMC_TILE_4096_LINEAR == mdmc::GP_SHAPE_LAST,
+              "tile shape numbering");
+
+mdmc_status mdmc_set_tile_shape(mdmc_ctx* ctx, int32_t shape) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    if (shape < MDMC_TILE_AUTO || shape > MDMC_TILE_4096_LINEAR) return fail(ctx, MDMC_ERR_INVALID_ARG, "unknown tile shape");
+    ctx->p.tileShape = shape;
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_get_tile_shape(const mdmc_ctx* ctx, int32_t* last_shape, double* tune_ms) {
+    if (!ctx || !last_shape) return MDMC_ERR_INVALID_ARG;
+    *last_shape = ctx->p.lastShape;
+    if (tune_ms)
+        for (int i = MDMC_TILE_1024; i <= MDMC_TILE_4096_LINEAR; ++i) tune_ms[i - MDMC_TILE_1024] = ctx->p.tuneMs[i];
+    return MDMC_OK;
+}
+
 }  // extern "C"
 
 namespace {

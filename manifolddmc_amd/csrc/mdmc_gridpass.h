@@ -1,4 +1,4 @@
-// mdmc_gridpass.h -- launch interface of the single-kernel DMC grid pass.
+// mdmc_gridpass.h -- launch interface of the DMC grid pass (classify, scan, emit).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,7 +8,13 @@
 
 namespace mdmc {
 
-enum : int { GP_ERR_OVERFLOW = 1, GP_ERR_TIMEOUT = 2, GP_ERR_RANGE = 4 };
+// Tile shapes of the grid pass: cells per tile and the block -> tile order.  Every shape
+// gives the same mesh; the autotuner picks the fastest per workload class.
+//   1..3: 1024 / 2048 / 4096 cells per tile, tiles dealt to the 8 XCDs in contiguous chunks
+//   4..6: the same tiles in launch order (block b -> tile b)
+enum : int { GP_SHAPE_AUTO = 0, GP_SHAPE_FIRST = 1, GP_SHAPE_LAST = 6, GP_SHAPE_DEFAULT = 2 };
+int gridPassTile(int shape);
+const char* gridPassShapeName(int shape);
 
 // One (grid, iso-value) problem of an extraction.  A batch concatenates the problems'
 // cells (cellBase) and tiles (tileBase) in order; a single extraction is a batch of one.
@@ -25,22 +31,24 @@ struct GridPassArgs {
     const ProblemDev* probs;
     int nProblems;
     int surfIter;
-    int64_t totalCells;
-    int64_t reach;  // max over problems of CX * CY + CX: how far back a quad's cells lie
     int64_t numTiles;
-    unsigned long long* status;  // per tile: flag | quads | vertices (look-back)
-    int* done;                   // per tile: finished reading the ring
-    unsigned int* ticket;        // tile claim counter
-    unsigned long long* ring;    // tag | edge->contour map | first vertex id of cell c at c & (R-1)
-    int64_t R;                   // power of two, 1 << ringShift
-    int ringShift;
+    // classify -> emit: per tile its active cells' polygon word and (cell | owned edges << 12),
+    // at t * tile + active index; per tile its vertex / quad / active-cell counts
+    uint64_t* recPoly;
+    uint16_t* recInfo;
+    int32_t* tileV;   // [numTiles + 1], last entry 0 (scanned in place of a total)
+    int32_t* tileQ;   // [numTiles + 1]
+    int32_t* tileNA;  // [numTiles]
+    // per active cell (concatenated cell index): vertex index inside its tile (14 bits) and the
+    // contour of each edge a later cell's quads read (2 bits each for edges 1,2,4,5,6,7,9,10,11)
+    uint32_t* cellMap;
+    // exclusive prefix of tileV / tileQ: each tile's first vertex / quad
+    const int64_t* tileVOff;
+    const int64_t* tileQOff;
     float* pos;
     float* nrm;
-    int64_t capV;
     int4* quads;
     uint32_t* qattr;
-    int64_t capQ;
-    int* err;
     // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1,
     // per vertex its cell (single-problem extractions only)
     int32_t* adjMap;
@@ -49,10 +57,7 @@ struct GridPassArgs {
     int32_t* vpid;
 };
 
-int gridPassTile();
-// ring size (power of two) for totalCells cells whose quads reach back `reach` cells
-void gridPassRing(int64_t totalCells, int64_t reach, int64_t* R, int* shift);
-void launchGridPass(const GridPassArgs& a, hipStream_t s);
-void launchGridPassTotals(const unsigned long long* status, int64_t numTiles, int64_t* out, hipStream_t s);
+void launchGridClassify(int shape, const GridPassArgs& a, hipStream_t s);
+void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s);
 
 }  // namespace mdmc

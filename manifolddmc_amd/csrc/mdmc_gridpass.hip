@@ -1,4 +1,4 @@
-// mdmc_gridpass.hip -- the DMC grid pass in one kernel.
+// mdmc_gridpass.hip -- the DMC grid pass: classify, scan, emit.
 //
 // Restates (MarchingCubes/VisualDMC/src/DMC):
 //   performDualMarchingCubes        DualMarchingCubes.cu:38-67
@@ -7,29 +7,26 @@
 //   QuadrilateralDeviceHashTable / mapQuadrilaterals   DeviceHashTables.h:288-348, DualMarchingCubes.cu:72-85
 //
 // The reference appends vertices with one global atomic counter and scatters
-// their ids into a 64-bit keyed quad hash that a second kernel compacts.  Here
-// one kernel walks the cells in linear order in tiles of 2048:
-//   1. find the tile's active cells (MC case neither 0 nor 255) and compact
-//      them, in cell order, into LDS; then compute polygon and owned edges of
-//      the active cells only, with every lane busy;
-//   2. scan the per-cell vertex and quad counts over the active cells;
-//   3. get the tile's global vertex / quad offsets by decoupled look-back over
-//      the tiles before it (tiles are claimed with an atomic ticket, so every
-//      predecessor is already running and the wait always ends);
-//   4. publish each active cell's first vertex id and edge->contour map in a
-//      ring buffer that later tiles read for the quads they own;
-//   5. compute the tile's dual vertices, one vertex per thread;
-//   6. write the quads whose edge starts at one of the tile's cells, one per
-//      thread, taking the other three vertex ids from LDS or from the ring.
-// Vertices come out ordered by (cell, contour) and quads by edge key
-// 3*point+axis, with no hash table and no global append counter.
-//
-// Cross-tile communication uses relaxed device-scope atomics only: every word
-// another tile reads carries its own validity (the look-back status packs flag
-// and both counts; a ring entry carries a tag naming the cell it holds), so no
-// release/acquire fence -- which on this chip writes back or invalidates whole
-// caches -- is needed.  Every wait is bounded; a timeout raises an error flag
-// instead of hanging.
+// their ids into a 64-bit keyed quad hash that a second kernel compacts.  Here the
+// cells are cut into tiles of consecutive cells (1024, 2048 or 4096) and three
+// launches with no waits between blocks build the same arrays:
+//   classify  per tile: MC case of every cell, a wave ballot per 64 cells and a
+//             block scan compact the active cells (case neither 0 nor 255) in
+//             cell order; their polygon and owned edges are computed with every
+//             lane busy and stored per tile, with the tile's vertex / quad counts;
+//             each active cell also gets a 32-bit map entry (its vertex index in
+//             the tile, the contour of the edges later cells read);
+//   scan      exclusive prefix of the tile counts: every tile's first vertex and
+//             quad id (the only host read-back: the exact output sizes);
+//   emit      per tile: dual vertices (edge mean, move-to-surface, gradient
+//             normal), one per thread, and the quads whose edge starts at one of
+//             the tile's cells, taking the other three vertex ids from LDS or from
+//             the map entries of earlier tiles.
+// Vertices come out ordered by (cell, contour) and quads by edge key 3*point+axis,
+// with no hash table and no global append counter.  Blocks take tiles either in
+// launch order or XCD-chunked (block b runs on XCD b % 8; each XCD gets one
+// contiguous run of tiles, so the rows and slices a tile shares with its
+// neighbours stay in that XCD's L2).
 #include <algorithm>
 
 #include "mdmc_device.h"
@@ -40,76 +37,36 @@ namespace mdmc {
 namespace {
 
 constexpr int GP_THREADS = 256;
-constexpr int GP_CPT = 8;
-constexpr int GP_TILE = GP_THREADS * GP_CPT;
 constexpr int GP_WAVES = GP_THREADS / 64;
-constexpr int GP_RUNS = GP_TILE / 64;  // runs of 64 consecutive cells, one ballot each
-constexpr int kSpinLimit = 1 << 22;
 
-constexpr uint64_t FLAG_AGG = 1ULL, FLAG_INC = 2ULL;
-__device__ __forceinline__ uint64_t packStatus(uint64_t flag, uint64_t v, uint64_t q) { return (flag << 62) | (q << 31) | v; }
-__device__ __forceinline__ uint64_t stFlag(uint64_t s) { return s >> 62; }
-__device__ __forceinline__ uint64_t stV(uint64_t s) { return s & 0x7FFFFFFFULL; }
-__device__ __forceinline__ uint64_t stQ(uint64_t s) { return (s >> 31) & 0x7FFFFFFFULL; }
-
-__device__ __forceinline__ uint64_t loadRelaxed(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void storeRelaxed(unsigned long long* p, uint64_t v) {
-    __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int loadRelaxedI(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void storeRelaxedI(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
-// bounded wait for a flag set by an earlier tile
-__device__ __forceinline__ void waitFlag(const int* flag, int* err) {
-    for (int it = 0; it < kSpinLimit; ++it) {
-        if (loadRelaxedI(flag)) return;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    atomicOr(err, GP_ERR_TIMEOUT);
-}
-
-// Ring entry of cell c: bits 0-30 first vertex id, 31-54 the contour (2 bits) of each of
-// the 12 edges, 55-63 tag = (c >> ringShift) + 1.  The ring is cleared before the pass and
-// holds fewer than 511 generations, so a tag never matches an older or missing entry.
-__device__ __forceinline__ uint64_t ringTag(int64_t c, int shift) { return (uint64_t)((c >> shift) + 1) & 0x1FFULL; }
-
-// the problem that holds global tile t, or global cell c (binary search over the bases)
-__device__ __forceinline__ int problemOfTile(const GridPassArgs& a, int64_t t) {
-    int lo = 0, hi = a.nProblems;  // probs[lo].tileBase <= t
+// the problem that holds global tile t (binary search over the tile bases)
+__device__ __forceinline__ int problemOfTile(const ProblemDev* probs, int n, int64_t t) {
+    int lo = 0, hi = n;  // probs[lo].tileBase <= t
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (a.probs[mid].tileBase <= t) lo = mid;
+        if (probs[mid].tileBase <= t) lo = mid;
         else hi = mid;
     }
     return lo;
 }
-__device__ __forceinline__ int64_t tileOfCell(const GridPassArgs& a, int64_t c) {
-    int lo = 0, hi = a.nProblems;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (a.probs[mid].cellBase <= c) lo = mid;
-        else hi = mid;
-    }
-    const ProblemDev& p = a.probs[lo];
-    const int64_t local = c - p.cellBase;
-    return p.tileBase + (local < p.cells ? local : p.cells - 1) / GP_TILE;
+
+// block b -> tile: launch order, or XCD-chunked (blocks b, b + 8, b + 16, ... run on one XCD
+// and take one contiguous run of numTiles / 8 tiles, the first numTiles % 8 runs one longer)
+__device__ __forceinline__ int64_t tileOfBlock(int64_t b, int64_t numTiles, bool chunked) {
+    if (!chunked) return b;
+    const int64_t x = b & 7, i = b >> 3, q = numTiles >> 3, r = numTiles & 7;
+    return x * q + (x < r ? x : r) + i;
 }
 
-__device__ __forceinline__ int ringVertex(const GridPassArgs& a, int64_t n, int e) {
-    const uint64_t tag = ringTag(n, a.ringShift);
-    const unsigned long long* p = a.ring + (n & (a.R - 1));
-    uint64_t v = loadRelaxed(p);
-    for (int it = 0; (v >> 55) != tag; ++it) {
-        if (it >= kSpinLimit) {
-            atomicOr(a.err, GP_ERR_TIMEOUT);
-            return -1;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        v = loadRelaxed(p);
-    }
-    return (int)(v & 0x7FFFFFFFULL) + (int)((v >> (31 + 2 * e)) & 3ULL);
+// map entry: bits 0-13 vertex index in the tile, then 2 bits per edge 1,2,4,5,6,7,9,10,11
+__device__ __forceinline__ uint32_t mapEntry(int vlocal, uint32_t cmap) {
+    const uint32_t nb = ((cmap >> 2) & 0xFu) | (((cmap >> 8) & 0xFFu) << 4) | (((cmap >> 18) & 0x3Fu) << 12);
+    return (uint32_t)vlocal | (nb << 14);
+}
+// slot of edge e in the map entry (edges 0, 3, 8 are never read from a neighbour)
+__device__ __forceinline__ int mapContour(uint32_t m, int e) {
+    const int k = (int)((0x876054320100ULL >> (4 * e)) & 0xF);
+    return (int)((m >> (14 + 2 * k)) & 3u);
 }
 
 __device__ __forceinline__ void loadCorners(const GridDev& g, int x, int y, int z, float s[8]) {
@@ -198,7 +155,7 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
     wp = mk3(g.ox, g.oy, g.oz) + (mk3((float)x, (float)y, (float)z) + p) * mk3(g.dx, g.dy, g.dz);
 }
 
-// coordinates of cell c0 + l from those of c0 (x0, y0, z0) and l < GP_TILE, in 32-bit arithmetic
+// coordinates of cell c0 + l from those of c0 (x0, y0, z0) and l < tile size, in 32-bit arithmetic
 struct TileBase {
     int x0, y0, z0;
 };
@@ -212,58 +169,105 @@ __device__ __forceinline__ void cellXYZ(const TileBase& b, int l, int CX, int CY
     z = b.z0 + (int)zz;
 }
 
-__global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
-    // per active cell (index in cell order): polygon, owned-edge bits, tile cell, scans
-    __shared__ uint64_t s_poly[GP_TILE];
-    __shared__ uint16_t s_cell[GP_TILE];
-    __shared__ uint16_t s_vpre[GP_TILE];
-    __shared__ uint16_t s_qpre[GP_TILE];  // also holds the MC case of each tile cell in phase 1
-    __shared__ uint8_t s_own[GP_TILE];
-    // per tile cell: its active index (valid for active cells)
-    __shared__ uint16_t s_aidx[GP_TILE];
-    __shared__ unsigned long long s_mask[GP_RUNS];
-    __shared__ int s_mpre[GP_RUNS + 1];
+// Block scan of the per-active-cell counts (vertices | quads << 16); thread tid owns active
+// indices CPT*tid .. CPT*tid + CPT - 1.  Writes the exclusive prefixes (s_qpre may be null)
+// and returns the tile's totals.  A tile has at most 4*4096 vertices and 3*4096 quads, so
+// both fit 16 bits and the packed sum never carries.
+template <int CPT>
+__device__ __forceinline__ void scanCounts(const uint64_t* s_poly, const uint8_t* s_own, int NA, uint16_t* s_vpre, uint16_t* s_qpre, int* s_wave,
+                                           int& VT, int& QT) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int cnt = 0;
+    int nvq[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int ai = tid * CPT + j;
+        nvq[j] = 0;
+        if (ai < NA) nvq[j] = polyCount(s_poly[ai]) | (__popc(s_own[ai] & 7) << 16);
+        cnt += nvq[j];
+    }
+    int x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    int add = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < GP_WAVES; ++q) {
+        const int v = s_wave[q];
+        if (q < wid) add += v;
+        tot += v;
+    }
+    int run = x - cnt + add;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int ai = tid * CPT + j;
+        if (ai < NA) {
+            s_vpre[ai] = (uint16_t)(run & 0xFFFF);
+            if (s_qpre) s_qpre[ai] = (uint16_t)(run >> 16);
+        }
+        run += nvq[j];
+    }
+    VT = tot & 0xFFFF;
+    QT = tot >> 16;
+}
+
+struct TileCtx {
+    int64_t t;
+    int prob;
+    int64_t c0;   // first cell of the tile inside its problem
+    int64_t gc0;  // ... and in the concatenation
+    TileBase tb;
+};
+
+template <int TILE>
+__device__ __forceinline__ TileCtx tileCtx(const GridPassArgs& a, int chunked) {
+    TileCtx c;
+    c.t = tileOfBlock(blockIdx.x, a.numTiles, chunked != 0);
+    c.prob = problemOfTile(a.probs, a.nProblems, c.t);
+    const ProblemDev& P = a.probs[c.prob];
+    c.c0 = (c.t - P.tileBase) * TILE;
+    c.gc0 = P.cellBase + c.c0;
+    const int64_t r0 = c.c0 / P.CX;
+    c.tb.x0 = (int)(c.c0 - r0 * P.CX);
+    c.tb.y0 = (int)(r0 % P.CY);
+    c.tb.z0 = (int)(r0 / P.CY);
+    return c;
+}
+
+// ---- classify: active cells of one tile, their polygons, counts and map entries
+template <int CPT>
+__global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chunked) {
+    constexpr int TILE = GP_THREADS * CPT, RUNS = TILE / 64;
+    __shared__ uint64_t s_poly[TILE];
+    __shared__ uint16_t s_cell[TILE];
+    __shared__ uint16_t s_vpre[TILE];  // holds the MC case of each tile cell (bytes) until the scan
+    __shared__ uint8_t s_own[TILE];
+    __shared__ unsigned long long s_mask[RUNS];
+    __shared__ int s_mpre[RUNS + 1];
     __shared__ int s_wave[GP_WAVES];
-    __shared__ long long s_tile;
-    __shared__ int s_prob;
-    __shared__ long long s_voff, s_qoff;
-    uint8_t* const s_mc = reinterpret_cast<uint8_t*>(s_qpre);
+    uint8_t* const s_mc = reinterpret_cast<uint8_t*>(s_vpre);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) {
-        const long long tk = (long long)atomicAdd(a.ticket, 1u);
-        s_tile = tk;
-        s_prob = tk < a.numTiles ? problemOfTile(a, tk) : 0;
-    }
-    __syncthreads();
-    const int64_t t = s_tile;
-    if (t >= a.numTiles) return;  // uniform across the block
-    const int prob = s_prob;
-    const ProblemDev& P = a.probs[prob];
+    const TileCtx tc = tileCtx<TILE>(a, chunked);
+    const ProblemDev& P = a.probs[tc.prob];
     const GridDev g = P.g;
     const float iso = P.iso;
     const int CX = P.CX, CY = P.CY;
     const int64_t C = P.cells;
-    const int64_t c0 = (t - P.tileBase) * GP_TILE;  // first cell of the tile inside its problem
-    const int64_t gc0 = P.cellBase + c0;             // ... and in the concatenation
 
-    TileBase tb;
-    {
-        const int64_t r0 = c0 / CX;
-        tb.x0 = (int)(c0 - r0 * CX);
-        tb.y0 = (int)(r0 % CY);
-        tb.z0 = (int)(r0 / CY);
-    }
-
-    // ---- 1a. MC case of every cell; a ballot per run of 64 cells marks the active ones
+    // MC case of every cell; a ballot per run of 64 cells marks the active ones
 #pragma unroll
-    for (int k = 0; k < GP_CPT; ++k) {
+    for (int k = 0; k < CPT; ++k) {
         const int l = k * GP_THREADS + tid;
         int mc = 0;
-        if (c0 + l < C) {
+        if (tc.c0 + l < C) {
             int x, y, z;
-            cellXYZ(tb, l, CX, CY, x, y, z);
+            cellXYZ(tc.tb, l, CX, CY, x, y, z);
             float s[8];
             loadCorners(g, x, y, z, s);
 #pragma unroll
@@ -275,201 +279,134 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
     }
     __syncthreads();
     if (tid < 64) {
-        const int v = tid < GP_RUNS ? __popcll(s_mask[tid]) : 0;
+        const int v = tid < RUNS ? __popcll(s_mask[tid]) : 0;
         int x = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const int y = __shfl_up(x, d, 64);
             if (tid >= d) x += y;
         }
-        if (tid < GP_RUNS) s_mpre[tid] = x - v;
-        if (tid == GP_RUNS - 1) s_mpre[GP_RUNS] = x;
+        if (tid < RUNS) s_mpre[tid] = x - v;
+        if (tid == RUNS - 1) s_mpre[RUNS] = x;
     }
     __syncthreads();
-    const int NA = s_mpre[GP_RUNS];
+    const int NA = s_mpre[RUNS];
 #pragma unroll
-    for (int k = 0; k < GP_CPT; ++k) {
+    for (int k = 0; k < CPT; ++k) {
         const int run = k * GP_WAVES + wid;
         const uint64_t m = s_mask[run];
-        if ((m >> lane) & 1ULL) {
-            const int ai = s_mpre[run] + __popcll(m & ((1ULL << lane) - 1ULL));
-            const int l = k * GP_THREADS + tid;
-            s_cell[ai] = (uint16_t)l;
-            s_aidx[l] = (uint16_t)ai;
-        }
+        if ((m >> lane) & 1ULL) s_cell[s_mpre[run] + __popcll(m & ((1ULL << lane) - 1ULL))] = (uint16_t)(k * GP_THREADS + tid);
     }
     __syncthreads();
 
-    // ---- 1b. polygon and owned edges of the active cells
+    // polygon and owned edges of the active cells, every lane busy
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
         const int l = s_cell[ai];
         int x, y, z;
-        cellXYZ(tb, l, CX, CY, x, y, z);
+        cellXYZ(tc.tb, l, CX, CY, x, y, z);
         float s[8];
         loadCorners(g, x, y, z, s);
         s_poly[ai] = cellPolygon(s_mc[l], s, iso);
         s_own[ai] = (uint8_t)ownedBits(s, iso, x, y, z);
     }
     __syncthreads();
+    int VT, QT;
+    scanCounts<CPT>(s_poly, s_own, NA, s_vpre, nullptr, s_wave, VT, QT);
+    __syncthreads();
 
-    // ---- 2. prefix of vertex / quad counts over the active cells (thread tid owns 8*tid .. 8*tid+7)
-    int cnt = 0, VT = 0, QT = 0;
-    int nvq[GP_CPT];
-#pragma unroll
-    for (int j = 0; j < GP_CPT; ++j) {
-        const int ai = tid * GP_CPT + j;
-        nvq[j] = 0;
-        if (ai < NA) nvq[j] = polyCount(s_poly[ai]) | (__popc(s_own[ai] & 7) << 16);
-        cnt += nvq[j];
+    const int64_t rb = tc.t * TILE;
+    for (int ai = tid; ai < NA; ai += GP_THREADS) {
+        const uint64_t w = s_poly[ai];
+        const int l = s_cell[ai];
+        a.recPoly[rb + ai] = w;
+        a.recInfo[rb + ai] = (uint16_t)(l | (s_own[ai] << 12));
+        a.cellMap[tc.gc0 + l] = mapEntry(s_vpre[ai], contourMap(w));
     }
-    {
-        int x = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
+    if (tid == 0) {
+        a.tileV[tc.t] = VT;
+        a.tileQ[tc.t] = QT;
+        a.tileNA[tc.t] = NA;
+        if (tc.t == a.numTiles - 1) {
+            a.tileV[a.numTiles] = 0;
+            a.tileQ[a.numTiles] = 0;
         }
-        if (lane == 63) s_wave[wid] = x;
+    }
+}
+
+// ---- emit: the tile's dual vertices and the quads its cells own
+template <int CPT>
+__global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked) {
+    constexpr int TILE = GP_THREADS * CPT;
+    __shared__ uint64_t s_poly[TILE];
+    __shared__ uint16_t s_cell[TILE];
+    __shared__ uint16_t s_vpre[TILE];
+    __shared__ uint16_t s_qpre[TILE];
+    __shared__ uint8_t s_own[TILE];
+    __shared__ uint16_t s_aidx[TILE];  // per tile cell: its active index (0xFFFF: inactive, adjoint map only)
+    __shared__ int s_wave[GP_WAVES];
+
+    const int tid = threadIdx.x;
+    const TileCtx tc = tileCtx<TILE>(a, chunked);
+    const ProblemDev& P = a.probs[tc.prob];
+    const GridDev g = P.g;
+    const float iso = P.iso;
+    const int CX = P.CX, CY = P.CY;
+    const int64_t C = P.cells;
+    const int64_t c0 = tc.c0;
+    const int NA = a.tileNA[tc.t];
+    const int64_t rb = tc.t * TILE;
+    if (a.adjMap) {
+        for (int l = tid; l < TILE; l += GP_THREADS) s_aidx[l] = 0xFFFF;
         __syncthreads();
-        int add = 0, tot = 0;
-#pragma unroll
-        for (int q = 0; q < GP_WAVES; ++q) {
-            const int v = s_wave[q];
-            if (q < wid) add += v;
-            tot += v;
-        }
-        int run = x - cnt + add;
-#pragma unroll
-        for (int j = 0; j < GP_CPT; ++j) {
-            const int ai = tid * GP_CPT + j;
-            if (ai < NA) {
-                s_vpre[ai] = (uint16_t)(run & 0xFFFF);
-                s_qpre[ai] = (uint16_t)(run >> 16);
-            }
-            run += nvq[j];
-        }
-        VT = tot & 0xFFFF;
-        QT = tot >> 16;
     }
-
-    // ---- 3. decoupled look-back (wave 0)
-    if (tid < 64) {
-        if (t == 0) {
-            if (lane == 0) {
-                storeRelaxed(&a.status[0], packStatus(FLAG_INC, (uint64_t)VT, (uint64_t)QT));
-                s_voff = 0;
-                s_qoff = 0;
-            }
-        } else {
-            if (lane == 0) storeRelaxed(&a.status[t], packStatus(FLAG_AGG, (uint64_t)VT, (uint64_t)QT));
-            int64_t accV = 0, accQ = 0;
-            int64_t i = t - 1;
-            int spins = 0;
-            bool doneLB = false;
-            while (!doneLB) {
-                const int64_t jdx = i - lane;
-                const uint64_t st = (jdx >= 0) ? loadRelaxed(&a.status[jdx]) : packStatus(FLAG_INC, 0, 0);
-                const uint64_t f = stFlag(st);
-                const uint64_t incMask = __ballot(f == FLAG_INC);
-                const uint64_t zeroMask = __ballot(f == 0);
-                const int firstInc = incMask ? __ffsll((unsigned long long)incMask) - 1 : 64;
-                const uint64_t below = (firstInc >= 64) ? ~0ULL : ((1ULL << firstInc) - 1ULL);
-                if (zeroMask & (below | (firstInc < 64 ? (1ULL << firstInc) : 0ULL))) {
-                    if (++spins > kSpinLimit) {
-                        if (lane == 0) atomicOr(a.err, GP_ERR_TIMEOUT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                int64_t v = (lane <= firstInc) ? (int64_t)stV(st) : 0;
-                int64_t q = (lane <= firstInc) ? (int64_t)stQ(st) : 0;
-#pragma unroll
-                for (int d = 32; d > 0; d >>= 1) {
-                    v += __shfl_down(v, d, 64);
-                    q += __shfl_down(q, d, 64);
-                }
-                v = __shfl(v, 0, 64);
-                q = __shfl(q, 0, 64);
-                accV += v;
-                accQ += q;
-                if (firstInc < 64) doneLB = true;
-                else i -= 64;
-            }
-            if (lane == 0) {
-                if (accV + VT > 0x7FFFFFFFLL || accQ + QT > 0x7FFFFFFFLL) atomicOr(a.err, GP_ERR_RANGE);
-                storeRelaxed(&a.status[t], packStatus(FLAG_INC, (uint64_t)(accV + VT) & 0x7FFFFFFFULL, (uint64_t)(accQ + QT) & 0x7FFFFFFFULL));
-                s_voff = accV;
-                s_qoff = accQ;
-            }
-        }
-    }
-    // ring reuse: the slots this tile overwrites held cells [gc0 - R, gc0 + T - R) of the
-    // concatenation; every tile that reads them (cells up to `reach` further on) must have finished
-    {
-        const int64_t oldHi = gc0 + GP_TILE - 1 - a.R;
-        if (oldHi >= 0) {
-            const int64_t lo = gc0 - a.R + 1;
-            const int64_t firstReader = tileOfCell(a, lo < 0 ? 0 : lo);
-            const int64_t hiCell = oldHi + a.reach;
-            int64_t lastReader = tileOfCell(a, hiCell < a.totalCells ? hiCell : a.totalCells - 1);
-            if (lastReader > t - 1) lastReader = t - 1;
-            for (int64_t tr = firstReader + tid; tr <= lastReader; tr += GP_THREADS) waitFlag(&a.done[tr], a.err);
-        }
+    for (int ai = tid; ai < NA; ai += GP_THREADS) {
+        s_poly[ai] = a.recPoly[rb + ai];
+        const int info = a.recInfo[rb + ai];
+        const int l = info & 0xFFF;
+        s_cell[ai] = (uint16_t)l;
+        s_own[ai] = (uint8_t)(info >> 12);
+        s_aidx[l] = (uint16_t)ai;
     }
     __syncthreads();
-    const int64_t voff = s_voff, qoff = s_qoff;
+    int VT, QT;
+    scanCounts<CPT>(s_poly, s_own, NA, s_vpre, s_qpre, s_wave, VT, QT);
+    __syncthreads();
+    const int64_t voff = a.tileVOff[tc.t], qoff = a.tileQOff[tc.t];
 
-    // ---- 4. publish first vertex id + contour map of the tile's active cells
-    for (int ai = tid; ai < NA; ai += GP_THREADS) {
-        const int64_t c = gc0 + s_cell[ai];
-        const uint64_t ent = (ringTag(c, a.ringShift) << 55) | ((uint64_t)contourMap(s_poly[ai]) << 31) | (uint64_t)((voff + s_vpre[ai]) & 0x7FFFFFFF);
-        storeRelaxed(a.ring + (c & (a.R - 1)), ent);
-    }
     // for the adjoint: cell -> (first vertex << 2 | contours - 1), -1 for empty cells
     if (a.adjMap) {
-#pragma unroll
-        for (int k = 0; k < GP_CPT; ++k) {
-            const int l = k * GP_THREADS + tid;
+        for (int l = tid; l < TILE; l += GP_THREADS) {
             const int64_t c = c0 + l;
             if (c < C) {
-                int val = -1;
-                if ((s_mask[k * GP_WAVES + wid] >> lane) & 1ULL) {
-                    const int ai = s_aidx[l];
-                    val = (int)(((voff + s_vpre[ai]) << 2) | (int64_t)(polyCount(s_poly[ai]) - 1));
-                }
-                a.adjMap[c] = val;
+                const int ai = s_aidx[l];
+                a.adjMap[c] = ai == 0xFFFF ? -1 : (int)(((voff + s_vpre[ai]) << 2) | (int64_t)(polyCount(s_poly[ai]) - 1));
             }
         }
     }
 
-    // ---- 5. dual vertices, one per thread
+    // dual vertices, one per thread
     for (int i = tid; i < VT; i += GP_THREADS) {
         const int ai = upperIndex(s_vpre, NA, i);
         const int tt = i - s_vpre[ai];
         int x, y, z;
-        cellXYZ(tb, s_cell[ai], CX, CY, x, y, z);
+        cellXYZ(tc.tb, s_cell[ai], CX, CY, x, y, z);
         float s[8];
         loadCorners(g, x, y, z, s);
         f3 wp, nn;
         dualVertex(g, iso, a.surfIter, x, y, z, s, s_poly[ai], tt, wp, nn);
         const int64_t v = voff + i;
-        if (v < a.capV) {
-            a.pos[3 * v + 0] = wp.x;
-            a.pos[3 * v + 1] = wp.y;
-            a.pos[3 * v + 2] = wp.z;
-            a.nrm[3 * v + 0] = nn.x;
-            a.nrm[3 * v + 1] = nn.y;
-            a.nrm[3 * v + 2] = nn.z;
-            if (a.adjCell) a.adjCell[v] = (int)(c0 + s_cell[ai]);
-            if (a.vpid) a.vpid[v] = prob;
-        } else {
-            atomicOr(a.err, GP_ERR_OVERFLOW);
-        }
+        a.pos[3 * v + 0] = wp.x;
+        a.pos[3 * v + 1] = wp.y;
+        a.pos[3 * v + 2] = wp.z;
+        a.nrm[3 * v + 0] = nn.x;
+        a.nrm[3 * v + 1] = nn.y;
+        a.nrm[3 * v + 2] = nn.z;
+        if (a.adjCell) a.adjCell[v] = (int)(c0 + s_cell[ai]);
+        if (a.vpid) a.vpid[v] = tc.prob;
     }
 
-    // ---- 6. quads whose edge starts at a cell of this tile.  Their other cells lie
-    // 1, CX, CX+1, CX*CY, CX*CY+1 or CX*CY+CX cells back: in LDS or in the ring.
+    // quads whose edge starts at a cell of this tile.  Their other cells lie 1, CX, CX+1,
+    // CX*CY, CX*CY+1 or CX*CY+CX cells back: in this tile (LDS) or an earlier one (map).
     for (int jq = tid; jq < QT; jq += GP_THREADS) {
         const int ai = upperIndex(s_qpre, NA, jq);
         int r = jq - s_qpre[ai];
@@ -486,7 +423,7 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
         }
         const int edgy = (own >> 3) & 1;
         int x, y, z;
-        cellXYZ(tb, s_cell[ai], CX, CY, x, y, z);
+        cellXYZ(tc.tb, s_cell[ai], CX, CY, x, y, z);
         // own edge, then (dx, dy, dz, edge) of the three other cells around it, packed
         // 4 bits each (dx | dy<<1 | dz<<2, edge) per cell: x: (y-1: e2) (z-1: e4) (y-1,z-1: e6);
         // y: (x-1: e1) (z-1: e7) (x-1,z-1: e5); z: (x-1: e9) (y-1: e11) (x-1,y-1: e10)
@@ -511,7 +448,8 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
                 const int an = s_aidx[n - c0];
                 vid = (int)(voff + s_vpre[an] + polyContourOf(s_poly[an], e));
             } else {
-                vid = ringVertex(a, P.cellBase + n, e);
+                const uint32_t m = a.cellMap[P.cellBase + n];
+                vid = (int)(a.tileVOff[P.tileBase + n / TILE] + (int64_t)(m & 0x3FFFu) + mapContour(m, e));
             }
             const int sl = edgeSlot(e, edgy);
             s0 = sl == 0 ? vid : s0;
@@ -520,48 +458,41 @@ __global__ void __launch_bounds__(GP_THREADS) kGridPass(GridPassArgs a) {
             s3 = sl == 3 ? vid : s3;
         }
         const int64_t q = qoff + jq;
-        if (q < a.capQ) {
-            a.quads[q] = make_int4(s0, s1, s2, s3);
-            a.qattr[q] = (uint32_t)(3 * ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)) + axis);
-        } else {
-            atomicOr(a.err, GP_ERR_OVERFLOW);
-        }
-    }
-    // every ring value read above has been consumed by a store, so the reads are complete
-    __syncthreads();
-    if (tid == 0) storeRelaxedI(&a.done[t], 1);
-}
-
-__global__ void kGridPassTotals(const unsigned long long* status, int64_t numTiles, int64_t* out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const uint64_t s = status[numTiles - 1];
-        out[0] = (int64_t)stV(s);
-        out[1] = (int64_t)stQ(s);
-        out[2] = (int64_t)stFlag(s);
+        a.quads[q] = make_int4(s0, s1, s2, s3);
+        a.qattr[q] = (uint32_t)(3 * ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)) + axis);
     }
 }
 
 }  // namespace
 
-int gridPassTile() { return GP_TILE; }
+int gridPassTile(int shape) { return 1024 << ((shape - 1) % 3); }
 
-void gridPassRing(int64_t totalCells, int64_t reach, int64_t* R, int* shift) {
-    // at least 8 z-slices of cells (the quads of a tile reach one slice back), and few
-    // enough generations that the 9-bit tag of a ring entry never wraps
-    const int64_t want = std::max<int64_t>({8 * (reach + GP_TILE), (totalCells + 509) / 510, (int64_t)1 << 16});
-    int s = 16;
-    while (((int64_t)1 << s) < want) ++s;
-    *R = (int64_t)1 << s;
-    *shift = s;
+const char* gridPassShapeName(int shape) {
+    static const char* names[] = {"auto", "1024 cells, XCD-chunked", "2048 cells, XCD-chunked", "4096 cells, XCD-chunked",
+                                  "1024 cells, launch order", "2048 cells, launch order", "4096 cells, launch order"};
+    return (shape >= 0 && shape <= GP_SHAPE_LAST) ? names[shape] : "?";
 }
 
-void launchGridPass(const GridPassArgs& a, hipStream_t s) {
+void launchGridClassify(int shape, const GridPassArgs& a, hipStream_t s) {
     if (a.numTiles <= 0) return;
-    kGridPass<<<(unsigned)a.numTiles, GP_THREADS, 0, s>>>(a);
+    const int chunked = shape <= 3 ? 1 : 0;
+    const unsigned nb = (unsigned)a.numTiles;
+    switch (gridPassTile(shape)) {
+        case 1024: kClassify<4><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        case 2048: kClassify<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        default: kClassify<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+    }
 }
 
-void launchGridPassTotals(const unsigned long long* status, int64_t numTiles, int64_t* out, hipStream_t s) {
-    kGridPassTotals<<<1, 1, 0, s>>>(status, numTiles, out);
+void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {
+    if (a.numTiles <= 0) return;
+    const int chunked = shape <= 3 ? 1 : 0;
+    const unsigned nb = (unsigned)a.numTiles;
+    switch (gridPassTile(shape)) {
+        case 1024: kEmit<4><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        case 2048: kEmit<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        default: kEmit<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+    }
 }
 
 }  // namespace mdmc

@@ -4,8 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
-
 #include <vector>
 
 #include "mdmc_arena.h"
@@ -69,16 +69,23 @@ class Pipeline {
     void triangles(uint32_t* idx, bool dstDevice);
 
     int64_t readI64(const int64_t* dev);
-    // duration of the last grid-pass kernel, from events on the launch stream
+    // duration of the last grid pass (classify + scan, then emit), from events on the
+    // launch stream; the host read-back of the sizes between them is not counted
     void kernelTimes(double* msGridPass);
-    int64_t lastGridV = 0, lastGridQ = 0;
     hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // grid-pass tile shape: GP_SHAPE_AUTO (autotuned per workload class) or a fixed shape;
+    // lastShape is the one the last extraction used, tuneMs the last tuning's timings
+    int tileShape = GP_SHAPE_AUTO;
+    int lastShape = GP_SHAPE_DEFAULT;
+    std::map<int, int> tuned;
+    double tuneMs[GP_SHAPE_LAST + 1] = {};
 
    private:
     int64_t* counters();
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
+    void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
     void buildBuckets(bool fromQuads, const int32_t* list, int64_t nList);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);
     void checkValid(const char* what);

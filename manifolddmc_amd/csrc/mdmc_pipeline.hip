@@ -1365,9 +1365,10 @@ Pipeline::~Pipeline() {
 }
 
 void Pipeline::kernelTimes(double* msGridPass) {
-    float a = 0.f;
+    float a = 0.f, b = 0.f;
     MDMC_HIP(hipEventElapsedTime(&a, kev[0], kev[1]));
-    *msGridPass = a;
+    MDMC_HIP(hipEventElapsedTime(&b, kev[2], kev[3]));
+    *msGridPass = (double)a + (double)b;
 }
 
 int64_t Pipeline::readI64(const int64_t* dev) {
@@ -1401,85 +1402,111 @@ int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
 
 // ---- grid pass
 void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint) {
-    const int T = gridPassTile();
     vertHeV = vertHeQ = 0;
     batchVertHe = false;
     nProblems = (int)problems.size();
-    int64_t C = 0, numTiles = 0, reach = 0;
+    if (recordAdjoint && nProblems != 1) throw HipError("adjoint recording needs a single problem", -4);
+    int64_t C = 0;
     for (ProblemDev& pr : problems) {
         pr.CX = pr.g.nx - 1;
         pr.CY = pr.g.ny - 1;
         pr.cells = (int64_t)pr.CX * pr.CY * (pr.g.nz - 1);
         pr.cellBase = C;
-        pr.tileBase = numTiles;
         C += pr.cells;
-        numTiles += (pr.cells + T - 1) / T;
-        reach = std::max<int64_t>(reach, (int64_t)pr.CX * pr.CY + pr.CX);
     }
-    if (recordAdjoint && nProblems != 1) throw HipError("adjoint recording needs a single problem", -4);
+    if (!kev[0])
+        for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
+    lastShape = tileShape;
+    if (lastShape == GP_SHAPE_AUTO) {
+        // autotuner: on the first extraction of a workload class, time every tile shape on
+        // this very input and keep the fastest for the class
+        int lc = 0, lp = 0;
+        while (((int64_t)1 << lc) < C + 1) ++lc;
+        while ((1 << lp) < nProblems) ++lp;
+        const int key = (lc << 16) | (lp << 1) | (surfIter ? 1 : 0);
+        auto it = tuned.find(key);
+        if (it != tuned.end()) {
+            lastShape = it->second;
+        } else {
+            double best = 1e30;
+            for (int sh = GP_SHAPE_FIRST; sh <= GP_SHAPE_LAST; ++sh) {
+                gridPass(problems, C, sh, surfIter, recordAdjoint);
+                MDMC_HIP(hipEventSynchronize(kev[3]));
+                double ms = 0.0;
+                kernelTimes(&ms);
+                tuneMs[sh] = ms;
+                if (ms < best) {
+                    best = ms;
+                    lastShape = sh;
+                }
+            }
+            tuned[key] = lastShape;
+            if (lastShape == GP_SHAPE_LAST) return;  // the last candidate's output is in place
+        }
+    }
+    gridPass(problems, C, lastShape, surfIter, recordAdjoint);
+}
+
+// classify -> scan of the tile counts -> exact output sizes -> emit
+void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint) {
+    const int T = gridPassTile(shape);
+    int64_t numTiles = 0;
+    for (ProblemDev& pr : problems) {
+        pr.tileBase = numTiles;
+        numTiles += (pr.cells + T - 1) / T;
+    }
     probsDev = arena.get<ProblemDev>("gp_probs", problems.size());
     MDMC_HIP(hipMemcpyAsync(probsDev, problems.data(), problems.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, stream));
     GridPassArgs a;
     a.probs = probsDev;
     a.nProblems = nProblems;
     a.surfIter = surfIter ? 1 : 0;
-    a.totalCells = C;
-    a.reach = reach;
     a.numTiles = numTiles;
-    // flags: status (u64) | done | ticket | err, zeroed together
-    const size_t flagBytes = (size_t)numTiles * (8 + 4) + 64;
-    char* flags = arena.get<char>("gp_flags", flagBytes);
-    a.status = (unsigned long long*)flags;
-    a.done = (int*)(flags + 8 * (size_t)numTiles);
-    a.ticket = (unsigned int*)(a.done + numTiles);
-    a.err = (int*)(a.ticket + 4);
-    gridPassRing(C, reach, &a.R, &a.ringShift);
-    a.ring = arena.get<unsigned long long>("gp_ring", (size_t)a.R);
-    int64_t* tot = arena.get<int64_t>("gp_totals", 4);
-    // output capacity: what the buffers hold, at least an estimate; exact sizes on overflow
-    int64_t capV = std::max<int64_t>(lastGridV + lastGridV / 16 + 1024, C / 6 + 4096);
-    int64_t capQ = std::max<int64_t>(lastGridQ + lastGridQ / 16 + 1024, C / 6 + 4096);
-    if (!kev[0])
-        for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        a.pos = arena.get<float>("pos", 3 * (size_t)capV);
-        a.nrm = arena.get<float>("nrm", 3 * (size_t)capV);
-        a.quads = arena.get<int4>("quads", (size_t)capQ);
-        a.qattr = arena.get<uint32_t>("qattr", (size_t)capQ);
-        a.capV = capV;
-        a.capQ = capQ;
-        a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)C) : nullptr;
-        a.adjCell = recordAdjoint ? arena.get<int32_t>("adj_cell", (size_t)capV) : nullptr;
-        a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)capV) : nullptr;
-        zero(flags, flagBytes);
-        zero(a.ring, (size_t)a.R * sizeof(unsigned long long));
-        MDMC_HIP(hipEventRecord(kev[0], stream));
-        launchGridPass(a, stream);
-        MDMC_HIP(hipEventRecord(kev[1], stream));
-        launchGridPassTotals(a.status, numTiles, tot, stream);
-        int64_t h[4] = {0, 0, 0, 0};
-        int err = 0;
-        MDMC_HIP(hipMemcpyAsync(h, tot, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-        MDMC_HIP(hipMemcpyAsync(&err, a.err, sizeof(int), hipMemcpyDeviceToHost, stream));
-        MDMC_HIP(hipStreamSynchronize(stream));
-        if (err & GP_ERR_TIMEOUT) throw HipError("grid pass: a tile wait timed out", -4);
-        if (err & GP_ERR_RANGE) throw HipError("mesh exceeds the int32 id range", -3);
-        V = h[0];
-        Q = h[1];
-        if (!(err & GP_ERR_OVERFLOW)) break;
-        if (attempt == 1) throw HipError("grid pass: output overflow after resize", -4);
-        capV = V + 1;
-        capQ = Q + 1;
+    a.recPoly = arena.get<uint64_t>("gp_rpoly", (size_t)std::max<int64_t>(numTiles * T, 1));
+    a.recInfo = arena.get<uint16_t>("gp_rinfo", (size_t)std::max<int64_t>(numTiles * T, 1));
+    a.tileV = arena.get<int32_t>("gp_tv", (size_t)numTiles + 1);
+    a.tileQ = arena.get<int32_t>("gp_tq", (size_t)numTiles + 1);
+    a.tileNA = arena.get<int32_t>("gp_tna", (size_t)numTiles + 1);
+    a.cellMap = arena.get<uint32_t>("gp_map", (size_t)std::max<int64_t>(C, 1));
+    int64_t* tvo = arena.get<int64_t>("gp_tvo", (size_t)numTiles + 1);
+    int64_t* tqo = arena.get<int64_t>("gp_tqo", (size_t)numTiles + 1);
+    a.tileVOff = tvo;
+    a.tileQOff = tqo;
+    void* scratchV = arena.get<char>("scan_scratch", scanScratchBytes(numTiles + 1));
+    void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(numTiles + 1));
+    int64_t* tot = arena.get<int64_t>("gp_totals", 2);
+    MDMC_HIP(hipEventRecord(kev[0], stream));
+    launchGridClassify(shape, a, stream);
+    if (numTiles > 0) {
+        scanExclusiveI32(a.tileV, tvo, numTiles + 1, tot, scratchV, stream);
+        scanExclusiveI32(a.tileQ, tqo, numTiles + 1, tot + 1, scratchQ, stream);
     }
-    MDMC_HIP(hipEventRecord(kev[2], stream));
-    MDMC_HIP(hipEventRecord(kev[3], stream));
+    MDMC_HIP(hipEventRecord(kev[1], stream));
+    int64_t h[4] = {0, 0, 0, 0};
+    if (numTiles > 0) {
+        MDMC_HIP(hipMemcpyAsync(h, tot, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+        MDMC_HIP(hipMemcpyAsync(&h[2], scanTimeoutFlag(scratchV), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+        MDMC_HIP(hipMemcpyAsync(&h[3], scanTimeoutFlag(scratchQ), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    }
+    MDMC_HIP(hipStreamSynchronize(stream));
+    if (h[2] || h[3]) throw HipError("prefix sum: a look-back wait timed out", -4);
+    V = h[0];
+    Q = h[1];
     if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
     if (recordAdjoint && (V >= ((int64_t)1 << 29) || C >= (int64_t)INT32_MAX))
         throw HipError("adjoint recording needs fewer than 2^29 vertices and 2^31 cells", -3);
+    a.pos = arena.get<float>("pos", 3 * (size_t)std::max<int64_t>(V, 1));
+    a.nrm = arena.get<float>("nrm", 3 * (size_t)std::max<int64_t>(V, 1));
+    a.quads = arena.get<int4>("quads", (size_t)std::max<int64_t>(Q, 1));
+    a.qattr = arena.get<uint32_t>("qattr", (size_t)std::max<int64_t>(Q, 1));
+    a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
+    a.adjCell = recordAdjoint ? arena.get<int32_t>("adj_cell", (size_t)std::max<int64_t>(V, 1)) : nullptr;
+    a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
+    MDMC_HIP(hipEventRecord(kev[2], stream));
+    launchGridEmit(shape, a, stream);
+    MDMC_HIP(hipEventRecord(kev[3], stream));
     dmcV = V;
     dmcQ = Q;
-    lastGridV = V;
-    lastGridQ = Q;
 }
 
 int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nullptr; }

@@ -27,3 +27,20 @@ def test_library_exports_every_declared_symbol():
 def test_options_struct_layout_matches_header():
     assert ctypes.sizeof(mdmc._Options) == 16
     assert ctypes.sizeof(mdmc._Grid) == 3 * 4 * 3 + 4 + 8 + 4 + 4  # dims, origin, delta, pad, ptr, flag, tail pad
+
+
+def test_reference_member_names_map_to_flags():
+    # DualMarchingCubes.cu:110-133 defaults; the reference names alias the snake_case options
+    d = object.__new__(mdmc.DualMarchingCubes)  # no context: flags() is host logic only
+    d.enable_surface_to_vertex_iteration = True
+    d.simplify_p3x3y_color = d.simplify_p3x3y_old = d.simplify_p3333 = False
+    d.split_non_manifold_halfedges = d.detect_non_manifold_vertices = True
+    d.splitting_step = mdmc.SplittingStep.ALL
+    assert d.flags() == mdmc.DEFAULT_FLAGS
+    d.m_simplifyP3x3yColor = True
+    d.m_simplifyP3333 = True
+    d.m_enableSurfaceToVertexIteration = False
+    d.m_programmaticSplittingStep = mdmc.SplittingStep.MERGE_VERTICES
+    assert d.simplify_p3x3y_color and d.simplify_p3333 and not d.enable_surface_to_vertex_iteration
+    assert d.splitting_step == mdmc.SplittingStep.MERGE_VERTICES
+    assert d.flags(add_zero_boundary=False) == (mdmc.P3X3Y_COLOR | mdmc.P3333 | mdmc.SPLIT_NM | mdmc.DETECT_NM_VERTS)
