@@ -101,6 +101,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libmdmc.so is missing: build it with `python -m manifolddmc_amd.build` "
                                "(there is no CPU fallback)")
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) but loads it
+        # by another file name; loaded first, it satisfies this library's dependency and the
+        # process keeps one runtime.  Loaded after ours, it would bring a second runtime that
+        # finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         L.mdmc_create.argtypes = [ctypes.POINTER(vp), i32]

@@ -8,6 +8,7 @@
 //   -> [P3333] -> fixManifoldTwins (always) -> [split] -> [detect NM vertices]
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -80,10 +81,25 @@ mdmc_status mdmc_create(mdmc_ctx** out, int32_t device) {
     int dev = device;
     if (dev < 0) (void)hipGetDevice(&dev);
     if (dev >= count) return MDMC_ERR_INVALID_ARG;
+    DeviceGuard g(dev);
+    {
+        // per-device constant tables of the grid pass, uploaded once per process
+        static std::mutex mu;
+        static std::vector<char> ready;
+        std::lock_guard<std::mutex> lock(mu);
+        if ((int)ready.size() <= dev) ready.resize(dev + 1, 0);
+        if (!ready[dev]) {
+            try {
+                mdmc::initGridPassTables();
+            } catch (const std::exception&) {
+                return MDMC_ERR_HIP;
+            }
+            ready[dev] = 1;
+        }
+    }
     mdmc_ctx* ctx = new (std::nothrow) mdmc_ctx();
     if (!ctx) return MDMC_ERR_OOM;
     ctx->device = dev;
-    DeviceGuard g(dev);
     if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return MDMC_ERR_HIP;
@@ -396,7 +412,7 @@ mdmc_status mdmc_backward(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield,
     try {
         hipStream_t s = p.stream;
         const int32_t* map = p.arena.ptr<int32_t>("adj_map");
-        const int32_t* cell = p.arena.ptr<int32_t>("adj_cell");
+        const int32_t* cell = p.arena.ptr<int32_t>("gp_vcell");
         float* contrib = p.arena.get<float>("adj_contrib", 8 * (size_t)std::max<int64_t>(a.V, 1));
         mdmc::launchAdjointVertex(a.g, a.iso, map, cell, a.V, dL_dpos, contrib, s);
         if (a.pad == 0 && a.refinement == 0) {

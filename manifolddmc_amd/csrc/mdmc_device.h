@@ -79,12 +79,26 @@ __device__ __forceinline__ int polyContourOf(uint64_t w, int e) {
     return -1;
 }
 
+// Faces of the cell: corners and edges in the walk order of faceVertexLookup /
+// faceEdgeLookup (CellIntersection.inl:290-292), one nibble each.
+#define MDMC_FACE_CORNERS {0x3120u, 0x6475u, 0x1504u, 0x7362u, 0x2064u, 0x7531u}
+#define MDMC_FACE_EDGES {0x0123u, 0x4765u, 0x4908u, 0x2A6Bu, 0x83B7u, 0x95A1u}
+
+// Face case of face f in MC case mc: bit i = corner i of the face is inside (>= iso).
+__host__ __device__ __forceinline__ int faceCase(int mc, int f) {
+    const uint32_t fcorner[6] = MDMC_FACE_CORNERS;
+    int fc = 0;
+    for (int i = 0; i < 4; ++i) fc |= ((mc >> ((fcorner[f] >> (4 * i)) & 0xF)) & 1) << i;
+    return fc;
+}
+
 // CellIntersection::marchingCubesAsymptoticPolygon + handleAsymptoticFaceCase,
-// CellIntersection.inl:203-338.  Links are kept as nibbles: out[e] in `outw`.
-__device__ __forceinline__ uint64_t polygonAsymptotic(const float s[8], float iso) {
-    // faces: corners and edges in the walk order of faceVertexLookup / faceEdgeLookup
-    const uint32_t fcorner[6] = {0x3120u, 0x6475u, 0x1504u, 0x7362u, 0x2064u, 0x7531u};
-    const uint32_t fedge[6] = {0x0123u, 0x4765u, 0x4908u, 0x2A6Bu, 0x83B7u, 0x95A1u};
+// CellIntersection.inl:203-338, with the asymptotic decider's outcomes given:
+// bit f of `dec` = (decider value >= iso) on face f, read only for the faces whose
+// case is 6 or 9.  Links are kept as nibbles: out[e] in `outw`; contours are walked
+// from the smallest edge that has a link.
+__host__ __device__ __forceinline__ uint64_t polygonFromFaceDecisions(int mc, uint32_t dec) {
+    const uint32_t fedge[6] = MDMC_FACE_EDGES;
     uint64_t outw = 0xFFFFFFFFFFFFULL;  // 12 nibbles of 0xF
     uint32_t set = 0;
 #define MDMC_CONNECT(a, b)                                                        \
@@ -93,15 +107,10 @@ __device__ __forceinline__ uint64_t polygonAsymptotic(const float s[8], float is
         set |= (1u << (a)) | (1u << (b));                                         \
     } while (0)
     for (int f = 0; f < 6; ++f) {
-        float v[4];
         int e[4];
-        int fc = 0;
-        for (int i = 0; i < 4; ++i) {
-            v[i] = s[(fcorner[f] >> (4 * i)) & 0xF];
-            e[i] = (int)((fedge[f] >> (4 * i)) & 0xF);
-            fc |= (v[i] >= iso ? 1 : 0) << i;
-        }
-        switch (fc) {
+        for (int i = 0; i < 4; ++i) e[i] = (int)((fedge[f] >> (4 * i)) & 0xF);
+        const bool up = (dec >> f) & 1u;
+        switch (faceCase(mc, f)) {
             case 1: MDMC_CONNECT(e[0], e[3]); break;
             case 2: MDMC_CONNECT(e[1], e[0]); break;
             case 3: MDMC_CONNECT(e[1], e[3]); break;
@@ -114,16 +123,14 @@ __device__ __forceinline__ uint64_t polygonAsymptotic(const float s[8], float is
             case 12: MDMC_CONNECT(e[3], e[1]); break;
             case 13: MDMC_CONNECT(e[0], e[1]); break;
             case 14: MDMC_CONNECT(e[3], e[0]); break;
-            case 6: {
-                const float d = (v[0] * v[3] - v[1] * v[2]) / (v[0] + v[3] - v[1] - v[2]);
-                if (d >= iso) { MDMC_CONNECT(e[3], e[0]); MDMC_CONNECT(e[1], e[2]); }
+            case 6:
+                if (up) { MDMC_CONNECT(e[3], e[0]); MDMC_CONNECT(e[1], e[2]); }
                 else { MDMC_CONNECT(e[1], e[0]); MDMC_CONNECT(e[3], e[2]); }
-            } break;
-            case 9: {
-                const float d = (v[0] * v[3] - v[1] * v[2]) / (v[0] + v[3] - v[1] - v[2]);
-                if (d >= iso) { MDMC_CONNECT(e[0], e[1]); MDMC_CONNECT(e[2], e[3]); }
+                break;
+            case 9:
+                if (up) { MDMC_CONNECT(e[0], e[1]); MDMC_CONNECT(e[2], e[3]); }
                 else { MDMC_CONNECT(e[0], e[3]); MDMC_CONNECT(e[2], e[1]); }
-            } break;
+                break;
             default: break;
         }
     }
@@ -149,6 +156,35 @@ __device__ __forceinline__ uint64_t polygonAsymptotic(const float s[8], float is
         ++n;
     }
     return w;
+}
+
+// asymptotic decider of face f (CellIntersection.inl:203-210): (ad - bc) / (a + d - b - c) >= iso
+template <int F>
+__device__ __forceinline__ bool faceDecider(const float s[8], float iso) {
+    constexpr uint32_t fc[6] = MDMC_FACE_CORNERS;
+    const float v0 = s[fc[F] & 0xF], v1 = s[(fc[F] >> 4) & 0xF], v2 = s[(fc[F] >> 8) & 0xF], v3 = s[(fc[F] >> 12) & 0xF];
+    const float d = (v0 * v3 - v1 * v2) / (v0 + v3 - v1 - v2);
+    return d >= iso;
+}
+
+// the decider outcome of every face whose case is 6 or 9 (bit f), for a cell of case mc
+__device__ __forceinline__ uint32_t faceDecisions(int mc, const float s[8], float iso) {
+    uint32_t dec = 0;
+#define MDMC_FACE(F)                                                  \
+    {                                                                 \
+        const int fc = faceCase(mc, F);                               \
+        if ((fc == 6 || fc == 9) && faceDecider<F>(s, iso)) dec |= 1u << F; \
+    }
+    MDMC_FACE(0) MDMC_FACE(1) MDMC_FACE(2) MDMC_FACE(3) MDMC_FACE(4) MDMC_FACE(5)
+#undef MDMC_FACE
+    return dec;
+}
+
+__device__ __forceinline__ uint64_t polygonAsymptotic(const float s[8], float iso) {
+    int mc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mc |= (s[k] >= iso ? 1 : 0) << k;
+    return polygonFromFaceDecisions(mc, faceDecisions(mc, s, iso));
 }
 
 __device__ __forceinline__ bool caseAmbiguous(int c) { return (c_ambiguous[c >> 6] >> (c & 63)) & 1ULL; }

@@ -49,15 +49,21 @@ struct GridPassArgs {
     float* nrm;
     int4* quads;
     uint32_t* qattr;
-    // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1,
-    // per vertex its cell (single-problem extractions only)
+    // per vertex: its cell inside its problem and its contour in that cell (emit -> vertices)
+    int32_t* vcell;
+    uint8_t* vcontour;
+    int64_t V;
+    // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1
+    // (single-problem extractions only)
     int32_t* adjMap;
-    int32_t* adjCell;
     // per vertex its problem, when not null
     int32_t* vpid;
 };
 
+// uploads the polygon table of the classify kernel; once per device, with that device current
+void initGridPassTables();
 void launchGridClassify(int shape, const GridPassArgs& a, hipStream_t s);
+// emit (quads, vertex cells) and the dual vertices
 void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s);
 
 }  // namespace mdmc

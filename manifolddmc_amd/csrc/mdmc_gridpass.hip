@@ -18,10 +18,12 @@
 //             the tile, the contour of the edges later cells read);
 //   scan      exclusive prefix of the tile counts: every tile's first vertex and
 //             quad id (the only host read-back: the exact output sizes);
-//   emit      per tile: dual vertices (edge mean, move-to-surface, gradient
-//             normal), one per thread, and the quads whose edge starts at one of
-//             the tile's cells, taking the other three vertex ids from LDS or from
-//             the map entries of earlier tiles.
+//   emit      per tile: each vertex's cell and contour, and the quads whose edge
+//             starts at one of the tile's cells, taking the other three vertex ids
+//             from LDS or from the map entries of earlier tiles; then one thread
+//             per vertex over the whole pass computes the dual vertex (edge mean,
+//             move-to-surface, gradient normal) -- a tile holds too few vertices
+//             to keep a block's lanes busy on that long computation.
 // Vertices come out ordered by (cell, contour) and quads by edge key 3*point+axis,
 // with no hash table and no global append counter.  Blocks take tiles either in
 // launch order or XCD-chunked (block b runs on XCD b % 8; each XCD gets one
@@ -29,6 +31,7 @@
 // neighbours stay in that XCD's L2).
 #include <algorithm>
 
+#include "mdmc_arena.h"
 #include "mdmc_device.h"
 #include "mdmc_gridpass.h"
 
@@ -56,6 +59,29 @@ __device__ __forceinline__ int64_t tileOfBlock(int64_t b, int64_t numTiles, bool
     if (!chunked) return b;
     const int64_t x = b & 7, i = b >> 3, q = numTiles >> 3, r = numTiles & 7;
     return x * q + (x < r ? x : r) + i;
+}
+
+// Polygons of all 256 cases with the decider outcomes folded in: case mc's polygon is
+// c_polyTab[(c_polyBase[mc] & 0xFFFF) + k], k = the decider bits of the faces in the
+// mask c_polyBase[mc] >> 16, packed in face order.  Cases the table marks unambiguous
+// have an empty mask and their table polygon; the others hold the asymptotic
+// polygon of every outcome (656 entries in all).  Filled by initGridPassTables.
+constexpr int kPolyTabSize = 656;
+__constant__ uint64_t c_polyTab[kPolyTabSize];
+__constant__ uint32_t c_polyBase[256];
+
+__device__ __forceinline__ uint64_t cellPolygonTab(int mc, const float s[8], float iso) {
+    const uint32_t b = c_polyBase[mc];
+    const uint32_t mask = b >> 16;
+    int k = 0, bit = 0;
+#define MDMC_FACE(F)                                          \
+    if ((mask >> F) & 1u) {                                   \
+        if (faceDecider<F>(s, iso)) k |= 1 << bit;            \
+        ++bit;                                                \
+    }
+    MDMC_FACE(0) MDMC_FACE(1) MDMC_FACE(2) MDMC_FACE(3) MDMC_FACE(4) MDMC_FACE(5)
+#undef MDMC_FACE
+    return c_polyTab[(b & 0xFFFFu) + k];
 }
 
 // map entry: bits 0-13 vertex index in the tile, then 2 bits per edge 1,2,4,5,6,7,9,10,11
@@ -306,7 +332,7 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
         cellXYZ(tc.tb, l, CX, CY, x, y, z);
         float s[8];
         loadCorners(g, x, y, z, s);
-        s_poly[ai] = cellPolygon(s_mc[l], s, iso);
+        s_poly[ai] = cellPolygonTab(s_mc[l], s, iso);
         s_own[ai] = (uint8_t)ownedBits(s, iso, x, y, z);
     }
     __syncthreads();
@@ -384,24 +410,12 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
         }
     }
 
-    // dual vertices, one per thread
+    // per vertex: its cell and contour; kVertices computes position and normal
     for (int i = tid; i < VT; i += GP_THREADS) {
         const int ai = upperIndex(s_vpre, NA, i);
-        const int tt = i - s_vpre[ai];
-        int x, y, z;
-        cellXYZ(tc.tb, s_cell[ai], CX, CY, x, y, z);
-        float s[8];
-        loadCorners(g, x, y, z, s);
-        f3 wp, nn;
-        dualVertex(g, iso, a.surfIter, x, y, z, s, s_poly[ai], tt, wp, nn);
         const int64_t v = voff + i;
-        a.pos[3 * v + 0] = wp.x;
-        a.pos[3 * v + 1] = wp.y;
-        a.pos[3 * v + 2] = wp.z;
-        a.nrm[3 * v + 0] = nn.x;
-        a.nrm[3 * v + 1] = nn.y;
-        a.nrm[3 * v + 2] = nn.z;
-        if (a.adjCell) a.adjCell[v] = (int)(c0 + s_cell[ai]);
+        a.vcell[v] = (int)(c0 + s_cell[ai]);
+        a.vcontour[v] = (uint8_t)(i - s_vpre[ai]);
         if (a.vpid) a.vpid[v] = tc.prob;
     }
 
@@ -463,9 +477,72 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     }
 }
 
+// ---- dual vertices (sliceP), one per thread over all vertices of the pass
+__global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a) {
+    const int64_t v = (int64_t)blockIdx.x * GP_THREADS + threadIdx.x;
+    if (v >= a.V) return;
+    const ProblemDev& P = a.probs[a.vpid ? a.vpid[v] : 0];
+    const GridDev g = P.g;
+    const float iso = P.iso;
+    const unsigned c = (unsigned)a.vcell[v];
+    const unsigned r = c / (unsigned)P.CX;
+    const int x = (int)(c - r * (unsigned)P.CX);
+    const int y = (int)(r % (unsigned)P.CY);
+    const int z = (int)(r / (unsigned)P.CY);
+    float s[8];
+    loadCorners(g, x, y, z, s);
+    int mc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mc |= (s[k] >= iso ? 1 : 0) << k;
+    f3 wp, nn;
+    dualVertex(g, iso, a.surfIter, x, y, z, s, cellPolygonTab(mc, s, iso), a.vcontour[v], wp, nn);
+    a.pos[3 * v + 0] = wp.x;
+    a.pos[3 * v + 1] = wp.y;
+    a.pos[3 * v + 2] = wp.z;
+    a.nrm[3 * v + 0] = nn.x;
+    a.nrm[3 * v + 1] = nn.y;
+    a.nrm[3 * v + 2] = nn.z;
+}
+
 }  // namespace
 
 int gridPassTile(int shape) { return 1024 << ((shape - 1) % 3); }
+
+void initGridPassTables() {
+    static const uint64_t poly[256] = {MDMC_POLYGON_TABLE_DATA};
+    static const uint64_t amb[4] = {MDMC_AMBIGUOUS_MASK_DATA};
+    uint64_t tab[kPolyTabSize];
+    uint32_t base[256];
+    int n = 0;
+    for (int mc = 0; mc < 256; ++mc) {
+        uint32_t mask = 0;
+        if ((amb[mc >> 6] >> (mc & 63)) & 1ULL)
+            for (int f = 0; f < 6; ++f) {
+                const int fc = faceCase(mc, f);
+                if (fc == 6 || fc == 9) mask |= 1u << f;
+            }
+        base[mc] = (uint32_t)n | (mask << 16);
+        const int bits = __builtin_popcount(mask);
+        if (!((amb[mc >> 6] >> (mc & 63)) & 1ULL)) {
+            tab[n++] = poly[mc];
+            continue;
+        }
+        for (int k = 0; k < (1 << bits); ++k) {
+            // spread the packed outcome bits k over the faces of the mask
+            uint32_t dec = 0;
+            int bit = 0;
+            for (int f = 0; f < 6; ++f)
+                if ((mask >> f) & 1u) {
+                    if ((k >> bit) & 1) dec |= 1u << f;
+                    ++bit;
+                }
+            tab[n++] = polygonFromFaceDecisions(mc, dec);
+        }
+    }
+    if (n != kPolyTabSize) throw HipError("grid pass: polygon table size mismatch", -4);
+    MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyTab), tab, sizeof tab));
+    MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyBase), base, sizeof base));
+}
 
 const char* gridPassShapeName(int shape) {
     static const char* names[] = {"auto", "1024 cells, XCD-chunked", "2048 cells, XCD-chunked", "4096 cells, XCD-chunked",
@@ -493,6 +570,7 @@ void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {
         case 2048: kEmit<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
         default: kEmit<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
     }
+    if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a);
 }
 
 }  // namespace mdmc

@@ -39,8 +39,8 @@ class Pipeline {
     int64_t anomalies();
 
     // the grid pass over one or more (grid, iso) problems (their cellBase / tileBase and
-    // cell counts are filled in); recordAdjoint keeps the cell -> vertex map and vertex ->
-    // cell array ("adj_map", "adj_cell") that the adjoint reads (one problem only)
+    // cell counts are filled in); recordAdjoint keeps the cell -> vertex map ("adj_map") that
+    // the adjoint reads with the vertex -> cell array ("gp_vcell") (one problem only)
     void dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint = false);
     int nProblems = 1;
     ProblemDev* probsDev = nullptr;

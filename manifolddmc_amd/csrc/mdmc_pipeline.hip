@@ -1413,6 +1413,7 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
         pr.cells = (int64_t)pr.CX * pr.CY * (pr.g.nz - 1);
         pr.cellBase = C;
         C += pr.cells;
+        if (pr.cells >= (int64_t)INT32_MAX) throw HipError("a grid has 2^31 cells or more", -3);
     }
     if (!kev[0])
         for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
@@ -1500,7 +1501,9 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     a.quads = arena.get<int4>("quads", (size_t)std::max<int64_t>(Q, 1));
     a.qattr = arena.get<uint32_t>("qattr", (size_t)std::max<int64_t>(Q, 1));
     a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
-    a.adjCell = recordAdjoint ? arena.get<int32_t>("adj_cell", (size_t)std::max<int64_t>(V, 1)) : nullptr;
+    a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
+    a.vcontour = arena.get<uint8_t>("gp_vcontour", (size_t)std::max<int64_t>(V, 1));
+    a.V = V;
     a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
     MDMC_HIP(hipEventRecord(kev[2], stream));
     launchGridEmit(shape, a, stream);
