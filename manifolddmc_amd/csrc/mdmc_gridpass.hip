@@ -478,8 +478,11 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
 }
 
 // ---- dual vertices (sliceP), one per thread over all vertices of the pass
-__global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a) {
-    const int64_t v = (int64_t)blockIdx.x * GP_THREADS + threadIdx.x;
+// (blocks XCD-chunked like the tiles when `chunked`: consecutive vertices share rows and
+// slices of the field, which then stay in one XCD's L2)
+__global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chunked) {
+    const int64_t nb = (a.V + GP_THREADS - 1) / GP_THREADS;
+    const int64_t v = tileOfBlock(blockIdx.x, nb, chunked != 0) * GP_THREADS + threadIdx.x;
     if (v >= a.V) return;
     const ProblemDev& P = a.probs[a.vpid ? a.vpid[v] : 0];
     const GridDev g = P.g;
@@ -570,7 +573,7 @@ void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {
         case 2048: kEmit<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
         default: kEmit<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
     }
-    if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a);
+    if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a, chunked);
 }
 
 }  // namespace mdmc
