@@ -31,9 +31,25 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--tile-shape", type=int, default=0, help="grid-pass tile shape 1-6 (0: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers 5 times (reference for the HBM traffic counters) and exit")
     return ap.parse_args()
+
+
+def profiled_traffic(n, shape):
+    """HBM bytes of one grid pass (FETCH_SIZE and WRITE_SIZE, copy-calibrated) for the tile
+    shape used, from the committed rocprofv3 --pmc summary of the 512^3 workload; None
+    for other sizes or when the summary is missing."""
+    path = os.path.join(ROOT, "profiles", "gridpass_gyroid512_traffic.md")
+    if n != 512 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == "shape=%d" % shape:
+                return int(parts[2])
+    return None
 
 
 def make_field(n):
@@ -229,6 +245,8 @@ def main():
     torch.cuda.synchronize()
     flags = mdmc.PAD1 | mdmc.SURF_ITER | mdmc.ALL_POST
     d = mdmc.DualMarchingCubes(device=0)
+    if args.tile_shape:
+        d.set_tile_shape(args.tile_shape)
 
     def step():
         return d.extract_raw(t_field.data_ptr(), (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags, 0, True)
@@ -272,7 +290,7 @@ def main():
                    "dmc_quads": s["n_dmc_quads"], "split_iterations": s["split_iterations"], "nm_edges4": s["nm_edges4"],
                    "nm_vertices": s["nm_vertices"]},
         "roofline": {"kernel": "grid pass: kClassify + tile-count scan + kEmit", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(n, shape),
                      "algorithmic_bytes": alg_bytes, "kernel_ms": round(gen_ms, 4),
                      "tile_shape": mdmc.TILE_SHAPES.get(shape, shape), "tune_ms": {mdmc.TILE_SHAPES[k]: round(v, 4) for k, v in tune_ms.items()}},
         "stages_ms": {k: round(float(np.mean([x[k] for x in stats])), 3) for k in stage_keys},
