@@ -12,6 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdmc.so")
+CLI = os.path.join(HERE, "mdmc_cli")
+ROCM_LIB = "/opt/rocm/lib"
 SOURCES = ["mdmc_scan.hip", "mdmc_dmc.hip", "mdmc_gridpass.hip", "mdmc_pipeline.hip", "mdmc_adjoint.hip", "mdmc_batch.hip", "mdmc_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -52,6 +54,13 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB)
+    # the command-line front end (host C++ over the C ABI; finds libmdmc.so next to itself)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "mdmc_cli.cpp"),
+           "-o", CLI + ".tmp", "-L" + HERE, "-lmdmc", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + ROCM_LIB]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("mdmc_cli build failed:\n" + r.stderr[-4000:])
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 
