@@ -85,20 +85,26 @@ __global__ void kPlace(const int32_t* __restrict__ key, int64_t n0, int64_t n, c
     inv[old] = (int32_t)dst;
 }
 
-// per-key counts; keys arrive sorted, so a wave usually holds one key and adds once
-__global__ void kCountSorted(const int32_t* __restrict__ key, const int32_t* __restrict__ perm, int64_t n, unsigned long long* __restrict__ count,
-                             int stride, int slot) {
+// Per-key counts of keys that arrive sorted (key(i) = key[perm ? perm[i] : i], values in
+// [0, B)): thread i in [0, n] writes first[k] = i for every k in (key(i-1), key(i)], with
+// key(-1) = -1 and key(n) = B, so first[k] is the first index whose key is >= k and the
+// count of key k is first[k + 1] - first[k].  No atomics; a key out of order or out of
+// range raises *bad.
+__global__ void kKeyStarts(const int32_t* __restrict__ key, const int32_t* __restrict__ perm, int64_t n, int B, int64_t* __restrict__ first,
+                           int* __restrict__ bad) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < n;
-    const int k = live ? key[perm ? perm[i] : i] : -1;
-    const int k0 = __shfl(k, 0, 64);
-    const uint64_t same = __ballot(live && k == k0);
-    const uint64_t alive = __ballot(live);
-    if (same == alive) {
-        if ((threadIdx.x & 63) == 0 && k0 >= 0) atomicAdd(&count[(int64_t)k0 * stride + slot], (unsigned long long)__popcll(alive));
-    } else if (live) {
-        atomicAdd(&count[(int64_t)k * stride + slot], 1ULL);
+    if (i > n) return;
+    const int k = i < n ? key[perm ? perm[i] : i] : B;
+    const int kp = i > 0 ? key[perm ? perm[i - 1] : i - 1] : -1;
+    if (k < kp || k > B || (i < n && (k == B || k < 0))) {
+        atomicOr(bad, 1);
+        return;
     }
+    for (int kk = kp + 1; kk <= k; ++kk) first[kk] = i;
+}
+__global__ void kCountsFromStarts(const int64_t* __restrict__ first, int B, unsigned long long* __restrict__ count, int stride, int slot) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < B) count[(int64_t)k * stride + slot] = (unsigned long long)(first[k + 1] - first[k]);
 }
 
 __global__ void kPermuteVertices(const int32_t* __restrict__ perm, int64_t V, const float* __restrict__ pos, const float* __restrict__ nrm,
@@ -169,7 +175,17 @@ __global__ void kVertexHalfedgeLocal(const int32_t* __restrict__ vhInit, int64_t
 
 // bucket / face / vertex counters of HalfedgeMesh.cu:1164-1239 per problem.  Elements are
 // sorted by problem, so a wave usually holds one problem: it then adds its sums once.
-__device__ __forceinline__ void addPer(unsigned long long* out, bool live, int p, int slot, unsigned v) {
+// Per-problem statistics of one element kind over its regrouped range [0, n), where every
+// problem's elements are contiguous and problems ascend: each block reduces a chunk of
+// kStatChunk elements in LDS, one row per problem the chunk touches (up to kStatRows; the
+// elements of further ones add to global memory directly), and adds every non-zero row
+// entry once.  KIND 0: halfedges (slots 0-2: NM-4 / NM-3 / all edges, counted at the
+// bucket's first member), 1: faces (slots 3-7: quads with k NM edges), 2: vertices (slot 8:
+// non-manifold vertices).  out[9 * problem + slot].
+constexpr int kStatChunk = 4096, kStatRows = 32;
+
+__device__ __forceinline__ void statAdd(unsigned* acc, unsigned long long* out, bool live, int p, int pbase, int slot, int accSlot, unsigned v) {
+    // one LDS atomic per wave when all its live lanes share the problem
     const int p0 = __shfl(p, 0, 64);
     const uint64_t alive = __ballot(live);
     const uint64_t same = __ballot(live && p == p0);
@@ -179,39 +195,67 @@ __device__ __forceinline__ void addPer(unsigned long long* out, bool live, int p
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
         const int pk = __shfl(p, __ffsll((unsigned long long)alive) - 1, 64);
-        if ((threadIdx.x & 63) == 0 && sum) atomicAdd(&out[(int64_t)pk * 9 + slot], (unsigned long long)sum);
+        if ((threadIdx.x & 63) == 0 && sum) {
+            const int r = pk - pbase;
+            if (r >= 0 && r < kStatRows) atomicAdd(&acc[r * 5 + accSlot], sum);
+            else atomicAdd(&out[(int64_t)pk * 9 + slot], (unsigned long long)sum);
+        }
     } else if (live && v) {
-        atomicAdd(&out[(int64_t)p * 9 + slot], (unsigned long long)v);
+        const int r = p - pbase;
+        if (r >= 0 && r < kStatRows) atomicAdd(&acc[r * 5 + accSlot], v);
+        else atomicAdd(&out[(int64_t)p * 9 + slot], (unsigned long long)v);
     }
 }
-__global__ void kStatsBatch(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, const int32_t* __restrict__ bsize,
-                            const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags, const int32_t* __restrict__ pid, int64_t V,
+
+template <int KIND>
+__global__ void kStatsBatch(const int4* __restrict__ he, const int32_t* __restrict__ face_he, const int32_t* __restrict__ bsize,
+                            const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags, const int32_t* __restrict__ pid, int64_t n,
                             unsigned long long* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    {
-        const bool live = i < H && brank[i] == 0;
-        const int p = live ? pid[heO(he, (int)i)] : 0;
-        const int n = live ? bsize[i] : 0;
-        addPer(out, live, p, 0, n == 4);
-        addPer(out, live, p, 1, n == 3);
-        addPer(out, live, p, 2, n > 0);
-    }
-    {
-        const bool live = i < F;
-        int p = 0, k = 0;
-        if (live) {
-            int h = face_he[i];
-            p = pid[heO(he, h)];
-            for (int j = 0; j < 4; ++j) {
-                if (bsize[h] > 2) ++k;
-                h = ((const int*)he)[4 * (int64_t)h + 2];
+    constexpr int nslots = KIND == 0 ? 3 : (KIND == 1 ? 5 : 1);
+    constexpr int base = KIND == 0 ? 0 : (KIND == 1 ? 3 : 8);
+    __shared__ unsigned acc[kStatRows * 5];
+    const int64_t c0 = (int64_t)blockIdx.x * kStatChunk;
+    const int64_t c1 = c0 + kStatChunk < n ? c0 + kStatChunk : n;
+    for (int t = threadIdx.x; t < kStatRows * 5; t += blockDim.x) acc[t] = 0;
+    auto problemOf = [&](int64_t i) -> int {
+        if (KIND == 0) return pid[heO(he, (int)i)];
+        if (KIND == 1) return pid[heO(he, face_he[i])];
+        return pid[i];
+    };
+    const int pbase = problemOf(c0);
+    __syncthreads();
+    // uniform trip count so that every lane takes part in the wave reductions
+    for (int64_t i0 = c0; i0 < c1; i0 += blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool inRange = i < c1;
+        if (KIND == 0) {
+            const bool live = inRange && brank[i] == 0;
+            const int p = live ? problemOf(i) : 0;
+            const int nb = live ? bsize[i] : 0;
+            statAdd(acc, out, live, p, pbase, 0, 0, nb == 4);
+            statAdd(acc, out, live, p, pbase, 1, 1, nb == 3);
+            statAdd(acc, out, live, p, pbase, 2, 2, nb > 0);
+        } else if (KIND == 1) {
+            int p = 0, k = 0;
+            if (inRange) {
+                int h = face_he[i];
+                p = pid[heO(he, h)];
+                for (int j = 0; j < 4; ++j) {
+                    if (bsize[h] > 2) ++k;
+                    h = ((const int*)he)[4 * (int64_t)h + 2];
+                }
             }
+            for (int kk = 0; kk < 5; ++kk) statAdd(acc, out, inRange, p, pbase, 3 + kk, kk, k == kk);
+        } else {
+            const int p = inRange ? pid[i] : 0;
+            statAdd(acc, out, inRange, p, pbase, 8, 0, inRange && (vflags[i] & VF_FANS) ? 1u : 0u);
         }
-        for (int kk = 0; kk < 5; ++kk) addPer(out, live, p, 3 + kk, k == kk);
     }
-    {
-        const bool live = i < V;
-        addPer(out, live, live ? pid[i] : 0, 8, live && (vflags[i] & VF_FANS) ? 1u : 0u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kStatRows * nslots; t += blockDim.x) {
+        const int r = t / nslots, s = t % nslots;
+        const unsigned v = acc[r * 5 + s];
+        if (v) atomicAdd(&out[(int64_t)(pbase + r) * 9 + base + s], (unsigned long long)v);
     }
 }
 
@@ -238,19 +282,35 @@ int32_t* Pipeline::stableSortByKey(const int32_t* key, int64_t n, int B, const s
     return a;
 }
 
+// counts of keys that arrive sorted, into count[k * stride + slot] (k in [0, B))
+void Pipeline::countSorted(const int32_t* key, const int32_t* perm, int64_t n, int B, unsigned long long* count, int stride, int slot) {
+    int64_t* first = arena.get<int64_t>("key_first", (size_t)B + 1);
+    int* bad = arena.get<int>("batch_bad", 1);
+    kKeyStarts<<<gridFor(n + 1), kBlock, 0, stream>>>(key, perm, n, B, first, bad);
+    kCountsFromStarts<<<gridFor(B), kBlock, 0, stream>>>(first, B, count, stride, slot);
+}
+
+void Pipeline::checkBatchKeys() {
+    int bad = 0;
+    MDMC_HIP(hipMemcpyAsync(&bad, arena.get<int>("batch_bad", 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    if (bad) throw HipError("batched form: elements out of problem order", -4);
+}
+
 void Pipeline::problemDmcCounts(int B) {
+    zero(arena.get<int>("batch_bad", 1), sizeof(int));
     batchDmc.assign(2 * (size_t)B, 0);
     unsigned long long* cnt = arena.get<unsigned long long>("batch_dmc", 2 * (size_t)B);
     zero(cnt, 2 * (size_t)B * sizeof(unsigned long long));
-    if (V > 0) kCountSorted<<<gridFor(V), kBlock, 0, stream>>>(pid(), nullptr, V, cnt, 2, 0);
+    if (V > 0) countSorted(pid(), nullptr, V, B, cnt, 2, 0);
     if (Q > 0) {
         int32_t* qk = arena.get<int32_t>("batch_qkey", (size_t)Q);
         // a quad's problem is its first vertex's: reuse the halfedge-key kernel on the quads (x = first vertex)
         kHalfedgeKeys<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, pid(), qk);
-        kCountSorted<<<gridFor(Q), kBlock, 0, stream>>>(qk, nullptr, Q, cnt, 2, 1);
+        countSorted(qk, nullptr, Q, B, cnt, 2, 1);
     }
     MDMC_HIP(hipMemcpyAsync(batchDmc.data(), cnt, 2 * (size_t)B * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    MDMC_HIP(hipStreamSynchronize(stream));
+    checkBatchKeys();
 }
 
 // permutation (new -> old) and its inverse for one array of n elements whose first n0
@@ -263,8 +323,8 @@ void Pipeline::placeByProblem(const int32_t* key, int64_t n0, int64_t n, int B, 
     const int32_t* tp = stableSortByKey(key + n0, nt, B, tag + "_tail");
     unsigned long long* cnt = arena.get<unsigned long long>(tag + "_cnt", 2 * (size_t)B);
     zero(cnt, 2 * (size_t)B * sizeof(unsigned long long));
-    if (n0 > 0) kCountSorted<<<gridFor(n0), kBlock, 0, stream>>>(key, nullptr, n0, cnt, 2, 0);
-    if (nt > 0) kCountSorted<<<gridFor(nt), kBlock, 0, stream>>>(key + n0, tp, nt, cnt, 2, 1);
+    if (n0 > 0) countSorted(key, nullptr, n0, B, cnt, 2, 0);
+    if (nt > 0) countSorted(key + n0, tp, nt, B, cnt, 2, 1);
     std::vector<unsigned long long> c(2 * (size_t)B);
     MDMC_HIP(hipMemcpyAsync(c.data(), cnt, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));
@@ -282,6 +342,7 @@ void Pipeline::placeByProblem(const int32_t* key, int64_t n0, int64_t n, int B, 
 }
 
 void Pipeline::regroupBatch(int B) {
+    zero(arena.get<int>("batch_bad", 1), sizeof(int));
     int32_t* vp = pid();
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
@@ -291,7 +352,7 @@ void Pipeline::regroupBatch(int B) {
     if (vertHeQ > 0) {
         int32_t* qk = arena.get<int32_t>("batch_qkey", (size_t)vertHeQ);
         kHalfedgeKeys<<<gridFor(vertHeQ), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), vertHeQ, vp, qk);
-        kCountSorted<<<gridFor(vertHeQ), kBlock, 0, stream>>>(qk, nullptr, vertHeQ, qc, 1, 0);
+        countSorted(qk, nullptr, vertHeQ, B, qc, 1, 0);
     }
     int32_t* vhInit = arena.get<int32_t>("batch_vh_init", (size_t)std::max<int64_t>(vertHeV, 1));
     if (vertHeV > 0) kVertexHalfedgeInitBatch<<<gridFor(vertHeV), kBlock, 0, stream>>>(vhInit, vertHeV);
@@ -345,13 +406,20 @@ void Pipeline::regroupBatch(int B) {
     // per-problem statistics
     unsigned long long* st = arena.get<unsigned long long>("batch_stats", 9 * (size_t)B);
     zero(st, 9 * (size_t)B * sizeof(unsigned long long));
-    const int64_t n = std::max(H, std::max(F, V));
-    if (n > 0)
-        kStatsBatch<<<gridFor(n), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int32_t>("face_he"), F, arena.ptr<int32_t>("bsize"),
-                                                       arena.ptr<int32_t>("brank"), arena.ptr<uint32_t>("vflags"), pid(), V, st);
+    {
+        const int4* h4 = arena.ptr<int4>("he");
+        const int32_t* fh = arena.ptr<int32_t>("face_he");
+        const int32_t* bs = arena.ptr<int32_t>("bsize");
+        const int32_t* br = arena.ptr<int32_t>("brank");
+        const uint32_t* vf = arena.ptr<uint32_t>("vflags");
+        auto chunks = [](int64_t n) { return (unsigned)((n + kStatChunk - 1) / kStatChunk); };
+        if (H > 0) kStatsBatch<0><<<chunks(H), kBlock, 0, stream>>>(h4, fh, bs, br, vf, pid(), H, st);
+        if (F > 0) kStatsBatch<1><<<chunks(F), kBlock, 0, stream>>>(h4, fh, bs, br, vf, pid(), F, st);
+        if (V > 0) kStatsBatch<2><<<chunks(V), kBlock, 0, stream>>>(h4, fh, bs, br, vf, pid(), V, st);
+    }
     batchStats.assign(9 * (size_t)B, 0);
     MDMC_HIP(hipMemcpyAsync(batchStats.data(), st, batchStats.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    MDMC_HIP(hipStreamSynchronize(stream));
+    checkBatchKeys();
 }
 
 }  // namespace mdmc

@@ -93,6 +93,8 @@ class Pipeline {
     void simplifyCompactAndRebuild(int dropType, bool p3333);
     void buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag);
     int32_t* stableSortByKey(const int32_t* key, int64_t n, int B, const std::string& tag);
+    void countSorted(const int32_t* key, const int32_t* perm, int64_t n, int B, unsigned long long* count, int stride, int slot);
+    void checkBatchKeys();
     void placeByProblem(const int32_t* key, int64_t n0, int64_t n, int B, const std::string& tag, int32_t** perm, int32_t** inv,
                         std::vector<int64_t>& off);
 };

@@ -64,7 +64,9 @@ def test_autotuner_picks_a_shape_once_per_class_and_keeps_the_mesh():
         shape, ms = auto.tile_shape()
         assert shape in mdmc.TILE_SHAPES
         assert all(v > 0 for v in ms.values()), ms
-        assert ms[shape] == min(ms.values())
+        # the fastest shape, a launch-order shape (4-6) counted 3% slower than its time
+        cost = {k: v * (1.0 if k <= 3 else 1.03) for k, v in ms.items()}
+        assert cost[shape] == min(cost.values())
         g2, _ = extract(auto, f, 0.1, flags)  # same class: no new tuning
         assert auto.tile_shape() == (shape, ms)
         fixed.set_tile_shape(2 if shape != 2 else 5)
