@@ -86,7 +86,7 @@ class Pipeline {
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
     void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
-    void buildBuckets(bool fromQuads, const int32_t* list, int64_t nList);
+    void buildBuckets(bool fromQuads, const uint8_t* only);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);
     void checkValid(const char* what);
     void simplifyInit(int32_t** valence, int32_t** count);

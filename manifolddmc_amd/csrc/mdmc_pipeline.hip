@@ -49,15 +49,13 @@ __device__ __forceinline__ void setT(int4* he, int h, int t) { ((int*)he)[4 * (i
 // ---------------------------------------------------------------- halfedge construction
 // bucket order after refreshHashTable: (face, position along the face from its first halfedge)
 // and ResetFaceFlags (refreshFaceFlagsGPU) ahead of the NM bits the bucket pass sets again
-__global__ void kFaceKeys(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, int32_t* __restrict__ hkey,
-                          uint32_t* __restrict__ fattr) {
+// refreshHashTable's per-face part: the face's first halfedge must be a halfedge id, and
+// the pattern / remove / NM bits are cleared (NM is set again from the bucket sizes)
+__global__ void kFaceClear(const int32_t* __restrict__ face_he, int64_t F, int64_t H, uint32_t* __restrict__ fattr, int64_t* ctr) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
-    int h = face_he[f];
-    for (int i = 0; i < 4; ++i) {
-        hkey[h] = (int32_t)(4 * f + i);
-        h = heN(he, h);
-    }
+    const int h = face_he[f];
+    if (h < 0 || h >= H) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
     fattr[f] &= ~(uint32_t)(QA_PATTERN | QA_REMOVE | QA_NM);
 }
 
@@ -112,25 +110,48 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
     vert_he[i] = i < n0 ? -1 : 0;
 }
 
-// first list pass of refreshHashTable: R[h] = (rank in m's list, m, own entry, key), for
-// every halfedge or for the ones in `list`
-__global__ void kCountMin(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ list, const int32_t* __restrict__ hkey,
-                          int32_t* __restrict__ degM, int4* __restrict__ R) {
+// refreshHashTable's first list pass after a split step.  Each halfedge is checked (ids
+// in range, itself on its face's cycle); it is dirty -- its bucket can have changed --
+// when one of its endpoints was touched or it was appended.  A dirty halfedge gets its key
+// 4 * face + position in the face's cycle and R[h] = (rank in m's list, m, own entry, key).
+__global__ void kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t oldH, const int32_t* __restrict__ face_he, int64_t F, int64_t V,
+                              const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ degM, int4* __restrict__ R, uint8_t* __restrict__ dirty,
+                              int64_t* ctr) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= H) return;
-    const int h = list ? list[t] : (int)t;
+    const int h = (int)t;
     const int4 e = he[h];
-    const int a = e.x, b = heO(he, e.z);
-    const int m = min(a, b);
-    R[h] = make_int4(atomicAdd(&degM[m], 1), m, pairEntry(a, b), hkey[h]);
+    uint8_t d = 0;
+    bool bad = e.x < 0 || e.x >= V || e.y < 0 || e.y >= F || e.z < 0 || e.z >= H || e.w < -1 || e.w >= H;
+    int b = 0;
+    if (!bad) {
+        b = heO(he, e.z);
+        bad = b < 0 || b >= V;
+    }
+    if (!bad && (t >= oldH || dirtyV[e.x] || dirtyV[b])) {
+        int k = face_he[e.y], pos = 0;
+        while (pos < 4 && k != h && k >= 0 && k < H) {
+            k = heN(he, k);
+            ++pos;
+        }
+        if (k != h) {
+            bad = true;
+        } else {
+            const int m = min(e.x, b);
+            R[h] = make_int4(atomicAdd(&degM[m], 1), m, pairEntry(e.x, b), 4 * e.y + pos);
+            d = 1;
+        }
+    }
+    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+    dirty[h] = d;
 }
 
 // second list pass, after the scan of the counts: the entry goes to offM[m] + rank and
 // R[h] becomes (list start, list length, own entry, key)
-__global__ void kPlaceEntries(int64_t H, const int32_t* __restrict__ list, const int64_t* __restrict__ offM, int4* __restrict__ R, int2* __restrict__ A) {
+__global__ void kPlaceEntries(int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM, int4* __restrict__ R, int2* __restrict__ A) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= H) return;
-    const int h = list ? list[t] : (int)t;
+    if (t >= H || (only && !only[t])) return;
+    const int h = (int)t;
     const int4 r = R[h];
     const int64_t lo = offM[r.y];
     const int len = (int)(offM[r.y + 1] - lo);
@@ -180,12 +201,12 @@ __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF
 //     as OR-ed bits on the origin of h (vflags cleared beforehand),
 //   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given),
 //   the simplifications' vertex valence when valence is given.
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const int32_t* __restrict__ list, const int4* __restrict__ R, const int2* __restrict__ A,
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int4* __restrict__ R, const int2* __restrict__ A,
                          int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
                          uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H) return;
-    const int h = list ? list[hh] : (int)hh;
+    if (hh >= H || (only && !only[hh])) return;
+    const int h = (int)hh;
     const int4 r = R[h];
     const int o = entryOther(r.z), key = r.w;
     int n = 0, rk = 0, partner = -1;
@@ -258,19 +279,6 @@ __global__ void kFlagsFromBuckets(const int4* __restrict__ he, int64_t H, const 
     const int4 e = he[h];
     atomicOr(&vflags[e.x], vf);
     if (n == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
-}
-
-// refreshHashTable after a split step: the halfedges whose bucket can have changed are
-// those with an endpoint the splitter touched, and the ones it appended
-__global__ void kDirtyHalfedges(const int4* __restrict__ he, int64_t H, int64_t oldH, const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ flag) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
-    int f = 1;
-    if (h < oldH) {
-        const int4 e = he[h];
-        f = (dirtyV[e.x] || dirtyV[heO(he, e.z)]) ? 1 : 0;
-    }
-    flag[h] = f;
 }
 
 // outgoing-halfedge lists (by origin) for the fan detection, of the vertices in `only`
@@ -1522,15 +1530,15 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 // then the per-vertex bucket pass
 // bucket lists by smaller endpoint from the counts in degM and the records of the first
 // list pass, then the per-halfedge bucket pass
-void Pipeline::buildBuckets(bool fromQuads, const int32_t* list, int64_t nList) {
+void Pipeline::buildBuckets(bool fromQuads, const uint8_t* only) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
     int4* R = arena.ptr<int4>("bl_rec");
-    const int64_t n = list ? nList : H;
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
-    int2* A = arena.get<int2>("listA", (size_t)std::max<int64_t>(n, 1));
+    // entries of every halfedge, or of the flagged ones (at most H either way)
+    int2* A = arena.get<int2>("listA", (size_t)std::max<int64_t>(H, 1));
     int32_t* bsize = arena.ptr<int32_t>("bsize");
     int32_t* brank = arena.ptr<int32_t>("brank");
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
@@ -1542,12 +1550,12 @@ void Pipeline::buildBuckets(bool fromQuads, const int32_t* list, int64_t nList) 
         zero(valence, (size_t)V * sizeof(int32_t));
         valenceReady = true;
     }
-    if (n > 0) {
-        kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(n, list, offM, R, A);
-        kBuckets<<<gridFor(n), kBlock, 0, stream>>>(he, n, list, R, A, fromQuads ? 1 : 0, bsize, brank, list ? nullptr : vflags,
+    if (H > 0) {
+        kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(H, only, offM, R, A);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, R, A, fromQuads ? 1 : 0, bsize, brank, only ? nullptr : vflags,
                                                     arena.ptr<uint32_t>("fattr"), fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
     }
-    if (list && H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, arena.ptr<uint32_t>("fattr"));
+    if (only && H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, arena.ptr<uint32_t>("fattr"));
 }
 
 void Pipeline::halfedgeInitialize() {
@@ -1569,7 +1577,7 @@ void Pipeline::halfedgeInitialize() {
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
         kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, R);
-    buildBuckets(true, nullptr, 0);
+    buildBuckets(true, nullptr);
 }
 
 void Pipeline::checkValid(const char* what) {
@@ -1577,36 +1585,23 @@ void Pipeline::checkValid(const char* what) {
     if (bad) throw HipError(std::string("inconsistent mesh after ") + what + " (" + std::to_string(bad) + " bad ids)", -4);
 }
 
-// refreshHashTable (HalfedgeMesh.cu:1063-1100).  With dirtyV (vertices the splitter
-// touched) only the halfedges whose bucket can have changed are re-bucketed; the others
-// keep their sizes and ranks (same members, same keys), and the flags are rebuilt from
-// all sizes.
+// refreshHashTable (HalfedgeMesh.cu:1063-1100) after a split step: only the halfedges
+// whose bucket can have changed (an endpoint in dirtyV, or appended after oldH) are
+// re-bucketed; the others keep their sizes and ranks (same members, same keys), and the
+// flags are rebuilt from all sizes.
 void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
-    const int64_t n = std::max(H, F);
-    if (n > 0) kValidateMesh<<<gridFor(n), kBlock, 0, stream>>>(he, H, face_he, F, V, counters());
-    checkValid("splitting");
-    int32_t* hkey = arena.get<int32_t>("hkey", (size_t)H);
-    if (F > 0) kFaceKeys<<<gridFor(F), kBlock, 0, stream>>>(he, face_he, F, hkey, arena.ptr<uint32_t>("fattr"));
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
     int4* R = arena.get<int4>("bl_rec", (size_t)H);
     arena.grow<int32_t>("bsize", (size_t)H, (size_t)std::min(oldH, H), stream);
     arena.grow<int32_t>("brank", (size_t)H, (size_t)std::min(oldH, H), stream);
+    uint8_t* dirty = arena.get<uint8_t>("dirty_he", (size_t)std::max<int64_t>(H, 1));
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
-    if (!dirtyV) {
-        if (H > 0) kCountMin<<<gridFor(H), kBlock, 0, stream>>>(he, H, nullptr, hkey, degM, R);
-        buildBuckets(false, nullptr, 0);
-        return;
-    }
-    int32_t* flag = arena.get<int32_t>("dirty_flag", (size_t)std::max<int64_t>(H, 1));
-    if (H > 0) kDirtyHalfedges<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, dirtyV, flag);
-    int64_t* addr = nullptr;
-    const int64_t nd = scan32(flag, H, "dirty_addr", &addr);
-    int32_t* list = arena.get<int32_t>("dirty_list", (size_t)std::max<int64_t>(nd, 1));
-    if (H > 0) kScatterIndex<<<gridFor(H), kBlock, 0, stream>>>(flag, addr, H, list);
-    if (nd > 0) kCountMin<<<gridFor(nd), kBlock, 0, stream>>>(he, nd, list, hkey, degM, R);
-    buildBuckets(false, list, nd);
+    if (F > 0) kFaceClear<<<gridFor(F), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
+    if (H > 0) kDirtyRecords<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, R, dirty, counters());
+    checkValid("splitting");
+    buildBuckets(false, dirty);
 }
 
 // ---- simplifications
