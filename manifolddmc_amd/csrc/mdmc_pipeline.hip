@@ -62,16 +62,27 @@ __global__ void kFaceClear(const int32_t* __restrict__ face_he, int64_t F, int64
 // ---- buckets: the halfedges joining one vertex pair.  They are listed under the
 // smaller endpoint m: one short contiguous list per vertex holds every bucket of m.
 // A list entry A[i] is (other endpoint | 0x80000000 when the halfedge starts at the
-// other endpoint, bucket key); after an initialize the key is the halfedge id.  Each
-// halfedge keeps a record R[h] = (list start, list length, own entry) so that the
-// bucket pass reads its list with no index lookups.
+// other endpoint, bucket key); after an initialize the key is the halfedge id, after a
+// refresh it is kept in key[h].  A halfedge finds its list from its own endpoints and the
+// list offsets; the first list pass leaves only its rank in the list (rank[h]).
 __device__ __forceinline__ int pairEntry(int a, int b) { return a <= b ? b : (int)(0x80000000u | (uint32_t)a); }
 __device__ __forceinline__ int entryOther(int x) { return x & 0x7FFFFFFF; }
 
+// list of the smaller endpoint of h = (e.x, origin of e.z): start, length and h's own entry
+struct ListRef {
+    int lo, len, entry;
+};
+__device__ __forceinline__ ListRef listOf(const int4* __restrict__ he, const int4& e, const int64_t* __restrict__ offM) {
+    const int b = heO(he, e.z);
+    const int m = min(e.x, b);
+    const int64_t lo = offM[m];
+    return ListRef{(int)lo, (int)(offM[m + 1] - lo), pairEntry(e.x, b)};
+}
+
 // HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
-// first list pass: R[h] = (rank in m's list, m, own entry, key = h)
+// first list pass: rank[h] = rank of h in the list of its smaller endpoint m
 __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM, int4* __restrict__ R) {
+                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM, int32_t* __restrict__ rank) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
@@ -85,8 +96,7 @@ __global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restr
     const int c[5] = {v.x, v.y, v.z, v.w, v.x};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int m = min(c[i], c[i + 1]);
-        R[b + i] = make_int4(atomicAdd(&degM[m], 1), m, pairEntry(c[i], c[i + 1]), b + i);
+        rank[b + i] = atomicAdd(&degM[min(c[i], c[i + 1])], 1);
     }
 }
 
@@ -113,10 +123,10 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
 // refreshHashTable's first list pass after a split step.  Each halfedge is checked (ids
 // in range, itself on its face's cycle); it is dirty -- its bucket can have changed --
 // when one of its endpoints was touched or it was appended.  A dirty halfedge gets its key
-// 4 * face + position in the face's cycle and R[h] = (rank in m's list, m, own entry, key).
+// key[h] = 4 * face + position in the face's cycle and rank[h] = its rank in m's list.
 __global__ void kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t oldH, const int32_t* __restrict__ face_he, int64_t F, int64_t V,
-                              const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ degM, int4* __restrict__ R, uint8_t* __restrict__ dirty,
-                              int64_t* ctr) {
+                              const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ degM, int32_t* __restrict__ rank, int32_t* __restrict__ key,
+                              uint8_t* __restrict__ dirty, int64_t* ctr) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= H) return;
     const int h = (int)t;
@@ -137,8 +147,8 @@ __global__ void kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t ol
         if (k != h) {
             bad = true;
         } else {
-            const int m = min(e.x, b);
-            R[h] = make_int4(atomicAdd(&degM[m], 1), m, pairEntry(e.x, b), 4 * e.y + pos);
+            rank[h] = atomicAdd(&degM[min(e.x, b)], 1);
+            key[h] = 4 * e.y + pos;
             d = 1;
         }
     }
@@ -146,26 +156,24 @@ __global__ void kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t ol
     dirty[h] = d;
 }
 
-// second list pass, after the scan of the counts: the entry goes to offM[m] + rank and
-// R[h] becomes (list start, list length, own entry, key)
-__global__ void kPlaceEntries(int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM, int4* __restrict__ R, int2* __restrict__ A) {
+// second list pass, after the scan of the counts: the entry (own entry, key) goes to
+// offM[m] + rank[h]; the key is h itself when key is null
+__global__ void kPlaceEntries(const int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM,
+                              const int32_t* __restrict__ rank, const int32_t* __restrict__ key, int2* __restrict__ A) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= H || (only && !only[t])) return;
     const int h = (int)t;
-    const int4 r = R[h];
-    const int64_t lo = offM[r.y];
-    const int len = (int)(offM[r.y + 1] - lo);
-    A[lo + r.x] = make_int2(r.z, r.w);
-    R[h] = make_int4((int)lo, len, r.z, r.w);
+    const ListRef l = listOf(he, he[h], offM);
+    A[l.lo + rank[h]] = make_int2(l.entry, key ? key[h] : h);
 }
 
 // the halfedges joining the two vertices of h, sorted by bucket key; returns the true
 // count.  Member ids are the keys, so this holds between an initialize and the next refresh.
-__device__ int gatherBucket(int h, const int4* __restrict__ R, const int2* __restrict__ A, int mem[kMaxBucket]) {
-    const int4 r = R[h];
-    const int o = entryOther(r.z);
+__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int2* __restrict__ A, int mem[kMaxBucket]) {
+    const ListRef l = listOf(he, he[h], offM);
+    const int o = entryOther(l.entry);
     int n = 0;
-    for (int i = r.x; i < r.x + r.y; ++i) {
+    for (int i = l.lo; i < l.lo + l.len; ++i) {
         const int2 e = A[i];
         if (entryOther(e.x) == o) {
             if (n < kMaxBucket) mem[n] = e.y;
@@ -201,16 +209,18 @@ __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF
 //     as OR-ed bits on the origin of h (vflags cleared beforehand),
 //   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given),
 //   the simplifications' vertex valence when valence is given.
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int4* __restrict__ R, const int2* __restrict__ A,
-                         int setTwins, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
-                         uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
+__global__ void kBuckets(int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM,
+                         const int32_t* __restrict__ keys, const int2* __restrict__ A, int setTwins, int32_t* __restrict__ bsize,
+                         int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr,
+                         int32_t* __restrict__ valence) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H || (only && !only[hh])) return;
     const int h = (int)hh;
-    const int4 r = R[h];
-    const int o = entryOther(r.z), key = r.w;
+    const int4 e = he[h];
+    const ListRef l = listOf(he, e, offM);
+    const int o = entryOther(l.entry), key = keys ? keys[h] : h;
     int n = 0, rk = 0, partner = -1;
-    for (int i = r.x; i < r.x + r.y; ++i) {
+    for (int i = l.lo; i < l.lo + l.len; ++i) {
         const int2 f = A[i];
         if (entryOther(f.x) != o) continue;
         ++n;
@@ -219,7 +229,6 @@ __global__ void kBuckets(int4* __restrict__ he, int64_t H, const uint8_t* __rest
     }
     bsize[h] = n;
     brank[h] = rk;
-    const int4 e = he[h];
     if (setTwins) {
         int t = -1;
         if (n == 2) {
@@ -229,11 +238,11 @@ __global__ void kBuckets(int4* __restrict__ he, int64_t H, const uint8_t* __rest
             // vertex: the one after h (n == 3), or the pairing of all four
             int byRank[4] = {-1, -1, -1, -1};
             int dirRank[4] = {0, 0, 0, 0};
-            for (int i = r.x; i < r.x + r.y; ++i) {
+            for (int i = l.lo; i < l.lo + l.len; ++i) {
                 const int2 f = A[i];
                 if (entryOther(f.x) != o) continue;
                 int rf = 0;
-                for (int k = r.x; k < r.x + r.y; ++k) {
+                for (int k = l.lo; k < l.lo + l.len; ++k) {
                     const int2 g = A[k];
                     if (entryOther(g.x) == o && g.y < f.y) ++rf;
                 }
@@ -554,7 +563,7 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restrict__ R, const int2* __restrict__ A,
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ A,
                           const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                           const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,7 +571,7 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restri
     const int h = (int)hh;
     if (bsize[h] < 4 || brank[h] != 0) return;
     int mem[kMaxBucket];
-    const int n = gatherBucket(h, R, A, mem);
+    const int n = gatherBucket(h, he, offM, A, mem);
     if (n > kMaxBucket) anomaly(ctr);
     int s[4];
     bucketSlots(mem, n, s);
@@ -1533,7 +1542,9 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 void Pipeline::buildBuckets(bool fromQuads, const uint8_t* only) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
-    int4* R = arena.ptr<int4>("bl_rec");
+    int32_t* rank = arena.ptr<int32_t>("bl_rank");
+    // bucket keys: the halfedge ids after an initialize, the refresh's keys otherwise
+    int32_t* keys = only ? arena.ptr<int32_t>("bl_key") : nullptr;
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
@@ -1551,8 +1562,8 @@ void Pipeline::buildBuckets(bool fromQuads, const uint8_t* only) {
         valenceReady = true;
     }
     if (H > 0) {
-        kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(H, only, offM, R, A);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, R, A, fromQuads ? 1 : 0, bsize, brank, only ? nullptr : vflags,
+        kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, rank, keys, A);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, keys, A, fromQuads ? 1 : 0, bsize, brank, only ? nullptr : vflags,
                                                     arena.ptr<uint32_t>("fattr"), fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
     }
     if (only && H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, arena.ptr<uint32_t>("fattr"));
@@ -1566,7 +1577,7 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
-    int4* R = arena.get<int4>("bl_rec", (size_t)H);
+    int32_t* rank = arena.get<int32_t>("bl_rank", (size_t)H);
     arena.get<int32_t>("bsize", (size_t)H);
     arena.get<int32_t>("brank", (size_t)H);
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
@@ -1576,7 +1587,7 @@ void Pipeline::halfedgeInitialize() {
     vertHeQ = Q;
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (Q > 0)
-        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, R);
+        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, rank);
     buildBuckets(true, nullptr);
 }
 
@@ -1593,13 +1604,14 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
-    int4* R = arena.get<int4>("bl_rec", (size_t)H);
+    int32_t* rank = arena.get<int32_t>("bl_rank", (size_t)H);
+    int32_t* key = arena.get<int32_t>("bl_key", (size_t)H);
     arena.grow<int32_t>("bsize", (size_t)H, (size_t)std::min(oldH, H), stream);
     arena.grow<int32_t>("brank", (size_t)H, (size_t)std::min(oldH, H), stream);
     uint8_t* dirty = arena.get<uint8_t>("dirty_he", (size_t)std::max<int64_t>(H, 1));
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     if (F > 0) kFaceClear<<<gridFor(F), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
-    if (H > 0) kDirtyRecords<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, R, dirty, counters());
+    if (H > 0) kDirtyRecords<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dirty, counters());
     checkValid("splitting");
     buildBuckets(false, dirty);
 }
@@ -1608,33 +1620,47 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
 void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* vtype = arena.ptr<int32_t>("vtype");
     int32_t* vtarget = arena.ptr<int32_t>("vtarget");
-    int32_t* keep = arena.get<int32_t>("keepV", (size_t)V);
+    const uint32_t* attr = p3333 ? arena.ptr<uint32_t>("fattr") : arena.ptr<uint32_t>("qattr");
+    // keep flags and their prefix sums for vertices and quads, with one read-back of both totals
+    int32_t* keep = arena.get<int32_t>("keepV", (size_t)std::max<int64_t>(V, 1));
+    int32_t* keepq = arena.get<int32_t>("keepQ", (size_t)std::max<int64_t>(Q, 1));
+    int64_t* vaddr = arena.get<int64_t>("vaddr", (size_t)V + 1);
+    int64_t* qaddr = arena.get<int64_t>("qaddr", (size_t)Q + 1);
+    int64_t* tot = arena.get<int64_t>("keep_totals", 2);
+    void* sv = arena.get<char>("scan_scratch_v", scanScratchBytes(V + 1));
+    void* sq = arena.get<char>("scan_scratch_q", scanScratchBytes(Q + 1));
     if (V > 0) kKeepVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, V, dropType, keep);
-    int64_t* vaddr = nullptr;
-    const int64_t nV = scan32(keep, V, "vaddr", &vaddr);
-    int32_t* address = arena.get<int32_t>("vaddress", (size_t)V);
+    if (Q > 0) kKeepQuads<<<gridFor(Q), kBlock, 0, stream>>>(attr, Q, keepq);
+    scanExclusiveI32(keep, vaddr, V, tot, sv, stream);
+    scanExclusiveI32(keepq, qaddr, Q, tot + 1, sq, stream);
+    int64_t h[4] = {0, 0, 0, 0};
+    MDMC_HIP(hipMemcpyAsync(h, tot, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipMemcpyAsync(&h[2], scanTimeoutFlag(sv), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipMemcpyAsync(&h[3], scanTimeoutFlag(sq), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    if (h[2] || h[3]) throw HipError("prefix sum: a look-back wait timed out", -4);
+    const int64_t nV = h[0], nQ = h[1];
+    int32_t* address = arena.get<int32_t>("vaddress", (size_t)std::max<int64_t>(V, 1));
     fillNeg(address, (size_t)V * sizeof(int32_t));
-    float* npos = arena.get<float>("pos2", 3 * (size_t)nV);
-    float* nnrm = arena.get<float>("nrm2", 3 * (size_t)nV);
-    int32_t* npid = pid() ? arena.get<int32_t>("pid2", (size_t)nV) : nullptr;
+    // the compacted arrays go to the second buffers, which then swap with the primary ones;
+    // both are sized for the grid pass's output so that later extractions of the same
+    // workload reallocate neither
+    const size_t capV = (size_t)std::max(nV, dmcV), capQ = (size_t)std::max(nQ, dmcQ);
+    float* npos = arena.get<float>("pos2", 3 * capV);
+    float* nnrm = arena.get<float>("nrm2", 3 * capV);
+    int32_t* npid = pid() ? arena.get<int32_t>("pid2", capV) : nullptr;
     if (V > 0)
         kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), keep, vaddr, V, npos, nnrm, npid,
                                                             address);
-    const uint32_t* attr = p3333 ? arena.ptr<uint32_t>("fattr") : arena.ptr<uint32_t>("qattr");
-    int32_t* keepq = arena.get<int32_t>("keepQ", (size_t)Q);
-    if (Q > 0) kKeepQuads<<<gridFor(Q), kBlock, 0, stream>>>(attr, Q, keepq);
-    int64_t* qaddr = nullptr;
-    const int64_t nQ = scan32(keepq, Q, "qaddr", &qaddr);
-    int4* nquads = arena.get<int4>("quads2", (size_t)nQ);
-    uint32_t* nattr = arena.get<uint32_t>("qattr2", (size_t)nQ);
+    int4* nquads = arena.get<int4>("quads2", capQ);
+    uint32_t* nattr = arena.get<uint32_t>("qattr2", capQ);
     if (Q > 0)
         kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr);
-    // back into the primary buffers (they keep the grid pass's capacity)
-    MDMC_HIP(hipMemcpyAsync(arena.ptr<float>("pos"), npos, 3 * (size_t)nV * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    MDMC_HIP(hipMemcpyAsync(arena.ptr<float>("nrm"), nnrm, 3 * (size_t)nV * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    MDMC_HIP(hipMemcpyAsync(arena.ptr<int4>("quads"), nquads, (size_t)nQ * sizeof(int4), hipMemcpyDeviceToDevice, stream));
-    MDMC_HIP(hipMemcpyAsync(arena.ptr<uint32_t>("qattr"), nattr, (size_t)nQ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-    if (npid) MDMC_HIP(hipMemcpyAsync(pid(), npid, (size_t)nV * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
+    arena.swap("pos", "pos2");
+    arena.swap("nrm", "nrm2");
+    arena.swap("quads", "quads2");
+    arena.swap("qattr", "qattr2");
+    if (npid) arena.swap("pid", "pid2");
     V = nV;
     Q = nQ;
     halfedgeInitialize();
@@ -1686,7 +1712,7 @@ void Pipeline::simplify3333() {
 
 void Pipeline::fixManifoldTwins() {
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("bl_rec"), arena.ptr<int2>("listA"),
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listA"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
                                                  counters());
 }
