@@ -211,6 +211,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
         float* up = PinTot ? p.arena.get<float>("input", (size_t)PinTot) : nullptr;
         std::vector<mdmc::ProblemDev> problems;
         problems.reserve((size_t)B);
+        std::vector<mdmc::PrepareDesc> prep((size_t)G);
         mdmc::GridDev g0{};
         for (int gi = 0; gi < G; ++gi) {
             const mdmc_grid* grid = &grids[gi];
@@ -222,7 +223,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             }
             int od[3];
             processedDims(grid, r, pad, od);
-            mdmc::launchPrepareGrid(in, grid->dims, gp + pOff[gi], od, r, pad, s);
+            prep[gi] = mdmc::PrepareDesc{in, gp + pOff[gi], grid->dims[0], grid->dims[1], grid->dims[2], od[0], od[1], od[2], pOff[gi]};
             mdmc::GridDev g;
             g.v = gp + pOff[gi];
             g.nx = od[0];
@@ -247,6 +248,11 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
                 pr.iso = isos[(int64_t)gi * I + k];
                 problems.push_back(pr);
             }
+        }
+        {
+            mdmc::PrepareDesc* pd = p.arena.get<mdmc::PrepareDesc>("prep_desc", (size_t)G);
+            MDMC_HIP(hipMemcpyAsync(pd, prep.data(), (size_t)G * sizeof(mdmc::PrepareDesc), hipMemcpyHostToDevice, s));
+            mdmc::launchPrepareGrids(pd, G, Ptot, r, pad, s);
         }
         MDMC_HIP(hipEventRecord(ctx->ev[1], s));
         // ---- DMC

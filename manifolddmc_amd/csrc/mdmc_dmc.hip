@@ -12,11 +12,19 @@ namespace {
 
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
-__global__ void __launch_bounds__(kBlock) kPrepareGrid(const float* __restrict__ in, int inx, int iny, int inz, float* __restrict__ out,
-                                                       int onx, int ony, int onz, int refinement, int pad) {
-    const int64_t total = (int64_t)onx * ony * onz;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+// one thread per point of every processed grid: grid g owns points [d[g].base, d[g + 1].base)
+__global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int64_t total, int refinement, int pad) {
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total) return;
+    int lo = 0, hi = G;  // d[lo].base <= gi
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (d[mid].base <= gi) lo = mid;
+        else hi = mid;
+    }
+    const PrepareDesc& p = d[lo];
+    const int onx = p.onx, ony = p.ony, onz = p.onz, inx = p.inx, iny = p.iny, inz = p.inz;
+    const int64_t i = gi - p.base;
     int x = (int)(i % onx);
     int y = (int)((i / onx) % ony);
     int z = (int)(i / ((int64_t)onx * ony));
@@ -32,10 +40,10 @@ __global__ void __launch_bounds__(kBlock) kPrepareGrid(const float* __restrict__
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int cx = clampi(mx + (k & 1), 0, inx - 1), cy = clampi(my + ((k >> 1) & 1), 0, iny - 1), cz = clampi(mz + ((k >> 2) & 1), 0, inz - 1);
-        s[k] = in[(int64_t)cz * inx * iny + (int64_t)cy * inx + cx];
+        s[k] = p.in[(int64_t)cz * inx * iny + (int64_t)cy * inx + cx];
     }
-    const f3 d = mk3(fx - (float)mx, fy - (float)my, fz - (float)mz);
-    out[i] = trilinear(s, d);
+    const f3 dd = mk3(fx - (float)mx, fy - (float)my, fz - (float)mz);
+    p.out[i] = trilinear(s, dd);
 }
 
 __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, uint32_t* __restrict__ idx) {
@@ -55,9 +63,8 @@ __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restric
 
 }  // namespace
 
-void launchPrepareGrid(const float* in, const int inDims[3], float* out, const int outDims[3], int refinement, int pad, hipStream_t s) {
-    const int64_t total = (int64_t)outDims[0] * outDims[1] * outDims[2];
-    kPrepareGrid<<<gridFor(total), kBlock, 0, s>>>(in, inDims[0], inDims[1], inDims[2], out, outDims[0], outDims[1], outDims[2], refinement, pad);
+void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t total, int refinement, int pad, hipStream_t s) {
+    if (total > 0) kPrepareGrids<<<gridFor(total), kBlock, 0, s>>>(descDev, G, total, refinement, pad);
 }
 
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s) {

@@ -143,6 +143,13 @@ __device__ __forceinline__ int upperIndex(const uint16_t* pre, int n, int key) {
     return lo;
 }
 
+// a power of two whose reciprocal is a normal float (exponent field 1..253, zero mantissa)
+__device__ __forceinline__ bool exactReciprocal(float d) {
+    const uint32_t b = __float_as_uint(d);
+    const uint32_t e = (b >> 23) & 0xFF;
+    return (b & 0x7FFFFF) == 0 && e >= 1 && e <= 253;
+}
+
 // sliceP for contour t of one cell: dual vertex position (world) and normal
 __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surfIter, int x, int y, int z, const float s[8], uint64_t w, int t, f3& wp,
                                            f3& nn) {
@@ -157,9 +164,27 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
         const float gx = cx ? at(clampi(px + 1, 0, nx - 1), py, pz) - s[k & ~1] : s[k | 1] - at(clampi(px - 1, 0, nx - 1), py, pz);
         const float gy = cy ? at(px, clampi(py + 1, 0, ny - 1), pz) - s[k & ~2] : s[k | 2] - at(px, clampi(py - 1, 0, ny - 1), pz);
         const float gz = cz ? at(px, py, clampi(pz + 1, 0, nz - 1)) - s[k & ~4] : s[k | 4] - at(px, py, clampi(pz - 1, 0, nz - 1));
-        gxv[k] = (0.5f * gx) / g.dx;
-        gyv[k] = (0.5f * gy) / g.dy;
-        gzv[k] = (0.5f * gz) / g.dz;
+        gxv[k] = 0.5f * gx;
+        gyv[k] = 0.5f * gy;
+        gzv[k] = 0.5f * gz;
+    }
+    // (0.5 g) / delta.  When every delta is a power of two its reciprocal is exact and the
+    // product equals the quotient bit for bit, so that case skips the division sequence.
+    if (exactReciprocal(g.dx) && exactReciprocal(g.dy) && exactReciprocal(g.dz)) {
+        const float rx = 1.0f / g.dx, ry = 1.0f / g.dy, rz = 1.0f / g.dz;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            gxv[k] *= rx;
+            gyv[k] *= ry;
+            gzv[k] *= rz;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            gxv[k] /= g.dx;
+            gyv[k] /= g.dy;
+            gzv[k] /= g.dz;
+        }
     }
     int j = 0;
     for (int c = 0; c < t; ++c) j += polySize(w, c);

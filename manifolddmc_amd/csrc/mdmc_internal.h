@@ -27,7 +27,15 @@ const int64_t* scanTimeoutFlag(const void* scratch);
 inline unsigned gridFor(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
 // --- DMC grid pass (mdmc_dmc.hip)
-void launchPrepareGrid(const float* in, const int inDims[3], float* out, const int outDims[3], int refinement, int pad, hipStream_t s);
+// padding / refinement (refineGrid) of several grids in one launch: grid g reads in[inx*iny*inz]
+// and writes out[onx*ony*onz], its points numbered from base in the launch
+struct PrepareDesc {
+    const float* in;
+    float* out;
+    int inx, iny, inz, onx, ony, onz;
+    int64_t base;
+};
+void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t total, int refinement, int pad, hipStream_t s);
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s);
 
 }  // namespace mdmc
