@@ -212,6 +212,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
         std::vector<mdmc::ProblemDev> problems;
         problems.reserve((size_t)B);
         std::vector<mdmc::PrepareDesc> prep((size_t)G);
+        int64_t prepBlocks = 0;
         mdmc::GridDev g0{};
         for (int gi = 0; gi < G; ++gi) {
             const mdmc_grid* grid = &grids[gi];
@@ -223,7 +224,8 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             }
             int od[3];
             processedDims(grid, r, pad, od);
-            prep[gi] = mdmc::PrepareDesc{in, gp + pOff[gi], grid->dims[0], grid->dims[1], grid->dims[2], od[0], od[1], od[2], pOff[gi]};
+            prep[gi] = mdmc::PrepareDesc{in, gp + pOff[gi], grid->dims[0], grid->dims[1], grid->dims[2], od[0], od[1], od[2], prepBlocks};
+            prepBlocks += mdmc::prepareBlocks((int64_t)od[0] * od[1] * od[2]);
             mdmc::GridDev g;
             g.v = gp + pOff[gi];
             g.nx = od[0];
@@ -251,8 +253,8 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
         }
         {
             mdmc::PrepareDesc* pd = p.arena.get<mdmc::PrepareDesc>("prep_desc", (size_t)G);
-            MDMC_HIP(hipMemcpyAsync(pd, prep.data(), (size_t)G * sizeof(mdmc::PrepareDesc), hipMemcpyHostToDevice, s));
-            mdmc::launchPrepareGrids(pd, G, Ptot, r, pad, s);
+            p.upload(pd, prep.data(), (size_t)G * sizeof(mdmc::PrepareDesc), 1);
+            mdmc::launchPrepareGrids(pd, G, prepBlocks, r, pad, s);
         }
         MDMC_HIP(hipEventRecord(ctx->ev[1], s));
         // ---- DMC

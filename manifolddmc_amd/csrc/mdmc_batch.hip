@@ -336,9 +336,8 @@ void Pipeline::placeByProblem(const int32_t* key, int64_t n0, int64_t n, int B, 
     }
     off.assign(offs.begin() + 2 * (B + 1), offs.end());
     int64_t* od = arena.get<int64_t>(tag + "_offs", offs.size());
-    MDMC_HIP(hipMemcpyAsync(od, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    upload(od, offs.data(), offs.size() * sizeof(int64_t), 0);
     if (n > 0) kPlace<<<gridFor(n), kBlock, 0, stream>>>(key, n0, n, tp, od, B, *perm, *inv);
-    MDMC_HIP(hipStreamSynchronize(stream));  // offs lifetime
 }
 
 void Pipeline::regroupBatch(int B) {
@@ -371,7 +370,7 @@ void Pipeline::regroupBatch(int B) {
     std::vector<int64_t> qBase((size_t)B + 1, 0);
     for (int b = 0; b < B; ++b) qBase[b + 1] = qBase[b] + (int64_t)qcount[b];
     int64_t* qBaseDev = arena.get<int64_t>("batch_qbase", (size_t)B + 1);
-    MDMC_HIP(hipMemcpyAsync(qBaseDev, qBase.data(), qBase.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    upload(qBaseDev, qBase.data(), qBase.size() * sizeof(int64_t), 1);
     float* npos = arena.get<float>("pos2", 3 * (size_t)std::max<int64_t>(V, 1));
     float* nnrm = arena.get<float>("nrm2", 3 * (size_t)std::max<int64_t>(V, 1));
     uint32_t* nvf = arena.get<uint32_t>("vflags2", (size_t)std::max<int64_t>(V, 1));

@@ -12,19 +12,20 @@ namespace {
 
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
-// one thread per point of every processed grid: grid g owns points [d[g].base, d[g + 1].base)
-__global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int64_t total, int refinement, int pad) {
-    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= total) return;
-    int lo = 0, hi = G;  // d[lo].base <= gi
+// one thread per point of every processed grid; grid g owns blocks [d[g].base, d[g + 1].base),
+// so the grid a block works on, and with it every size below, is uniform across the block
+__global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int refinement, int pad) {
+    const int64_t blk = blockIdx.x;
+    int lo = 0, hi = G;  // d[lo].base <= blk
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (d[mid].base <= gi) lo = mid;
+        if (d[mid].base <= blk) lo = mid;
         else hi = mid;
     }
     const PrepareDesc& p = d[lo];
     const int onx = p.onx, ony = p.ony, onz = p.onz, inx = p.inx, iny = p.iny, inz = p.inz;
-    const int64_t i = gi - p.base;
+    const int64_t i = (blk - p.base) * kBlock + threadIdx.x;
+    if (i >= (int64_t)onx * ony * onz) return;
     int x = (int)(i % onx);
     int y = (int)((i / onx) % ony);
     int z = (int)(i / ((int64_t)onx * ony));
@@ -63,8 +64,10 @@ __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restric
 
 }  // namespace
 
-void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t total, int refinement, int pad, hipStream_t s) {
-    if (total > 0) kPrepareGrids<<<gridFor(total), kBlock, 0, s>>>(descDev, G, total, refinement, pad);
+int64_t prepareBlocks(int64_t points) { return (points + kBlock - 1) / kBlock; }
+
+void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, hipStream_t s) {
+    if (blocks > 0) kPrepareGrids<<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad);
 }
 
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s) {

@@ -21,21 +21,24 @@ struct GridDev {
 // returns the total through *total_dev (device int64).  scratch is sized by scanScratchBytes;
 // after the scan, *scanTimeoutFlag(scratch) is nonzero if a look-back wait timed out.
 size_t scanScratchBytes(int64_t n);
-void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s);
+// total_dev and timeoutOut may point into mapped pinned host memory; a timed-out look-back
+// also sets *timeoutOut (when given) to 1
+void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s, int64_t* timeoutOut = nullptr);
 const int64_t* scanTimeoutFlag(const void* scratch);
 
 inline unsigned gridFor(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
 // --- DMC grid pass (mdmc_dmc.hip)
 // padding / refinement (refineGrid) of several grids in one launch: grid g reads in[inx*iny*inz]
-// and writes out[onx*ony*onz], its points numbered from base in the launch
+// and writes out[onx*ony*onz] with the blocks from base on (prepareBlocks(points) of them)
 struct PrepareDesc {
     const float* in;
     float* out;
     int inx, iny, inz, onx, ony, onz;
     int64_t base;
 };
-void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t total, int refinement, int pad, hipStream_t s);
+int64_t prepareBlocks(int64_t points);
+void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, hipStream_t s);
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s);
 
 }  // namespace mdmc

@@ -69,6 +69,14 @@ class Pipeline {
     void triangles(uint32_t* idx, bool dstDevice);
 
     int64_t readI64(const int64_t* dev);
+    // mapped pinned host words for small read-backs: kernels and copies write them
+    // directly, so a read-back is one stream synchronisation (slots 0..kPinSlots-1)
+    static constexpr int kPinSlots = 64;
+    int64_t* pinHost();
+    int64_t* pinDev();
+    // host -> device copy of a small host array through pinned staging (region 0 or 1, so
+    // that two uploads of one stage do not share a buffer); returns after queueing the copy
+    void upload(void* dst, const void* src, size_t bytes, int region);
     // duration of the last grid pass (classify + scan, then emit), from events on the
     // launch stream; the host read-back of the sizes between them is not counted
     void kernelTimes(double* msGridPass);
@@ -81,6 +89,11 @@ class Pipeline {
     double tuneMs[GP_SHAPE_LAST + 1] = {};
 
    private:
+    int64_t* pinH_ = nullptr;
+    int64_t* pinD_ = nullptr;
+    void* up_[2] = {nullptr, nullptr};
+    hipEvent_t upEv_[2] = {nullptr, nullptr};
+    size_t upCap_[2] = {0, 0};
     int64_t* counters();
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
     void zero(void* p, size_t bytes);

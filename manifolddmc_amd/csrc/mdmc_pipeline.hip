@@ -1379,6 +1379,45 @@ Pipeline::Pipeline() {}
 Pipeline::~Pipeline() {
     for (auto& e : kev)
         if (e) (void)hipEventDestroy(e);
+    if (pinH_) (void)hipHostFree(pinH_);
+    for (auto& e : upEv_)
+        if (e) (void)hipEventSynchronize(e), (void)hipEventDestroy(e);
+    for (void* u : up_)
+        if (u) (void)hipHostFree(u);
+}
+
+void Pipeline::upload(void* dst, const void* src, size_t bytes, int region) {
+    if (bytes == 0) return;
+    if (!upEv_[region]) MDMC_HIP(hipEventCreateWithFlags(&upEv_[region], hipEventDisableTiming));
+    // the last copy out of this region must be done before its bytes change (normally long done)
+    MDMC_HIP(hipEventSynchronize(upEv_[region]));
+    if (upCap_[region] < bytes) {
+        if (up_[region]) MDMC_HIP(hipHostFree(up_[region]));
+        up_[region] = nullptr;
+        upCap_[region] = 0;
+        const size_t want = bytes + bytes / 2 + 4096;
+        MDMC_HIP(hipHostMalloc(&up_[region], want, hipHostMallocDefault));
+        upCap_[region] = want;
+    }
+    std::memcpy(up_[region], src, bytes);
+    MDMC_HIP(hipMemcpyAsync(dst, up_[region], bytes, hipMemcpyHostToDevice, stream));
+    MDMC_HIP(hipEventRecord(upEv_[region], stream));
+}
+
+int64_t* Pipeline::pinHost() {
+    if (!pinH_) {
+        void* h = nullptr;
+        MDMC_HIP(hipHostMalloc(&h, kPinSlots * sizeof(int64_t), hipHostMallocMapped));
+        pinH_ = (int64_t*)h;
+        void* d = nullptr;
+        MDMC_HIP(hipHostGetDevicePointer(&d, h, 0));
+        pinD_ = (int64_t*)d;
+    }
+    return pinH_;
+}
+int64_t* Pipeline::pinDev() {
+    pinHost();
+    return pinD_;
 }
 
 void Pipeline::kernelTimes(double* msGridPass) {
@@ -1389,10 +1428,10 @@ void Pipeline::kernelTimes(double* msGridPass) {
 }
 
 int64_t Pipeline::readI64(const int64_t* dev) {
-    int64_t v = 0;
-    MDMC_HIP(hipMemcpyAsync(&v, dev, sizeof v, hipMemcpyDeviceToHost, stream));
+    int64_t* h = pinHost();
+    MDMC_HIP(hipMemcpyAsync(h, dev, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));
-    return v;
+    return h[0];
 }
 
 int64_t* Pipeline::counters() { return arena.get<int64_t>("counters", A_NCOUNTERS); }
@@ -1400,14 +1439,14 @@ int64_t* Pipeline::counters() { return arena.get<int64_t>("counters", A_NCOUNTER
 int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out) {
     *out = arena.get<int64_t>(outName, (size_t)n + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(n + 1));
-    int64_t* tot = arena.get<int64_t>("scan_total", 1);
-    scanExclusiveI32(in, *out, n, tot, scratch, stream);
-    int64_t h[2] = {0, 0};
-    MDMC_HIP(hipMemcpyAsync(&h[0], tot, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    MDMC_HIP(hipMemcpyAsync(&h[1], scanTimeoutFlag(scratch), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    // total and timeout flag straight into pinned host memory
+    int64_t* h = pinHost();
+    int64_t* d = pinDev();
+    h[0] = 0;
+    scanExclusiveI32(in, *out, n, d + 1, scratch, stream, d);
     MDMC_HIP(hipStreamSynchronize(stream));
-    if (h[1]) throw HipError("prefix sum: a look-back wait timed out", -4);
-    return h[0];
+    if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
+    return h[1];
 }
 
 void Pipeline::zero(void* p, size_t bytes) { MDMC_HIP(hipMemsetAsync(p, 0, bytes, stream)); }
@@ -1477,7 +1516,7 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
         numTiles += (pr.cells + T - 1) / T;
     }
     probsDev = arena.get<ProblemDev>("gp_probs", problems.size());
-    MDMC_HIP(hipMemcpyAsync(probsDev, problems.data(), problems.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, stream));
+    upload(probsDev, problems.data(), problems.size() * sizeof(ProblemDev), 0);
     GridPassArgs a;
     a.probs = probsDev;
     a.nProblems = nProblems;
@@ -1495,24 +1534,21 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     a.tileQOff = tqo;
     void* scratchV = arena.get<char>("scan_scratch", scanScratchBytes(numTiles + 1));
     void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(numTiles + 1));
-    int64_t* tot = arena.get<int64_t>("gp_totals", 2);
+    // totals and timeout flag straight into pinned host memory: [0] timeout, [1] V, [2] Q
+    int64_t* h = pinHost();
+    int64_t* pd = pinDev();
+    h[0] = h[1] = h[2] = 0;
     MDMC_HIP(hipEventRecord(kev[0], stream));
     launchGridClassify(shape, a, stream);
     if (numTiles > 0) {
-        scanExclusiveI32(a.tileV, tvo, numTiles + 1, tot, scratchV, stream);
-        scanExclusiveI32(a.tileQ, tqo, numTiles + 1, tot + 1, scratchQ, stream);
+        scanExclusiveI32(a.tileV, tvo, numTiles + 1, pd + 1, scratchV, stream, pd);
+        scanExclusiveI32(a.tileQ, tqo, numTiles + 1, pd + 2, scratchQ, stream, pd);
     }
     MDMC_HIP(hipEventRecord(kev[1], stream));
-    int64_t h[4] = {0, 0, 0, 0};
-    if (numTiles > 0) {
-        MDMC_HIP(hipMemcpyAsync(h, tot, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-        MDMC_HIP(hipMemcpyAsync(&h[2], scanTimeoutFlag(scratchV), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-        MDMC_HIP(hipMemcpyAsync(&h[3], scanTimeoutFlag(scratchQ), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    }
     MDMC_HIP(hipStreamSynchronize(stream));
-    if (h[2] || h[3]) throw HipError("prefix sum: a look-back wait timed out", -4);
-    V = h[0];
-    Q = h[1];
+    if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
+    V = h[1];
+    Q = h[2];
     if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
     if (recordAdjoint && (V >= ((int64_t)1 << 29) || C >= (int64_t)INT32_MAX))
         throw HipError("adjoint recording needs fewer than 2^29 vertices and 2^31 cells", -3);
@@ -1626,20 +1662,19 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* keepq = arena.get<int32_t>("keepQ", (size_t)std::max<int64_t>(Q, 1));
     int64_t* vaddr = arena.get<int64_t>("vaddr", (size_t)V + 1);
     int64_t* qaddr = arena.get<int64_t>("qaddr", (size_t)Q + 1);
-    int64_t* tot = arena.get<int64_t>("keep_totals", 2);
     void* sv = arena.get<char>("scan_scratch_v", scanScratchBytes(V + 1));
     void* sq = arena.get<char>("scan_scratch_q", scanScratchBytes(Q + 1));
     if (V > 0) kKeepVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, V, dropType, keep);
     if (Q > 0) kKeepQuads<<<gridFor(Q), kBlock, 0, stream>>>(attr, Q, keepq);
-    scanExclusiveI32(keep, vaddr, V, tot, sv, stream);
-    scanExclusiveI32(keepq, qaddr, Q, tot + 1, sq, stream);
-    int64_t h[4] = {0, 0, 0, 0};
-    MDMC_HIP(hipMemcpyAsync(h, tot, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    MDMC_HIP(hipMemcpyAsync(&h[2], scanTimeoutFlag(sv), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    MDMC_HIP(hipMemcpyAsync(&h[3], scanTimeoutFlag(sq), sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    // totals and timeout flag straight into pinned host memory: [0] timeout, [1] V, [2] Q
+    int64_t* h = pinHost();
+    int64_t* pd = pinDev();
+    h[0] = h[1] = h[2] = 0;
+    scanExclusiveI32(keep, vaddr, V, pd + 1, sv, stream, pd);
+    scanExclusiveI32(keepq, qaddr, Q, pd + 2, sq, stream, pd);
     MDMC_HIP(hipStreamSynchronize(stream));
-    if (h[2] || h[3]) throw HipError("prefix sum: a look-back wait timed out", -4);
-    const int64_t nV = h[0], nQ = h[1];
+    if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
+    const int64_t nV = h[1], nQ = h[2];
     int32_t* address = arena.get<int32_t>("vaddress", (size_t)std::max<int64_t>(V, 1));
     fillNeg(address, (size_t)V * sizeof(int32_t));
     // the compacted arrays go to the second buffers, which then swap with the primary ones;

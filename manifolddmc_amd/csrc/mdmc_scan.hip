@@ -32,7 +32,8 @@ __global__ void kScanReset(unsigned long long* scratch, int64_t words) {
 }
 
 __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ in, int64_t* __restrict__ out, int64_t n, int64_t numTiles,
-                                                    unsigned long long* __restrict__ scratch, int64_t* __restrict__ total) {
+                                                    unsigned long long* __restrict__ scratch, int64_t* __restrict__ total,
+                                                    int64_t* __restrict__ timeoutOut) {
     __shared__ int32_t s_in[kTile];
     __shared__ int64_t s_wave[kScanBlock / 64];
     __shared__ long long s_tile, s_off;
@@ -97,7 +98,10 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
                 const uint64_t upto = firstInc >= 63 ? ~0ULL : ((1ULL << (firstInc + 1)) - 1ULL);
                 if (zeroMask & upto) {
                     if (++spins > kSpinLimit) {
-                        if (lane == 0) storeRelaxed(&scratch[1], 1ULL);
+                        if (lane == 0) {
+                            storeRelaxed(&scratch[1], 1ULL);
+                            if (timeoutOut) *timeoutOut = 1;
+                        }
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -146,7 +150,7 @@ size_t scanScratchBytes(int64_t n) {
     return (size_t)(tiles + 2) * sizeof(unsigned long long) + 256;
 }
 
-void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total, void* scratch, hipStream_t s) {
+void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total, void* scratch, hipStream_t s, int64_t* timeoutOut) {
     unsigned long long* w = (unsigned long long*)scratch;
     const int64_t tiles = n > 0 ? (n + kTile - 1) / kTile : 0;
     const int64_t words = tiles + 2;
@@ -155,7 +159,7 @@ void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total
         kScanReset<<<1, 1, 0, s>>>((unsigned long long*)total, 1);
         return;
     }
-    kScan<<<(unsigned)tiles, kScanBlock, 0, s>>>(in, out, n, tiles, w, total);
+    kScan<<<(unsigned)tiles, kScanBlock, 0, s>>>(in, out, n, tiles, w, total, timeoutOut);
 }
 
 const int64_t* scanTimeoutFlag(const void* scratch) { return (const int64_t*)scratch + 1; }
