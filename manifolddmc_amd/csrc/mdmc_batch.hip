@@ -55,11 +55,6 @@ __global__ void kSplitScatter(const int32_t* __restrict__ perm, const int32_t* _
     out[zero[i] ? p : *zeros + (i - p)] = perm[i];
 }
 
-__global__ void kInverse(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) inv[perm[i]] = (int32_t)i;
-}
-
 // final position of every element: prefix element i of problem b -> out[b] + i - pre[b];
 // tail element at sorted tail position j -> out[b] + preCount[b] + j - tail[b]
 // (offsets: [0, B] prefix starts, [B+1, 2B+1] tail starts, [2B+2, 3B+2] output starts)

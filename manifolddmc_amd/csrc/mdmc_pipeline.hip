@@ -1312,20 +1312,6 @@ __global__ void kValidateQuads(const int4* __restrict__ quads, int64_t Q, int64_
     if (v.x < 0 || v.x >= V || v.y < 0 || v.y >= V || v.z < 0 || v.z >= V || v.w < 0 || v.w >= V)
         atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
 }
-__global__ void kValidateMesh(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F, int64_t V, int64_t* ctr) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool bad = false;
-    if (i < H) {
-        const int4 e = he[i];
-        bad = bad || e.x < 0 || e.x >= V || e.y < 0 || e.y >= F || e.z < 0 || e.z >= H || e.w < -1 || e.w >= H;
-    }
-    if (i < F) {
-        const int h = face_he[i];
-        bad = bad || h < 0 || h >= H;
-    }
-    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-}
-
 // ---------------------------------------------------------------- statistics
 // counts over the vertex-pair buckets (HalfedgeMesh.cu:1164-1239, ManifoldCreator.cu:545-549)
 __global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F,
