@@ -547,11 +547,14 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
 }
 
 // ---------------------------------------------------------------- fixManifoldTwins
+// floor of a cell coordinate; a non-finite or huge one (a NaN or Inf sample upstream)
+// becomes -2^30, outside every grid, instead of an undefined conversion
+__device__ __forceinline__ int cellIndex(float q) { return (q > -1.0e9f && q < 1.0e9f) ? (int)floorf(q) : -(1 << 30); }
 __device__ __forceinline__ void cellOf(const GridDev& g, f3 p, int c[3]) {
     const f3 q = (p - mk3(g.ox, g.oy, g.oz)) / mk3(g.dx, g.dy, g.dz);
-    c[0] = (int)floorf(q.x);
-    c[1] = (int)floorf(q.y);
-    c[2] = (int)floorf(q.z);
+    c[0] = cellIndex(q.x);
+    c[1] = cellIndex(q.y);
+    c[2] = cellIndex(q.z);
 }
 __device__ __forceinline__ f3 nudge(int it) {
     float x = (it & 1) ? 1.f : 0.f, y = (it & 2) ? 1.f : 0.f, z = (it & 4) ? 1.f : 0.f;
@@ -624,7 +627,7 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __res
         xo[1] = 1;
         yo[2] = 1;
     }
-    if (mi[0] < 0 || mi[1] < 0 || mi[2] < 0 || mi[0] + 1 >= g.nx || mi[1] + 1 >= g.ny || mi[2] + 1 >= g.nz) {
+    if (mi[0] < 0 || mi[1] < 0 || mi[2] < 0 || mi[0] >= g.nx - 1 || mi[1] >= g.ny - 1 || mi[2] >= g.nz - 1) {
         anomaly(ctr);
         return;
     }

@@ -767,11 +767,15 @@ static void simplify3333(Mesh<R>& m) {
 
 // --------------------------------------------------------------------------
 // ManifoldCreator::fixManifoldTwins -> reconnectHalfedges, ManifoldCreator.cu:39-312
+// floor of a cell coordinate; a non-finite or huge one becomes -2^30 (outside every grid),
+// the same value the HIP path uses, instead of an undefined conversion
+template <class R>
+static inline int cellIndex(R q) { return (q > R(-1.0e9) && q < R(1.0e9)) ? (int)std::floor(q) : -(1 << 30); }
 template <class R>
 static inline void cellOf(const Grid& g, V3<R> p, int c[3]) {
     V3<R> o = {R(g.ox), R(g.oy), R(g.oz)}, d = {R(g.dx), R(g.dy), R(g.dz)};
     V3<R> q = (p - o) / d;
-    c[0] = (int)std::floor(q.x); c[1] = (int)std::floor(q.y); c[2] = (int)std::floor(q.z);
+    c[0] = cellIndex(q.x); c[1] = cellIndex(q.y); c[2] = cellIndex(q.z);
 }
 template <class R>
 static inline V3<R> nudge(int it) {
@@ -826,7 +830,7 @@ static void fixManifoldTwins(Mesh<R>& m, const Grid& g, R iso) {
         } else { xo[1] = 1; yo[2] = 1; }
         int mi[3];
         for (int i = 0; i < 3; ++i) mi[i] = std::max(ca[i], cb[i]);
-        if (mi[0] < 0 || mi[1] < 0 || mi[2] < 0 || mi[0] + 1 >= g.nx || mi[1] + 1 >= g.ny || mi[2] + 1 >= g.nz) { m.anomalies++; continue; }
+        if (mi[0] < 0 || mi[1] < 0 || mi[2] < 0 || mi[0] >= g.nx - 1 || mi[1] >= g.ny - 1 || mi[2] >= g.nz - 1) { m.anomalies++; continue; }
         auto gv = [&](int dx, int dy, int dz) { return R(g.at(mi[0] + dx, mi[1] + dy, mi[2] + dz)); };
         R wall[4] = {gv(0, 0, 0), gv(xo[0], xo[1], xo[2]), gv(xo[0] + yo[0], xo[1] + yo[1], xo[2] + yo[2]), gv(yo[0], yo[1], yo[2])};
         V3<R> o = {R(g.ox), R(g.oy), R(g.oz)}, d = {R(g.dx), R(g.dy), R(g.dz)};

@@ -116,3 +116,39 @@ def test_empty_and_degenerate():
     import manifolddmc_amd as mdmc
     with pytest.raises(mdmc.MdmcError):
         run_gpu(np.zeros((1, 1, 1), np.float32), (0, 0, 0), (1, 1, 1), 0.0, 0)
+
+
+def nan_equal(name, a, b):
+    """Floats equal bit for bit, except that any NaN matches any NaN."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        return "%s: shape %s vs %s" % (name, a.shape, b.shape)
+    both = np.isnan(a) & np.isnan(b)
+    same = (a.view(np.uint32) == b.view(np.uint32)) | both
+    if not same.all():
+        i = np.argwhere(~same)[0]
+        return "%s: first difference at %s: %r vs %r" % (name, tuple(i), a[tuple(i)], b[tuple(i)])
+    return None
+
+
+@pytest.mark.parametrize("bad", ["nan", "inf"])
+def test_non_finite_samples_match_oracle(bad):
+    """NaN / +-Inf samples (the reference only zeroes NaN when it reads a file,
+    UniformGrid.inl:553-560; a grid handed over in memory keeps them): the mesh
+    arrays equal the oracle's, positions and normals up to NaN payloads."""
+    rng = np.random.default_rng(77)
+    v = rng.uniform(-1, 1, size=(12, 13, 14)).astype(np.float32)
+    idx = rng.choice(v.size, size=40, replace=False)
+    v.reshape(-1)[idx] = np.nan if bad == "nan" else np.where(rng.random(40) < 0.5, np.inf, -np.inf).astype(np.float32)
+    for flags in (oa.PAD1 | oa.SURF_ITER, oa.PAD1 | oa.SURF_ITER | oa.ALL_POST):
+        print("case non-finite", bad, "flags", flags, flush=True)
+        g, gst = run_gpu(v, (0, 0, 0), (1, 1, 1), 0.0, flags)
+        o = oa.extract(v, (0, 0, 0), (1, 1, 1), 0.0, flags=flags)
+        errs = [e for e in (nan_equal("positions", g.positions, o.pos), nan_equal("normals", g.normals, o.nrm)) if e]
+        for name, a, b in [("halfedges", g.halfedges, o.he), ("face_halfedge", g.face_halfedge, o.face_he),
+                           ("face_attr", g.face_attr, o.face_attr), ("vertex_flags", g.vertex_flags, o.vflags)]:
+            e = meshcheck.exact_diff(name, a, b)
+            if e:
+                errs.append(e)
+        assert not errs, errs
