@@ -297,7 +297,7 @@ __global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, 
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int o = heO(he, (int)h);
-    if (o < 0 || o >= V) { anomaly(ctr); return; }
+    if (o < 0 || o >= V) return;  // counted as an anomaly by kOutDegree
     if (only && !only[o]) return;
     atomicAdd(&deg[o], 1);
 }
@@ -314,11 +314,15 @@ __global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, co
 }
 
 // out-degree of every vertex and one of its outgoing halfedges
-__global__ void kOutDegree(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ outdeg, int32_t* __restrict__ rep) {
+__global__ void kOutDegree(const int4* __restrict__ he, int64_t H, int64_t V, int32_t* __restrict__ outdeg, int32_t* __restrict__ rep,
+                           int64_t* ctr) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int o = heO(he, (int)h);
-    if (o < 0 || o >= V) return;  // counted as an anomaly by kCountOrigin
+    if (o < 0 || o >= V) {
+        anomaly(ctr);
+        return;
+    }
     if (atomicAdd(&outdeg[o], 1) == 0) rep[o] = (int32_t)h;
 }
 
@@ -326,7 +330,7 @@ __global__ void kOutDegree(const int4* __restrict__ he, int64_t H, int64_t V, in
 // any of them visits all and returns) has a single fan: every fan walk of fanWalk reaches
 // the same minimum.  The others are decided by kDetectFans over their outgoing lists.
 __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
-                         uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow) {
+                         uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, int64_t* __restrict__ nslow) {
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vv >= V) return;
     const int v = (int)vv;
@@ -348,6 +352,9 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* 
     }
     slow[v] = single ? 0 : 1;
     if (single) vflags[v] &= ~(uint32_t)VF_FANS;
+    // count of the vertices left for the outgoing-list pass (one atomic per wave)
+    const uint64_t m = __ballot(!single);
+    if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) atomicAdd((unsigned long long*)nslow, (unsigned long long)__popcll(m));
 }
 
 // ---------------------------------------------------------------- simplifications
@@ -1757,6 +1764,7 @@ void Pipeline::split(int step, int maxIters) {
         int64_t* faddr = nullptr;
         const int64_t n64 = scan32(pred, F, "nmaddr", &faddr);
         if (n64 == 0) break;
+        pinHost()[kPinMergeTimeout] = 0;  // the stream is idle here; checked at the copy-back read-back
         if (20 * n64 >= (int64_t)INT32_MAX / 2) throw HipError("working copy exceeds the int32 id range", -3);
         const int n = (int)n64;
         int32_t* nm = arena.get<int32_t>("nmlist", (size_t)n);
@@ -1845,11 +1853,28 @@ void Pipeline::split(int step, int maxIters) {
         if (step >= 5) kMoveToSurface<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, probsDev);
 
         if (copyDataBack || step >= 6) {
+            // the new vertices, halfedges and faces of the working copy: three prefix sums
+            // and one read-back of their totals ([0] timeout, [1..3] counts)
+            int32_t* pv = arena.get<int32_t>("cb_pred_v", (size_t)std::max(vsize, 1));
+            int32_t* ph = arena.get<int32_t>("cb_pred_h", (size_t)std::max(hsize, 1));
+            int32_t* pf = arena.get<int32_t>("cb_pred_f", (size_t)std::max(fsize, 1));
+            int64_t* av = arena.get<int64_t>("cb_addr_v", (size_t)vsize + 1);
+            int64_t* ah = arena.get<int64_t>("cb_addr_h", (size_t)hsize + 1);
+            int64_t* af = arena.get<int64_t>("cb_addr_f", (size_t)fsize + 1);
+            kNewPred<<<gridFor(vsize), kBlock, 0, stream>>>(w.vflags, w.v_c2o, vsize, pv);
+            kNewPred<<<gridFor(hsize), kBlock, 0, stream>>>(w.hflags, w.h_c2o, hsize, ph);
+            kNewPred<<<gridFor(fsize), kBlock, 0, stream>>>(w.fflags, w.f_c2o, fsize, pf);
+            int64_t* hp = pinHost();
+            int64_t* dp = pinDev();
+            hp[0] = hp[1] = hp[2] = hp[3] = 0;
+            scanExclusiveI32(pv, av, vsize, dp + 1, arena.get<char>("cb_scan_v", scanScratchBytes(vsize + 1)), stream, dp);
+            scanExclusiveI32(ph, ah, hsize, dp + 2, arena.get<char>("cb_scan_h", scanScratchBytes(hsize + 1)), stream, dp);
+            scanExclusiveI32(pf, af, fsize, dp + 3, arena.get<char>("cb_scan_f", scanScratchBytes(fsize + 1)), stream, dp);
+            MDMC_HIP(hipStreamSynchronize(stream));
+            if (hp[0] || hp[kPinMergeTimeout]) throw HipError("prefix sum: a look-back wait timed out", -4);
+            const int64_t addV = hp[1], addH = hp[2], addF = hp[3];
             // vertices
-            int32_t* p = arena.get<int32_t>("cb_pred", (size_t)std::max(vsize, std::max(hsize, fsize)));
-            kNewPred<<<gridFor(vsize), kBlock, 0, stream>>>(w.vflags, w.v_c2o, vsize, p);
-            int64_t* a = nullptr;
-            const int64_t addV = scan32(p, vsize, "cb_addr", &a);
+            int64_t* a = av;
             const int64_t V0 = V;
             float* pos = arena.grow<float>("pos", 3 * (size_t)(V0 + addV), 3 * (size_t)V0, stream);
             float* nrm = arena.grow<float>("nrm", 3 * (size_t)(V0 + addV), 3 * (size_t)V0, stream);
@@ -1859,14 +1884,12 @@ void Pipeline::split(int step, int maxIters) {
             kLastCopy<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, last);
             kCopyBackVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, a, V0, pos, nrm, vp, last);
             // halfedges
-            kNewPred<<<gridFor(hsize), kBlock, 0, stream>>>(w.hflags, w.h_c2o, hsize, p);
-            const int64_t addH = scan32(p, hsize, "cb_addr", &a);
+            a = ah;
             const int64_t H0 = H;
             int4* he = arena.grow<int4>("he", (size_t)(H0 + addH), (size_t)H0, stream);
             kCopyBackHalfedges<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, a, H0, he);
             // faces
-            kNewPred<<<gridFor(fsize), kBlock, 0, stream>>>(w.fflags, w.f_c2o, fsize, p);
-            const int64_t addF = scan32(p, fsize, "cb_addr", &a);
+            a = af;
             const int64_t F0 = F;
             int32_t* face_he = arena.grow<int32_t>("face_he", (size_t)(F0 + addF), (size_t)F0, stream);
             uint32_t* fattr = arena.grow<uint32_t>("fattr", (size_t)(F0 + addF), (size_t)F0, stream);
@@ -1895,8 +1918,10 @@ void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* nu
     int32_t* cnt = arena.get<int32_t>(tag + "_cnt", (size_t)count + 1);
     zero(cnt, ((size_t)count + 1) * sizeof(int32_t));
     if (R > 0) kRecCount<<<gridFor(R), kBlock, 0, stream>>>(rec, R, count, cnt, counters());
-    int64_t* roff = nullptr;
-    scan32(cnt, count + 1, tag + "_off", &roff);
+    // offsets only: the total is not needed on the host
+    int64_t* roff = arena.get<int64_t>(tag + "_off", (size_t)count + 2);
+    scanExclusiveI32(cnt, roff, count + 1, arena.get<int64_t>(tag + "_tot", 1), arena.get<char>(tag + "_scan", scanScratchBytes(count + 1)), stream,
+                     pinDev() + kPinMergeTimeout);
     int32_t* cursor = arena.get<int32_t>(tag + "_cur", (size_t)count + 1);
     zero(cursor, ((size_t)count + 1) * sizeof(int32_t));
     int32_t* idx = arena.get<int32_t>(tag + "_idx", (size_t)R);
@@ -1913,8 +1938,13 @@ void Pipeline::detectNonManifoldVertices() {
     int32_t* rep = arena.get<int32_t>("outrep", (size_t)V);
     uint8_t* slow = arena.get<uint8_t>("fan_slow", (size_t)V);
     zero(outdeg, (size_t)V * sizeof(int32_t));
-    kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep);
-    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow);
+    // the slow-vertex count is accumulated in device memory (atomics on mapped host memory
+    // cross the bus one by one) and read back once
+    int64_t* nslow = arena.get<int64_t>("fan_nslow", 1);
+    zero(nslow, sizeof(int64_t));
+    kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
+    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, nslow);
+    if (readI64(nslow) == 0) return;  // every vertex has one closed fan, or none
     // the others over their outgoing lists
     int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
     zero(deg, ((size_t)V + 1) * sizeof(int32_t));
