@@ -156,14 +156,42 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int64_t NX = nx, NXY = (int64_t)nx * ny;
     auto at = [&](int px, int py, int pz) { return g.v[(int64_t)pz * NXY + (int64_t)py * NX + px]; };
-    float gxv[8], gyv[8], gzv[8];
+    // the 24 neighbour samples of the corner gradients (computeGradients): loaded first and
+    // used after the position, so that their latency overlaps the edge mean and the
+    // move-to-surface iteration
+    float nbx[8], nby[8], nbz[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
         const int px = x + cx, py = y + cy, pz = z + cz;
-        const float gx = cx ? at(clampi(px + 1, 0, nx - 1), py, pz) - s[k & ~1] : s[k | 1] - at(clampi(px - 1, 0, nx - 1), py, pz);
-        const float gy = cy ? at(px, clampi(py + 1, 0, ny - 1), pz) - s[k & ~2] : s[k | 2] - at(px, clampi(py - 1, 0, ny - 1), pz);
-        const float gz = cz ? at(px, py, clampi(pz + 1, 0, nz - 1)) - s[k & ~4] : s[k | 4] - at(px, py, clampi(pz - 1, 0, nz - 1));
+        nbx[k] = at(clampi(cx ? px + 1 : px - 1, 0, nx - 1), py, pz);
+        nby[k] = at(px, clampi(cy ? py + 1 : py - 1, 0, ny - 1), pz);
+        nbz[k] = at(px, py, clampi(cz ? pz + 1 : pz - 1, 0, nz - 1));
+    }
+    int j = 0;
+    for (int c = 0; c < t; ++c) j += polySize(w, c);
+    const int sz = polySize(w, t);
+    f3 p = mk3(0.f, 0.f, 0.f);
+    for (int i = 0; i < sz; ++i, ++j) {
+        const int e = polyEdge(w, j);
+        const int a = edgeA(e), b = edgeB(e), ax = edgeAxis(e);
+        const float tt = (iso - s[a]) / (s[b] - s[a]);
+        f3 off;
+        off.x = 0.f + (tt * (ax == 0 ? 1.f : 0.f) + (float)(a & 1));
+        off.y = 0.f + (tt * (ax == 1 ? 1.f : 0.f) + (float)((a >> 1) & 1));
+        off.z = 0.f + (tt * (ax == 2 ? 1.f : 0.f) + (float)((a >> 2) & 1));
+        p = p + off;
+    }
+    p = divs(p, (float)sz);
+    if (surfIter) movePointToSurface(s, iso, p);
+    // central differences ((upper - lower) * 0.5 / delta) at the 8 corners
+    float gxv[8], gyv[8], gzv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
+        const float gx = cx ? nbx[k] - s[k & ~1] : s[k | 1] - nbx[k];
+        const float gy = cy ? nby[k] - s[k & ~2] : s[k | 2] - nby[k];
+        const float gz = cz ? nbz[k] - s[k & ~4] : s[k | 4] - nbz[k];
         gxv[k] = 0.5f * gx;
         gyv[k] = 0.5f * gy;
         gzv[k] = 0.5f * gz;
@@ -186,22 +214,6 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
             gzv[k] /= g.dz;
         }
     }
-    int j = 0;
-    for (int c = 0; c < t; ++c) j += polySize(w, c);
-    const int sz = polySize(w, t);
-    f3 p = mk3(0.f, 0.f, 0.f);
-    for (int i = 0; i < sz; ++i, ++j) {
-        const int e = polyEdge(w, j);
-        const int a = edgeA(e), b = edgeB(e), ax = edgeAxis(e);
-        const float tt = (iso - s[a]) / (s[b] - s[a]);
-        f3 off;
-        off.x = 0.f + (tt * (ax == 0 ? 1.f : 0.f) + (float)(a & 1));
-        off.y = 0.f + (tt * (ax == 1 ? 1.f : 0.f) + (float)((a >> 1) & 1));
-        off.z = 0.f + (tt * (ax == 2 ? 1.f : 0.f) + (float)((a >> 2) & 1));
-        p = p + off;
-    }
-    p = divs(p, (float)sz);
-    if (surfIter) movePointToSurface(s, iso, p);
     nn = normalize3(mk3(trilinear(gxv, p), trilinear(gyv, p), trilinear(gzv, p)));
     wp = mk3(g.ox, g.oy, g.oz) + (mk3((float)x, (float)y, (float)z) + p) * mk3(g.dx, g.dy, g.dz);
 }
