@@ -13,7 +13,13 @@ namespace {
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 // one thread per point of every processed grid; grid g owns blocks [d[g].base, d[g + 1].base),
-// so the grid a block works on, and with it every size below, is uniform across the block
+// so the grid a block works on, and with it every size below, is uniform across the block.
+// The block's first point is split into (x0, y0, z0) once (uniform, scalar); each thread
+// carries x0 + tid into y and z.  Without refinement the sample position is the point
+// itself: x / 1 = x exactly, floor(x) = x and the weights are +0, the same values the
+// general path computes, so the 8 corner loads and the trilinear blend stay (a non-finite
+// neighbour still reaches the result through 0 * s).
+template <bool REFINE>
 __global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int refinement, int pad) {
     const int64_t blk = blockIdx.x;
     int lo = 0, hi = G;  // d[lo].base <= blk
@@ -24,26 +30,44 @@ __global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __res
     }
     const PrepareDesc& p = d[lo];
     const int onx = p.onx, ony = p.ony, onz = p.onz, inx = p.inx, iny = p.iny, inz = p.inz;
-    const int64_t i = (blk - p.base) * kBlock + threadIdx.x;
+    const int64_t i0 = (blk - p.base) * kBlock;
+    const int64_t i = i0 + threadIdx.x;
     if (i >= (int64_t)onx * ony * onz) return;
-    int x = (int)(i % onx);
-    int y = (int)((i / onx) % ony);
-    int z = (int)(i / ((int64_t)onx * ony));
+    const int64_t r0 = i0 / onx;
+    int x = (int)(i0 - r0 * onx) + (int)threadIdx.x;
+    int y = (int)(r0 % ony);
+    int z = (int)(r0 / ony);
+    while (x >= onx) {  // once at most when onx >= kBlock
+        x -= onx;
+        if (++y == ony) {
+            y = 0;
+            ++z;
+        }
+    }
     if (pad) {
         x = clampi(x, 1, onx - 2) - 1;
         y = clampi(y, 1, ony - 2) - 1;
         z = clampi(z, 1, onz - 2) - 1;
     }
-    const float rr = (float)(refinement + 1);
-    const float fx = (float)x / rr, fy = (float)y / rr, fz = (float)z / rr;
-    const int mx = (int)floorf(fx), my = (int)floorf(fy), mz = (int)floorf(fz);
-    float s[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int cx = clampi(mx + (k & 1), 0, inx - 1), cy = clampi(my + ((k >> 1) & 1), 0, iny - 1), cz = clampi(mz + ((k >> 2) & 1), 0, inz - 1);
-        s[k] = p.in[(int64_t)cz * inx * iny + (int64_t)cy * inx + cx];
+    int mx = x, my = y, mz = z;
+    f3 dd = mk3(0.f, 0.f, 0.f);
+    if (REFINE) {
+        const float rr = (float)(refinement + 1);
+        const float fx = (float)x / rr, fy = (float)y / rr, fz = (float)z / rr;
+        mx = (int)floorf(fx);
+        my = (int)floorf(fy);
+        mz = (int)floorf(fz);
+        dd = mk3(fx - (float)mx, fy - (float)my, fz - (float)mz);
     }
-    const f3 dd = mk3(fx - (float)mx, fy - (float)my, fz - (float)mz);
+    const int x0 = clampi(mx, 0, inx - 1), x1 = clampi(mx + 1, 0, inx - 1);
+    const int y0 = clampi(my, 0, iny - 1), y1 = clampi(my + 1, 0, iny - 1);
+    const int z0 = clampi(mz, 0, inz - 1), z1 = clampi(mz + 1, 0, inz - 1);
+    const int64_t nxy = (int64_t)inx * iny;
+    const float* r00 = p.in + (int64_t)z0 * nxy + (int64_t)y0 * inx;
+    const float* r10 = p.in + (int64_t)z0 * nxy + (int64_t)y1 * inx;
+    const float* r01 = p.in + (int64_t)z1 * nxy + (int64_t)y0 * inx;
+    const float* r11 = p.in + (int64_t)z1 * nxy + (int64_t)y1 * inx;
+    const float s[8] = {r00[x0], r00[x1], r10[x0], r10[x1], r01[x0], r01[x1], r11[x0], r11[x1]};
     p.out[i] = trilinear(s, dd);
 }
 
@@ -67,7 +91,9 @@ __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restric
 int64_t prepareBlocks(int64_t points) { return (points + kBlock - 1) / kBlock; }
 
 void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, hipStream_t s) {
-    if (blocks > 0) kPrepareGrids<<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad);
+    if (blocks <= 0) return;
+    if (refinement > 0) kPrepareGrids<true><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad);
+    else kPrepareGrids<false><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad);
 }
 
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s) {

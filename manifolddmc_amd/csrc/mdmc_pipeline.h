@@ -100,7 +100,8 @@ class Pipeline {
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
     void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
-    void buildBuckets(bool fromQuads, const uint8_t* only);
+    void buildBuckets(const uint8_t* only);
+    int32_t* bucketPassValence(const int64_t* off, int64_t nV);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);
     void checkValid(const char* what);
     void simplifyInit(int32_t** valence, int32_t** count);

@@ -79,27 +79,6 @@ __device__ __forceinline__ ListRef listOf(const int4* __restrict__ he, const int
     return ListRef{(int)lo, (int)(offM[m + 1] - lo), pairEntry(e.x, b)};
 }
 
-// HalfedgeMesh::initialize's collectHalfedgeElements (HalfedgeMesh.cu:14-58) plus the
-// first list pass: rank[h] = rank of h in the list of its smaller endpoint m
-__global__ void kCollect(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int4* __restrict__ he,
-                         int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int32_t* __restrict__ degM, int32_t* __restrict__ rank) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    const int4 v = quads[q];
-    const int b = (int)(4 * q);
-    he[b + 0] = make_int4(v.x, (int)q, b + 1, -1);
-    he[b + 1] = make_int4(v.y, (int)q, b + 2, -1);
-    he[b + 2] = make_int4(v.z, (int)q, b + 3, -1);
-    he[b + 3] = make_int4(v.w, (int)q, b + 0, -1);
-    face_he[q] = b;
-    fattr[q] = qattr[q];
-    const int c[5] = {v.x, v.y, v.z, v.w, v.x};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        rank[b + i] = atomicAdd(&degM[min(c[i], c[i + 1])], 1);
-    }
-}
-
 // HalfedgeMesh's vertex -> halfedge array as collectHalfedgeElements leaves it: the
 // highest halfedge starting at each vertex ("last writer wins" in ascending thread order)
 __global__ void kVertexHalfedge(const int4* __restrict__ quads, int64_t Q, int32_t* __restrict__ vert_he) {
@@ -167,32 +146,6 @@ __global__ void kPlaceEntries(const int4* __restrict__ he, int64_t H, const uint
     A[l.lo + rank[h]] = make_int2(l.entry, key ? key[h] : h);
 }
 
-// the halfedges joining the two vertices of h, sorted by bucket key; returns the true
-// count.  Member ids are the keys, so this holds between an initialize and the next refresh.
-__device__ int gatherBucket(int h, const int4* __restrict__ he, const int64_t* __restrict__ offM, const int2* __restrict__ A, int mem[kMaxBucket]) {
-    const ListRef l = listOf(he, he[h], offM);
-    const int o = entryOther(l.entry);
-    int n = 0;
-    for (int i = l.lo; i < l.lo + l.len; ++i) {
-        const int2 e = A[i];
-        if (entryOther(e.x) == o) {
-            if (n < kMaxBucket) mem[n] = e.y;
-            ++n;
-        }
-    }
-    const int cnt = n < kMaxBucket ? n : kMaxBucket;
-    for (int i = 1; i < cnt; ++i) {  // insertion sort by key (= id)
-        const int x = mem[i];
-        int j = i - 1;
-        while (j >= 0 && mem[j] > x) {
-            mem[j + 1] = mem[j];
-            --j;
-        }
-        mem[j + 1] = x;
-    }
-    return n;
-}
-
 // the four slots of a bucket: entries beyond the fourth land in slot 0 (glm::ivec4::operator[] default)
 __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, int s[4]) {
     for (int i = 0; i < 4; ++i) s[i] = (i < n) ? mem[i] : -1;
@@ -201,79 +154,24 @@ __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, in
 
 __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF_NM : (n == 3 ? VF_WEIRD : (n == 1 ? VF_BOUNDARY : 0u)); }
 
-// One thread per halfedge h, reading only the short list of its smaller endpoint:
-//   bucket size and rank of h (rank = position in key order),
-//   connectHalfedgeTwins (HalfedgeMesh.cu:173-291) when setTwins (right after an
-//     initialize, where keys are halfedge ids),
-//   refreshNonManifoldFlags / refreshWeirdFlags / refreshBoundaryFlags (HalfedgeMesh.cu:92-148)
-//     as OR-ed bits on the origin of h (vflags cleared beforehand),
-//   the NM bit of the face of h when its bucket has 4 halfedges (fattr, and qattr when given),
-//   the simplifications' vertex valence when valence is given.
-__global__ void kBuckets(int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM,
-                         const int32_t* __restrict__ keys, const int2* __restrict__ A, int setTwins, int32_t* __restrict__ bsize,
-                         int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr,
-                         int32_t* __restrict__ valence) {
+// One thread per flagged halfedge h of a refresh, reading only the short list of its smaller
+// endpoint: the size of its bucket and its rank (position in key order).
+__global__ void kBuckets(const int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM,
+                         const int32_t* __restrict__ keys, const int2* __restrict__ A, int32_t* __restrict__ bsize, int32_t* __restrict__ brank) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H || (only && !only[hh])) return;
+    if (hh >= H || !only[hh]) return;
     const int h = (int)hh;
-    const int4 e = he[h];
-    const ListRef l = listOf(he, e, offM);
-    const int o = entryOther(l.entry), key = keys ? keys[h] : h;
-    int n = 0, rk = 0, partner = -1;
+    const ListRef l = listOf(he, he[h], offM);
+    const int o = entryOther(l.entry), key = keys[h];
+    int n = 0, rk = 0;
     for (int i = l.lo; i < l.lo + l.len; ++i) {
         const int2 f = A[i];
         if (entryOther(f.x) != o) continue;
         ++n;
         if (f.y < key) ++rk;
-        if (f.y != key) partner = f.y;
     }
     bsize[h] = n;
     brank[h] = rk;
-    if (setTwins) {
-        int t = -1;
-        if (n == 2) {
-            t = partner;
-        } else if (n == 3 || n == 4) {
-            // members by rank (their keys are their ids) and whether they start at the list
-            // vertex: the one after h (n == 3), or the pairing of all four
-            int byRank[4] = {-1, -1, -1, -1};
-            int dirRank[4] = {0, 0, 0, 0};
-            for (int i = l.lo; i < l.lo + l.len; ++i) {
-                const int2 f = A[i];
-                if (entryOther(f.x) != o) continue;
-                int rf = 0;
-                for (int k = l.lo; k < l.lo + l.len; ++k) {
-                    const int2 g = A[k];
-                    if (entryOther(g.x) == o && g.y < f.y) ++rf;
-                }
-                byRank[rf] = f.y;
-                dirRank[rf] = f.x < 0 ? 1 : 0;
-            }
-            if (n == 3) {
-                t = byRank[(rk + 1) % 3];
-            } else {
-                // pair the first with the first of another origin (HalfedgeMesh.cu:253-280)
-                int pr;
-                if (dirRank[0] != dirRank[1]) pr = rk ^ 1;       // (0,1) (2,3)
-                else if (dirRank[0] != dirRank[2]) pr = rk ^ 2;  // (0,2) (1,3)
-                else pr = 3 - rk;                                // (0,3) (1,2)
-                t = byRank[pr];
-            }
-        }
-        setT(he, h, t);
-        if (valence) {
-            // outgoing halfedges, plus incoming boundary ones
-            atomicAdd(&valence[e.x], 1);
-            if (t == -1) atomicAdd(&valence[heO(he, e.z)], 1);
-        }
-    }
-    if (!vflags) return;
-    const uint32_t vf = bucketVertexFlag(n);
-    if (vf) atomicOr(&vflags[e.x], vf);
-    if (n == 4) {
-        atomicOr(&fattr[e.y], (uint32_t)QA_NM);
-        if (qattr) atomicOr(&qattr[e.y], (uint32_t)QA_NM);
-    }
 }
 
 // vertex flags and face NM bits from the bucket sizes of all halfedges (vflags and the
@@ -288,6 +186,300 @@ __global__ void kFlagsFromBuckets(const int4* __restrict__ he, int64_t H, const 
     const int4 e = he[h];
     atomicOr(&vflags[e.x], vf);
     if (n == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+}
+
+// ---------------------------------------------------------------- halfedge build from outgoing lists
+// HalfedgeMesh::initialize (HalfedgeMesh.cu:14-291) from a quad list: halfedge 4f + i runs
+// from corner i of quad f to corner i + 1.  The build keeps, per vertex v, the list of the
+// halfedges starting at v (out[off[v] .. off[v + 1])).  A bucket -- the halfedges joining
+// two vertices a and b -- is read from the list of m = min(a, b): the halfedges x of that
+// list that end at o = max(a, b) start at m, and (o != m) the halfedge before such an x in
+// its quad starts at o when that quad has o just before m.  Members are ordered by id, as
+// the reference's bucket keys are the halfedge ids right after an initialize.
+__device__ __forceinline__ int quadCorner(const int4& q, int j) { return j == 0 ? q.x : (j == 1 ? q.y : (j == 2 ? q.z : q.w)); }
+
+struct BucketOf {
+    int n, rank, twin;
+};
+
+// bucket size, rank of h and the twin connectHalfedgeTwins (HalfedgeMesh.cu:173-291) gives h:
+// the partner of a pair, the next by id of three, the first other-direction member paired
+// with the first of four (the same rules kBuckets applies to the bucket lists)
+__device__ BucketOf bucketFromOut(int h, int a, int b, const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off,
+                                  const int32_t* __restrict__ out, int64_t* ctr) {
+    const int m = min(a, b), o = max(a, b);
+    int mem[4], dir[4];
+    int n = 0, rk = 0, partner = -1;
+    bool self = false;
+    auto add = [&](int id, int d) {
+        if (n < 4) {
+            mem[n] = id;
+            dir[n] = d;
+        }
+        ++n;
+        if (id < h) ++rk;
+        if (id != h) partner = id;
+        else self = true;
+    };
+    const int64_t k1 = off[m + 1];
+    for (int64_t k = off[m]; k < k1; ++k) {
+        const int x = out[k];
+        if (x < 0 || x >= H) {
+            anomaly(ctr);
+            continue;
+        }
+        const int4 q = quads[x >> 2];
+        const int j = x & 3;
+        if (quadCorner(q, j) != m) anomaly(ctr);
+        if (quadCorner(q, (j + 1) & 3) == o) add(x, 0);
+        if (o != m && quadCorner(q, (j + 3) & 3) == o) add((x & ~3) | ((j + 3) & 3), 1);
+    }
+    if (!self) anomaly(ctr);
+    int t = -1;
+    if (n == 2) {
+        t = partner;
+    } else if (n == 3 || n == 4) {
+        for (int i = 1; i < n; ++i) {  // by id
+            const int x = mem[i], dx = dir[i];
+            int j = i - 1;
+            while (j >= 0 && mem[j] > x) {
+                mem[j + 1] = mem[j];
+                dir[j + 1] = dir[j];
+                --j;
+            }
+            mem[j + 1] = x;
+            dir[j + 1] = dx;
+        }
+        if (n == 3) {
+            t = mem[(rk + 1) % 3];
+        } else {
+            int pr;
+            if (dir[0] != dir[1]) pr = rk ^ 1;       // (0,1) (2,3)
+            else if (dir[0] != dir[2]) pr = rk ^ 2;  // (0,2) (1,3)
+            else pr = 3 - rk;                        // (0,3) (1,2)
+            t = mem[pr];
+        }
+    }
+    return BucketOf{n, rk, t};
+}
+
+// the members of h's bucket sorted by id (first kMaxBucket), returns the true count
+__device__ int gatherBucketOut(int h, const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                               int mem[kMaxBucket]) {
+    const int4 qh = quads[h >> 2];
+    const int a = quadCorner(qh, h & 3), b = quadCorner(qh, (h + 1) & 3);
+    const int m = min(a, b), o = max(a, b);
+    int n = 0;
+    for (int64_t k = off[m]; k < off[m + 1]; ++k) {
+        const int x = out[k];
+        if (x < 0 || x >= H) continue;
+        const int4 q = quads[x >> 2];
+        const int j = x & 3;
+        if (quadCorner(q, (j + 1) & 3) == o) {
+            if (n < kMaxBucket) mem[n] = x;
+            ++n;
+        }
+        if (o != m && quadCorner(q, (j + 3) & 3) == o) {
+            if (n < kMaxBucket) mem[n] = (x & ~3) | ((j + 3) & 3);
+            ++n;
+        }
+    }
+    const int cnt = n < kMaxBucket ? n : kMaxBucket;
+    for (int i = 1; i < cnt; ++i) {
+        const int x = mem[i];
+        int j = i - 1;
+        while (j >= 0 && mem[j] > x) {
+            mem[j + 1] = mem[j];
+            --j;
+        }
+        mem[j + 1] = x;
+    }
+    return n;
+}
+
+// the record of halfedge h = 4f + i from a to b with its bucket, and what the bucket pass of
+// an initialize derives from it: vertex flags of the origin, the face's NM bit (fattr and the
+// quad attributes), an incoming boundary halfedge's share of the destination's valence
+__device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketOf& bk, int4* __restrict__ he, int32_t* __restrict__ bsize,
+                                              int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
+                                              uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
+    const int f = h >> 2;
+    he[h] = make_int4(a, f, (h & ~3) | ((h + 1) & 3), bk.twin);
+    bsize[h] = bk.n;
+    brank[h] = bk.rank;
+    if (valence && bk.twin == -1) atomicAdd(&valence[b], 1);
+    const uint32_t vf = bucketVertexFlag(bk.n);
+    if (vf) atomicOr(&vflags[a], vf);
+    if (bk.n == 4) {
+        atomicOr(&fattr[f], (uint32_t)QA_NM);
+        atomicOr(&qattr[f], (uint32_t)QA_NM);
+    }
+}
+
+// first list pass over the quads: the rank of every halfedge in its origin's list, the
+// face -> halfedge array and the face attributes (collectHalfedgeElements, HalfedgeMesh.cu:14-58)
+__global__ void kCollectOut(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int32_t* __restrict__ cnt,
+                            int32_t* __restrict__ rank, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v = quads[q];
+    const int b = (int)(4 * q);
+    rank[b + 0] = atomicAdd(&cnt[v.x], 1);
+    rank[b + 1] = atomicAdd(&cnt[v.y], 1);
+    rank[b + 2] = atomicAdd(&cnt[v.z], 1);
+    rank[b + 3] = atomicAdd(&cnt[v.w], 1);
+    face_he[q] = b;
+    fattr[q] = qattr[q];
+}
+
+__global__ void kFillOut(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ rank,
+                         int32_t* __restrict__ out) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int v = quadCorner(quads[h >> 2], (int)(h & 3));
+    out[off[v] + rank[h]] = (int32_t)h;
+}
+
+// valence = outgoing halfedges (the list length); incoming boundary halfedges are added by
+// the bucket pass
+__global__ void kValenceFromOff(const int64_t* __restrict__ off, int64_t V, int32_t* __restrict__ valence) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    valence[v] = (int32_t)(off[v + 1] - off[v]);
+}
+
+__global__ void kFaceInit(const uint32_t* __restrict__ qattr, int64_t Q, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    face_he[q] = (int32_t)(4 * q);
+    fattr[q] = qattr[q];
+}
+
+// every halfedge from its bucket
+__global__ void kBucketsAll(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                            int4* __restrict__ he, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
+                            uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int h = (int)hh;
+    const int4 q = quads[h >> 2];
+    const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+    storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads, H, off, out, ctr), he, bsize, brank, vflags, fattr, qattr, valence);
+}
+
+// ---- rebuild after a simplification's compaction.  New vertex v' = address[v]; a dropped
+// vertex u (vtype[u] == dropType) reappears as t' = address[vtarget[u]] in the faces that
+// kCompactQuads relabels; every such t' is touched.  Every edge of a removed quad has a
+// dropped or a target endpoint: (x, y, z, w) of a P3X3Y merge drops z into x, and the four
+// neighbours (v', v, c, c') of a P3333 centre drop v, v' (into c, c').  So a bucket whose
+// two endpoints are untouched has the same members as before (no relabelled vertex maps
+// onto either endpoint and no member lay in a removed quad), renumbered in the same order,
+// and its halfedges keep size, rank and (renumbered) twin; the others are recomputed from
+// the new outgoing lists.  qnew[f] is the new id of old quad f (-1: removed).
+__global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, const int32_t* __restrict__ address, int64_t V,
+                         int dropType, uint8_t* __restrict__ touched, int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= V || vtype[u] != dropType) return;
+    const int t = vtarget[u];
+    if (t < 0 || t >= V) return;
+    const int tn = address[t];
+    if (tn < 0) return;  // the relabelled quads then hold -1 and fail validation
+    touched[tn] = 1;
+    srcNext[u] = atomicExch(&srcHead[tn], (int)u);
+}
+
+// new vertex v': the halfedges of kept quads in the old lists of its own vertex and of the
+// dropped vertices relabelled to it (a chain through srcNext), counted, then written with
+// their new ids (the bucket pass checks every entry it reads against the quad's corner)
+__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                          const int32_t* __restrict__ qnew, const int32_t* __restrict__ srcHead, const int32_t* __restrict__ srcNext,
+                          int32_t* __restrict__ cnt) {
+    const int64_t vn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vn >= Vn) return;
+    int c = 0;
+    int u = oldV[vn], s = srcHead[vn];
+    for (;;) {
+        for (int64_t k = off[u]; k < off[u + 1]; ++k) c += qnew[out[k] >> 2] >= 0 ? 1 : 0;
+        if (s < 0) break;
+        u = s;
+        s = srcNext[s];
+    }
+    cnt[vn] = c;
+}
+
+__global__ void kOutFill(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                         const int32_t* __restrict__ qnew, const int32_t* __restrict__ srcHead,
+                         const int32_t* __restrict__ srcNext, const int64_t* __restrict__ off2, int32_t* __restrict__ out2, int64_t cap,
+                         int64_t* ctr) {
+    const int64_t vn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vn >= Vn) return;
+    int64_t w = off2[vn];
+    int u = oldV[vn], s = srcHead[vn];
+    for (;;) {
+        for (int64_t k = off[u]; k < off[u + 1]; ++k) {
+            const int x = out[k];
+            const int fn = qnew[x >> 2];
+            if (fn < 0) continue;
+            if (w < cap) out2[w++] = 4 * fn + (x & 3);
+            else anomaly(ctr);
+        }
+        if (s < 0) break;
+        u = s;
+        s = srcNext[s];
+    }
+}
+
+// a halfedge with two untouched endpoints takes size, rank and renumbered twin from its old
+// halfedge; the block's other halfedges are gathered in LDS and recomputed from their
+// buckets by the block's first lanes, so that the long path runs on full waves
+__global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__ quads2, int64_t H2, const uint8_t* __restrict__ touched,
+                                                        const int32_t* __restrict__ oldQ, const int4* __restrict__ heOld, int64_t Hold,
+                                                        const int32_t* __restrict__ bsizeOld, const int32_t* __restrict__ brankOld,
+                                                        const int32_t* __restrict__ qnew, const int64_t* __restrict__ off2,
+                                                        const int32_t* __restrict__ out2, int4* __restrict__ he, int32_t* __restrict__ bsize,
+                                                        int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
+                                                        uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool later = false;
+    if (hh < H2) {
+        const int h = (int)hh;
+        const int4 q = quads2[h >> 2];
+        const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+        if (touched[a] || touched[b]) {
+            later = true;
+        } else {
+            const int ho = 4 * oldQ[h >> 2] + (h & 3);
+            const int tt = heOld[ho].w;
+            int t = -1;
+            if (tt >= Hold) {
+                anomaly(ctr);
+            } else if (tt >= 0) {
+                const int fo = qnew[tt >> 2];
+                if (fo < 0) anomaly(ctr);
+                else t = 4 * fo + (tt & 3);
+            }
+            storeHalfedge(h, a, b, BucketOf{bsizeOld[ho], brankOld[ho], t}, he, bsize, brank, vflags, fattr, qattr, valence);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t m = __ballot(later);
+    if (lane == 0) s_wave[wid] = __popcll(m);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
+    }
+    if (later) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)hh;
+    __syncthreads();
+    if ((int)threadIdx.x < total) {
+        const int h = s_list[threadIdx.x];
+        const int4 q = quads2[h >> 2];
+        const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+        storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads2, H2, off2, out2, ctr), he, bsize, brank, vflags, fattr, qattr, valence);
+    }
 }
 
 // outgoing-halfedge lists (by origin) for the fan detection, of the vertices in `only`
@@ -511,12 +703,13 @@ __global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int 
 
 __global__ void kCompactVertices(const float* __restrict__ pos, const float* __restrict__ nrm, const int32_t* __restrict__ pid,
                                  const int32_t* __restrict__ keep, const int64_t* __restrict__ addr, int64_t V, float* __restrict__ npos,
-                                 float* __restrict__ nnrm, int32_t* __restrict__ npid, int32_t* __restrict__ address) {
+                                 float* __restrict__ nnrm, int32_t* __restrict__ npid, int32_t* __restrict__ address, int32_t* __restrict__ oldV) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     if (!keep[v]) return;
     const int64_t o = addr[v];
     address[v] = (int32_t)o;
+    oldV[o] = (int32_t)v;
     for (int k = 0; k < 3; ++k) {
         npos[3 * o + k] = pos[3 * v + k];
         nnrm[3 * o + k] = nrm[3 * v + k];
@@ -533,10 +726,15 @@ __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t
 // removeQuadrilateralsP3X3Y (p3333 = 0) / removeQuadsP3333 (p3333 = 1), HalfedgeMesh.cu:513-538, :657-683
 __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __restrict__ attr, int64_t Q, const int32_t* __restrict__ keep,
                               const int64_t* __restrict__ qaddr, const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
-                              const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr) {
+                              const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr,
+                              int32_t* __restrict__ oldQ, int32_t* __restrict__ qnew) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
-    if (!keep[q]) return;
+    if (!keep[q]) {
+        qnew[q] = -1;
+        return;
+    }
+    qnew[q] = (int32_t)qaddr[q];
     const uint32_t a = attr[q];
     const int4 v4 = quads[q];
     const int v[4] = {v4.x, v4.y, v4.z, v4.w};
@@ -551,6 +749,7 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
     if (a & QA_NM) na |= QA_NM;
     if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
     nattr[d] = na;
+    oldQ[d] = (int32_t)q;
 }
 
 // ---------------------------------------------------------------- fixManifoldTwins
@@ -573,7 +772,7 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __restrict__ offM, const int2* __restrict__ A,
+__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restrict__ quads, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
                           const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                           const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -581,7 +780,7 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int64_t* __res
     const int h = (int)hh;
     if (bsize[h] < 4 || brank[h] != 0) return;
     int mem[kMaxBucket];
-    const int n = gatherBucket(h, he, offM, A, mem);
+    const int n = gatherBucketOut(h, quads, H, off, out, mem);
     if (n > kMaxBucket) anomaly(ctr);
     int s[4];
     bucketSlots(mem, n, s);
@@ -1571,56 +1770,70 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 // then the per-vertex bucket pass
 // bucket lists by smaller endpoint from the counts in degM and the records of the first
 // list pass, then the per-halfedge bucket pass
-void Pipeline::buildBuckets(bool fromQuads, const uint8_t* only) {
+// the split's refresh: lists of the flagged halfedges under their smaller endpoint, with
+// the refresh's keys; sizes and ranks of the flagged halfedges, then the flags from all sizes
+void Pipeline::buildBuckets(const uint8_t* only) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
     int32_t* rank = arena.ptr<int32_t>("bl_rank");
-    // bucket keys: the halfedge ids after an initialize, the refresh's keys otherwise
-    int32_t* keys = only ? arena.ptr<int32_t>("bl_key") : nullptr;
+    int32_t* keys = arena.ptr<int32_t>("bl_key");
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
-    // entries of every halfedge, or of the flagged ones (at most H either way)
     int2* A = arena.get<int2>("listA", (size_t)std::max<int64_t>(H, 1));
     int32_t* bsize = arena.ptr<int32_t>("bsize");
     int32_t* brank = arena.ptr<int32_t>("brank");
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
+    uint32_t* fattr = arena.ptr<uint32_t>("fattr");
     zero(vflags, (size_t)V * sizeof(uint32_t));
-    int32_t* valence = nullptr;
     valenceReady = false;
-    if (fromQuads && simplifyQueue > 0) {
-        valence = arena.get<int32_t>("valence", (size_t)V);
-        zero(valence, (size_t)V * sizeof(int32_t));
-        valenceReady = true;
-    }
     if (H > 0) {
         kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, rank, keys, A);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, keys, A, fromQuads ? 1 : 0, bsize, brank, only ? nullptr : vflags,
-                                                    arena.ptr<uint32_t>("fattr"), fromQuads ? arena.ptr<uint32_t>("qattr") : nullptr, valence);
+        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, keys, A, bsize, brank);
+        kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, fattr);
     }
-    if (only && H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, arena.ptr<uint32_t>("fattr"));
 }
 
+// the valence the next simplification starts from, when one follows: the outgoing list
+// lengths here, incoming boundary halfedges added by the bucket pass
+int32_t* Pipeline::bucketPassValence(const int64_t* off, int64_t nV) {
+    valenceReady = false;
+    if (simplifyQueue <= 0) return nullptr;
+    int32_t* valence = arena.get<int32_t>("valence", (size_t)std::max<int64_t>(nV, 1));
+    if (nV > 0) kValenceFromOff<<<gridFor(nV), kBlock, 0, stream>>>(off, nV, valence);
+    valenceReady = true;
+    return valence;
+}
+
+// HalfedgeMesh::initialize from the quad list alone (the grid pass's output): outgoing lists
+// by counting, then every halfedge from its bucket
 void Pipeline::halfedgeInitialize() {
     H = 4 * Q;
     F = Q;
     int4* he = arena.get<int4>("he", (size_t)H);
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)F);
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
-    arena.get<uint32_t>("vflags", (size_t)V);
-    int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
+    uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)V);
+    int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)V + 1);
     int32_t* rank = arena.get<int32_t>("bl_rank", (size_t)H);
-    arena.get<int32_t>("bsize", (size_t)H);
-    arena.get<int32_t>("brank", (size_t)H);
-    if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), Q, V, counters());
+    int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
+    int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
+    const int4* quads = arena.ptr<int4>("quads");
+    uint32_t* qattr = arena.ptr<uint32_t>("qattr");
+    if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, Q, V, counters());
     checkValid("quad vertex ids");
     // the vertex -> halfedge array is derived from these quads when it is copied out
     vertHeV = V;
     vertHeQ = Q;
-    zero(degM, ((size_t)V + 1) * sizeof(int32_t));
-    if (Q > 0)
-        kCollect<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<uint32_t>("qattr"), Q, he, face_he, fattr, degM, rank);
-    buildBuckets(true, nullptr);
+    zero(cnt, ((size_t)V + 1) * sizeof(int32_t));
+    if (Q > 0) kCollectOut<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, cnt, rank, face_he, fattr);
+    int64_t* off = arena.get<int64_t>("out_off", (size_t)V + 1);
+    scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream);
+    int32_t* out = arena.get<int32_t>("out_list", (size_t)std::max<int64_t>(H, 1));
+    if (H > 0) kFillOut<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, rank, out);
+    zero(vflags, (size_t)V * sizeof(uint32_t));
+    int32_t* valence = bucketPassValence(off, V);
+    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, he, bsize, brank, vflags, fattr, qattr, valence, counters());
 }
 
 void Pipeline::checkValid(const char* what) {
@@ -1645,7 +1858,7 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     if (F > 0) kFaceClear<<<gridFor(F), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
     if (H > 0) kDirtyRecords<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dirty, counters());
     checkValid("splitting");
-    buildBuckets(false, dirty);
+    buildBuckets(dirty);
 }
 
 // ---- simplifications
@@ -1680,21 +1893,71 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     float* npos = arena.get<float>("pos2", 3 * capV);
     float* nnrm = arena.get<float>("nrm2", 3 * capV);
     int32_t* npid = pid() ? arena.get<int32_t>("pid2", capV) : nullptr;
+    int32_t* oldV = arena.get<int32_t>("oldV", (size_t)std::max<int64_t>(nV, 1));
     if (V > 0)
         kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), keep, vaddr, V, npos, nnrm, npid,
-                                                            address);
+                                                            address, oldV);
+    int4* quads = arena.ptr<int4>("quads");
     int4* nquads = arena.get<int4>("quads2", capQ);
     uint32_t* nattr = arena.get<uint32_t>("qattr2", capQ);
+    int32_t* oldQ = arena.get<int32_t>("oldQ", (size_t)std::max<int64_t>(nQ, 1));
+    int32_t* qnew = arena.get<int32_t>("qnew", (size_t)std::max<int64_t>(Q, 1));
     if (Q > 0)
-        kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr);
+        kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ,
+                                                         qnew);
+    if (nQ > 0) kValidateQuads<<<gridFor(nQ), kBlock, 0, stream>>>(nquads, nQ, nV, counters());
+    checkValid("quad vertex ids");
+
+    // HalfedgeMesh::initialize of the compacted quads, from the previous build: the new
+    // outgoing lists are the old ones of each new vertex's own and relabelled vertices,
+    // renumbered; halfedges with untouched endpoints keep their bucket results
+    const int64_t nH = 4 * nQ;
+    uint8_t* touched = arena.get<uint8_t>("touched", (size_t)std::max<int64_t>(nV, 1));
+    int32_t* srcHead = arena.get<int32_t>("src_head", (size_t)std::max<int64_t>(nV, 1));
+    int32_t* srcNext = arena.get<int32_t>("src_next", (size_t)std::max<int64_t>(V, 1));
+    zero(touched, (size_t)nV);
+    fillNeg(srcHead, (size_t)nV * sizeof(int32_t));
+    if (V > 0) kSources<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext);
+    const int64_t* off = arena.ptr<int64_t>("out_off");
+    const int32_t* out = arena.ptr<int32_t>("out_list");
+    int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
+    zero(cnt + nV, sizeof(int32_t));
+    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, qnew, srcHead, srcNext, cnt);
+    int64_t* off2 = arena.get<int64_t>("out_off2", (size_t)std::max<int64_t>(nV, dmcV) + 1);
+    scanExclusiveI32(cnt, off2, nV + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(nV + 1)), stream);
+    int32_t* out2 = arena.get<int32_t>("out_list2", 4 * capQ);
+    if (nV > 0)
+        kOutFill<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, qnew, srcHead, srcNext, off2, out2, nH, counters());
+
+    int4* he2 = arena.get<int4>("he2", 4 * capQ);
+    int32_t* bsize2 = arena.get<int32_t>("bsize2", 4 * capQ);
+    int32_t* brank2 = arena.get<int32_t>("brank2", 4 * capQ);
+    int32_t* face_he = arena.get<int32_t>("face_he", (size_t)std::max<int64_t>(nQ, 1));
+    uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)std::max<int64_t>(nQ, 1));
+    uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)std::max<int64_t>(nV, 1));
+    if (nQ > 0) kFaceInit<<<gridFor(nQ), kBlock, 0, stream>>>(nattr, nQ, face_he, fattr);
+    zero(vflags, (size_t)nV * sizeof(uint32_t));
+    int32_t* valence = bucketPassValence(off2, nV);
+    if (nH > 0)
+        kRemapRebuild<<<gridFor(nH), kBlock, 0, stream>>>(nquads, nH, touched, oldQ, arena.ptr<int4>("he"), H, arena.ptr<int32_t>("bsize"),
+                                                         arena.ptr<int32_t>("brank"), qnew, off2, out2, he2, bsize2, brank2, vflags, fattr, nattr,
+                                                         valence, counters());
     arena.swap("pos", "pos2");
     arena.swap("nrm", "nrm2");
     arena.swap("quads", "quads2");
     arena.swap("qattr", "qattr2");
     if (npid) arena.swap("pid", "pid2");
+    arena.swap("he", "he2");
+    arena.swap("bsize", "bsize2");
+    arena.swap("brank", "brank2");
+    arena.swap("out_off", "out_off2");
+    arena.swap("out_list", "out_list2");
     V = nV;
     Q = nQ;
-    halfedgeInitialize();
+    H = nH;
+    F = nQ;
+    vertHeV = V;
+    vertHeQ = Q;
 }
 
 void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
@@ -1743,7 +2006,7 @@ void Pipeline::simplify3333() {
 
 void Pipeline::fixManifoldTwins() {
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int64_t>("offM"), arena.ptr<int2>("listA"),
+    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("quads"), arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
                                                  counters());
 }

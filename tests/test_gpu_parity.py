@@ -40,7 +40,7 @@ def assert_same(vals, origin, delta, iso, flags, refinement=0, split_step=oa.STE
         if e:
             errs.append(e)
     for k in ("n_vertices", "n_halfedges", "n_faces", "n_dmc_vertices", "n_dmc_quads", "nm_edges4", "nm_edges3", "n_edges",
-              "nm_vertices", "quads_with_nm", "split_iterations"):
+              "nm_vertices", "quads_with_nm", "split_iterations", "anomalies"):
         if gst[k] != o.stats[k]:
             errs.append("stat %s: gpu %s oracle %s" % (k, gst[k], o.stats[k]))
     assert not errs, "\n".join(errs)
