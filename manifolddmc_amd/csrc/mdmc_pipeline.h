@@ -99,6 +99,7 @@ class Pipeline {
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
+    void launchFill(void* p, uint8_t byte, size_t bytes);
     void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
     void buildBuckets(const uint8_t* only);
     int32_t* bucketPassValence(const int64_t* off, int64_t nV);

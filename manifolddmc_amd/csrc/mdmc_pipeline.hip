@@ -46,6 +46,20 @@ __device__ __forceinline__ int heN(const int4* he, int h) { return ((const int*)
 __device__ __forceinline__ int heT(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h + 3]; }
 __device__ __forceinline__ void setT(int4* he, int h, int t) { ((int*)he)[4 * (int64_t)h + 3] = t; }
 
+// bytes [0, head) and [head + body, n) one per thread of the first block, [head, head + body)
+// (16-byte aligned) 16 per thread
+__global__ void kFillBytes(uint8_t* __restrict__ p, uint8_t byte, size_t head, size_t body, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (16 * t < body) {
+        const uint32_t w = 0x01010101u * byte;
+        *reinterpret_cast<uint4*>(p + head + 16 * t) = make_uint4(w, w, w, w);
+    }
+    if (t < 16) {
+        if (t < head) p[t] = byte;
+        if (head + body + t < n) p[head + body + t] = byte;
+    }
+}
+
 // ---------------------------------------------------------------- halfedge construction
 // bucket order after refreshHashTable: (face, position along the face from its first halfedge)
 // and ResetFaceFlags (refreshFaceFlagsGPU) ahead of the NM bits the bucket pass sets again
@@ -389,22 +403,22 @@ __global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __res
 }
 
 // new vertex v': the halfedges of kept quads in the old lists of its own vertex and of the
-// dropped vertices relabelled to it (a chain through srcNext), counted, then written with
-// their new ids (the bucket pass checks every entry it reads against the quad's corner)
-__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                          const int32_t* __restrict__ qnew, const int32_t* __restrict__ srcHead, const int32_t* __restrict__ srcNext,
-                          int32_t* __restrict__ cnt) {
+// dropped vertices relabelled to it (a chain through srcNext), counted (list lengths less
+// the halfedges of removed quads), then written with their new ids (the bucket pass checks
+// every entry it reads against the quad's corner)
+__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ removedAt,
+                          const int32_t* __restrict__ srcHead, const int32_t* __restrict__ srcNext, int32_t* __restrict__ cnt) {
     const int64_t vn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vn >= Vn) return;
-    int c = 0;
+    int64_t c = 0;
     int u = oldV[vn], s = srcHead[vn];
     for (;;) {
-        for (int64_t k = off[u]; k < off[u + 1]; ++k) c += qnew[out[k] >> 2] >= 0 ? 1 : 0;
+        c += off[u + 1] - off[u] - removedAt[u];
         if (s < 0) break;
         u = s;
         s = srcNext[s];
     }
-    cnt[vn] = c;
+    cnt[vn] = (int32_t)c;
 }
 
 __global__ void kOutFill(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
@@ -727,11 +741,17 @@ __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t
 __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __restrict__ attr, int64_t Q, const int32_t* __restrict__ keep,
                               const int64_t* __restrict__ qaddr, const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
                               const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr,
-                              int32_t* __restrict__ oldQ, int32_t* __restrict__ qnew) {
+                              int32_t* __restrict__ oldQ, int32_t* __restrict__ qnew, int32_t* __restrict__ removedAt, int64_t V, int64_t* ctr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     if (!keep[q]) {
+        // the outgoing halfedges this quad takes from its corners' lists
         qnew[q] = -1;
+        const int4 r = quads[q];
+        atomicAdd(&removedAt[r.x], 1);
+        atomicAdd(&removedAt[r.y], 1);
+        atomicAdd(&removedAt[r.z], 1);
+        atomicAdd(&removedAt[r.w], 1);
         return;
     }
     qnew[q] = (int32_t)qaddr[q];
@@ -739,10 +759,14 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
     const int4 v4 = quads[q];
     const int v[4] = {v4.x, v4.y, v4.z, v4.w};
     int o[4];
+    bool bad = false;
     for (int i = 0; i < 4; ++i) {
-        if (p3333) o[i] = (a & QA_PATTERN) ? address[vtarget[v[i]]] : address[v[i]];
-        else o[i] = (vtype[v[i]] == 3) ? address[vtarget[v[i]]] : address[v[i]];
+        const bool moved = p3333 ? (a & QA_PATTERN) != 0 : vtype[v[i]] == 3;
+        const int src = moved ? vtarget[v[i]] : v[i];
+        o[i] = (src >= 0 && src < V) ? address[src] : -1;
+        bad = bad || o[i] < 0;  // a dropped vertex or a missing target: the quad is invalid
     }
+    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
     const int64_t d = qaddr[q];
     nquads[d] = make_int4(o[0], o[1], o[2], o[3]);
     uint32_t na = QA_COLOR;
@@ -1644,8 +1668,16 @@ int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outNam
     return h[1];
 }
 
-void Pipeline::zero(void* p, size_t bytes) { MDMC_HIP(hipMemsetAsync(p, 0, bytes, stream)); }
-void Pipeline::fillNeg(void* p, size_t bytes) { MDMC_HIP(hipMemsetAsync(p, 0xFF, bytes, stream)); }
+// fills with one launch (the runtime's memset launches a body and a tail kernel)
+void Pipeline::zero(void* p, size_t bytes) { launchFill(p, 0x00, bytes); }
+void Pipeline::fillNeg(void* p, size_t bytes) { launchFill(p, 0xFF, bytes); }
+void Pipeline::launchFill(void* p, uint8_t byte, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t head = std::min<size_t>((16 - ((uintptr_t)p & 15)) & 15, bytes);
+    const size_t body = (bytes - head) & ~(size_t)15;
+    const size_t vec = body / 16;
+    kFillBytes<<<(unsigned)std::max<size_t>((vec + kBlock - 1) / kBlock, 1), kBlock, 0, stream>>>((uint8_t*)p, byte, head, body, bytes);
+}
 
 void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t)); }
 
@@ -1902,10 +1934,11 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     uint32_t* nattr = arena.get<uint32_t>("qattr2", capQ);
     int32_t* oldQ = arena.get<int32_t>("oldQ", (size_t)std::max<int64_t>(nQ, 1));
     int32_t* qnew = arena.get<int32_t>("qnew", (size_t)std::max<int64_t>(Q, 1));
+    int32_t* removedAt = arena.get<int32_t>("removed_at", (size_t)std::max<int64_t>(V, 1));
+    zero(removedAt, (size_t)V * sizeof(int32_t));
     if (Q > 0)
         kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ,
-                                                         qnew);
-    if (nQ > 0) kValidateQuads<<<gridFor(nQ), kBlock, 0, stream>>>(nquads, nQ, nV, counters());
+                                                         qnew, removedAt, V, counters());
     checkValid("quad vertex ids");
 
     // HalfedgeMesh::initialize of the compacted quads, from the previous build: the new
@@ -1922,7 +1955,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     const int32_t* out = arena.ptr<int32_t>("out_list");
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
     zero(cnt + nV, sizeof(int32_t));
-    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, qnew, srcHead, srcNext, cnt);
+    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, removedAt, srcHead, srcNext, cnt);
     int64_t* off2 = arena.get<int64_t>("out_off2", (size_t)std::max<int64_t>(nV, dmcV) + 1);
     scanExclusiveI32(cnt, off2, nV + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(nV + 1)), stream);
     int32_t* out2 = arena.get<int32_t>("out_list2", 4 * capQ);
