@@ -34,6 +34,10 @@ class Pipeline {
     // vertex valence they start from
     int simplifyQueue = 0;
     bool valenceReady = false;
+    // vertices the outgoing lists of the last halfedge build describe (-1: none), and
+    // whether a split since then keeps the per-vertex "touched" flags
+    int64_t listsV = -1;
+    bool splitTouched = false;
 
     void resetCounters();
     int64_t anomalies();

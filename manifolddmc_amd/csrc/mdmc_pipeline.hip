@@ -532,6 +532,37 @@ __global__ void kOutDegree(const int4* __restrict__ he, int64_t H, int64_t V, in
     if (atomicAdd(&outdeg[o], 1) == 0) rep[o] = (int32_t)h;
 }
 
+// out-degree and one outgoing halfedge from the outgoing lists of the last build, for the
+// vertices they still describe: those below listsV that no split step touched (a split
+// step rewrites only halfedges whose origin is in its working copy, before and after);
+// the others are counted by kOutDegreeTouched
+__global__ void kOutFromLists(const int64_t* __restrict__ off, const int32_t* __restrict__ out, int64_t listsV, const uint8_t* __restrict__ touched,
+                              int64_t V, int32_t* __restrict__ outdeg, int32_t* __restrict__ rep) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    if (v < listsV && !(touched && touched[v])) {
+        const int64_t o = off[v];
+        const int d = (int)(off[v + 1] - o);
+        outdeg[v] = d;
+        if (d > 0) rep[v] = out[o];
+    } else {
+        outdeg[v] = 0;
+    }
+}
+
+__global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_t V, int64_t listsV, const uint8_t* __restrict__ touched,
+                                  int32_t* __restrict__ outdeg, int32_t* __restrict__ rep, int64_t* ctr) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) {
+        anomaly(ctr);
+        return;
+    }
+    if (o >= listsV || (touched && touched[o]))
+        if (atomicAdd(&outdeg[o], 1) == 0) rep[o] = (int32_t)h;
+}
+
 // A vertex whose outgoing halfedges form one closed clockwise cycle (next(twin(h)) from
 // any of them visits all and returns) has a single fan: every fan walk of fanWalk reaches
 // the same minimum.  The others are decided by kDetectFans over their outgoing lists.
@@ -1410,11 +1441,14 @@ __global__ void kCopyBackFaces(WC w, int count, const int64_t* __restrict__ addr
 }
 
 // the vertices a split step touched (its working copy's, by their ids after the copy back)
-__global__ void kMarkWcVertices(WC w, int count, uint8_t* __restrict__ dirtyV) {
+__global__ void kMarkWcVertices(WC w, int count, uint8_t* __restrict__ dirtyV, uint8_t* __restrict__ touched) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= count) return;
     const int o = w.v_c2o[c];
-    if (o >= 0) dirtyV[o] = 1;
+    if (o >= 0) {
+        dirtyV[o] = 1;
+        touched[o] = 1;
+    }
 }
 
 __global__ void kUpdateBack(WC w, int count, int4* __restrict__ he) {
@@ -1866,6 +1900,8 @@ void Pipeline::halfedgeInitialize() {
     zero(vflags, (size_t)V * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off, V);
     if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, he, bsize, brank, vflags, fattr, qattr, valence, counters());
+    listsV = V;
+    splitTouched = false;
 }
 
 void Pipeline::checkValid(const char* what) {
@@ -1991,6 +2027,8 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     F = nQ;
     vertHeV = V;
     vertHeQ = Q;
+    listsV = V;
+    splitTouched = false;
 }
 
 void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
@@ -2048,6 +2086,11 @@ void Pipeline::fixManifoldTwins() {
 void Pipeline::split(int step, int maxIters) {
     const bool copyDataBack = true;
     int iter = 0;
+    // every vertex any step touches (for the NM-vertex detection's use of the outgoing lists)
+    int64_t touchedV = V;
+    uint8_t* touched = arena.get<uint8_t>("split_touched", (size_t)std::max<int64_t>(V, 1));
+    zero(touched, (size_t)V);
+    splitTouched = true;
     while (true) {
         if (iter >= maxIters) {
             extraAnomalies += 1;
@@ -2202,12 +2245,18 @@ void Pipeline::split(int step, int maxIters) {
         }
         uint8_t* dirtyV = arena.get<uint8_t>("dirty_v", (size_t)std::max<int64_t>(V, 1));
         zero(dirtyV, (size_t)std::max<int64_t>(V, 1));
-        if (vsize > 0) kMarkWcVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, dirtyV);
+        if (V > touchedV) {
+            touched = arena.grow<uint8_t>("split_touched", (size_t)V, (size_t)touchedV, stream);
+            zero(touched + touchedV, (size_t)(V - touchedV));
+            touchedV = V;
+        }
+        if (vsize > 0) kMarkWcVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, dirtyV, touched);
         refreshHashTable(dirtyV, iterH);
         ++iter;
         splitIterations = iter;
         if (!(step >= 6) || !copyDataBack) break;
     }
+    if (iter == 0) splitTouched = false;  // nothing changed: the lists describe every vertex
 }
 
 void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag) {
@@ -2238,7 +2287,14 @@ void Pipeline::detectNonManifoldVertices() {
     // cross the bus one by one) and read back once
     int64_t* nslow = arena.get<int64_t>("fan_nslow", 1);
     zero(nslow, sizeof(int64_t));
-    kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
+    if (listsV >= 0 && listsV <= V) {
+        const uint8_t* touched = splitTouched ? arena.ptr<uint8_t>("split_touched") : nullptr;
+        kOutFromLists<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"), listsV, touched, V, outdeg,
+                                                         rep);
+        if (touched || V > listsV) kOutDegreeTouched<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, listsV, touched, outdeg, rep, counters());
+    } else {
+        kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
+    }
     kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, nslow);
     if (readI64(nslow) == 0) return;  // every vertex has one closed fan, or none
     // the others over their outgoing lists
