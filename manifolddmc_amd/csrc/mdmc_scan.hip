@@ -1,6 +1,13 @@
 // mdmc_scan.hip -- device-wide exclusive prefix sum of int32 counts into int64 offsets,
-// in one pass: tiles of 4096 counts are claimed in order with an atomic ticket and get
-// their offset by decoupled look-back over the tiles before them.
+// in one pass: block b scans tile b of 4096 counts and gets its offset by decoupled
+// look-back over the tiles before it.
+//
+// Tile = block index, not an atomic ticket: the blocks of a launch go to the XCDs
+// round-robin and each XCD starts its blocks in increasing order, so every tile below the
+// lowest resident one has finished (a lower tile not yet started would sit on an XCD whose
+// resident blocks are all lower still), and the lowest resident tile never waits on an
+// unstarted one.  The ticket cost one same-address atomic per block: 100 us against 67 us
+// for 17 M counts.
 //
 // Look-back words pack flag (2 bits) and value (62 bits), so relaxed device-scope
 // atomics suffice: a reader needs nothing beyond the word it reads.  Every wait is
@@ -25,7 +32,7 @@ __device__ __forceinline__ void storeRelaxed(unsigned long long* p, uint64_t v) 
     __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// scratch layout: [0] ticket, [1] timeout flag, [2 ..] one look-back word per tile
+// scratch layout: [0] unused, [1] timeout flag, [2 ..] one look-back word per tile
 __global__ void kScanReset(unsigned long long* scratch, int64_t words) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < words) scratch[i] = 0;
@@ -36,11 +43,9 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
                                                     int64_t* __restrict__ timeoutOut) {
     __shared__ int32_t s_in[kTile];
     __shared__ int64_t s_wave[kScanBlock / 64];
-    __shared__ long long s_tile, s_off;
+    __shared__ long long s_off;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) s_tile = (long long)atomicAdd(&scratch[0], 1ULL);
-    __syncthreads();
-    const int64_t t = s_tile;
+    const int64_t t = blockIdx.x;
     if (t >= numTiles) return;
     unsigned long long* status = scratch + 2;
     const int64_t base = t * kTile;
