@@ -289,7 +289,7 @@ def main():
                    "cells": cells, "vertices": s["n_vertices"], "faces": s["n_faces"], "dmc_vertices": s["n_dmc_vertices"],
                    "dmc_quads": s["n_dmc_quads"], "split_iterations": s["split_iterations"], "nm_edges4": s["nm_edges4"],
                    "nm_vertices": s["nm_vertices"]},
-        "roofline": {"kernel": "grid pass: kClassify + tile-count scan + kEmit", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"kernel": "grid pass: kClassify + tile-count scan + kEmit + kVertices", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(n, shape),
                      "algorithmic_bytes": alg_bytes, "kernel_ms": round(gen_ms, 4),
                      "tile_shape": mdmc.TILE_SHAPES.get(shape, shape), "tune_ms": {mdmc.TILE_SHAPES[k]: round(v, 4) for k, v in tune_ms.items()}},
