@@ -651,22 +651,29 @@ __device__ __forceinline__ void handleMerge(float* pos, float* nrm, int32_t* vty
     vtarget[z] = x;
 }
 
-// the four edge-neighbour faces; false when an edge has no twin (the reference
+// The simplifications run right after a halfedge build, whose faces are the quads in order:
+// halfedge 4f + i is corner i of face f, its next is 4f + (i + 1) % 4 and face_he[f] = 4f.
+// So next and face come from the id; only twins and origins are read.
+__device__ __forceinline__ int quadNext(int h, int k) { return (h & ~3) | ((h + k) & 3); }
+
+// the four edge-neighbour faces of quad q; false when an edge has no twin (the reference
 // then reads halfedges[-1], which is undefined; here the quad is left alone)
-__device__ __forceinline__ bool neighborFaces(const int4* __restrict__ he, int first, int nb[4]) {
-    int h = first;
+__device__ __forceinline__ bool neighborFaces(const int4* __restrict__ he, int q, int nb[4]) {
+    int t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = heT(he, 4 * q + i);
+    bool ok = true;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int t = heT(he, h);
-        if (t < 0) return false;
-        nb[i] = heF(he, t);
-        h = heN(he, h);
+        ok = ok && t[i] >= 0;
+        nb[i] = t[i] >> 2;
     }
-    return true;
+    return ok;
 }
 
 // mergeVerticesP3X3YColor (colour = 1) and mergeVerticesP3X3Y (colour = 0), HalfedgeMesh.cu:391-498
 __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int4* __restrict__ he,
-                           const int32_t* __restrict__ face_he, const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
+                           const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
                            float* __restrict__ pos, float* __restrict__ nrm, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
@@ -674,7 +681,7 @@ __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict_
     if (!colour && ((a & QA_NM) || (a & QA_BOUNDARY))) return;
     if (!(a & QA_PATTERN)) return;
     int nb[4];
-    if (!neighborFaces(he, face_he[q], nb)) return;
+    if (!neighborFaces(he, (int)q, nb)) return;
     for (int i = 0; i < 4; ++i)
         if ((qattr[nb[i]] & QA_NM) || (a & QA_BOUNDARY)) return;
     if (colour) {
@@ -705,26 +712,25 @@ __global__ void kClear3333(uint32_t* __restrict__ fattr, int64_t F) {
 }
 
 // markElementsP3333 second half, HalfedgeMesh.cu:547-628
-__global__ void kMark3333(const int4* __restrict__ he, const int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int64_t F,
+__global__ void kMark3333(const int4* __restrict__ he, uint32_t* __restrict__ fattr, int64_t F,
                           const int32_t* __restrict__ valence, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const uint32_t a0 = fattr[f];
     if ((a0 & QA_NM) || (a0 & QA_BOUNDARY)) return;
     int qh[4];
-    qh[0] = face_he[f];
     for (int i = 0; i < 4; ++i) {
-        if (i < 3) qh[i + 1] = heN(he, qh[i]);
+        qh[i] = 4 * (int)f + i;
         if (valence[heO(he, qh[i])] != 3) return;
     }
     int nbf[4], nbv[4], nbval[4];
     for (int i = 0; i < 4; ++i) {
         const int tw = heT(he, qh[i]);
         if (tw < 0) return;  // no neighbour across a boundary edge
-        nbf[i] = heF(he, tw);
+        nbf[i] = tw >> 2;
         const uint32_t na = fattr[nbf[i]];
         if ((na & QA_NM) || (na & QA_BOUNDARY)) return;
-        nbv[i] = heO(he, heN(he, heN(he, tw)));
+        nbv[i] = heO(he, quadNext(tw, 2));
         nbval[i] = valence[nbv[i]];
         if (nbval[i] == 3) return;
     }
@@ -2056,7 +2062,7 @@ void Pipeline::simplify3X3Y(bool colour) {
     if (Q > 0) {
         if (colour) kMark3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence);
         else kCount3x3yOld<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence, count);
-        kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), valence, count,
+        kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int4>("he"), valence, count,
                                                       arena.ptr<float>("pos"), arena.ptr<float>("nrm"), arena.ptr<int32_t>("vtype"),
                                                       arena.ptr<int32_t>("vtarget"), colour ? 1 : 0);
     }
@@ -2069,7 +2075,7 @@ void Pipeline::simplify3333() {
     simplifyInit(&valence, &count);
     if (F > 0) {
         kClear3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F);
-        kMark3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"), F, valence,
+        kMark3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<int4>("he"), arena.ptr<uint32_t>("fattr"), F, valence,
                                                      arena.ptr<int32_t>("vtype"), arena.ptr<int32_t>("vtarget"));
     }
     simplifyCompactAndRebuild(1, true);
