@@ -219,27 +219,52 @@ __device__ __forceinline__ int moverUpdate(const float* s, float iso, f3& point,
     }
     return r;
 }
-__device__ __forceinline__ void movePointToSurface(const float* s, float iso, f3& point) {
-    f3 i1 = point;
-    float v1 = trilinear(s, i1);
+// The iteration as a resumable state (moverInit, then moverStep until done), so that a
+// kernel can run a few steps on every lane and hand the unfinished ones to other lanes.
+struct Mover {
+    f3 i1, point;
+    float v1;
+    int it;
+    bool done;
+};
+__device__ __forceinline__ void moverInit(Mover& m, const float* s, f3 point) {
+    m.i1 = point;
+    m.point = point;
+    m.v1 = trilinear(s, point);
+    m.it = 0;
+    m.done = false;
+}
+// one pass of the 30-step loop: the crossing found (done), or the next point
+__device__ __forceinline__ void moverStep(Mover& m, const float* s, float iso) {
+    f3 g = moverGradient(s, m.i1);
+    if (m.v1 > iso) g = f3{-g.x, -g.y, -g.z};
+    f3 i2 = m.i1;
     float v2 = 0.f;
-    for (int it = 0; it < 30; ++it) {
-        f3 g = moverGradient(s, i1);
-        if (v1 > iso) g = f3{-g.x, -g.y, -g.z};
-        f3 i2 = i1;
-        const int r = moverUpdate(s, iso, point, g, i1, i2, v1, v2);
-        if (r & 1) break;
-        if (r & 14) {
-            if (r & 2) g.x = 0.f;
-            else if (r & 4) g.y = 0.f;
-            else g.z = 0.f;
-            const int r2 = moverUpdate(s, iso, point, g, i1, i2, v1, v2);
-            if (r2 & 1) break;
-        }
-        i1 = i2;
-        v1 = v2;
+    const int r = moverUpdate(s, iso, m.point, g, m.i1, i2, m.v1, v2);
+    if (r & 1) {
+        m.done = true;
+        return;
     }
-    point = vclamp(point, 0.0001f, 1.f - 0.0001f);
+    if (r & 14) {
+        if (r & 2) g.x = 0.f;
+        else if (r & 4) g.y = 0.f;
+        else g.z = 0.f;
+        const int r2 = moverUpdate(s, iso, m.point, g, m.i1, i2, m.v1, v2);
+        if (r2 & 1) {
+            m.done = true;
+            return;
+        }
+    }
+    m.i1 = i2;
+    m.v1 = v2;
+    if (++m.it == 30) m.done = true;
+}
+__device__ __forceinline__ f3 moverResult(const Mover& m) { return vclamp(m.point, 0.0001f, 1.f - 0.0001f); }
+__device__ __forceinline__ void movePointToSurface(const float* s, float iso, f3& point) {
+    Mover m;
+    moverInit(m, s, point);
+    while (!m.done) moverStep(m, s, iso);
+    point = moverResult(m);
 }
 
 // QuadrilateralAttribute bits (DeviceVectors.h:334-450) and vertex Flags bits (Halfedge.h:14-111)

@@ -150,16 +150,13 @@ __device__ __forceinline__ bool exactReciprocal(float d) {
     return (b & 0x7FFFFF) == 0 && e >= 1 && e <= 253;
 }
 
-// sliceP for contour t of one cell: dual vertex position (world) and normal
-__device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surfIter, int x, int y, int z, const float s[8], uint64_t w, int t, f3& wp,
-                                           f3& nn) {
+// sliceP for contour t of one cell, in three parts: the 24 neighbour samples of the corner
+// gradients (computeGradients), the edge mean of the contour's crossings (cell-local), and --
+// after the move to the surface -- the normal and the world position.
+__device__ __forceinline__ void neighbourSamples(const GridDev& g, int x, int y, int z, float nbx[8], float nby[8], float nbz[8]) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int64_t NX = nx, NXY = (int64_t)nx * ny;
     auto at = [&](int px, int py, int pz) { return g.v[(int64_t)pz * NXY + (int64_t)py * NX + px]; };
-    // the 24 neighbour samples of the corner gradients (computeGradients): loaded first and
-    // used after the position, so that their latency overlaps the edge mean and the
-    // move-to-surface iteration
-    float nbx[8], nby[8], nbz[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
@@ -168,6 +165,9 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
         nby[k] = at(px, clampi(cy ? py + 1 : py - 1, 0, ny - 1), pz);
         nbz[k] = at(px, py, clampi(cz ? pz + 1 : pz - 1, 0, nz - 1));
     }
+}
+
+__device__ __forceinline__ f3 edgeMean(const float s[8], float iso, uint64_t w, int t) {
     int j = 0;
     for (int c = 0; c < t; ++c) j += polySize(w, c);
     const int sz = polySize(w, t);
@@ -182,8 +182,11 @@ __device__ __forceinline__ void dualVertex(const GridDev& g, float iso, int surf
         off.z = 0.f + (tt * (ax == 2 ? 1.f : 0.f) + (float)((a >> 2) & 1));
         p = p + off;
     }
-    p = divs(p, (float)sz);
-    if (surfIter) movePointToSurface(s, iso, p);
+    return divs(p, (float)sz);
+}
+
+__device__ __forceinline__ void finishVertex(const GridDev& g, int x, int y, int z, const float s[8], const float nbx[8], const float nby[8],
+                                             const float nbz[8], f3 p, f3& wp, f3& nn) {
     // central differences ((upper - lower) * 0.5 / delta) at the 8 corners
     float gxv[8], gyv[8], gzv[8];
 #pragma unroll
@@ -516,26 +519,96 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
 
 // ---- dual vertices (sliceP), one per thread over all vertices of the pass
 // (blocks XCD-chunked like the tiles when `chunked`: consecutive vertices share rows and
-// slices of the field, which then stay in one XCD's L2)
+// slices of the field, which then stay in one XCD's L2).  The move to the surface takes
+// 1.5 steps on average but a wave waits for its slowest lane (5.8 steps at the 512^3
+// gyroid), so every lane runs at most kMoveSteps steps; the block's unfinished lanes leave
+// their state in LDS, consecutive lanes finish them on full waves, and each vertex then
+// computes its normal from its final point.  Each vertex's arithmetic is unchanged.
+constexpr int kMoveSteps = 2;
+constexpr int kMoveState = 17;  // s[8], iso, i1, v1, point, step
+
 __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chunked) {
+    __shared__ float s_st[kMoveState][GP_THREADS];
+    __shared__ int s_wave[GP_WAVES];
     const int64_t nb = (a.V + GP_THREADS - 1) / GP_THREADS;
     const int64_t v = tileOfBlock(blockIdx.x, nb, chunked != 0) * GP_THREADS + threadIdx.x;
-    if (v >= a.V) return;
-    const ProblemDev& P = a.probs[a.vpid ? a.vpid[v] : 0];
-    const GridDev g = P.g;
-    const float iso = P.iso;
-    const unsigned c = (unsigned)a.vcell[v];
-    const unsigned r = c / (unsigned)P.CX;
-    const int x = (int)(c - r * (unsigned)P.CX);
-    const int y = (int)(r % (unsigned)P.CY);
-    const int z = (int)(r / (unsigned)P.CY);
-    float s[8];
-    loadCorners(g, x, y, z, s);
-    int mc = 0;
+    const bool live = v < a.V;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    GridDev g{};
+    float iso = 0.f;
+    int x = 0, y = 0, z = 0;
+    float s[8], nbx[8], nby[8], nbz[8];
+    Mover m{};
+    m.done = true;
+    f3 p = mk3(0.f, 0.f, 0.f);
+    if (live) {
+        const ProblemDev& P = a.probs[a.vpid ? a.vpid[v] : 0];
+        g = P.g;
+        iso = P.iso;
+        const unsigned c = (unsigned)a.vcell[v];
+        const unsigned r = c / (unsigned)P.CX;
+        x = (int)(c - r * (unsigned)P.CX);
+        y = (int)(r % (unsigned)P.CY);
+        z = (int)(r / (unsigned)P.CY);
+        loadCorners(g, x, y, z, s);
+        int mc = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) mc |= (s[k] >= iso ? 1 : 0) << k;
+        for (int k = 0; k < 8; ++k) mc |= (s[k] >= iso ? 1 : 0) << k;
+        p = edgeMean(s, iso, cellPolygonTab(mc, s, iso), a.vcontour[v]);
+        if (a.surfIter) {
+            moverInit(m, s, p);
+            for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);
+        }
+    }
+    // unfinished lanes -> LDS slots, in lane order
+    const bool later = live && !m.done;
+    const uint64_t bal = __ballot(later);
+    if (lane == 0) s_wave[wid] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < GP_WAVES; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
+    }
+    const int slot = base + __popcll(bal & ((1ULL << lane) - 1ULL));
+    if (later) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s_st[k][slot] = s[k];
+        s_st[8][slot] = iso;
+        s_st[9][slot] = m.i1.x;
+        s_st[10][slot] = m.i1.y;
+        s_st[11][slot] = m.i1.z;
+        s_st[12][slot] = m.v1;
+        s_st[13][slot] = m.point.x;
+        s_st[14][slot] = m.point.y;
+        s_st[15][slot] = m.point.z;
+        s_st[16][slot] = __int_as_float(m.it);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < total) {
+        const int k0 = threadIdx.x;
+        float ss[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ss[k] = s_st[k][k0];
+        const float is = s_st[8][k0];
+        Mover mm;
+        mm.i1 = mk3(s_st[9][k0], s_st[10][k0], s_st[11][k0]);
+        mm.v1 = s_st[12][k0];
+        mm.point = mk3(s_st[13][k0], s_st[14][k0], s_st[15][k0]);
+        mm.it = __float_as_int(s_st[16][k0]);
+        mm.done = false;
+        while (!mm.done) moverStep(mm, ss, is);
+        s_st[13][k0] = mm.point.x;
+        s_st[14][k0] = mm.point.y;
+        s_st[15][k0] = mm.point.z;
+    }
+    __syncthreads();
+    if (!live) return;
+    neighbourSamples(g, x, y, z, nbx, nby, nbz);
+    if (a.surfIter) p = later ? moverResult(Mover{mk3(0.f, 0.f, 0.f), mk3(s_st[13][slot], s_st[14][slot], s_st[15][slot]), 0.f, 0, true}) : moverResult(m);
     f3 wp, nn;
-    dualVertex(g, iso, a.surfIter, x, y, z, s, cellPolygonTab(mc, s, iso), a.vcontour[v], wp, nn);
+    finishVertex(g, x, y, z, s, nbx, nby, nbz, p, wp, nn);
     a.pos[3 * v + 0] = wp.x;
     a.pos[3 * v + 1] = wp.y;
     a.pos[3 * v + 2] = wp.z;
