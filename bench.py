@@ -27,7 +27,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--cpu-planes", type=int, default=20, help="z-planes of the field in the CPU baseline sample")
+    ap.add_argument("--cpu-planes", type=int, default=96, help="z-planes of the field in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
