@@ -496,27 +496,26 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
     }
 }
 
-// outgoing-halfedge lists (by origin) for the fan detection, of the vertices in `only`
-// (all when null)
-__global__ void kCountOrigin(const int4* __restrict__ he, int64_t H, int64_t V, const uint8_t* __restrict__ only, int32_t* __restrict__ deg,
-                             int64_t* ctr) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
-    const int o = heO(he, (int)h);
-    if (o < 0 || o >= V) return;  // counted as an anomaly by kOutDegree
-    if (only && !only[o]) return;
-    atomicAdd(&deg[o], 1);
+// list lengths of the fan detection's slow vertices: their out-degree (the halfedges
+// kOutDegree / the outgoing lists counted per origin), 0 for the others
+__global__ void kSlowDegree(const uint8_t* __restrict__ slow, const int32_t* __restrict__ outdeg, int64_t V, int32_t* __restrict__ deg) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    deg[v] = (v < V && slow[v]) ? outdeg[v] : 0;
 }
 
+// outgoing-halfedge lists (by origin) for the fan detection, of the vertices in `only`
+// (all when null)
+// (deg counts down to 0; a vertex whose count does not end at 0 is skipped by kDetectFans)
 __global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const uint8_t* __restrict__ only, const int64_t* __restrict__ off,
-                           int32_t* __restrict__ deg, int32_t* __restrict__ list) {
+                           int32_t* __restrict__ deg, int32_t* __restrict__ list, int64_t* ctr) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     const int o = heO(he, (int)h);
     if (o < 0 || o >= V) return;
     if (only && !only[o]) return;
     const int slot = atomicSub(&deg[o], 1) - 1;
-    list[off[o] + slot] = (int32_t)h;
+    if (slot >= 0) list[off[o] + slot] = (int32_t)h;
 }
 
 // out-degree of every vertex and one of its outgoing halfedges
@@ -1508,7 +1507,7 @@ __device__ int fanWalk(const int4* __restrict__ he, int start, bool cw, int64_t*
 // halfedge that does not start at v ends the walk, as the origin test of fanWalk does.
 constexpr int kFanLocal = 16;
 __global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ he, int64_t V, const uint8_t* __restrict__ only, const int64_t* __restrict__ off,
-                                                      const int32_t* __restrict__ list, uint32_t* __restrict__ vflags,
+                                                      const int32_t* __restrict__ list, const int32_t* __restrict__ left, uint32_t* __restrict__ vflags,
                                                       int32_t* __restrict__ keyScratch, int64_t* ctr) {
     __shared__ int s_h[kFanLocal * kBlock];
     __shared__ int8_t s_cw[kFanLocal * kBlock];
@@ -1516,6 +1515,10 @@ __global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ h
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vv >= V) return;
     if (only && !only[vv]) return;
+    if (left[vv] != 0) {  // the list and the out-degree disagree
+        anomaly(ctr);
+        return;
+    }
     const int tid = threadIdx.x;
     const int64_t b0 = off[vv];
     const int d = (int)(off[vv + 1] - b0);
@@ -2305,15 +2308,14 @@ void Pipeline::detectNonManifoldVertices() {
     if (readI64(nslow) == 0) return;  // every vertex has one closed fan, or none
     // the others over their outgoing lists
     int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
-    zero(deg, ((size_t)V + 1) * sizeof(int32_t));
-    kCountOrigin<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, deg, counters());
+    kSlowDegree<<<gridFor(V + 1), kBlock, 0, stream>>>(slow, outdeg, V, deg);
     int64_t* off = arena.get<int64_t>("off", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(deg, off, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
     int32_t* list = arena.get<int32_t>("list", (size_t)H);
-    kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, off, deg, list);
+    kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, off, deg, list, counters());
     int32_t* key = arena.get<int32_t>("fankey", (size_t)H);
-    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, slow, off, list, vflags, key, counters());
+    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, slow, off, list, deg, vflags, key, counters());
 }
 
 void Pipeline::computeStats(int64_t out[9]) {
