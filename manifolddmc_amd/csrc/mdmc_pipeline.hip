@@ -506,6 +506,15 @@ __global__ void kSlowDegree(const uint8_t* __restrict__ slow, const int32_t* __r
 
 // outgoing-halfedge lists (by origin) for the fan detection, of the vertices in `only`
 // (all when null)
+// the lists of slow vertices the build's outgoing lists still describe, copied from them
+__global__ void kCopySlowLists(const uint8_t* __restrict__ slow, int64_t V, const int64_t* __restrict__ outOff, const int32_t* __restrict__ outList,
+                               const int64_t* __restrict__ off, int32_t* __restrict__ list) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || !slow[v]) return;
+    const int64_t o = off[v], d = off[v + 1] - o, src = outOff[v];
+    for (int64_t k = 0; k < d; ++k) list[o + k] = outList[src + k];
+}
+
 // (deg counts down to 0; a vertex whose count does not end at 0 is skipped by kDetectFans)
 __global__ void kFillLists(const int4* __restrict__ he, int64_t H, int64_t V, const uint8_t* __restrict__ only, const int64_t* __restrict__ off,
                            int32_t* __restrict__ deg, int32_t* __restrict__ list, int64_t* ctr) {
@@ -566,7 +575,8 @@ __global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_
 // any of them visits all and returns) has a single fan: every fan walk of fanWalk reaches
 // the same minimum.  The others are decided by kDetectFans over their outgoing lists.
 __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
-                         uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, int64_t* __restrict__ nslow) {
+                         uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, const uint8_t* __restrict__ touched, int64_t listsV,
+                         int64_t* __restrict__ nslow) {
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vv >= V) return;
     const int v = (int)vv;
@@ -588,9 +598,13 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* 
     }
     slow[v] = single ? 0 : 1;
     if (single) vflags[v] &= ~(uint32_t)VF_FANS;
-    // count of the vertices left for the outgoing-list pass (one atomic per wave)
+    // counts of the vertices left for the outgoing-list pass and of those among them whose
+    // outgoing lists are not the build's (one atomic per wave each)
     const uint64_t m = __ballot(!single);
-    if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) atomicAdd((unsigned long long*)nslow, (unsigned long long)__popcll(m));
+    const uint64_t mt = __ballot(!single && (v >= listsV || (touched && touched[v])));
+    const int leader = (int)(threadIdx.x & 63) == __ffsll((unsigned long long)(m ? m : 1ULL)) - 1;
+    if (m && leader) atomicAdd((unsigned long long*)nslow, (unsigned long long)__popcll(m));
+    if (mt && leader) atomicAdd((unsigned long long*)(nslow + 1), (unsigned long long)__popcll(mt));
 }
 
 // ---------------------------------------------------------------- simplifications
@@ -1515,7 +1529,7 @@ __global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ h
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vv >= V) return;
     if (only && !only[vv]) return;
-    if (left[vv] != 0) {  // the list and the out-degree disagree
+    if (left && left[vv] != 0) {  // the list and the out-degree disagree
         anomaly(ctr);
         return;
     }
@@ -2294,18 +2308,23 @@ void Pipeline::detectNonManifoldVertices() {
     zero(outdeg, (size_t)V * sizeof(int32_t));
     // the slow-vertex count is accumulated in device memory (atomics on mapped host memory
     // cross the bus one by one) and read back once
-    int64_t* nslow = arena.get<int64_t>("fan_nslow", 1);
-    zero(nslow, sizeof(int64_t));
-    if (listsV >= 0 && listsV <= V) {
-        const uint8_t* touched = splitTouched ? arena.ptr<uint8_t>("split_touched") : nullptr;
+    int64_t* nslow = arena.get<int64_t>("fan_nslow", 2);
+    zero(nslow, 2 * sizeof(int64_t));
+    const bool fromLists = listsV >= 0 && listsV <= V;
+    const uint8_t* touched = (fromLists && splitTouched) ? arena.ptr<uint8_t>("split_touched") : nullptr;
+    if (fromLists) {
         kOutFromLists<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"), listsV, touched, V, outdeg,
                                                          rep);
         if (touched || V > listsV) kOutDegreeTouched<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, listsV, touched, outdeg, rep, counters());
     } else {
         kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
     }
-    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, nslow);
-    if (readI64(nslow) == 0) return;  // every vertex has one closed fan, or none
+    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, touched, fromLists ? listsV : 0, nslow);
+    int64_t* hc = pinHost();
+    MDMC_HIP(hipMemcpyAsync(hc, nslow, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    const int64_t nSlow = hc[0], nSlowOff = hc[1];
+    if (nSlow == 0) return;  // every vertex has one closed fan, or none
     // the others over their outgoing lists
     int32_t* deg = arena.get<int32_t>("deg", (size_t)V + 1);
     kSlowDegree<<<gridFor(V + 1), kBlock, 0, stream>>>(slow, outdeg, V, deg);
@@ -2313,9 +2332,11 @@ void Pipeline::detectNonManifoldVertices() {
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(deg, off, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
     int32_t* list = arena.get<int32_t>("list", (size_t)H);
-    kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, off, deg, list, counters());
+    const bool copyLists = nSlowOff == 0;  // every slow vertex keeps the build's list
+    if (copyLists) kCopySlowLists<<<gridFor(V), kBlock, 0, stream>>>(slow, V, arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"), off, list);
+    else kFillLists<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, slow, off, deg, list, counters());
     int32_t* key = arena.get<int32_t>("fankey", (size_t)H);
-    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, slow, off, list, deg, vflags, key, counters());
+    kDetectFans<<<gridFor(V), kBlock, 0, stream>>>(he, V, slow, off, list, copyLists ? nullptr : deg, vflags, key, counters());
 }
 
 void Pipeline::computeStats(int64_t out[9]) {
