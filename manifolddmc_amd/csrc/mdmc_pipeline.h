@@ -38,6 +38,10 @@ class Pipeline {
     // whether a split since then keeps the per-vertex "touched" flags
     int64_t listsV = -1;
     bool splitTouched = false;
+    // per halfedge whether its record still has the build's layout (next = quadNext, face =
+    // id / 4): tracked once a split step writes records back; before that every one has
+    bool canonTracked = false;
+    const uint8_t* canonFlags();
 
     void resetCounters();
     int64_t anomalies();

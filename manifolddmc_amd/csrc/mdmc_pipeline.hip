@@ -45,6 +45,11 @@ __device__ __forceinline__ int heF(const int4* he, int h) { return ((const int*)
 __device__ __forceinline__ int heN(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h + 2]; }
 __device__ __forceinline__ int heT(const int4* he, int h) { return ((const int*)he)[4 * (int64_t)h + 3]; }
 __device__ __forceinline__ void setT(int4* he, int h, int t) { ((int*)he)[4 * (int64_t)h + 3] = t; }
+// the k-th next of h in a face laid out as a halfedge build leaves it (4f .. 4f + 3)
+__device__ __forceinline__ int quadNext(int h, int k) { return (h & ~3) | ((h + k) & 3); }
+// next of h: from the id where canon marks h as laid out by the build (canon null: all are),
+// else from its record (the split rewrites the records of the halfedges it copies back)
+__device__ __forceinline__ int nextOf(const int4* he, const uint8_t* canon, int h) { return (!canon || canon[h]) ? quadNext(h, 1) : heN(he, h); }
 
 // bytes [0, head) and [head + body, n) one per thread of the first block, [head, head + body)
 // (16-byte aligned) 16 per thread
@@ -576,7 +581,7 @@ __global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_
 // the same minimum.  The others are decided by kDetectFans over their outgoing lists.
 __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
                          uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, const uint8_t* __restrict__ touched, int64_t listsV,
-                         int64_t* __restrict__ nslow) {
+                         const uint8_t* __restrict__ canon, int64_t* __restrict__ nslow) {
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vv >= V) return;
     const int v = (int)vv;
@@ -588,7 +593,7 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* 
         for (int k = 1; k <= d; ++k) {
             const int t = heT(he, cur);
             if (t < 0) break;
-            cur = heN(he, t);
+            cur = nextOf(he, canon, t);
             if (heO(he, cur) != v) break;
             if (cur == start) {
                 single = k == d;
@@ -666,8 +671,7 @@ __device__ __forceinline__ void handleMerge(float* pos, float* nrm, int32_t* vty
 
 // The simplifications run right after a halfedge build, whose faces are the quads in order:
 // halfedge 4f + i is corner i of face f, its next is 4f + (i + 1) % 4 and face_he[f] = 4f.
-// So next and face come from the id; only twins and origins are read.
-__device__ __forceinline__ int quadNext(int h, int k) { return (h & ~3) | ((h + k) & 3); }
+// So next and face come from the id (quadNext); only twins and origins are read.
 
 // the four edge-neighbour faces of quad q; false when an edge has no twin (the reference
 // then reads halfedges[-1], which is undefined; here the quad is left alone)
@@ -1434,7 +1438,7 @@ __global__ void kCopyBackVertices(WC w, int count, const int64_t* __restrict__ a
     if (pid) pid[o] = w.pid[v];
 }
 
-__global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ addr, int64_t H0, int4* __restrict__ he) {
+__global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ addr, int64_t H0, int4* __restrict__ he, uint8_t* __restrict__ canon) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= count) return;
     if (w.hflags[h] & VF_REMOVE) return;
@@ -1444,6 +1448,7 @@ __global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ 
         w.h_c2o[h] = (int32_t)o;
     }
     he[o] = w.he[h];
+    canon[o] = 0;  // this record (and kUpdateBack's rewrite of it) may leave the build's layout
 }
 
 __global__ void kCopyBackFaces(WC w, int count, const int64_t* __restrict__ addr, int64_t F0, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
@@ -1606,7 +1611,7 @@ __global__ void kValidateQuads(const int4* __restrict__ quads, int64_t Q, int64_
 // counts over the vertex-pair buckets (HalfedgeMesh.cu:1164-1239, ManifoldCreator.cu:545-549)
 __global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F,
                                                  const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, const uint32_t* __restrict__ vflags,
-                                                 int64_t V, unsigned long long* __restrict__ out) {
+                                                 int64_t V, const uint8_t* __restrict__ canon, unsigned long long* __restrict__ out) {
     __shared__ unsigned int acc[9];
     if (threadIdx.x < 9) acc[threadIdx.x] = 0;
     __syncthreads();
@@ -1623,7 +1628,7 @@ __global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, in
             int k = 0;
             for (int j = 0; j < 4; ++j) {
                 if (bsize[h] > 2) ++k;
-                h = heN(he, h);
+                h = nextOf(he, canon, h);
             }
             loc[3 + k] += 1;
         }
@@ -1711,6 +1716,8 @@ int64_t Pipeline::readI64(const int64_t* dev) {
 }
 
 int64_t* Pipeline::counters() { return arena.get<int64_t>("counters", A_NCOUNTERS); }
+
+const uint8_t* Pipeline::canonFlags() { return canonTracked ? arena.ptr<uint8_t>("he_canon") : nullptr; }
 
 int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out) {
     *out = arena.get<int64_t>(outName, (size_t)n + 1);
@@ -1925,6 +1932,7 @@ void Pipeline::halfedgeInitialize() {
     if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, he, bsize, brank, vflags, fattr, qattr, valence, counters());
     listsV = V;
     splitTouched = false;
+    canonTracked = false;
 }
 
 void Pipeline::checkValid(const char* what) {
@@ -2052,6 +2060,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     vertHeQ = Q;
     listsV = V;
     splitTouched = false;
+    canonTracked = false;
 }
 
 void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
@@ -2249,7 +2258,15 @@ void Pipeline::split(int step, int maxIters) {
             a = ah;
             const int64_t H0 = H;
             int4* he = arena.grow<int4>("he", (size_t)(H0 + addH), (size_t)H0, stream);
-            kCopyBackHalfedges<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, a, H0, he);
+            // layout flags: all set after the build, cleared for every record written back
+            if (!canonTracked) {
+                uint8_t* c0 = arena.get<uint8_t>("he_canon", (size_t)std::max<int64_t>(H0, 1));
+                launchFill(c0, 1, (size_t)H0);
+                canonTracked = true;
+            }
+            uint8_t* canon = arena.grow<uint8_t>("he_canon", (size_t)std::max<int64_t>(H0 + addH, 1), (size_t)H0, stream);
+            zero(canon + H0, (size_t)addH);
+            kCopyBackHalfedges<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, a, H0, he, canon);
             // faces
             a = af;
             const int64_t F0 = F;
@@ -2319,7 +2336,7 @@ void Pipeline::detectNonManifoldVertices() {
     } else {
         kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
     }
-    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, touched, fromLists ? listsV : 0, nslow);
+    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, touched, fromLists ? listsV : 0, canonFlags(), nslow);
     int64_t* hc = pinHost();
     MDMC_HIP(hipMemcpyAsync(hc, nslow, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));
@@ -2345,7 +2362,7 @@ void Pipeline::computeStats(int64_t out[9]) {
     const int64_t n = std::max(H, std::max(F, V));
     if (n > 0 && H > 0)
         kStats<<<(unsigned)std::min<int64_t>(gridFor(n), 4096), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int32_t>("face_he"), F, arena.ptr<int32_t>("bsize"),
-                                                  arena.ptr<int32_t>("brank"), arena.ptr<uint32_t>("vflags"), V, d);
+                                                  arena.ptr<int32_t>("brank"), arena.ptr<uint32_t>("vflags"), V, canonFlags(), d);
     MDMC_HIP(hipMemcpyAsync(out, d, 9 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));
 }
