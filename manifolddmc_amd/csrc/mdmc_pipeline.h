@@ -42,6 +42,10 @@ class Pipeline {
     // id / 4): tracked once a split step writes records back; before that every one has
     bool canonTracked = false;
     const uint8_t* canonFlags();
+    // the halfedge records (he) are written from the quads and the twins after the
+    // simplifications, which read only those
+    bool heMaterialized = false;
+    void materializeHalfedges();
 
     void resetCounters();
     int64_t anomalies();

@@ -316,14 +316,17 @@ __device__ int gatherBucketOut(int h, const int4* __restrict__ quads, int64_t H,
     return n;
 }
 
-// the record of halfedge h = 4f + i from a to b with its bucket, and what the bucket pass of
-// an initialize derives from it: vertex flags of the origin, the face's NM bit (fattr and the
-// quad attributes), an incoming boundary halfedge's share of the destination's valence
-__device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketOf& bk, int4* __restrict__ he, int32_t* __restrict__ bsize,
+// the bucket of halfedge h = 4f + i from a to b, and what the bucket pass of an initialize
+// derives from it: vertex flags of the origin, the face's NM bit (fattr and the quad
+// attributes), an incoming boundary halfedge's share of the destination's valence.  Until the
+// simplifications are done a halfedge is its id (origin = corner i of quad f, face f, next
+// 4f + (i + 1) % 4) and its twin; the full records are written once after them
+// (kMaterialize).
+__device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketOf& bk, int32_t* __restrict__ twin, int32_t* __restrict__ bsize,
                                               int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
                                               uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
     const int f = h >> 2;
-    he[h] = make_int4(a, f, (h & ~3) | ((h + 1) & 3), bk.twin);
+    twin[h] = bk.twin;
     bsize[h] = bk.n;
     brank[h] = bk.rank;
     if (valence && bk.twin == -1) atomicAdd(&valence[b], 1);
@@ -376,14 +379,22 @@ __global__ void kFaceInit(const uint32_t* __restrict__ qattr, int64_t Q, int32_t
 
 // every halfedge from its bucket
 __global__ void kBucketsAll(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                            int4* __restrict__ he, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
+                            int32_t* __restrict__ twin, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
                             uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
     const int h = (int)hh;
     const int4 q = quads[h >> 2];
     const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-    storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads, H, off, out, ctr), he, bsize, brank, vflags, fattr, qattr, valence);
+    storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads, H, off, out, ctr), twin, bsize, brank, vflags, fattr, qattr, valence);
+}
+
+// the halfedge records of the quads, from the ids and the twins
+__global__ void kMaterialize(const int4* __restrict__ quads, int64_t H, const int32_t* __restrict__ twin, int4* __restrict__ he) {
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hh >= H) return;
+    const int h = (int)hh;
+    he[h] = make_int4(quadCorner(quads[h >> 2], h & 3), h >> 2, quadNext(h, 1), twin[h]);
 }
 
 // ---- rebuild after a simplification's compaction.  New vertex v' = address[v]; a dropped
@@ -452,10 +463,10 @@ __global__ void kOutFill(const int32_t* __restrict__ oldV, int64_t Vn, const int
 // halfedge; the block's other halfedges are gathered in LDS and recomputed from their
 // buckets by the block's first lanes, so that the long path runs on full waves
 __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__ quads2, int64_t H2, const uint8_t* __restrict__ touched,
-                                                        const int32_t* __restrict__ oldQ, const int4* __restrict__ heOld, int64_t Hold,
+                                                        const int32_t* __restrict__ oldQ, const int32_t* __restrict__ twinOld, int64_t Hold,
                                                         const int32_t* __restrict__ bsizeOld, const int32_t* __restrict__ brankOld,
                                                         const int32_t* __restrict__ qnew, const int64_t* __restrict__ off2,
-                                                        const int32_t* __restrict__ out2, int4* __restrict__ he, int32_t* __restrict__ bsize,
+                                                        const int32_t* __restrict__ out2, int32_t* __restrict__ twin, int32_t* __restrict__ bsize,
                                                         int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
                                                         uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
     __shared__ int s_list[kBlock];
@@ -470,7 +481,7 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
             later = true;
         } else {
             const int ho = 4 * oldQ[h >> 2] + (h & 3);
-            const int tt = heOld[ho].w;
+            const int tt = twinOld[ho];
             int t = -1;
             if (tt >= Hold) {
                 anomaly(ctr);
@@ -479,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
                 if (fo < 0) anomaly(ctr);
                 else t = 4 * fo + (tt & 3);
             }
-            storeHalfedge(h, a, b, BucketOf{bsizeOld[ho], brankOld[ho], t}, he, bsize, brank, vflags, fattr, qattr, valence);
+            storeHalfedge(h, a, b, BucketOf{bsizeOld[ho], brankOld[ho], t}, twin, bsize, brank, vflags, fattr, qattr, valence);
         }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -497,7 +508,7 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
         const int h = s_list[threadIdx.x];
         const int4 q = quads2[h >> 2];
         const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-        storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads2, H2, off2, out2, ctr), he, bsize, brank, vflags, fattr, qattr, valence);
+        storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads2, H2, off2, out2, ctr), twin, bsize, brank, vflags, fattr, qattr, valence);
     }
 }
 
@@ -613,11 +624,12 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* 
 }
 
 // ---------------------------------------------------------------- simplifications
-__global__ void kValence(const int4* __restrict__ he, int64_t H, int32_t* __restrict__ valence) {
+__global__ void kValence(const int4* __restrict__ quads, const int32_t* __restrict__ twin, int64_t H, int32_t* __restrict__ valence) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
-    atomicAdd(&valence[heO(he, (int)h)], 1);
-    if (heT(he, (int)h) == -1) atomicAdd(&valence[heO(he, heN(he, (int)h))], 1);
+    const int4 q = quads[h >> 2];
+    atomicAdd(&valence[quadCorner(q, (int)(h & 3))], 1);
+    if (twin[h] == -1) atomicAdd(&valence[quadCorner(q, (int)((h + 1) & 3))], 1);
 }
 
 __device__ __forceinline__ bool pattern1(const int v[4]) { return v[0] == 3 && v[1] >= 5 && v[2] == 3 && v[3] >= 5; }
@@ -675,10 +687,9 @@ __device__ __forceinline__ void handleMerge(float* pos, float* nrm, int32_t* vty
 
 // the four edge-neighbour faces of quad q; false when an edge has no twin (the reference
 // then reads halfedges[-1], which is undefined; here the quad is left alone)
-__device__ __forceinline__ bool neighborFaces(const int4* __restrict__ he, int q, int nb[4]) {
-    int t[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) t[i] = heT(he, 4 * q + i);
+__device__ __forceinline__ bool neighborFaces(const int32_t* __restrict__ twin, int q, int nb[4]) {
+    const int4 tq = *reinterpret_cast<const int4*>(twin + 4 * (int64_t)q);
+    const int t[4] = {tq.x, tq.y, tq.z, tq.w};
     bool ok = true;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -689,7 +700,7 @@ __device__ __forceinline__ bool neighborFaces(const int4* __restrict__ he, int q
 }
 
 // mergeVerticesP3X3YColor (colour = 1) and mergeVerticesP3X3Y (colour = 0), HalfedgeMesh.cu:391-498
-__global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int4* __restrict__ he,
+__global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ twin,
                            const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
                            float* __restrict__ pos, float* __restrict__ nrm, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -698,7 +709,7 @@ __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict_
     if (!colour && ((a & QA_NM) || (a & QA_BOUNDARY))) return;
     if (!(a & QA_PATTERN)) return;
     int nb[4];
-    if (!neighborFaces(he, (int)q, nb)) return;
+    if (!neighborFaces(twin, (int)q, nb)) return;
     for (int i = 0; i < 4; ++i)
         if ((qattr[nb[i]] & QA_NM) || (a & QA_BOUNDARY)) return;
     if (colour) {
@@ -729,25 +740,24 @@ __global__ void kClear3333(uint32_t* __restrict__ fattr, int64_t F) {
 }
 
 // markElementsP3333 second half, HalfedgeMesh.cu:547-628
-__global__ void kMark3333(const int4* __restrict__ he, uint32_t* __restrict__ fattr, int64_t F,
+__global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restrict__ twin, uint32_t* __restrict__ fattr, int64_t F,
                           const int32_t* __restrict__ valence, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const uint32_t a0 = fattr[f];
     if ((a0 & QA_NM) || (a0 & QA_BOUNDARY)) return;
-    int qh[4];
-    for (int i = 0; i < 4; ++i) {
-        qh[i] = 4 * (int)f + i;
-        if (valence[heO(he, qh[i])] != 3) return;
-    }
+    const int4 qv = quads[f];
+    const int org[4] = {qv.x, qv.y, qv.z, qv.w};
+    for (int i = 0; i < 4; ++i)
+        if (valence[org[i]] != 3) return;
     int nbf[4], nbv[4], nbval[4];
     for (int i = 0; i < 4; ++i) {
-        const int tw = heT(he, qh[i]);
+        const int tw = twin[4 * f + i];
         if (tw < 0) return;  // no neighbour across a boundary edge
         nbf[i] = tw >> 2;
         const uint32_t na = fattr[nbf[i]];
         if ((na & QA_NM) || (na & QA_BOUNDARY)) return;
-        nbv[i] = heO(he, quadNext(tw, 2));
+        nbv[i] = quadCorner(quads[tw >> 2], (tw + 2) & 3);
         nbval[i] = valence[nbv[i]];
         if (nbval[i] == 3) return;
     }
@@ -757,7 +767,7 @@ __global__ void kMark3333(const int4* __restrict__ he, uint32_t* __restrict__ fa
     atomicOr(&fattr[f], (uint32_t)QA_PATTERN);
     for (int i = 0; i < 4; ++i) {
         atomicOr(&fattr[nbf[i]], (uint32_t)QA_REMOVE);
-        const int v = heO(he, qh[i]);
+        const int v = org[i];
         vtype[v] = 1;  // VertexType::Pattern3333
         vtarget[v] = nbv[i];
     }
@@ -1906,7 +1916,7 @@ int32_t* Pipeline::bucketPassValence(const int64_t* off, int64_t nV) {
 void Pipeline::halfedgeInitialize() {
     H = 4 * Q;
     F = Q;
-    int4* he = arena.get<int4>("he", (size_t)H);
+    int32_t* twin = arena.get<int32_t>("twin", (size_t)std::max<int64_t>(H, 1));
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)F);
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)V);
@@ -1929,7 +1939,8 @@ void Pipeline::halfedgeInitialize() {
     if (H > 0) kFillOut<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, rank, out);
     zero(vflags, (size_t)V * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off, V);
-    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, he, bsize, brank, vflags, fattr, qattr, valence, counters());
+    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, twin, bsize, brank, vflags, fattr, qattr, valence, counters());
+    heMaterialized = false;
     listsV = V;
     splitTouched = false;
     canonTracked = false;
@@ -2029,7 +2040,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     if (nV > 0)
         kOutFill<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, qnew, srcHead, srcNext, off2, out2, nH, counters());
 
-    int4* he2 = arena.get<int4>("he2", 4 * capQ);
+    int32_t* twin2 = arena.get<int32_t>("twin2", 4 * capQ);
     int32_t* bsize2 = arena.get<int32_t>("bsize2", 4 * capQ);
     int32_t* brank2 = arena.get<int32_t>("brank2", 4 * capQ);
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)std::max<int64_t>(nQ, 1));
@@ -2039,15 +2050,15 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     zero(vflags, (size_t)nV * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off2, nV);
     if (nH > 0)
-        kRemapRebuild<<<gridFor(nH), kBlock, 0, stream>>>(nquads, nH, touched, oldQ, arena.ptr<int4>("he"), H, arena.ptr<int32_t>("bsize"),
-                                                         arena.ptr<int32_t>("brank"), qnew, off2, out2, he2, bsize2, brank2, vflags, fattr, nattr,
+        kRemapRebuild<<<gridFor(nH), kBlock, 0, stream>>>(nquads, nH, touched, oldQ, arena.ptr<int32_t>("twin"), H, arena.ptr<int32_t>("bsize"),
+                                                         arena.ptr<int32_t>("brank"), qnew, off2, out2, twin2, bsize2, brank2, vflags, fattr, nattr,
                                                          valence, counters());
     arena.swap("pos", "pos2");
     arena.swap("nrm", "nrm2");
     arena.swap("quads", "quads2");
     arena.swap("qattr", "qattr2");
     if (npid) arena.swap("pid", "pid2");
-    arena.swap("he", "he2");
+    arena.swap("twin", "twin2");
     arena.swap("bsize", "bsize2");
     arena.swap("brank", "brank2");
     arena.swap("out_off", "out_off2");
@@ -2058,6 +2069,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     F = nQ;
     vertHeV = V;
     vertHeQ = Q;
+    heMaterialized = false;
     listsV = V;
     splitTouched = false;
     canonTracked = false;
@@ -2074,7 +2086,7 @@ void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
     fillNeg(vtarget, (size_t)V * sizeof(int32_t));
     if (!valenceReady) {  // not computed by the last bucket pass
         zero(*valence, (size_t)V * sizeof(int32_t));
-        if (H > 0) kValence<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, *valence);
+        if (H > 0) kValence<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<int32_t>("twin"), H, *valence);
     }
     valenceReady = false;
 }
@@ -2088,7 +2100,7 @@ void Pipeline::simplify3X3Y(bool colour) {
     if (Q > 0) {
         if (colour) kMark3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence);
         else kCount3x3yOld<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence, count);
-        kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int4>("he"), valence, count,
+        kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int32_t>("twin"), valence, count,
                                                       arena.ptr<float>("pos"), arena.ptr<float>("nrm"), arena.ptr<int32_t>("vtype"),
                                                       arena.ptr<int32_t>("vtarget"), colour ? 1 : 0);
     }
@@ -2101,13 +2113,22 @@ void Pipeline::simplify3333() {
     simplifyInit(&valence, &count);
     if (F > 0) {
         kClear3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F);
-        kMark3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<int4>("he"), arena.ptr<uint32_t>("fattr"), F, valence,
+        kMark3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<int32_t>("twin"), arena.ptr<uint32_t>("fattr"), F, valence,
                                                      arena.ptr<int32_t>("vtype"), arena.ptr<int32_t>("vtarget"));
     }
     simplifyCompactAndRebuild(1, true);
 }
 
+// the full halfedge records, once the simplifications (which read ids and twins only) are done
+void Pipeline::materializeHalfedges() {
+    if (heMaterialized) return;
+    int4* he = arena.get<int4>("he", (size_t)std::max<int64_t>(H, 1));
+    if (H > 0) kMaterialize<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), H, arena.ptr<int32_t>("twin"), he);
+    heMaterialized = true;
+}
+
 void Pipeline::fixManifoldTwins() {
+    materializeHalfedges();
     if (H == 0) return;
     kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("quads"), arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
