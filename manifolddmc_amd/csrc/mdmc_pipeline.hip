@@ -860,13 +860,10 @@ __device__ __forceinline__ f3 nudge(int it) {
 __device__ __forceinline__ f3 ld3(const float* p, int i) { return mk3(p[3 * (int64_t)i], p[3 * (int64_t)i + 1], p[3 * (int64_t)i + 2]); }
 
 // reconnectHalfedges, ManifoldCreator.cu:257-312
-__global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restrict__ quads, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                          const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
-                          const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid, int64_t* ctr) {
-    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H) return;
-    const int h = (int)hh;
-    if (bsize[h] < 4 || brank[h] != 0) return;
+// the bucket of h, a bucket of four or more whose first member (rank 0) h is
+__device__ void fixBucket(int h, int4* __restrict__ he, int64_t H, const int4* __restrict__ quads, const int64_t* __restrict__ off,
+                          const int32_t* __restrict__ out, const float* __restrict__ pos, const ProblemDev* __restrict__ probs,
+                          const int32_t* __restrict__ pid, int64_t* ctr) {
     int mem[kMaxBucket];
     const int n = gatherBucketOut(h, quads, H, off, out, mem);
     if (n > kMaxBucket) anomaly(ctr);
@@ -966,6 +963,42 @@ __global__ void kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restri
     } else {
         setT(he, sorted[0], sorted[3]); setT(he, sorted[3], sorted[0]);
         setT(he, sorted[1], sorted[2]); setT(he, sorted[2], sorted[1]);
+    }
+}
+
+// Buckets of four or more are rare (one halfedge in several hundred), and repairing one is
+// a long dependent chain.  Each block of a fixed grid takes a contiguous range of halfedges,
+// gathers the range's rank-0 members of such buckets in LDS (up to 1024 at a time) and
+// repairs them on consecutive lanes, so that the chains run on full waves.
+constexpr int kFixList = 1024;
+__global__ void __launch_bounds__(kBlock) kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restrict__ quads, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ out, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
+                                                    const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid,
+                                                    int64_t* ctr) {
+    __shared__ int s_list[kFixList];
+    __shared__ int s_n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t per = (H + gridDim.x - 1) / gridDim.x;
+    const int64_t h0 = (int64_t)blockIdx.x * per, h1 = std::min<int64_t>(H, h0 + per);
+    int64_t cur = h0;
+    while (cur < h1) {
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        // fill while a whole chunk still fits
+        while (cur < h1 && s_n <= kFixList - kBlock) {
+            const int64_t h = cur + tid;
+            const bool hit = h < h1 && bsize[h] >= 4 && brank[h] == 0;
+            const uint64_t m = __ballot(hit);
+            int base = 0;
+            if (m && lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(&s_n, __popcll(m));
+            base = __shfl(base, __ffsll((unsigned long long)(m ? m : 1ULL)) - 1, 64);
+            if (hit) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)h;
+            cur += kBlock;
+            __syncthreads();
+        }
+        const int n = s_n;
+        for (int k = tid; k < n; k += kBlock) fixBucket(s_list[k], he, H, quads, off, out, pos, probs, pid, ctr);
+        __syncthreads();
     }
 }
 
@@ -2130,7 +2163,7 @@ void Pipeline::materializeHalfedges() {
 void Pipeline::fixManifoldTwins() {
     materializeHalfedges();
     if (H == 0) return;
-    kFixTwins<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("quads"), arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"),
+    kFixTwins<<<(unsigned)std::min<int64_t>(gridFor(H), 4096), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("quads"), arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
                                                  counters());
 }
