@@ -132,13 +132,15 @@ __device__ __forceinline__ uint32_t contourMap(uint64_t w) {
     return map;
 }
 
-// the last index l in [0, n) with pre[l] <= key (pre non-decreasing, pre[0] = 0 <= key)
+// the last index l in [0, n) with pre[l] <= key (pre non-decreasing, pre[0] = 0 <= key, n <=
+// TILE): a fixed number of halving steps with selects, so that the lanes of a wave stay in step
+template <int TILE>
 __device__ __forceinline__ int upperIndex(const uint16_t* pre, int n, int key) {
-    int lo = 0, hi = n;  // invariant: pre[lo] <= key, answer in [lo, hi)
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if ((int)pre[mid] <= key) lo = mid;
-        else hi = mid;
+    int lo = 0;
+#pragma unroll
+    for (int step = TILE / 2; step > 0; step >>= 1) {
+        const int m = lo + step;
+        lo = (m < n && (int)pre[m] <= key) ? m : lo;
     }
     return lo;
 }
@@ -409,6 +411,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     __shared__ uint16_t s_qpre[TILE];
     __shared__ uint8_t s_own[TILE];
     __shared__ uint16_t s_aidx[TILE];  // per tile cell: its active index (0xFFFF: inactive, adjoint map only)
+    __shared__ uint32_t s_cmap[TILE];  // per active cell: contour of each crossed edge, 2 bits per edge
     __shared__ int s_wave[GP_WAVES];
 
     const int tid = threadIdx.x;
@@ -426,7 +429,9 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
         __syncthreads();
     }
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
-        s_poly[ai] = a.recPoly[rb + ai];
+        const uint64_t w = a.recPoly[rb + ai];
+        s_poly[ai] = w;
+        s_cmap[ai] = contourMap(w);
         const int info = a.recInfo[rb + ai];
         const int l = info & 0xFFF;
         s_cell[ai] = (uint16_t)l;
@@ -452,7 +457,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
 
     // per vertex: its cell and contour; kVertices computes position and normal
     for (int i = tid; i < VT; i += GP_THREADS) {
-        const int ai = upperIndex(s_vpre, NA, i);
+        const int ai = upperIndex<TILE>(s_vpre, NA, i);
         const int64_t v = voff + i;
         a.vcell[v] = (int)(c0 + s_cell[ai]);
         a.vcontour[v] = (uint8_t)(i - s_vpre[ai]);
@@ -462,7 +467,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     // quads whose edge starts at a cell of this tile.  Their other cells lie 1, CX, CX+1,
     // CX*CY, CX*CY+1 or CX*CY+CX cells back: in this tile (LDS) or an earlier one (map).
     for (int jq = tid; jq < QT; jq += GP_THREADS) {
-        const int ai = upperIndex(s_qpre, NA, jq);
+        const int ai = upperIndex<TILE>(s_qpre, NA, jq);
         int r = jq - s_qpre[ai];
         const int own = s_own[ai];
         int axis = 0;
@@ -486,7 +491,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
         int s0 = -1, s1 = -1, s2 = -1, s3 = -1;
         {
             const int sl = edgeSlot(ownE, edgy);
-            const int vid = (int)(voff + s_vpre[ai] + polyContourOf(s_poly[ai], ownE));
+            const int vid = (int)(voff + s_vpre[ai] + (int)((s_cmap[ai] >> (2 * ownE)) & 3u));
             s0 = sl == 0 ? vid : s0;
             s1 = sl == 1 ? vid : s1;
             s2 = sl == 2 ? vid : s2;
@@ -500,7 +505,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
             int vid;
             if (n >= c0) {
                 const int an = s_aidx[n - c0];
-                vid = (int)(voff + s_vpre[an] + polyContourOf(s_poly[an], e));
+                vid = (int)(voff + s_vpre[an] + (int)((s_cmap[an] >> (2 * e)) & 3u));
             } else {
                 const uint32_t m = a.cellMap[P.cellBase + n];
                 vid = (int)(a.tileVOff[P.tileBase + n / TILE] + (int64_t)(m & 0x3FFFu) + mapContour(m, e));
