@@ -220,6 +220,9 @@ __device__ __forceinline__ int quadCorner(const int4& q, int j) { return j == 0 
 struct BucketOf {
     int n, rank, twin;
 };
+// size and rank of a halfedge's bucket in one word while the simplifications run (both are
+// at most the bucket size; a bucket of 65536 or more halfedges saturates)
+__device__ __forceinline__ uint32_t packBucket(int n, int rank) { return (uint32_t)min(n, 0xFFFF) | ((uint32_t)min(rank, 0xFFFF) << 16); }
 
 // bucket size, rank of h and the twin connectHalfedgeTwins (HalfedgeMesh.cu:173-291) gives h:
 // the partner of a pair, the next by id of three, the first other-direction member paired
@@ -322,13 +325,12 @@ __device__ int gatherBucketOut(int h, const int4* __restrict__ quads, int64_t H,
 // simplifications are done a halfedge is its id (origin = corner i of quad f, face f, next
 // 4f + (i + 1) % 4) and its twin; the full records are written once after them
 // (kMaterialize).
-__device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketOf& bk, int32_t* __restrict__ twin, int32_t* __restrict__ bsize,
-                                              int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
+__device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketOf& bk, int32_t* __restrict__ twin, uint32_t* __restrict__ bpack,
+                                              uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
                                               uint32_t* __restrict__ qattr, int32_t* __restrict__ valence) {
     const int f = h >> 2;
     twin[h] = bk.twin;
-    bsize[h] = bk.n;
-    brank[h] = bk.rank;
+    bpack[h] = packBucket(bk.n, bk.rank);
     if (valence && bk.twin == -1) atomicAdd(&valence[b], 1);
     const uint32_t vf = bucketVertexFlag(bk.n);
     if (vf) atomicOr(&vflags[a], vf);
@@ -379,22 +381,26 @@ __global__ void kFaceInit(const uint32_t* __restrict__ qattr, int64_t Q, int32_t
 
 // every halfedge from its bucket
 __global__ void kBucketsAll(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                            int32_t* __restrict__ twin, int32_t* __restrict__ bsize, int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
+                            int32_t* __restrict__ twin, uint32_t* __restrict__ bpack, uint32_t* __restrict__ vflags,
                             uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
     const int h = (int)hh;
     const int4 q = quads[h >> 2];
     const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-    storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads, H, off, out, ctr), twin, bsize, brank, vflags, fattr, qattr, valence);
+    storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads, H, off, out, ctr), twin, bpack, vflags, fattr, qattr, valence);
 }
 
-// the halfedge records of the quads, from the ids and the twins
-__global__ void kMaterialize(const int4* __restrict__ quads, int64_t H, const int32_t* __restrict__ twin, int4* __restrict__ he) {
+// the halfedge records of the quads, from the ids and the twins, and the bucket sizes / ranks
+__global__ void kMaterialize(const int4* __restrict__ quads, int64_t H, const int32_t* __restrict__ twin, const uint32_t* __restrict__ bpack,
+                             int4* __restrict__ he, int32_t* __restrict__ bsize, int32_t* __restrict__ brank) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
     const int h = (int)hh;
     he[h] = make_int4(quadCorner(quads[h >> 2], h & 3), h >> 2, quadNext(h, 1), twin[h]);
+    const uint32_t b = bpack[h];
+    bsize[h] = (int32_t)(b & 0xFFFFu);
+    brank[h] = (int32_t)(b >> 16);
 }
 
 // ---- rebuild after a simplification's compaction.  New vertex v' = address[v]; a dropped
@@ -464,10 +470,10 @@ __global__ void kOutFill(const int32_t* __restrict__ oldV, int64_t Vn, const int
 // buckets by the block's first lanes, so that the long path runs on full waves
 __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__ quads2, int64_t H2, const uint8_t* __restrict__ touched,
                                                         const int32_t* __restrict__ oldQ, const int32_t* __restrict__ twinOld, int64_t Hold,
-                                                        const int32_t* __restrict__ bsizeOld, const int32_t* __restrict__ brankOld,
+                                                        const uint32_t* __restrict__ bpackOld,
                                                         const int32_t* __restrict__ qnew, const int64_t* __restrict__ off2,
-                                                        const int32_t* __restrict__ out2, int32_t* __restrict__ twin, int32_t* __restrict__ bsize,
-                                                        int32_t* __restrict__ brank, uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
+                                                        const int32_t* __restrict__ out2, int32_t* __restrict__ twin, uint32_t* __restrict__ bpack,
+                                                        uint32_t* __restrict__ vflags, uint32_t* __restrict__ fattr,
                                                         uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
     __shared__ int s_list[kBlock];
     __shared__ int s_wave[kBlock / 64];
@@ -490,7 +496,8 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
                 if (fo < 0) anomaly(ctr);
                 else t = 4 * fo + (tt & 3);
             }
-            storeHalfedge(h, a, b, BucketOf{bsizeOld[ho], brankOld[ho], t}, twin, bsize, brank, vflags, fattr, qattr, valence);
+            const uint32_t bo = bpackOld[ho];
+            storeHalfedge(h, a, b, BucketOf{(int)(bo & 0xFFFFu), (int)(bo >> 16), t}, twin, bpack, vflags, fattr, qattr, valence);
         }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -508,7 +515,7 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
         const int h = s_list[threadIdx.x];
         const int4 q = quads2[h >> 2];
         const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-        storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads2, H2, off2, out2, ctr), twin, bsize, brank, vflags, fattr, qattr, valence);
+        storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads2, H2, off2, out2, ctr), twin, bpack, vflags, fattr, qattr, valence);
     }
 }
 
@@ -1955,8 +1962,7 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)V + 1);
     int32_t* rank = arena.get<int32_t>("bl_rank", (size_t)H);
-    int32_t* bsize = arena.get<int32_t>("bsize", (size_t)H);
-    int32_t* brank = arena.get<int32_t>("brank", (size_t)H);
+    uint32_t* bpack = arena.get<uint32_t>("bpack", (size_t)std::max<int64_t>(H, 1));
     const int4* quads = arena.ptr<int4>("quads");
     uint32_t* qattr = arena.ptr<uint32_t>("qattr");
     if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, Q, V, counters());
@@ -1972,7 +1978,7 @@ void Pipeline::halfedgeInitialize() {
     if (H > 0) kFillOut<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, rank, out);
     zero(vflags, (size_t)V * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off, V);
-    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, twin, bsize, brank, vflags, fattr, qattr, valence, counters());
+    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, twin, bpack, vflags, fattr, qattr, valence, counters());
     heMaterialized = false;
     listsV = V;
     splitTouched = false;
@@ -2074,8 +2080,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
         kOutFill<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, qnew, srcHead, srcNext, off2, out2, nH, counters());
 
     int32_t* twin2 = arena.get<int32_t>("twin2", 4 * capQ);
-    int32_t* bsize2 = arena.get<int32_t>("bsize2", 4 * capQ);
-    int32_t* brank2 = arena.get<int32_t>("brank2", 4 * capQ);
+    uint32_t* bpack2 = arena.get<uint32_t>("bpack2", 4 * capQ);
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)std::max<int64_t>(nQ, 1));
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)std::max<int64_t>(nQ, 1));
     uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)std::max<int64_t>(nV, 1));
@@ -2083,8 +2088,8 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     zero(vflags, (size_t)nV * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off2, nV);
     if (nH > 0)
-        kRemapRebuild<<<gridFor(nH), kBlock, 0, stream>>>(nquads, nH, touched, oldQ, arena.ptr<int32_t>("twin"), H, arena.ptr<int32_t>("bsize"),
-                                                         arena.ptr<int32_t>("brank"), qnew, off2, out2, twin2, bsize2, brank2, vflags, fattr, nattr,
+        kRemapRebuild<<<gridFor(nH), kBlock, 0, stream>>>(nquads, nH, touched, oldQ, arena.ptr<int32_t>("twin"), H, arena.ptr<uint32_t>("bpack"),
+                                                         qnew, off2, out2, twin2, bpack2, vflags, fattr, nattr,
                                                          valence, counters());
     arena.swap("pos", "pos2");
     arena.swap("nrm", "nrm2");
@@ -2092,8 +2097,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     arena.swap("qattr", "qattr2");
     if (npid) arena.swap("pid", "pid2");
     arena.swap("twin", "twin2");
-    arena.swap("bsize", "bsize2");
-    arena.swap("brank", "brank2");
+    arena.swap("bpack", "bpack2");
     arena.swap("out_off", "out_off2");
     arena.swap("out_list", "out_list2");
     V = nV;
@@ -2156,7 +2160,11 @@ void Pipeline::simplify3333() {
 void Pipeline::materializeHalfedges() {
     if (heMaterialized) return;
     int4* he = arena.get<int4>("he", (size_t)std::max<int64_t>(H, 1));
-    if (H > 0) kMaterialize<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), H, arena.ptr<int32_t>("twin"), he);
+    int32_t* bsize = arena.get<int32_t>("bsize", (size_t)std::max<int64_t>(H, 1));
+    int32_t* brank = arena.get<int32_t>("brank", (size_t)std::max<int64_t>(H, 1));
+    if (H > 0)
+        kMaterialize<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), H, arena.ptr<int32_t>("twin"), arena.ptr<uint32_t>("bpack"), he, bsize,
+                                                        brank);
     heMaterialized = true;
 }
 
