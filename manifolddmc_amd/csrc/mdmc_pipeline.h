@@ -46,6 +46,10 @@ class Pipeline {
     // simplifications, which read only those
     bool heMaterialized = false;
     void materializeHalfedges();
+    // while the simplifications run, vertex v's position and normal stay at slot vslot[v] of
+    // pos / nrm (not moved by each compaction); materializeHalfedges gathers them back
+    bool posSlotted = false;
+    const int32_t* vertexSlots();
 
     void resetCounters();
     int64_t anomalies();

@@ -673,17 +673,19 @@ __global__ void kCount3x3yOld(const int4* __restrict__ quads, uint32_t* __restri
     qattr[q] = a;
 }
 
-// HalfedgeMesh.cu:368-388
-__device__ __forceinline__ void handleMerge(float* pos, float* nrm, int32_t* vtype, int32_t* vtarget, int x, int y, int z, int w) {
-    f3 p = mk3(pos[3 * y], pos[3 * y + 1], pos[3 * y + 2]);
-    const f3 pw = mk3(pos[3 * w], pos[3 * w + 1], pos[3 * w + 2]);
+// HalfedgeMesh.cu:368-388.  Positions and normals stay where the grid pass wrote them while
+// the simplifications run: vertex v's are at slot vslot[v] (v itself when vslot is null).
+__device__ __forceinline__ void handleMerge(float* pos, float* nrm, const int32_t* vslot, int32_t* vtype, int32_t* vtarget, int x, int y, int z, int w) {
+    const int64_t sx = vslot ? vslot[x] : x, sy = vslot ? vslot[y] : y, sw = vslot ? vslot[w] : w;
+    f3 p = mk3(pos[3 * sy], pos[3 * sy + 1], pos[3 * sy + 2]);
+    const f3 pw = mk3(pos[3 * sw], pos[3 * sw + 1], pos[3 * sw + 2]);
     p = p + scl(0.5f, pw - p);
-    f3 n = mk3(nrm[3 * y], nrm[3 * y + 1], nrm[3 * y + 2]);
-    const f3 nw = mk3(nrm[3 * w], nrm[3 * w + 1], nrm[3 * w + 2]);
+    f3 n = mk3(nrm[3 * sy], nrm[3 * sy + 1], nrm[3 * sy + 2]);
+    const f3 nw = mk3(nrm[3 * sw], nrm[3 * sw + 1], nrm[3 * sw + 2]);
     n = n + scl(0.5f, nw - n);
     n = normalize3(n);
-    pos[3 * x] = p.x; pos[3 * x + 1] = p.y; pos[3 * x + 2] = p.z;
-    nrm[3 * x] = n.x; nrm[3 * x + 1] = n.y; nrm[3 * x + 2] = n.z;
+    pos[3 * sx] = p.x; pos[3 * sx + 1] = p.y; pos[3 * sx + 2] = p.z;
+    nrm[3 * sx] = n.x; nrm[3 * sx + 1] = n.y; nrm[3 * sx + 2] = n.z;
     vtype[z] = 3;  // VertexType::PatternRemovable
     vtarget[z] = x;
 }
@@ -709,7 +711,7 @@ __device__ __forceinline__ bool neighborFaces(const int32_t* __restrict__ twin, 
 // mergeVerticesP3X3YColor (colour = 1) and mergeVerticesP3X3Y (colour = 0), HalfedgeMesh.cu:391-498
 __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ twin,
                            const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
-                           float* __restrict__ pos, float* __restrict__ nrm, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour) {
+                           float* __restrict__ pos, float* __restrict__ nrm, const int32_t* __restrict__ vslot, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const uint32_t a = qattr[q];
@@ -729,10 +731,10 @@ __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict_
     const int4 v = quads[q];
     const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
     if (pattern1(val) && (colour || (count[v.x] == 1 && count[v.z] == 1))) {
-        handleMerge(pos, nrm, vtype, vtarget, v.x, v.y, v.z, v.w);
+        handleMerge(pos, nrm, vslot, vtype, vtarget, v.x, v.y, v.z, v.w);
         atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
     } else if (pattern2(val) && (colour || (count[v.y] == 1 && count[v.w] == 1))) {
-        handleMerge(pos, nrm, vtype, vtarget, v.y, v.x, v.w, v.z);
+        handleMerge(pos, nrm, vslot, vtype, vtarget, v.y, v.x, v.w, v.z);
         atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
     }
 }
@@ -786,20 +788,30 @@ __global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int 
     keep[v] = vtype[v] == dropType ? 0 : 1;
 }
 
-__global__ void kCompactVertices(const float* __restrict__ pos, const float* __restrict__ nrm, const int32_t* __restrict__ pid,
-                                 const int32_t* __restrict__ keep, const int64_t* __restrict__ addr, int64_t V, float* __restrict__ npos,
-                                 float* __restrict__ nnrm, int32_t* __restrict__ npid, int32_t* __restrict__ address, int32_t* __restrict__ oldV) {
+// (positions and normals are not moved: the new vertex o keeps the slot of its old vertex)
+__global__ void kCompactVertices(const int32_t* __restrict__ pid, const int32_t* __restrict__ keep, const int64_t* __restrict__ addr, int64_t V,
+                                 int32_t* __restrict__ npid, int32_t* __restrict__ address, int32_t* __restrict__ oldV, const int32_t* __restrict__ vslot,
+                                 int32_t* __restrict__ nvslot) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     if (!keep[v]) return;
     const int64_t o = addr[v];
     address[v] = (int32_t)o;
     oldV[o] = (int32_t)v;
-    for (int k = 0; k < 3; ++k) {
-        npos[3 * o + k] = pos[3 * v + k];
-        nnrm[3 * o + k] = nrm[3 * v + k];
-    }
+    nvslot[o] = vslot ? vslot[v] : (int32_t)v;
     if (pid) npid[o] = pid[v];
+}
+
+// positions and normals of the current vertices, gathered from their slots
+__global__ void kGatherVertices(const float* __restrict__ pos, const float* __restrict__ nrm, const int32_t* __restrict__ vslot, int64_t V,
+                                float* __restrict__ npos, float* __restrict__ nnrm) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int64_t sl = vslot[v];
+    for (int k = 0; k < 3; ++k) {
+        npos[3 * v + k] = pos[3 * sl + k];
+        nnrm[3 * v + k] = nrm[3 * sl + k];
+    }
 }
 
 __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t* __restrict__ keep) {
@@ -1769,6 +1781,8 @@ int64_t* Pipeline::counters() { return arena.get<int64_t>("counters", A_NCOUNTER
 
 const uint8_t* Pipeline::canonFlags() { return canonTracked ? arena.ptr<uint8_t>("he_canon") : nullptr; }
 
+const int32_t* Pipeline::vertexSlots() { return posSlotted ? arena.ptr<int32_t>("vslot") : nullptr; }
+
 int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out) {
     *out = arena.get<int64_t>(outName, (size_t)n + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(n + 1));
@@ -1980,6 +1994,7 @@ void Pipeline::halfedgeInitialize() {
     int32_t* valence = bucketPassValence(off, V);
     if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, twin, bpack, vflags, fattr, qattr, valence, counters());
     heMaterialized = false;
+    posSlotted = false;  // the grid pass wrote positions by vertex
     listsV = V;
     splitTouched = false;
     canonTracked = false;
@@ -2039,13 +2054,11 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     // both are sized for the grid pass's output so that later extractions of the same
     // workload reallocate neither
     const size_t capV = (size_t)std::max(nV, dmcV), capQ = (size_t)std::max(nQ, dmcQ);
-    float* npos = arena.get<float>("pos2", 3 * capV);
-    float* nnrm = arena.get<float>("nrm2", 3 * capV);
     int32_t* npid = pid() ? arena.get<int32_t>("pid2", capV) : nullptr;
     int32_t* oldV = arena.get<int32_t>("oldV", (size_t)std::max<int64_t>(nV, 1));
+    int32_t* nvslot = arena.get<int32_t>("vslot2", capV);
     if (V > 0)
-        kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), keep, vaddr, V, npos, nnrm, npid,
-                                                            address, oldV);
+        kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(pid(), keep, vaddr, V, npid, address, oldV, vertexSlots(), nvslot);
     int4* quads = arena.ptr<int4>("quads");
     int4* nquads = arena.get<int4>("quads2", capQ);
     uint32_t* nattr = arena.get<uint32_t>("qattr2", capQ);
@@ -2091,8 +2104,8 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
         kRemapRebuild<<<gridFor(nH), kBlock, 0, stream>>>(nquads, nH, touched, oldQ, arena.ptr<int32_t>("twin"), H, arena.ptr<uint32_t>("bpack"),
                                                          qnew, off2, out2, twin2, bpack2, vflags, fattr, nattr,
                                                          valence, counters());
-    arena.swap("pos", "pos2");
-    arena.swap("nrm", "nrm2");
+    arena.swap("vslot", "vslot2");
+    posSlotted = true;
     arena.swap("quads", "quads2");
     arena.swap("qattr", "qattr2");
     if (npid) arena.swap("pid", "pid2");
@@ -2138,7 +2151,7 @@ void Pipeline::simplify3X3Y(bool colour) {
         if (colour) kMark3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence);
         else kCount3x3yOld<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence, count);
         kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int32_t>("twin"), valence, count,
-                                                      arena.ptr<float>("pos"), arena.ptr<float>("nrm"), arena.ptr<int32_t>("vtype"),
+                                                      arena.ptr<float>("pos"), arena.ptr<float>("nrm"), vertexSlots(), arena.ptr<int32_t>("vtype"),
                                                       arena.ptr<int32_t>("vtarget"), colour ? 1 : 0);
     }
     simplifyCompactAndRebuild(3, false);
@@ -2158,6 +2171,16 @@ void Pipeline::simplify3333() {
 
 // the full halfedge records, once the simplifications (which read ids and twins only) are done
 void Pipeline::materializeHalfedges() {
+    if (posSlotted) {
+        // positions and normals by vertex again, from the slots the simplifications kept them in
+        const size_t cap = (size_t)std::max(V, dmcV);
+        float* npos = arena.get<float>("pos2", 3 * cap);
+        float* nnrm = arena.get<float>("nrm2", 3 * cap);
+        if (V > 0) kGatherVertices<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<float>("pos"), arena.ptr<float>("nrm"), vertexSlots(), V, npos, nnrm);
+        arena.swap("pos", "pos2");
+        arena.swap("nrm", "nrm2");
+        posSlotted = false;
+    }
     if (heMaterialized) return;
     int4* he = arena.get<int4>("he", (size_t)std::max<int64_t>(H, 1));
     int32_t* bsize = arena.get<int32_t>("bsize", (size_t)std::max<int64_t>(H, 1));
