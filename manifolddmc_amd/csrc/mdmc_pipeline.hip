@@ -36,7 +36,7 @@ constexpr int kMaxBucket = 16;
 constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
-enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_NCOUNTERS = 8 };
+enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_NCOUNTERS = 8 };
 
 __device__ __forceinline__ void anomaly(int64_t* ctr) { atomicAdd((unsigned long long*)&ctr[A_ANOMALY], 1ULL); }
 
@@ -122,47 +122,87 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
 // in range, itself on its face's cycle); it is dirty -- its bucket can have changed --
 // when one of its endpoints was touched or it was appended.  A dirty halfedge gets its key
 // key[h] = 4 * face + position in the face's cycle and rank[h] = its rank in m's list.
-__global__ void kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t oldH, const int32_t* __restrict__ face_he, int64_t F, int64_t V,
-                              const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ degM, int32_t* __restrict__ rank, int32_t* __restrict__ key,
-                              uint8_t* __restrict__ dirty, int64_t* ctr) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= H) return;
-    const int h = (int)t;
-    const int4 e = he[h];
-    uint8_t d = 0;
-    bool bad = e.x < 0 || e.x >= V || e.y < 0 || e.y >= F || e.z < 0 || e.z >= H || e.w < -1 || e.w >= H;
-    int b = 0;
-    if (!bad) {
-        b = heO(he, e.z);
-        bad = b < 0 || b >= V;
+// One block checks kDirtyPer * kBlock consecutive halfedges, gathers its dirty ones in LDS
+// (their face walks then run on consecutive lanes) and appends them to the dirty list
+// with one atomic on ctr[A_DIRTY]; the list order is free, since ranks only pick list slots
+// and bucket ranks come from the keys.
+constexpr int kDirtyPer = 16;
+constexpr int kDirtyBatch = 4;
+__global__ void __launch_bounds__(kBlock) kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t oldH, const int32_t* __restrict__ face_he,
+                                                        int64_t F, int64_t V, const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ degM,
+                                                        int32_t* __restrict__ rank, int32_t* __restrict__ key, int32_t* __restrict__ dlist, int64_t* ctr) {
+    __shared__ int s_list[kBlock * kDirtyPer];
+    __shared__ int s_n;
+    __shared__ long long s_base;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * (kBlock * kDirtyPer) + threadIdx.x;
+    int nbad = 0;
+    for (int k0 = 0; k0 < kDirtyPer; k0 += kDirtyBatch) {
+        int4 e[kDirtyBatch];
+        int b[kDirtyBatch];
+        bool bad[kDirtyBatch], d[kDirtyBatch];
+#pragma unroll
+        for (int j = 0; j < kDirtyBatch; ++j) {
+            const int64_t t = b0 + (int64_t)(k0 + j) * kBlock;
+            e[j] = t < H ? he[t] : make_int4(0, 0, 0, -1);
+        }
+#pragma unroll
+        for (int j = 0; j < kDirtyBatch; ++j) {
+            const int64_t t = b0 + (int64_t)(k0 + j) * kBlock;
+            bad[j] = t < H && (e[j].x < 0 || e[j].x >= V || e[j].y < 0 || e[j].y >= F || e[j].z < 0 || e[j].z >= H || e[j].w < -1 || e[j].w >= H);
+            b[j] = (t < H && !bad[j]) ? heO(he, e[j].z) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kDirtyBatch; ++j) {
+            const int64_t t = b0 + (int64_t)(k0 + j) * kBlock;
+            if (t < H && !bad[j]) bad[j] = b[j] < 0 || b[j] >= V;
+            d[j] = t < H && !bad[j] && (t >= oldH || dirtyV[e[j].x] || dirtyV[b[j]]);
+            nbad += bad[j] ? 1 : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kDirtyBatch; ++j) {
+            const uint64_t m = __ballot(d[j]);
+            int at = 0;
+            if (lane == 0 && m) at = atomicAdd(&s_n, __popcll(m));
+            at = __shfl(at, 0, 64);
+            if (d[j]) s_list[at + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)(b0 + (int64_t)(k0 + j) * kBlock);
+        }
     }
-    if (!bad && (t >= oldH || dirtyV[e.x] || dirtyV[b])) {
+    __syncthreads();
+    const int n = s_n;
+    if (threadIdx.x == 0 && n > 0) s_base = (long long)atomicAdd((unsigned long long*)&ctr[A_DIRTY], (unsigned long long)n);
+    __syncthreads();
+    const int64_t base = s_base;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        const int h = s_list[i];
+        const int4 e = he[h];
         int k = face_he[e.y], pos = 0;
         while (pos < 4 && k != h && k >= 0 && k < H) {
             k = heN(he, k);
             ++pos;
         }
         if (k != h) {
-            bad = true;
+            ++nbad;  // the refresh stops at the check that follows
         } else {
-            rank[h] = atomicAdd(&degM[min(e.x, b)], 1);
+            rank[h] = atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
             key[h] = 4 * e.y + pos;
-            d = 1;
         }
+        dlist[base + i] = h;
     }
-    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-    dirty[h] = d;
+    if (nbad) atomicAdd((unsigned long long*)&ctr[A_INVALID], (unsigned long long)nbad);
 }
 
-// second list pass, after the scan of the counts: the entry (own entry, key) goes to
-// offM[m] + rank[h]; the key is h itself when key is null
-__global__ void kPlaceEntries(const int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM,
+// second list pass, after the scan of the counts: the entry (own entry, key) of each listed
+// halfedge goes to offM[m] + rank[h]
+__global__ void kPlaceEntries(const int4* __restrict__ he, const int32_t* __restrict__ dlist, int64_t n, const int64_t* __restrict__ offM,
                               const int32_t* __restrict__ rank, const int32_t* __restrict__ key, int2* __restrict__ A) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= H || (only && !only[t])) return;
-    const int h = (int)t;
+    if (t >= n) return;
+    const int h = dlist[t];
     const ListRef l = listOf(he, he[h], offM);
-    A[l.lo + rank[h]] = make_int2(l.entry, key ? key[h] : h);
+    A[l.lo + rank[h]] = make_int2(l.entry, key[h]);
 }
 
 // the four slots of a bucket: entries beyond the fourth land in slot 0 (glm::ivec4::operator[] default)
@@ -173,13 +213,13 @@ __device__ __forceinline__ void bucketSlots(const int mem[kMaxBucket], int n, in
 
 __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF_NM : (n == 3 ? VF_WEIRD : (n == 1 ? VF_BOUNDARY : 0u)); }
 
-// One thread per flagged halfedge h of a refresh, reading only the short list of its smaller
+// One thread per listed halfedge h of a refresh, reading only the short list of its smaller
 // endpoint: the size of its bucket and its rank (position in key order).
-__global__ void kBuckets(const int4* __restrict__ he, int64_t H, const uint8_t* __restrict__ only, const int64_t* __restrict__ offM,
+__global__ void kBuckets(const int4* __restrict__ he, const int32_t* __restrict__ dlist, int64_t nList, const int64_t* __restrict__ offM,
                          const int32_t* __restrict__ keys, const int2* __restrict__ A, int32_t* __restrict__ bsize, int32_t* __restrict__ brank) {
-    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H || !only[hh]) return;
-    const int h = (int)hh;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nList) return;
+    const int h = dlist[t];
     const ListRef l = listOf(he, he[h], offM);
     const int o = entryOther(l.entry), key = keys[h];
     int n = 0, rk = 0;
@@ -243,18 +283,29 @@ __device__ BucketOf bucketFromOut(int h, int a, int b, const int4* __restrict__ 
         if (id != h) partner = id;
         else self = true;
     };
-    const int64_t k1 = off[m + 1];
-    for (int64_t k = off[m]; k < k1; ++k) {
-        const int x = out[k];
-        if (x < 0 || x >= H) {
-            anomaly(ctr);
-            continue;
+    // the list in groups of four entries, their quads loaded side by side
+    const int64_t k0 = off[m], k1 = off[m + 1];
+    for (int64_t kb = k0; kb < k1; kb += 4) {
+        int xs[4];
+        int4 qs[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xs[i] = kb + i < k1 ? out[kb + i] : 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) qs[i] = (kb + i < k1 && xs[i] >= 0 && xs[i] < H) ? quads[xs[i] >> 2] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (kb + i >= k1) continue;
+            const int x = xs[i];
+            if (x < 0 || x >= H) {
+                anomaly(ctr);
+                continue;
+            }
+            const int4 q = qs[i];
+            const int j = x & 3;
+            if (quadCorner(q, j) != m) anomaly(ctr);
+            if (quadCorner(q, (j + 1) & 3) == o) add(x, 0);
+            if (o != m && quadCorner(q, (j + 3) & 3) == o) add((x & ~3) | ((j + 3) & 3), 1);
         }
-        const int4 q = quads[x >> 2];
-        const int j = x & 3;
-        if (quadCorner(q, j) != m) anomaly(ctr);
-        if (quadCorner(q, (j + 1) & 3) == o) add(x, 0);
-        if (o != m && quadCorner(q, (j + 3) & 3) == o) add((x & ~3) | ((j + 3) & 3), 1);
     }
     if (!self) anomaly(ctr);
     int t = -1;
@@ -452,12 +503,20 @@ __global__ void kOutFill(const int32_t* __restrict__ oldV, int64_t Vn, const int
     int64_t w = off2[vn];
     int u = oldV[vn], s = srcHead[vn];
     for (;;) {
-        for (int64_t k = off[u]; k < off[u + 1]; ++k) {
-            const int x = out[k];
-            const int fn = qnew[x >> 2];
-            if (fn < 0) continue;
-            if (w < cap) out2[w++] = 4 * fn + (x & 3);
-            else anomaly(ctr);
+        // groups of four list entries, their new quad ids loaded side by side
+        const int64_t k1 = off[u + 1];
+        for (int64_t kb = off[u]; kb < k1; kb += 4) {
+            int xs[4], fns[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xs[i] = kb + i < k1 ? out[kb + i] : 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fns[i] = kb + i < k1 ? qnew[xs[i] >> 2] : -1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (fns[i] < 0) continue;
+                if (w < cap) out2[w++] = 4 * fns[i] + (xs[i] & 3);
+                else anomaly(ctr);
+            }
         }
         if (s < 0) break;
         u = s;
@@ -608,11 +667,13 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* 
     if (d > 0) {
         const int start = rep[v];
         int cur = start;
+        int4 e = he[start];  // the record of cur: one load gives the next step's origin check and twin
         for (int k = 1; k <= d; ++k) {
-            const int t = heT(he, cur);
+            const int t = e.w;
             if (t < 0) break;
             cur = nextOf(he, canon, t);
-            if (heO(he, cur) != v) break;
+            e = he[cur];
+            if (e.x != v) break;
             if (cur == start) {
                 single = k == d;
                 break;
@@ -717,19 +778,20 @@ __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict_
     const uint32_t a = qattr[q];
     if (!colour && ((a & QA_NM) || (a & QA_BOUNDARY))) return;
     if (!(a & QA_PATTERN)) return;
+    const int4 v = quads[q];
     int nb[4];
     if (!neighborFaces(twin, (int)q, nb)) return;
+    // the neighbours' attributes and the corner valences loaded side by side before the tests
+    // (other faces only add QA_REMOVE meanwhile, which no test reads)
+    const uint32_t na[4] = {qattr[nb[0]], qattr[nb[1]], qattr[nb[2]], qattr[nb[3]]};
+    const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
     for (int i = 0; i < 4; ++i)
-        if ((qattr[nb[i]] & QA_NM) || (a & QA_BOUNDARY)) return;
+        if ((na[i] & QA_NM) || (a & QA_BOUNDARY)) return;
     if (colour) {
         const uint32_t c = a & QA_COLOR;
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t na = qattr[nb[i]];
-            if ((na & QA_PATTERN) && ((na & QA_COLOR) <= c)) return;
-        }
+        for (int i = 0; i < 4; ++i)
+            if ((na[i] & QA_PATTERN) && ((na[i] & QA_COLOR) <= c)) return;
     }
-    const int4 v = quads[q];
-    const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
     if (pattern1(val) && (colour || (count[v.x] == 1 && count[v.z] == 1))) {
         handleMerge(pos, nrm, vslot, vtype, vtarget, v.x, v.y, v.z, v.w);
         atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
@@ -757,19 +819,29 @@ __global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restr
     if ((a0 & QA_NM) || (a0 & QA_BOUNDARY)) return;
     const int4 qv = quads[f];
     const int org[4] = {qv.x, qv.y, qv.z, qv.w};
+    // the four valences (and below the four neighbours) loaded side by side before any test
+    const int val[4] = {valence[org[0]], valence[org[1]], valence[org[2]], valence[org[3]]};
     for (int i = 0; i < 4; ++i)
-        if (valence[org[i]] != 3) return;
+        if (val[i] != 3) return;
+    const int4 tw4 = reinterpret_cast<const int4*>(twin)[f];
+    const int tw[4] = {tw4.x, tw4.y, tw4.z, tw4.w};
+    for (int i = 0; i < 4; ++i)
+        if (tw[i] < 0) return;  // no neighbour across a boundary edge
     int nbf[4], nbv[4], nbval[4];
+    uint32_t na[4];
+    int4 nq[4];
     for (int i = 0; i < 4; ++i) {
-        const int tw = twin[4 * f + i];
-        if (tw < 0) return;  // no neighbour across a boundary edge
-        nbf[i] = tw >> 2;
-        const uint32_t na = fattr[nbf[i]];
-        if ((na & QA_NM) || (na & QA_BOUNDARY)) return;
-        nbv[i] = quadCorner(quads[tw >> 2], (tw + 2) & 3);
-        nbval[i] = valence[nbv[i]];
-        if (nbval[i] == 3) return;
+        nbf[i] = tw[i] >> 2;
+        na[i] = fattr[nbf[i]];
+        nq[i] = quads[nbf[i]];
     }
+    for (int i = 0; i < 4; ++i) {
+        if ((na[i] & QA_NM) || (na[i] & QA_BOUNDARY)) return;
+        nbv[i] = quadCorner(nq[i], (tw[i] + 2) & 3);
+    }
+    for (int i = 0; i < 4; ++i) nbval[i] = valence[nbv[i]];
+    for (int i = 0; i < 4; ++i)
+        if (nbval[i] == 3) return;
     bool all4 = true;
     for (int i = 0; i < 4; ++i) all4 = all4 && nbval[i] == 4;
     if (all4) return;
@@ -841,11 +913,19 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
     const uint32_t a = attr[q];
     const int4 v4 = quads[q];
     const int v[4] = {v4.x, v4.y, v4.z, v4.w};
-    int o[4];
+    int o[4], vt[4], tg[4];
     bool bad = false;
+    // type and target side by side (the target is read whether or not the vertex moves)
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const bool moved = p3333 ? (a & QA_PATTERN) != 0 : vtype[v[i]] == 3;
-        const int src = moved ? vtarget[v[i]] : v[i];
+        const bool in = v[i] >= 0 && v[i] < V;
+        vt[i] = (!p3333 && in) ? vtype[v[i]] : 0;
+        tg[i] = in ? vtarget[v[i]] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const bool moved = p3333 ? (a & QA_PATTERN) != 0 : vt[i] == 3;
+        const int src = moved ? tg[i] : v[i];
         o[i] = (src >= 0 && src < V) ? address[src] : -1;
         bad = bad || o[i] < 0;  // a dropped vertex or a missing target: the quad is invalid
     }
@@ -1932,7 +2012,7 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 // list pass, then the per-halfedge bucket pass
 // the split's refresh: lists of the flagged halfedges under their smaller endpoint, with
 // the refresh's keys; sizes and ranks of the flagged halfedges, then the flags from all sizes
-void Pipeline::buildBuckets(const uint8_t* only) {
+void Pipeline::buildBuckets(const int32_t* list, int64_t n) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
     int32_t* rank = arena.ptr<int32_t>("bl_rank");
@@ -1947,11 +2027,11 @@ void Pipeline::buildBuckets(const uint8_t* only) {
     uint32_t* fattr = arena.ptr<uint32_t>("fattr");
     zero(vflags, (size_t)V * sizeof(uint32_t));
     valenceReady = false;
-    if (H > 0) {
-        kPlaceEntries<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, rank, keys, A);
-        kBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, only, offM, keys, A, bsize, brank);
-        kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, fattr);
+    if (n > 0) {
+        kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, rank, keys, A);
+        kBuckets<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, keys, A, bsize, brank);
     }
+    if (H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, fattr);
 }
 
 // the valence the next simplification starts from, when one follows: the outgoing list
@@ -2006,9 +2086,9 @@ void Pipeline::checkValid(const char* what) {
 }
 
 // refreshHashTable (HalfedgeMesh.cu:1063-1100) after a split step: only the halfedges
-// whose bucket can have changed (an endpoint in dirtyV, or appended after oldH) are
-// re-bucketed; the others keep their sizes and ranks (same members, same keys), and the
-// flags are rebuilt from all sizes.
+// whose bucket can have changed (an endpoint in dirtyV, or appended after oldH), which
+// kDirtyRecords lists, are re-bucketed; the others keep their sizes and ranks (same
+// members, same keys), and the flags are rebuilt from all sizes.
 void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
@@ -2017,12 +2097,20 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int32_t* key = arena.get<int32_t>("bl_key", (size_t)H);
     arena.grow<int32_t>("bsize", (size_t)H, (size_t)std::min(oldH, H), stream);
     arena.grow<int32_t>("brank", (size_t)H, (size_t)std::min(oldH, H), stream);
-    uint8_t* dirty = arena.get<uint8_t>("dirty_he", (size_t)std::max<int64_t>(H, 1));
+    int32_t* dlist = arena.get<int32_t>("dirty_list", (size_t)std::max<int64_t>(H, 1));
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
+    zero(counters() + A_DIRTY, sizeof(int64_t));
     if (F > 0) kFaceClear<<<gridFor(F), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
-    if (H > 0) kDirtyRecords<<<gridFor(H), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dirty, counters());
-    checkValid("splitting");
-    buildBuckets(dirty);
+    const int64_t per = (int64_t)kBlock * kDirtyPer;
+    if (H > 0) kDirtyRecords<<<(unsigned)((H + per - 1) / per), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dlist, counters());
+    // one read-back: the invalid-record count and the dirty-list length
+    int64_t* hp = pinHost();
+    MDMC_HIP(hipMemcpyAsync(hp, counters() + A_INVALID, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    if (hp[0]) throw HipError("inconsistent mesh after splitting (" + std::to_string(hp[0]) + " bad ids)", -4);
+    const int64_t nDirty = hp[1];
+    if (nDirty < 0 || nDirty > H) throw HipError("inconsistent mesh after splitting (dirty list of " + std::to_string(nDirty) + ")", -4);
+    buildBuckets(dlist, nDirty);
 }
 
 // ---- simplifications
