@@ -264,6 +264,31 @@ struct BucketOf {
 // at most the bucket size; a bucket of 65536 or more halfedges saturates)
 __device__ __forceinline__ uint32_t packBucket(int n, int rank) { return (uint32_t)min(n, 0xFFFF) | ((uint32_t)min(rank, 0xFFFF) << 16); }
 
+// the twin connectHalfedgeTwins (HalfedgeMesh.cu:173-291) gives the member of rank rk of a
+// bucket of n halfedges (mem / dir: its first four members and their directions, in any
+// order; partner: the other member of a pair)
+__device__ __forceinline__ int bucketTwin(int n, int rk, int partner, int mem[4], int dir[4]) {
+    if (n == 2) return partner;
+    if (n != 3 && n != 4) return -1;
+    for (int i = 1; i < n; ++i) {  // by id
+        const int x = mem[i], dx = dir[i];
+        int j = i - 1;
+        while (j >= 0 && mem[j] > x) {
+            mem[j + 1] = mem[j];
+            dir[j + 1] = dir[j];
+            --j;
+        }
+        mem[j + 1] = x;
+        dir[j + 1] = dx;
+    }
+    if (n == 3) return mem[(rk + 1) % 3];
+    int pr;
+    if (dir[0] != dir[1]) pr = rk ^ 1;       // (0,1) (2,3)
+    else if (dir[0] != dir[2]) pr = rk ^ 2;  // (0,2) (1,3)
+    else pr = 3 - rk;                        // (0,3) (1,2)
+    return mem[pr];
+}
+
 // bucket size, rank of h and the twin connectHalfedgeTwins (HalfedgeMesh.cu:173-291) gives h:
 // the partner of a pair, the next by id of three, the first other-direction member paired
 // with the first of four (the same rules kBuckets applies to the bucket lists)
@@ -308,32 +333,7 @@ __device__ BucketOf bucketFromOut(int h, int a, int b, const int4* __restrict__ 
         }
     }
     if (!self) anomaly(ctr);
-    int t = -1;
-    if (n == 2) {
-        t = partner;
-    } else if (n == 3 || n == 4) {
-        for (int i = 1; i < n; ++i) {  // by id
-            const int x = mem[i], dx = dir[i];
-            int j = i - 1;
-            while (j >= 0 && mem[j] > x) {
-                mem[j + 1] = mem[j];
-                dir[j + 1] = dir[j];
-                --j;
-            }
-            mem[j + 1] = x;
-            dir[j + 1] = dx;
-        }
-        if (n == 3) {
-            t = mem[(rk + 1) % 3];
-        } else {
-            int pr;
-            if (dir[0] != dir[1]) pr = rk ^ 1;       // (0,1) (2,3)
-            else if (dir[0] != dir[2]) pr = rk ^ 2;  // (0,2) (1,3)
-            else pr = 3 - rk;                        // (0,3) (1,2)
-            t = mem[pr];
-        }
-    }
-    return BucketOf{n, rk, t};
+    return BucketOf{n, rk, bucketTwin(n, rk, partner, mem, dir)};
 }
 
 // the members of h's bucket sorted by id (first kMaxBucket), returns the true count
@@ -540,13 +540,16 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
     bool later = false;
     if (hh < H2) {
         const int h = (int)hh;
+        // the old halfedge's twin and packed bucket are loaded alongside the touched flags
+        // (every new quad has an old one, so ho is in range either way)
         const int4 q = quads2[h >> 2];
+        const int ho = 4 * oldQ[h >> 2] + (h & 3);
         const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-        if (touched[a] || touched[b]) {
+        const int tt = twinOld[ho];
+        const uint32_t bo = bpackOld[ho];
+        if (touched[a] | touched[b]) {
             later = true;
         } else {
-            const int ho = 4 * oldQ[h >> 2] + (h & 3);
-            const int tt = twinOld[ho];
             int t = -1;
             if (tt >= Hold) {
                 anomaly(ctr);
@@ -555,7 +558,6 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
                 if (fo < 0) anomaly(ctr);
                 else t = 4 * fo + (tt & 3);
             }
-            const uint32_t bo = bpackOld[ho];
             storeHalfedge(h, a, b, BucketOf{(int)(bo & 0xFFFFu), (int)(bo >> 16), t}, twin, bpack, vflags, fattr, qattr, valence);
         }
     }

@@ -41,7 +41,7 @@ __global__ void kScanReset(unsigned long long* scratch, int64_t words) {
 __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ in, int64_t* __restrict__ out, int64_t n, int64_t numTiles,
                                                     unsigned long long* __restrict__ scratch, int64_t* __restrict__ total,
                                                     int64_t* __restrict__ timeoutOut) {
-    __shared__ int32_t s_in[kTile];
+    __shared__ __align__(16) int32_t s_in[kTile];
     __shared__ int64_t s_wave[kScanBlock / 64];
     __shared__ long long s_off;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -50,19 +50,32 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
     unsigned long long* status = scratch + 2;
     const int64_t base = t * kTile;
 
-    // coalesced load into LDS, then 16 consecutive counts per thread
-    for (int i = tid; i < kTile; i += kScanBlock) {
-        const int64_t j = base + i;
-        s_in[i] = j < n ? in[j] : 0;
-    }
-    __syncthreads();
+    // 16 consecutive counts per thread: four 16-byte loads each when the tile is full and
+    // the input 16-byte aligned, else a coalesced load through LDS
     int32_t v[kScanItems];
+    const bool vec = base + kTile <= n && (((uintptr_t)in) & 15) == 0;
+    if (vec) {
+        const int4* p = reinterpret_cast<const int4*>(in + base) + tid * (kScanItems / 4);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            const int4 q = p[k];
+            v[4 * k] = q.x;
+            v[4 * k + 1] = q.y;
+            v[4 * k + 2] = q.z;
+            v[4 * k + 3] = q.w;
+        }
+    } else {
+        for (int i = tid; i < kTile; i += kScanBlock) {
+            const int64_t j = base + i;
+            s_in[i] = j < n ? in[j] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = s_in[tid * kScanItems + k];
+    }
     int64_t acc = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = s_in[tid * kScanItems + k];
-        acc += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) acc += v[k];
     int64_t x = acc;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -143,8 +156,15 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
         }
         __syncthreads();
         const int64_t o = base + (int64_t)half * (kTile / 2);
-        for (int i = tid; i < kTile / 2; i += kScanBlock)
-            if (o + i < n) out[o + i] = s_out[i];
+        if (vec && (((uintptr_t)out) & 15) == 0) {
+            // pairs of offsets as 16-byte stores
+            const longlong2* s2 = reinterpret_cast<const longlong2*>(s_out);
+            longlong2* o2 = reinterpret_cast<longlong2*>(out + o);
+            for (int i = tid; i < kTile / 4; i += kScanBlock) o2[i] = s2[i];
+        } else {
+            for (int i = tid; i < kTile / 2; i += kScanBlock)
+                if (o + i < n) out[o + i] = s_out[i];
+        }
     }
     if (t == numTiles - 1 && tid == 0) *total = s_off + tileSum;
 }
