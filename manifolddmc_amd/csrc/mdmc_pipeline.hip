@@ -399,10 +399,12 @@ __global__ void kCollectOut(const int4* __restrict__ quads, const uint32_t* __re
     if (q >= Q) return;
     const int4 v = quads[q];
     const int b = (int)(4 * q);
-    rank[b + 0] = atomicAdd(&cnt[v.x], 1);
-    rank[b + 1] = atomicAdd(&cnt[v.y], 1);
-    rank[b + 2] = atomicAdd(&cnt[v.z], 1);
-    rank[b + 3] = atomicAdd(&cnt[v.w], 1);
+    int4 r;
+    r.x = atomicAdd(&cnt[v.x], 1);
+    r.y = atomicAdd(&cnt[v.y], 1);
+    r.z = atomicAdd(&cnt[v.z], 1);
+    r.w = atomicAdd(&cnt[v.w], 1);
+    reinterpret_cast<int4*>(rank)[q] = r;
     face_he[q] = b;
     fattr[q] = qattr[q];
 }
@@ -915,20 +917,21 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
     const uint32_t a = attr[q];
     const int4 v4 = quads[q];
     const int v[4] = {v4.x, v4.y, v4.z, v4.w};
-    int o[4], vt[4], tg[4];
+    // P3333: the corners of a pattern face move to their targets.  P3X3Y: a corner moves when
+    // it is dropped (vtype 3, the drop type), which is when it has no new address -- so the
+    // addresses are read first and a target only for the few dropped corners.
+    int o[4];
     bool bad = false;
-    // type and target side by side (the target is read whether or not the vertex moves)
+    const bool pat = p3333 && (a & QA_PATTERN) != 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = (v[i] >= 0 && v[i] < V && !pat) ? address[v[i]] : -1;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const bool in = v[i] >= 0 && v[i] < V;
-        vt[i] = (!p3333 && in) ? vtype[v[i]] : 0;
-        tg[i] = in ? vtarget[v[i]] : -1;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const bool moved = p3333 ? (a & QA_PATTERN) != 0 : vt[i] == 3;
-        const int src = moved ? tg[i] : v[i];
-        o[i] = (src >= 0 && src < V) ? address[src] : -1;
+        if (in && (pat || (!p3333 && o[i] < 0))) {
+            const int src = vtarget[v[i]];
+            o[i] = (src >= 0 && src < V) ? address[src] : -1;
+        }
         bad = bad || o[i] < 0;  // a dropped vertex or a missing target: the quad is invalid
     }
     if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
