@@ -1137,34 +1137,39 @@ struct WC {
     int n;
 };
 
-// copyVerticesToWorkingCopy / copyHalfedgesToWorkingCopy / copyFacesToWorkingCopy (ManifoldSplitter.cu:307-457)
+// copyVerticesToWorkingCopy / copyHalfedgesToWorkingCopy / copyFacesToWorkingCopy (ManifoldSplitter.cu:307-457),
+// one thread per face corner: thread 4r + i copies the i-th halfedge of face nm[r] and its
+// origin into slot c = 4r + i, so the copies' stores are consecutive across lanes
 __global__ void kWcCopy(const int32_t* __restrict__ nm, int n, const int4* __restrict__ he, const int32_t* __restrict__ face_he,
                         const uint32_t* __restrict__ fattr, const float* __restrict__ pos, const float* __restrict__ nrm,
                         const int32_t* __restrict__ pid, WC w) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 4 * n) return;
+    const int r = c >> 2, i = c & 3;
     const int f = nm[r];
     int h = face_he[f];
-    for (int i = 0; i < 4; ++i) {
-        const int4 e = he[h];
-        const int c = 4 * r + i;
-        const int v = e.x;
-        for (int k = 0; k < 3; ++k) {
-            w.pos[3 * c + k] = pos[3 * (int64_t)v + k];
-            w.nrm[3 * c + k] = nrm[3 * (int64_t)v + k];
-        }
-        w.v_c2o[c] = v;
-        if (w.pid) w.pid[c] = pid[v];
-        atomicMax(&w.v_o2c[v], c);  // racy "last writer" in ascending order
-        w.he[c] = e;
-        w.h_c2o[c] = h;
-        w.h_o2c[h] = c;
+    int4 e = he[h];
+    for (int k = 0; k < i; ++k) {
         h = e.z;
+        e = he[h];
     }
-    w.f_first[r] = 4 * r;
-    w.f_attr[r] = fattr[f];
-    w.f_c2o[r] = f;
-    w.f_o2c[f] = r;
+    const int v = e.x;
+    for (int k = 0; k < 3; ++k) {
+        w.pos[3 * c + k] = pos[3 * (int64_t)v + k];
+        w.nrm[3 * c + k] = nrm[3 * (int64_t)v + k];
+    }
+    w.v_c2o[c] = v;
+    if (w.pid) w.pid[c] = pid[v];
+    atomicMax(&w.v_o2c[v], c);  // racy "last writer" in ascending order
+    w.he[c] = e;
+    w.h_c2o[c] = h;
+    w.h_o2c[h] = c;
+    if (i == 0) {
+        w.f_first[r] = 4 * r;
+        w.f_attr[r] = fattr[f];
+        w.f_c2o[r] = f;
+        w.f_o2c[f] = r;
+    }
 }
 
 // updateWorkingCopyHalfedges, ManifoldSplitter.cu:463-505
@@ -1199,73 +1204,63 @@ __global__ void kWcNMFlags(int64_t H, const int32_t* __restrict__ bsize, const i
 
 // performSubdivideQuads, ManifoldSplitter.cu:570-731.  Face r appends 4 vertices at 4n+4r,
 // 16 halfedges at 4n+16r and 4 faces at n+4r, which is where the ascending schedule puts them.
+// Four threads per face (consecutive lanes of one wave): thread i writes inner vertex i, inner
+// halfedge ih[i], the three new halfedges of new face i (with the twins the reference's
+// setTwin calls leave on them) and the outer halfedge that joins that face.  Every lane of
+// the wave has walked the face before any lane rewrites one of its outer halfedges.
 __global__ void kSubdivide(const int32_t* __restrict__ nm, WC w) {
     const int n = w.n;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c0 >= 4 * n) return;
+    const int r = c0 >> 2, i = c0 & 3;
     const int faceID = w.f_o2c[nm[r]];
     int fh[4], vid[4];
     int cur = w.f_first[faceID];
-    for (int i = 0; i < 4; ++i) {
-        fh[i] = cur;
+    for (int k = 0; k < 4; ++k) {
+        fh[k] = cur;
         const int4 e = w.he[cur];
-        vid[i] = e.x;
+        vid[k] = e.x;
         cur = e.z;
     }
     f3 vp[4], vn[4];
-    for (int i = 0; i < 4; ++i) {
-        vp[i] = ld3(w.pos, vid[i]);
-        vn[i] = ld3(w.nrm, vid[i]);
+    for (int k = 0; k < 4; ++k) {
+        vp[k] = ld3(w.pos, vid[k]);
+        vn[k] = ld3(w.nrm, vid[k]);
     }
     const float us[4] = {0.33f, 0.67f, 0.67f, 0.33f}, vs[4] = {0.33f, 0.33f, 0.67f, 0.67f};
-    int inner[4];
-    for (int k = 0; k < 4; ++k) {
-        const f3 ap = mixv(vp[0], vp[1], us[k]), bp = mixv(vp[3], vp[2], us[k]);
-        const f3 an = mixv(vn[0], vn[1], us[k]), bn = mixv(vn[3], vn[2], us[k]);
-        const f3 p = mixv(ap, bp, vs[k]), q = mixv(an, bn, vs[k]);
-        const int c = 4 * n + 4 * r + k;
+    const int nx = (i + 1) & 3, pv = (i + 3) & 3;
+    const int innerBase = 4 * n + 4 * r;
+    {
+        const f3 ap = mixv(vp[0], vp[1], us[i]), bp = mixv(vp[3], vp[2], us[i]);
+        const f3 an = mixv(vn[0], vn[1], us[i]), bn = mixv(vn[3], vn[2], us[i]);
+        const f3 p = mixv(ap, bp, vs[i]), q = mixv(an, bn, vs[i]);
+        const int c = innerBase + i;
         w.pos[3 * c] = p.x; w.pos[3 * c + 1] = p.y; w.pos[3 * c + 2] = p.z;
         w.nrm[3 * c] = q.x; w.nrm[3 * c + 1] = q.y; w.nrm[3 * c + 2] = q.z;
         w.vflags[c] &= ~(uint32_t)VF_REMOVE;
         if (w.pid) w.pid[c] = w.pid[vid[0]];
-        inner[k] = c;
     }
     const int hb = 4 * n + 16 * r;
-    int ih[4];
-    for (int i = 0; i < 4; ++i) ih[i] = hb + i;
-    for (int i = 0; i < 4; ++i) w.he[ih[i]] = make_int4(inner[i], faceID, ih[(i + 1) & 3], -1);
-    w.f_first[faceID] = ih[0];
-    int of[4][4];
-    int prev = 3;
-    for (int i = 0; i < 4; ++i) {
-        for (int k = 0; k < 3; ++k) of[i][k] = hb + 4 + 3 * i + k;
-        of[i][3] = fh[prev];
-        const int nf = n + 4 * r + i;
-        const int qv[3] = {vid[i], inner[i], inner[prev]};
-        for (int k = 0; k < 3; ++k) w.he[of[i][k]] = make_int4(qv[k], nf, of[i][k + 1], -1);
-        int4 e3 = w.he[fh[prev]];
-        e3.y = nf;
-        e3.z = of[i][0];
-        w.he[fh[prev]] = e3;
-        w.f_first[nf] = of[i][0];
-        w.f_attr[nf] = QA_COLOR;  // HalfedgeFace()
-        prev = i;
-    }
-    prev = 3;
-    for (int i = 0; i < 4; ++i) {
-        setT(w.he, of[i][2], of[prev][0]);
-        setT(w.he, of[prev][0], of[i][2]);
-        setT(w.he, of[i][1], ih[prev]);
-        setT(w.he, ih[prev], of[i][1]);
-        prev = i;
-    }
-    for (int i = 0; i < 4; ++i) {  // swapHalfedgeMappings(outer, inner)
-        const int a = fh[i], b = ih[i];
-        const int ao = w.h_c2o[a], bo = w.h_c2o[b];
-        w.h_c2o[a] = bo;
-        w.h_c2o[b] = ao;
-        if (ao != -1 && bo == -1) w.h_o2c[ao] = b;
-    }
+    auto ih = [&](int k) { return hb + k; };
+    auto of = [&](int k, int j) { return hb + 4 + 3 * k + j; };
+    const int nf = n + 4 * r + i;
+    w.he[ih(i)] = make_int4(innerBase + i, faceID, ih(nx), of(nx, 1));
+    if (i == 0) w.f_first[faceID] = ih(0);
+    w.he[of(i, 0)] = make_int4(vid[i], nf, of(i, 1), of(nx, 2));
+    w.he[of(i, 1)] = make_int4(innerBase + i, nf, of(i, 2), ih(pv));
+    w.he[of(i, 2)] = make_int4(innerBase + pv, nf, fh[pv], of(pv, 0));
+    int4 e3 = w.he[fh[pv]];
+    e3.y = nf;
+    e3.z = of(i, 0);
+    w.he[fh[pv]] = e3;
+    w.f_first[nf] = of(i, 0);
+    w.f_attr[nf] = QA_COLOR;  // HalfedgeFace()
+    // swapHalfedgeMappings(outer, inner)
+    const int a = fh[i], b = ih(i);
+    const int ao = w.h_c2o[a], bo = w.h_c2o[b];
+    w.h_c2o[a] = bo;
+    w.h_c2o[b] = ao;
+    if (ao != -1 && bo == -1) w.h_o2c[ao] = b;
 }
 
 // performCollapseNonManifoldHalfedges: which working-copy halfedges collapse
@@ -2354,14 +2349,14 @@ void Pipeline::split(int step, int maxIters) {
         fillNeg(w.f_c2o, (size_t)nf * sizeof(int32_t));
         fillNeg(w.f_o2c, (size_t)(F + nf) * sizeof(int32_t));
 
-        kWcCopy<<<gridFor(n), kBlock, 0, stream>>>(nm, n, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"),
+        kWcCopy<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, n, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"),
                                                    arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), w);
         int vsize = 4 * n, hsize = 4 * n, fsize = n;
         kWcUpdate<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize);
         if (H > 0) kWcNMFlags<<<gridFor(H), kBlock, 0, stream>>>(H, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, counters());
 
         if (step >= 1) {
-            kSubdivide<<<gridFor(n), kBlock, 0, stream>>>(nm, w);
+            kSubdivide<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, w);
             vsize = 8 * n;
             hsize = 20 * n;
             fsize = 5 * n;
