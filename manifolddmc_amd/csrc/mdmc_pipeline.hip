@@ -1580,7 +1580,9 @@ __global__ void kCopyBackVertices(WC w, int count, const int64_t* __restrict__ a
     if (pid) pid[o] = w.pid[v];
 }
 
-__global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ addr, int64_t H0, int4* __restrict__ he, uint8_t* __restrict__ canon) {
+// the ids of the halfedges written back (the records themselves are written by kUpdateBack,
+// from the working copy, once every id is known)
+__global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ addr, int64_t H0, uint8_t* __restrict__ canon) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= count) return;
     if (w.hflags[h] & VF_REMOVE) return;
@@ -1589,8 +1591,7 @@ __global__ void kCopyBackHalfedges(WC w, int count, const int64_t* __restrict__ 
         o = H0 + addr[h];
         w.h_c2o[h] = (int32_t)o;
     }
-    he[o] = w.he[h];
-    canon[o] = 0;  // this record (and kUpdateBack's rewrite of it) may leave the build's layout
+    canon[o] = 0;  // this record may leave the build's layout
 }
 
 __global__ void kCopyBackFaces(WC w, int count, const int64_t* __restrict__ addr, int64_t F0, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
@@ -1622,7 +1623,7 @@ __global__ void kUpdateBack(WC w, int count, int4* __restrict__ he) {
     if (h >= count) return;
     if (w.hflags[h] & VF_REMOVE) return;
     const int64_t me = w.h_c2o[h];
-    int4 e = he[me];
+    int4 e = w.he[h];  // the record copied back to me: no other thread writes he[me]
     e.z = w.h_c2o[e.z];
     e.y = w.f_c2o[e.y];
     e.x = w.v_c2o[e.x];
@@ -2439,7 +2440,7 @@ void Pipeline::split(int step, int maxIters) {
             }
             uint8_t* canon = arena.grow<uint8_t>("he_canon", (size_t)std::max<int64_t>(H0 + addH, 1), (size_t)H0, stream);
             zero(canon + H0, (size_t)addH);
-            kCopyBackHalfedges<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, a, H0, he, canon);
+            kCopyBackHalfedges<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, a, H0, canon);
             // faces
             a = af;
             const int64_t F0 = F;
