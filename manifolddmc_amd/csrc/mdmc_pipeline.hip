@@ -336,6 +336,49 @@ __device__ BucketOf bucketFromOut(int h, int a, int b, const int4* __restrict__ 
     return BucketOf{n, rk, bucketTwin(n, rk, partner, mem, dir)};
 }
 
+// bucketFromOut for the initialize, whose lists carry each entry's destination and the origin
+// of the halfedge before it (ends[k], written with the list), so no quad is gathered
+__device__ BucketOf bucketFromOutEnds(int h, int a, int b, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                                      const int2* __restrict__ ends, int64_t* ctr) {
+    const int m = min(a, b), o = max(a, b);
+    int mem[4], dir[4];
+    int n = 0, rk = 0, partner = -1;
+    bool self = false;
+    auto add = [&](int id, int d) {
+        if (n < 4) {
+            mem[n] = id;
+            dir[n] = d;
+        }
+        ++n;
+        if (id < h) ++rk;
+        if (id != h) partner = id;
+        else self = true;
+    };
+    const int64_t k0 = off[m], k1 = off[m + 1];
+    for (int64_t kb = k0; kb < k1; kb += 4) {
+        int xs[4];
+        int2 es[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            xs[i] = kb + i < k1 ? out[kb + i] : 0;
+            es[i] = kb + i < k1 ? ends[kb + i] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (kb + i >= k1) continue;
+            const int x = xs[i];
+            if (x < 0 || x >= H) {
+                anomaly(ctr);
+                continue;
+            }
+            if (es[i].x == o) add(x, 0);
+            if (o != m && es[i].y == o) add((x & ~3) | ((x + 3) & 3), 1);
+        }
+    }
+    if (!self) anomaly(ctr);
+    return BucketOf{n, rk, bucketTwin(n, rk, partner, mem, dir)};
+}
+
 // the members of h's bucket sorted by id (first kMaxBucket), returns the true count
 __device__ int gatherBucketOut(int h, const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
                                int mem[kMaxBucket]) {
@@ -409,12 +452,17 @@ __global__ void kCollectOut(const int4* __restrict__ quads, const uint32_t* __re
     fattr[q] = qattr[q];
 }
 
+// the entry of h in its origin's list, and beside it h's destination and the origin of the
+// halfedge before h in its quad
 __global__ void kFillOut(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ rank,
-                         int32_t* __restrict__ out) {
+                         int32_t* __restrict__ out, int2* __restrict__ ends) {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
-    const int v = quadCorner(quads[h >> 2], (int)(h & 3));
-    out[off[v] + rank[h]] = (int32_t)h;
+    const int4 q = quads[h >> 2];
+    const int j = (int)(h & 3);
+    const int64_t k = off[quadCorner(q, j)] + rank[h];
+    out[k] = (int32_t)h;
+    ends[k] = make_int2(quadCorner(q, (j + 1) & 3), quadCorner(q, (j + 3) & 3));
 }
 
 // valence = outgoing halfedges (the list length); incoming boundary halfedges are added by
@@ -434,14 +482,14 @@ __global__ void kFaceInit(const uint32_t* __restrict__ qattr, int64_t Q, int32_t
 
 // every halfedge from its bucket
 __global__ void kBucketsAll(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                            int32_t* __restrict__ twin, uint32_t* __restrict__ bpack, uint32_t* __restrict__ vflags,
+                            const int2* __restrict__ ends, int32_t* __restrict__ twin, uint32_t* __restrict__ bpack, uint32_t* __restrict__ vflags,
                             uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence, int64_t* ctr) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (hh >= H) return;
     const int h = (int)hh;
     const int4 q = quads[h >> 2];
     const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-    storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads, H, off, out, ctr), twin, bpack, vflags, fattr, qattr, valence);
+    storeHalfedge(h, a, b, bucketFromOutEnds(h, a, b, H, off, out, ends, ctr), twin, bpack, vflags, fattr, qattr, valence);
 }
 
 // the halfedge records of the quads, from the ids and the twins, and the bucket sizes / ranks
@@ -2070,10 +2118,11 @@ void Pipeline::halfedgeInitialize() {
     int64_t* off = arena.get<int64_t>("out_off", (size_t)V + 1);
     scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream);
     int32_t* out = arena.get<int32_t>("out_list", (size_t)std::max<int64_t>(H, 1));
-    if (H > 0) kFillOut<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, rank, out);
+    int2* ends = arena.get<int2>("out_ends", (size_t)std::max<int64_t>(H, 1));
+    if (H > 0) kFillOut<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, rank, out, ends);
     zero(vflags, (size_t)V * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off, V);
-    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, twin, bpack, vflags, fattr, qattr, valence, counters());
+    if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, ends, twin, bpack, vflags, fattr, qattr, valence, counters());
     heMaterialized = false;
     posSlotted = false;  // the grid pass wrote positions by vertex
     listsV = V;
