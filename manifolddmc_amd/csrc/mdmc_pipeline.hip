@@ -1807,24 +1807,42 @@ __global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, in
     if (threadIdx.x < 9) acc[threadIdx.x] = 0;
     __syncthreads();
     unsigned int loc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < H || i < F || i < V; i += (int64_t)gridDim.x * blockDim.x) {
-        if (i < H && brank[i] == 0) {
-            const int n = bsize[i];
-            loc[0] += (n == 4);
-            loc[1] += (n == 3);
-            loc[2] += (n > 0);
-        }
-        if (i < F) {
-            int h = face_he[i];
-            int k = 0;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    auto bucket = [&](int r, int n) {
+        if (r != 0) return;
+        loc[0] += (n == 4);
+        loc[1] += (n == 3);
+        loc[2] += (n > 0);
+    };
+    // buckets: four halfedges per 16-byte load, then the tail
+    const int64_t H4 = H >> 2;
+    for (int64_t i = t0; i < H4; i += stride) {
+        const int4 r = reinterpret_cast<const int4*>(brank)[i];
+        const int4 n = reinterpret_cast<const int4*>(bsize)[i];
+        bucket(r.x, n.x);
+        bucket(r.y, n.y);
+        bucket(r.z, n.z);
+        bucket(r.w, n.w);
+    }
+    for (int64_t i = 4 * H4 + t0; i < H; i += stride) bucket(brank[i], bsize[i]);
+    // faces: a face whose four records all keep the build's layout is the four halfedges of
+    // one quad slot (next = quadNext from any of them), read with one load; others are walked
+    for (int64_t f = t0; f < F; f += stride) {
+        int h = face_he[f];
+        const int q4 = h & ~3;
+        int k = 0;
+        if (q4 + 3 < H && (!canon || *reinterpret_cast<const uint32_t*>(canon + q4) == 0x01010101u)) {
+            const int4 n = reinterpret_cast<const int4*>(bsize)[q4 >> 2];
+            k = (n.x > 2) + (n.y > 2) + (n.z > 2) + (n.w > 2);
+        } else {
             for (int j = 0; j < 4; ++j) {
                 if (bsize[h] > 2) ++k;
                 h = nextOf(he, canon, h);
             }
-            loc[3 + k] += 1;
         }
-        if (i < V) loc[8] += (vflags[i] & VF_FANS) ? 1u : 0u;
+        loc[3 + k] += 1;
     }
+    for (int64_t v = t0; v < V; v += stride) loc[8] += (vflags[v] & VF_FANS) ? 1u : 0u;
     for (int k = 0; k < 9; ++k) {
         unsigned int v = loc[k];
         for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
