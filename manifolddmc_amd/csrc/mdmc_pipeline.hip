@@ -235,16 +235,32 @@ __global__ void kBuckets(const int4* __restrict__ he, const int32_t* __restrict_
 
 // vertex flags and face NM bits from the bucket sizes of all halfedges (vflags and the
 // faces' NM bits cleared beforehand)
+// (four halfedges per thread, their sizes read with one 16-byte load)
+__device__ __forceinline__ void sizes4(const int32_t* __restrict__ bsize, int64_t H, int64_t h0, int n[4]) {
+    if (h0 + 3 < H) {
+        const int4 v = reinterpret_cast<const int4*>(bsize)[h0 >> 2];
+        n[0] = v.x;
+        n[1] = v.y;
+        n[2] = v.z;
+        n[3] = v.w;
+    } else {
+        for (int j = 0; j < 4; ++j) n[j] = h0 + j < H ? bsize[h0 + j] : 2;
+    }
+}
 __global__ void kFlagsFromBuckets(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ bsize, uint32_t* __restrict__ vflags,
                                   uint32_t* __restrict__ fattr) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
-    const int n = bsize[h];
-    const uint32_t vf = bucketVertexFlag(n);
-    if (!vf) return;
-    const int4 e = he[h];
-    atomicOr(&vflags[e.x], vf);
-    if (n == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+    const int64_t h0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (h0 >= H) return;
+    int n[4];
+    sizes4(bsize, H, h0, n);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t vf = bucketVertexFlag(n[j]);
+        if (!vf) continue;
+        const int4 e = he[h0 + j];
+        atomicOr(&vflags[e.x], vf);
+        if (n[j] == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+    }
 }
 
 // ---------------------------------------------------------------- halfedge build from outgoing lists
@@ -1237,17 +1253,26 @@ __global__ void kWcUpdate(WC w, int count) {
 
 // markHalfedgesAsNonManifold, ManifoldSplitter.cu:527-555: the four slotted halfedges of every full bucket
 __global__ void kWcNMFlags(int64_t H, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, WC w, int64_t* ctr) {
-    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H) return;
-    const int n = bsize[hh];
-    if (n < 4) return;
-    const int r = brank[hh];
-    const bool slotted = (n == 4) ? true : (r == 1 || r == 2 || r == 3 || r == n - 1);
-    if (!slotted) return;
-    const int c = w.h_o2c[hh];
-    if (c < 0) { anomaly(ctr); return; }
-    w.hflags[c] |= VF_NM;
-    atomicOr(&w.vflags[w.he[c].x], (uint32_t)VF_NM);
+    const int64_t h0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (h0 >= H) return;
+    int ns[4];
+    sizes4(bsize, H, h0, ns);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int n = ns[j];
+        if (n < 4) continue;
+        const int64_t hh = h0 + j;
+        const int r = brank[hh];
+        const bool slotted = (n == 4) ? true : (r == 1 || r == 2 || r == 3 || r == n - 1);
+        if (!slotted) continue;
+        const int c = w.h_o2c[hh];
+        if (c < 0) {
+            anomaly(ctr);
+            continue;
+        }
+        w.hflags[c] |= VF_NM;
+        atomicOr(&w.vflags[w.he[c].x], (uint32_t)VF_NM);
+    }
 }
 
 // performSubdivideQuads, ManifoldSplitter.cu:570-731.  Face r appends 4 vertices at 4n+4r,
@@ -2098,7 +2123,7 @@ void Pipeline::buildBuckets(const int32_t* list, int64_t n) {
         kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, rank, keys, A);
         kBuckets<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, keys, A, bsize, brank);
     }
-    if (H > 0) kFlagsFromBuckets<<<gridFor(H), kBlock, 0, stream>>>(he, H, bsize, vflags, fattr);
+    if (H > 0) kFlagsFromBuckets<<<gridFor((H + 3) / 4), kBlock, 0, stream>>>(he, H, bsize, vflags, fattr);
 }
 
 // the valence the next simplification starts from, when one follows: the outgoing list
@@ -2421,7 +2446,7 @@ void Pipeline::split(int step, int maxIters) {
                                                    arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), w);
         int vsize = 4 * n, hsize = 4 * n, fsize = n;
         kWcUpdate<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize);
-        if (H > 0) kWcNMFlags<<<gridFor(H), kBlock, 0, stream>>>(H, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, counters());
+        if (H > 0) kWcNMFlags<<<gridFor((H + 3) / 4), kBlock, 0, stream>>>(H, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, counters());
 
         if (step >= 1) {
             kSubdivide<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, w);
