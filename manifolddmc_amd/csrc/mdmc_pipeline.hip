@@ -65,17 +65,40 @@ __global__ void kFillBytes(uint8_t* __restrict__ p, uint8_t byte, size_t head, s
     }
 }
 
+// Elementwise kernels below take four elements per thread: one 16-byte access per array when
+// all four are in range and the arrays are 16-byte aligned (the arena's are), else one by one.
+__device__ __forceinline__ int64_t firstOfFour() { return 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
+__device__ __forceinline__ bool vec4(int64_t i0, int64_t n, const void* a, const void* b) {
+    return i0 + 3 < n && ((((uintptr_t)a) | ((uintptr_t)b)) & 15) == 0;
+}
+
 // ---------------------------------------------------------------- halfedge construction
 // bucket order after refreshHashTable: (face, position along the face from its first halfedge)
 // and ResetFaceFlags (refreshFaceFlagsGPU) ahead of the NM bits the bucket pass sets again
 // refreshHashTable's per-face part: the face's first halfedge must be a halfedge id, and
 // the pattern / remove / NM bits are cleared (NM is set again from the bucket sizes)
 __global__ void kFaceClear(const int32_t* __restrict__ face_he, int64_t F, int64_t H, uint32_t* __restrict__ fattr, int64_t* ctr) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    const int h = face_he[f];
-    if (h < 0 || h >= H) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-    fattr[f] &= ~(uint32_t)(QA_PATTERN | QA_REMOVE | QA_NM);
+    const int64_t f0 = firstOfFour();
+    if (f0 >= F) return;
+    constexpr uint32_t keep = ~(uint32_t)(QA_PATTERN | QA_REMOVE | QA_NM);
+    int bad = 0;
+    if (vec4(f0, F, face_he, fattr)) {
+        const int4 h = reinterpret_cast<const int4*>(face_he)[f0 >> 2];
+        uint4 a = reinterpret_cast<const uint4*>(fattr)[f0 >> 2];
+        bad = (h.x < 0 || h.x >= H) + (h.y < 0 || h.y >= H) + (h.z < 0 || h.z >= H) + (h.w < 0 || h.w >= H);
+        a.x &= keep;
+        a.y &= keep;
+        a.z &= keep;
+        a.w &= keep;
+        reinterpret_cast<uint4*>(fattr)[f0 >> 2] = a;
+    } else {
+        for (int64_t f = f0; f < f0 + 4 && f < F; ++f) {
+            const int h = face_he[f];
+            bad += (h < 0 || h >= H);
+            fattr[f] &= keep;
+        }
+    }
+    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], (unsigned long long)bad);
 }
 
 // ---- buckets: the halfedges joining one vertex pair.  They are listed under the
@@ -490,10 +513,18 @@ __global__ void kValenceFromOff(const int64_t* __restrict__ off, int64_t V, int3
 }
 
 __global__ void kFaceInit(const uint32_t* __restrict__ qattr, int64_t Q, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    face_he[q] = (int32_t)(4 * q);
-    fattr[q] = qattr[q];
+    const int64_t q0 = firstOfFour();
+    if (q0 >= Q) return;
+    if (vec4(q0, Q, qattr, face_he) && vec4(q0, Q, fattr, fattr)) {
+        const int b = (int)(4 * q0);
+        reinterpret_cast<int4*>(face_he)[q0 >> 2] = make_int4(b, b + 4, b + 8, b + 12);
+        reinterpret_cast<uint4*>(fattr)[q0 >> 2] = reinterpret_cast<const uint4*>(qattr)[q0 >> 2];
+    } else {
+        for (int64_t q = q0; q < q0 + 4 && q < Q; ++q) {
+            face_he[q] = (int32_t)(4 * q);
+            fattr[q] = qattr[q];
+        }
+    }
 }
 
 // every halfedge from its bucket
@@ -923,9 +954,14 @@ __global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restr
 }
 
 __global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int dropType, int32_t* __restrict__ keep) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    keep[v] = vtype[v] == dropType ? 0 : 1;
+    const int64_t v0 = firstOfFour();
+    if (v0 >= V) return;
+    if (vec4(v0, V, vtype, keep)) {
+        const int4 t = reinterpret_cast<const int4*>(vtype)[v0 >> 2];
+        reinterpret_cast<int4*>(keep)[v0 >> 2] = make_int4(t.x != dropType, t.y != dropType, t.z != dropType, t.w != dropType);
+    } else {
+        for (int64_t v = v0; v < v0 + 4 && v < V; ++v) keep[v] = vtype[v] == dropType ? 0 : 1;
+    }
 }
 
 // (positions and normals are not moved: the new vertex o keeps the slot of its old vertex)
@@ -955,9 +991,14 @@ __global__ void kGatherVertices(const float* __restrict__ pos, const float* __re
 }
 
 __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t* __restrict__ keep) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    keep[q] = (attr[q] & QA_REMOVE) ? 0 : 1;
+    const int64_t q0 = firstOfFour();
+    if (q0 >= Q) return;
+    if (vec4(q0, Q, attr, keep)) {
+        const uint4 a = reinterpret_cast<const uint4*>(attr)[q0 >> 2];
+        reinterpret_cast<int4*>(keep)[q0 >> 2] = make_int4(!(a.x & QA_REMOVE), !(a.y & QA_REMOVE), !(a.z & QA_REMOVE), !(a.w & QA_REMOVE));
+    } else {
+        for (int64_t q = q0; q < q0 + 4 && q < Q; ++q) keep[q] = (attr[q] & QA_REMOVE) ? 0 : 1;
+    }
 }
 
 // removeQuadrilateralsP3X3Y (p3333 = 0) / removeQuadsP3333 (p3333 = 1), HalfedgeMesh.cu:513-538, :657-683
@@ -1172,9 +1213,14 @@ __global__ void __launch_bounds__(kBlock) kFixTwins(int4* __restrict__ he, int64
 
 // ---------------------------------------------------------------- ManifoldSplitter
 __global__ void kNMFacePred(const uint32_t* __restrict__ fattr, int64_t F, int32_t* __restrict__ pred) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    pred[f] = (fattr[f] & QA_NM) ? 1 : 0;
+    const int64_t f0 = firstOfFour();
+    if (f0 >= F) return;
+    if (vec4(f0, F, fattr, pred)) {
+        const uint4 a = reinterpret_cast<const uint4*>(fattr)[f0 >> 2];
+        reinterpret_cast<int4*>(pred)[f0 >> 2] = make_int4((a.x & QA_NM) != 0, (a.y & QA_NM) != 0, (a.z & QA_NM) != 0, (a.w & QA_NM) != 0);
+    } else {
+        for (int64_t f = f0; f < f0 + 4 && f < F; ++f) pred[f] = (fattr[f] & QA_NM) ? 1 : 0;
+    }
 }
 
 __global__ void kScatterIndex(const int32_t* __restrict__ pred, const int64_t* __restrict__ addr, int64_t n, int32_t* __restrict__ out) {
@@ -2193,7 +2239,7 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int32_t* dlist = arena.get<int32_t>("dirty_list", (size_t)std::max<int64_t>(H, 1));
     zero(degM, ((size_t)V + 1) * sizeof(int32_t));
     zero(counters() + A_DIRTY, sizeof(int64_t));
-    if (F > 0) kFaceClear<<<gridFor(F), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
+    if (F > 0) kFaceClear<<<gridFor((F + 3) / 4), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
     const int64_t per = (int64_t)kBlock * kDirtyPer;
     if (H > 0) kDirtyRecords<<<(unsigned)((H + per - 1) / per), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dlist, counters());
     // one read-back: the invalid-record count and the dirty-list length
@@ -2218,8 +2264,8 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int64_t* qaddr = arena.get<int64_t>("qaddr", (size_t)Q + 1);
     void* sv = arena.get<char>("scan_scratch_v", scanScratchBytes(V + 1));
     void* sq = arena.get<char>("scan_scratch_q", scanScratchBytes(Q + 1));
-    if (V > 0) kKeepVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, V, dropType, keep);
-    if (Q > 0) kKeepQuads<<<gridFor(Q), kBlock, 0, stream>>>(attr, Q, keepq);
+    if (V > 0) kKeepVertices<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, V, dropType, keep);
+    if (Q > 0) kKeepQuads<<<gridFor((Q + 3) / 4), kBlock, 0, stream>>>(attr, Q, keepq);
     // totals and timeout flag straight into pinned host memory: [0] timeout, [1] V, [2] Q
     int64_t* h = pinHost();
     int64_t* pd = pinDev();
@@ -2278,7 +2324,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* face_he = arena.get<int32_t>("face_he", (size_t)std::max<int64_t>(nQ, 1));
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)std::max<int64_t>(nQ, 1));
     uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)std::max<int64_t>(nV, 1));
-    if (nQ > 0) kFaceInit<<<gridFor(nQ), kBlock, 0, stream>>>(nattr, nQ, face_he, fattr);
+    if (nQ > 0) kFaceInit<<<gridFor((nQ + 3) / 4), kBlock, 0, stream>>>(nattr, nQ, face_he, fattr);
     zero(vflags, (size_t)nV * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off2, nV);
     if (nH > 0)
@@ -2397,7 +2443,7 @@ void Pipeline::split(int step, int maxIters) {
         const int64_t iterH = H;
         // countNonManifoldFaces
         int32_t* pred = arena.get<int32_t>("nmpred", (size_t)F);
-        if (F > 0) kNMFacePred<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, pred);
+        if (F > 0) kNMFacePred<<<gridFor((F + 3) / 4), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, pred);
         int64_t* faddr = nullptr;
         const int64_t n64 = scan32(pred, F, "nmaddr", &faddr);
         if (n64 == 0) break;
