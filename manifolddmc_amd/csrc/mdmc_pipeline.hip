@@ -562,14 +562,28 @@ __global__ void kMaterialize(const int4* __restrict__ quads, int64_t H, const in
 // the new outgoing lists.  qnew[f] is the new id of old quad f (-1: removed).
 __global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, const int32_t* __restrict__ address, int64_t V,
                          int dropType, uint8_t* __restrict__ touched, int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext) {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= V || vtype[u] != dropType) return;
-    const int t = vtarget[u];
-    if (t < 0 || t >= V) return;
-    const int tn = address[t];
-    if (tn < 0) return;  // the relabelled quads then hold -1 and fail validation
-    touched[tn] = 1;
-    srcNext[u] = atomicExch(&srcHead[tn], (int)u);
+    const int64_t u0 = firstOfFour();
+    if (u0 >= V) return;
+    int ty[4];
+    if (vec4(u0, V, vtype, vtype)) {
+        const int4 t4 = reinterpret_cast<const int4*>(vtype)[u0 >> 2];
+        ty[0] = t4.x;
+        ty[1] = t4.y;
+        ty[2] = t4.z;
+        ty[3] = t4.w;
+    } else {
+        for (int j = 0; j < 4; ++j) ty[j] = u0 + j < V ? vtype[u0 + j] : dropType + 1;
+    }
+    for (int j = 0; j < 4; ++j) {
+        if (ty[j] != dropType) continue;
+        const int64_t u = u0 + j;
+        const int t = vtarget[u];
+        if (t < 0 || t >= V) continue;
+        const int tn = address[t];
+        if (tn < 0) continue;  // the relabelled quads then hold -1 and fail validation
+        touched[tn] = 1;
+        srcNext[u] = atomicExch(&srcHead[tn], (int)u);
+    }
 }
 
 // new vertex v': the halfedges of kept quads in the old lists of its own vertex and of the
@@ -968,9 +982,13 @@ __global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int 
 __global__ void kCompactVertices(const int32_t* __restrict__ pid, const int32_t* __restrict__ keep, const int64_t* __restrict__ addr, int64_t V,
                                  int32_t* __restrict__ npid, int32_t* __restrict__ address, int32_t* __restrict__ oldV, const int32_t* __restrict__ vslot,
                                  int32_t* __restrict__ nvslot) {
+    // every vertex gets its new address, -1 when dropped
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    if (!keep[v]) return;
+    if (!keep[v]) {
+        address[v] = -1;
+        return;
+    }
     const int64_t o = addr[v];
     address[v] = (int32_t)o;
     oldV[o] = (int32_t)v;
@@ -2275,8 +2293,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     MDMC_HIP(hipStreamSynchronize(stream));
     if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
     const int64_t nV = h[1], nQ = h[2];
-    int32_t* address = arena.get<int32_t>("vaddress", (size_t)std::max<int64_t>(V, 1));
-    fillNeg(address, (size_t)V * sizeof(int32_t));
+    int32_t* address = arena.get<int32_t>("vaddress", (size_t)std::max<int64_t>(V, 1));  // written for every vertex below
     // the compacted arrays go to the second buffers, which then swap with the primary ones;
     // both are sized for the grid pass's output so that later extractions of the same
     // workload reallocate neither
@@ -2307,7 +2324,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* srcNext = arena.get<int32_t>("src_next", (size_t)std::max<int64_t>(V, 1));
     zero(touched, (size_t)nV);
     fillNeg(srcHead, (size_t)nV * sizeof(int32_t));
-    if (V > 0) kSources<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext);
+    if (V > 0) kSources<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext);
     const int64_t* off = arena.ptr<int64_t>("out_off");
     const int32_t* out = arena.ptr<int32_t>("out_list");
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
