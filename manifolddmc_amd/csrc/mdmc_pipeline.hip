@@ -2490,17 +2490,15 @@ void Pipeline::split(int step, int maxIters) {
         w.f_c2o = arena.get<int32_t>("wc_f_c2o", (size_t)nf);
         w.f_o2c = arena.get<int32_t>("wc_f_o2c", (size_t)(F + nf));
         w.pid = pid() ? arena.get<int32_t>("wc_pid", (size_t)nv) : nullptr;
-        zero(w.pos, 3 * (size_t)nv * sizeof(float));
-        zero(w.nrm, 3 * (size_t)nv * sizeof(float));
+        // positions, normals, halfedge records, first halfedges and face attributes of the
+        // working copy are written in full by kWcCopy ([0, 4n) / [0, n)) and kSubdivide (the
+        // rest) before any read, so only the flags and the id maps are initialised here
         zero(w.vflags, (size_t)nv * sizeof(uint32_t));  // WorkingCopy ignores its initialFlags argument: flags start at 0
         fillNeg(w.v_c2o, (size_t)nv * sizeof(int32_t));
         fillNeg(w.v_o2c, (size_t)(V + nv) * sizeof(int32_t));
-        fillNeg(w.he, (size_t)nh * sizeof(int4));
         zero(w.hflags, (size_t)nh * sizeof(uint32_t));
         fillNeg(w.h_c2o, (size_t)nh * sizeof(int32_t));
         fillNeg(w.h_o2c, (size_t)(H + nh) * sizeof(int32_t));
-        fillNeg(w.f_first, (size_t)nf * sizeof(int32_t));
-        zero(w.f_attr, (size_t)nf * sizeof(uint32_t));
         zero(w.fflags, (size_t)nf * sizeof(uint32_t));
         fillNeg(w.f_c2o, (size_t)nf * sizeof(int32_t));
         fillNeg(w.f_o2c, (size_t)(F + nf) * sizeof(int32_t));
