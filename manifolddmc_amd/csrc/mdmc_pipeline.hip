@@ -475,11 +475,17 @@ __device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketO
 
 // first list pass over the quads: the rank of every halfedge in its origin's list, the
 // face -> halfedge array and the face attributes (collectHalfedgeElements, HalfedgeMesh.cu:14-58)
-__global__ void kCollectOut(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int32_t* __restrict__ cnt,
-                            int32_t* __restrict__ rank, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr) {
+// (a quad with a vertex id outside [0, V) is counted as invalid and takes no list slot; the
+// caller then stops)
+__global__ void kCollectOut(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int64_t V, int32_t* __restrict__ cnt,
+                            int32_t* __restrict__ rank, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int64_t* ctr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v = quads[q];
+    if (v.x < 0 || v.x >= V || v.y < 0 || v.y >= V || v.z < 0 || v.z >= V || v.w < 0 || v.w >= V) {
+        atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+        return;
+    }
     const int b = (int)(4 * q);
     int4 r;
     r.x = atomicAdd(&cnt[v.x], 1);
@@ -1877,16 +1883,6 @@ __global__ void __launch_bounds__(kBlock) kDetectFans(const int4* __restrict__ h
     else vflags[vv] &= ~(uint32_t)VF_FANS;
 }
 
-// ---------------------------------------------------------------- consistency checks
-// Every id the following kernels chase is checked first; a bad one stops the
-// pipeline with an error instead of an out-of-range access.
-__global__ void kValidateQuads(const int4* __restrict__ quads, int64_t Q, int64_t V, int64_t* ctr) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    const int4 v = quads[q];
-    if (v.x < 0 || v.x >= V || v.y < 0 || v.y >= V || v.z < 0 || v.z >= V || v.w < 0 || v.w >= V)
-        atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-}
 // ---------------------------------------------------------------- statistics
 // counts over the vertex-pair buckets (HalfedgeMesh.cu:1164-1239, ManifoldCreator.cu:545-549)
 __global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, int64_t F,
@@ -2215,13 +2211,12 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* bpack = arena.get<uint32_t>("bpack", (size_t)std::max<int64_t>(H, 1));
     const int4* quads = arena.ptr<int4>("quads");
     uint32_t* qattr = arena.ptr<uint32_t>("qattr");
-    if (Q > 0) kValidateQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, Q, V, counters());
-    checkValid("quad vertex ids");
     // the vertex -> halfedge array is derived from these quads when it is copied out
     vertHeV = V;
     vertHeQ = Q;
     zero(cnt, ((size_t)V + 1) * sizeof(int32_t));
-    if (Q > 0) kCollectOut<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, cnt, rank, face_he, fattr);
+    if (Q > 0) kCollectOut<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, V, cnt, rank, face_he, fattr, counters());
+    checkValid("quad vertex ids");
     int64_t* off = arena.get<int64_t>("out_off", (size_t)V + 1);
     scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream);
     int32_t* out = arena.get<int32_t>("out_list", (size_t)std::max<int64_t>(H, 1));
