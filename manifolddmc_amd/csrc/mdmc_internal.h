@@ -18,13 +18,12 @@ struct GridDev {
 };
 
 // --- scans (mdmc_scan.hip): exclusive prefix of n int32 values into int64,
-// returns the total through *total_dev (device int64).  scratch is sized by scanScratchBytes;
-// after the scan, *scanTimeoutFlag(scratch) is nonzero if a look-back wait timed out.
+// returns the total through *total_dev (device int64).  scratch is sized by scanScratchBytes.
+// The result is exact in every case; a tile whose look-back wait passed its spin limit
+// summed its prefix directly and sets *directOut (when given) to 1, a diagnostic only.
 size_t scanScratchBytes(int64_t n);
-// total_dev and timeoutOut may point into mapped pinned host memory; a timed-out look-back
-// also sets *timeoutOut (when given) to 1
-void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s, int64_t* timeoutOut = nullptr);
-const int64_t* scanTimeoutFlag(const void* scratch);
+// total_dev and directOut may point into mapped pinned host memory
+void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s, int64_t* directOut = nullptr);
 
 inline unsigned gridFor(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 

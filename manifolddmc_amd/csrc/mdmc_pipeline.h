@@ -88,7 +88,7 @@ class Pipeline {
     // mapped pinned host words for small read-backs: kernels and copies write them
     // directly, so a read-back is one stream synchronisation (slots 0..kPinSlots-1)
     static constexpr int kPinSlots = 64;
-    static constexpr int kPinMergeTimeout = 8;  // the splitter's merge-list scans
+    static constexpr int kPinMergeTimeout = 8;  // direct-sum diagnostic of the splitter's merge-list scans
     int64_t* pinHost();
     int64_t* pinDev();
     // host -> device copy of a small host array through pinned staging (region 0 or 1, so

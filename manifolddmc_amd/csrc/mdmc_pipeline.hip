@@ -2018,13 +2018,12 @@ const int32_t* Pipeline::vertexSlots() { return posSlotted ? arena.ptr<int32_t>(
 int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out) {
     *out = arena.get<int64_t>(outName, (size_t)n + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(n + 1));
-    // total and timeout flag straight into pinned host memory
+    // total (and the direct-sum diagnostic flag) straight into pinned host memory
     int64_t* h = pinHost();
     int64_t* d = pinDev();
     h[0] = 0;
     scanExclusiveI32(in, *out, n, d + 1, scratch, stream, d);
     MDMC_HIP(hipStreamSynchronize(stream));
-    if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
     return h[1];
 }
 
@@ -2121,7 +2120,7 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     a.tileQOff = tqo;
     void* scratchV = arena.get<char>("scan_scratch", scanScratchBytes(numTiles + 1));
     void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(numTiles + 1));
-    // totals and timeout flag straight into pinned host memory: [0] timeout, [1] V, [2] Q
+    // totals straight into pinned host memory: [0] direct-sum diagnostic, [1] V, [2] Q
     int64_t* h = pinHost();
     int64_t* pd = pinDev();
     h[0] = h[1] = h[2] = 0;
@@ -2133,7 +2132,6 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     }
     MDMC_HIP(hipEventRecord(kev[1], stream));
     MDMC_HIP(hipStreamSynchronize(stream));
-    if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
     V = h[1];
     Q = h[2];
     if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
@@ -2279,14 +2277,13 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     void* sq = arena.get<char>("scan_scratch_q", scanScratchBytes(Q + 1));
     if (V > 0) kKeepVertices<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, V, dropType, keep);
     if (Q > 0) kKeepQuads<<<gridFor((Q + 3) / 4), kBlock, 0, stream>>>(attr, Q, keepq);
-    // totals and timeout flag straight into pinned host memory: [0] timeout, [1] V, [2] Q
+    // totals straight into pinned host memory: [0] direct-sum diagnostic, [1] V, [2] Q
     int64_t* h = pinHost();
     int64_t* pd = pinDev();
     h[0] = h[1] = h[2] = 0;
     scanExclusiveI32(keep, vaddr, V, pd + 1, sv, stream, pd);
     scanExclusiveI32(keepq, qaddr, Q, pd + 2, sq, stream, pd);
     MDMC_HIP(hipStreamSynchronize(stream));
-    if (h[0]) throw HipError("prefix sum: a look-back wait timed out", -4);
     const int64_t nV = h[1], nQ = h[2];
     int32_t* address = arena.get<int32_t>("vaddress", (size_t)std::max<int64_t>(V, 1));  // written for every vertex below
     // the compacted arrays go to the second buffers, which then swap with the primary ones;
@@ -2448,10 +2445,6 @@ void Pipeline::split(int step, int maxIters) {
     zero(touched, (size_t)V);
     splitTouched = true;
     while (true) {
-        if (iter >= maxIters) {
-            extraAnomalies += 1;
-            break;
-        }
         const int64_t iterH = H;
         // countNonManifoldFaces
         int32_t* pred = arena.get<int32_t>("nmpred", (size_t)F);
@@ -2459,7 +2452,12 @@ void Pipeline::split(int step, int maxIters) {
         int64_t* faddr = nullptr;
         const int64_t n64 = scan32(pred, F, "nmaddr", &faddr);
         if (n64 == 0) break;
-        pinHost()[kPinMergeTimeout] = 0;  // the stream is idle here; checked at the copy-back read-back
+        // still non-manifold after the allowed iterations (the reference loops until clean)
+        if (iter >= maxIters) {
+            extraAnomalies += 1;
+            break;
+        }
+        pinHost()[kPinMergeTimeout] = 0;  // direct-sum diagnostic of the merge-list scans
         if (20 * n64 >= (int64_t)INT32_MAX / 2) throw HipError("working copy exceeds the int32 id range", -3);
         const int n = (int)n64;
         int32_t* nm = arena.get<int32_t>("nmlist", (size_t)n);
@@ -2547,7 +2545,7 @@ void Pipeline::split(int step, int maxIters) {
 
         if (copyDataBack || step >= 6) {
             // the new vertices, halfedges and faces of the working copy: three prefix sums
-            // and one read-back of their totals ([0] timeout, [1..3] counts)
+            // and one read-back of their totals ([0] direct-sum diagnostic, [1..3] counts)
             int32_t* pv = arena.get<int32_t>("cb_pred_v", (size_t)std::max(vsize, 1));
             int32_t* ph = arena.get<int32_t>("cb_pred_h", (size_t)std::max(hsize, 1));
             int32_t* pf = arena.get<int32_t>("cb_pred_f", (size_t)std::max(fsize, 1));
@@ -2564,7 +2562,6 @@ void Pipeline::split(int step, int maxIters) {
             scanExclusiveI32(ph, ah, hsize, dp + 2, arena.get<char>("cb_scan_h", scanScratchBytes(hsize + 1)), stream, dp);
             scanExclusiveI32(pf, af, fsize, dp + 3, arena.get<char>("cb_scan_f", scanScratchBytes(fsize + 1)), stream, dp);
             MDMC_HIP(hipStreamSynchronize(stream));
-            if (hp[0] || hp[kPinMergeTimeout]) throw HipError("prefix sum: a look-back wait timed out", -4);
             const int64_t addV = hp[1], addH = hp[2], addF = hp[3];
             // vertices
             int64_t* a = av;

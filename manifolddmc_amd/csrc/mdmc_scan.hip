@@ -7,11 +7,14 @@
 // lowest resident one has finished (a lower tile not yet started would sit on an XCD whose
 // resident blocks are all lower still), and the lowest resident tile never waits on an
 // unstarted one.  The ticket cost one same-address atomic per block: 100 us against 67 us
-// for 17 M counts.
+// for 17 M counts.  That dispatch order is observed behaviour, not a documented
+// guarantee, so correctness does not rest on it: a tile whose look-back wait passes its
+// spin limit sums every count before it straight from the input instead (exact, only
+// slower), then publishes as usual, so a tile that waits on an unstarted one still ends
+// and frees its slot.
 //
 // Look-back words pack flag (2 bits) and value (62 bits), so relaxed device-scope
-// atomics suffice: a reader needs nothing beyond the word it reads.  Every wait is
-// bounded; a timeout sets scratch word 1, which scanTimedOut reports.
+// atomics suffice: a reader needs nothing beyond the word it reads.
 #include "mdmc_internal.h"
 
 namespace mdmc {
@@ -32,7 +35,7 @@ __device__ __forceinline__ void storeRelaxed(unsigned long long* p, uint64_t v) 
     __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// scratch layout: [0] unused, [1] timeout flag, [2 ..] one look-back word per tile
+// scratch layout: [0], [1] unused, [2 ..] one look-back word per tile
 __global__ void kScanReset(unsigned long long* scratch, int64_t words) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < words) scratch[i] = 0;
@@ -40,7 +43,7 @@ __global__ void kScanReset(unsigned long long* scratch, int64_t words) {
 
 __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ in, int64_t* __restrict__ out, int64_t n, int64_t numTiles,
                                                     unsigned long long* __restrict__ scratch, int64_t* __restrict__ total,
-                                                    int64_t* __restrict__ timeoutOut) {
+                                                    int64_t* __restrict__ directOut) {
     __shared__ __align__(16) int32_t s_in[kTile];
     __shared__ int64_t s_wave[kScanBlock / 64];
     __shared__ long long s_off;
@@ -105,7 +108,7 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
             int64_t accum = 0;
             int64_t i = t - 1;
             int spins = 0;
-            bool done = false;
+            bool done = false, direct = false;
             while (!done) {
                 const int64_t j = i - lane;
                 const uint64_t st = j >= 0 ? loadRelaxed(&status[j]) : (FLAG_INC << 62);
@@ -115,11 +118,9 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
                 const int firstInc = incMask ? __ffsll((unsigned long long)incMask) - 1 : 64;
                 const uint64_t upto = firstInc >= 63 ? ~0ULL : ((1ULL << (firstInc + 1)) - 1ULL);
                 if (zeroMask & upto) {
+                    // the ballots make this branch and the counter uniform over the wave
                     if (++spins > kSpinLimit) {
-                        if (lane == 0) {
-                            storeRelaxed(&scratch[1], 1ULL);
-                            if (timeoutOut) *timeoutOut = 1;
-                        }
+                        direct = true;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -131,6 +132,15 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
                 accum += __shfl(val, 0, 64);
                 if (firstInc < 64) done = true;
                 else i -= 64;
+            }
+            if (direct) {
+                // the exclusive prefix straight from the counts before this tile
+                int64_t sum = 0;
+                for (int64_t k = lane; k < base; k += 64) sum += in[k];
+#pragma unroll
+                for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
+                accum = __shfl(sum, 0, 64);
+                if (lane == 0 && directOut) *directOut = 1;
             }
             if (lane == 0) {
                 storeRelaxed(&status[t], (FLAG_INC << 62) | ((uint64_t)(accum + tileSum) & VAL_MASK));
@@ -175,7 +185,7 @@ size_t scanScratchBytes(int64_t n) {
     return (size_t)(tiles + 2) * sizeof(unsigned long long) + 256;
 }
 
-void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total, void* scratch, hipStream_t s, int64_t* timeoutOut) {
+void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total, void* scratch, hipStream_t s, int64_t* directOut) {
     unsigned long long* w = (unsigned long long*)scratch;
     const int64_t tiles = n > 0 ? (n + kTile - 1) / kTile : 0;
     const int64_t words = tiles + 2;
@@ -184,9 +194,7 @@ void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total
         kScanReset<<<1, 1, 0, s>>>((unsigned long long*)total, 1);
         return;
     }
-    kScan<<<(unsigned)tiles, kScanBlock, 0, s>>>(in, out, n, tiles, w, total, timeoutOut);
+    kScan<<<(unsigned)tiles, kScanBlock, 0, s>>>(in, out, n, tiles, w, total, directOut);
 }
-
-const int64_t* scanTimeoutFlag(const void* scratch) { return (const int64_t*)scratch + 1; }
 
 }  // namespace mdmc

@@ -24,9 +24,12 @@
 // The same code instantiated with double gives the FP64 oracle.
 //
 // Parity status: every public stage is pinned by the reference's Data-and-Results
-// fixtures, closed forms or invariants in tests/, except the P3X3Y/P3333
-// simplifications, which no reference fixture covers: those are "parity
-// unpinned" beyond the invariants and the Euler-characteristic checks.
+// fixtures, closed forms or invariants in tests/.  The P3X3Y/P3333
+// simplifications, which no reference fixture covers, are pinned by hand-built
+// quad meshes whose simplified results were derived by hand
+// (tests/test_oracle_simplify_pins.py, through orc_simplify_quads); each of those
+// tests also runs against a planted mistake (orc_set_plant, below) and must fail
+// there.  The plants exist only in this CPU checker and are off by default.
 
 #include <algorithm>
 #include <cmath>
@@ -63,6 +66,21 @@ enum : uint8_t { VF_NM = 1, VF_BOUNDARY = 2, VF_WEIRD = 4, VF_FANS = 8, VF_REMOV
 // QuadrilateralAttribute bits, DeviceVectors.h:334-450
 enum : uint16_t { QA_COLOR = 0x1F, QA_NM = 0x20, QA_PATTERN = 0x40, QA_REMOVE = 0x80, QA_BOUNDARY = 0x100 };
 static inline uint16_t qaSetColor(uint16_t d, int color) { return (uint16_t)((d & 0xE0) | (uint16_t)color); }
+
+// Planted mistakes for the tests that pin the simplifications: each value turns one
+// plausible misreading of the reference on, so a test can show that it catches it.
+enum : int {
+    PLANT_NONE = 0,
+    PLANT_SWAP_SURVIVOR = 1,   // P3X3Y keeps quad.z at the midpoint and maps quad.x onto it
+    PLANT_COLOR_LT = 2,        // colour rule with '<' instead of '<=' (HalfedgeMesh.cu:425-428)
+    PLANT_NO_RENORM = 3,       // merged normal not renormalised
+    PLANT_FAR_CORNER = 4,      // P3333 far vertex taken one corner later (twin.next^3 instead of twin.next^2)
+    PLANT_NO_ALL_FOUR = 5,     // P3333 without the all-valence-4 rejection
+    PLANT_NO_NEIGHBOR_NM = 6,  // P3X3Y colour / P3333 without the neighbour NM test
+    PLANT_OLD_NO_COUNT = 7,    // P3X3Y old without the elementCount == 1 test
+    PLANT_MID_XZ = 8,          // P3X3Y midpoint of the two valence-3 corners instead of the two >=5 corners
+};
+static int g_plant = PLANT_NONE;
 
 // --------------------------------------------------------------------------
 template <class R>
@@ -589,12 +607,15 @@ static bool neighborFaces(const Mesh<R>& m, int face, int out[4]) {
 
 template <class R>
 static void handleMerge(Mesh<R>& m, VertexMap& vm, int x, int y, int z, int w, uint16_t& attr) {
-    V3<R> p = m.pos[y];
-    p = p + scl(R(0.5f), m.pos[w] - p);
+    if (g_plant == PLANT_SWAP_SURVIVOR) std::swap(x, z);
+    // the two corners averaged: the >=5 corners y and w
+    const int ma = (g_plant == PLANT_MID_XZ) ? x : y, mb = (g_plant == PLANT_MID_XZ) ? z : w;
+    V3<R> p = m.pos[ma];
+    p = p + scl(R(0.5f), m.pos[mb] - p);
     m.pos[x] = p;
-    V3<R> n = m.nrm[y];
-    n = n + scl(R(0.5f), m.nrm[w] - n);
-    n = normalize(n);
+    V3<R> n = m.nrm[ma];
+    n = n + scl(R(0.5f), m.nrm[mb] - n);
+    if (g_plant != PLANT_NO_RENORM) n = normalize(n);
     m.nrm[x] = n;
     vm.type[z] = VT_REMOVABLE;
     vm.target[z] = x;
@@ -661,11 +682,14 @@ static void simplify3X3YColor(Mesh<R>& m) {
         if (!neighborFaces(m, q, nb)) continue;
         bool stop = false;
         for (int i = 0; i < 4; ++i)
-            if ((m.qattr[nb[i]] & QA_NM) || (a & QA_BOUNDARY)) stop = true;
+            if (((m.qattr[nb[i]] & QA_NM) && g_plant != PLANT_NO_NEIGHBOR_NM) || (a & QA_BOUNDARY)) stop = true;
         if (stop) continue;
         const int color = a & QA_COLOR;
-        for (int i = 0; i < 4; ++i)
-            if ((m.qattr[nb[i]] & QA_PATTERN) && ((m.qattr[nb[i]] & QA_COLOR) <= color)) stop = true;
+        for (int i = 0; i < 4; ++i) {
+            const int nc = m.qattr[nb[i]] & QA_COLOR;
+            const bool lower = (g_plant == PLANT_COLOR_LT) ? (nc < color) : (nc <= color);
+            if ((m.qattr[nb[i]] & QA_PATTERN) && lower) stop = true;
+        }
         if (stop) continue;
         const int* v = &m.quad[4 * q];
         int val[4];
@@ -707,9 +731,10 @@ static void simplify3X3YOld(Mesh<R>& m) {
         const int* v = &m.quad[4 * q];
         int val[4];
         for (int i = 0; i < 4; ++i) val[i] = vm.valence[v[i]];
-        if (val[0] == 3 && val[1] >= 5 && val[2] == 3 && val[3] >= 5 && vm.count[v[0]] == 1 && vm.count[v[2]] == 1)
+        const bool anyCount = g_plant == PLANT_OLD_NO_COUNT;
+        if (val[0] == 3 && val[1] >= 5 && val[2] == 3 && val[3] >= 5 && ((vm.count[v[0]] == 1 && vm.count[v[2]] == 1) || anyCount))
             handleMerge(m, vm, v[0], v[1], v[2], v[3], a);
-        else if (val[0] >= 5 && val[1] == 3 && val[2] >= 5 && val[3] == 3 && vm.count[v[1]] == 1 && vm.count[v[3]] == 1)
+        else if (val[0] >= 5 && val[1] == 3 && val[2] >= 5 && val[3] == 3 && ((vm.count[v[1]] == 1 && vm.count[v[3]] == 1) || anyCount))
             handleMerge(m, vm, v[1], v[0], v[3], v[2], a);
     }
     simplifyRemoveAndRebuild(m, vm, false, false);
@@ -745,8 +770,9 @@ static void simplify3333(Mesh<R>& m) {
             if (tw < 0) { ok = false; break; }  // boundary: no neighbour (reference reads halfedges[-1])
             nbf[i] = m.he[tw].f;
             const uint16_t na = m.fattr[nbf[i]];
-            if ((na & QA_NM) || (na & QA_BOUNDARY)) { ok = false; break; }
-            const int hh = m.he[m.he[tw].n].n;
+            if (((na & QA_NM) && g_plant != PLANT_NO_NEIGHBOR_NM) || (na & QA_BOUNDARY)) { ok = false; break; }
+            int hh = m.he[m.he[tw].n].n;
+            if (g_plant == PLANT_FAR_CORNER) hh = m.he[hh].n;
             nbv[i] = m.he[hh].o;
             nbval[i] = vm.valence[nbv[i]];
             if (nbval[i] == 3) { ok = false; break; }
@@ -754,7 +780,7 @@ static void simplify3333(Mesh<R>& m) {
         if (!ok) continue;
         bool allFour = true;
         for (int i = 0; i < 4; ++i) allFour = allFour && nbval[i] == 4;
-        if (allFour) continue;
+        if (allFour && g_plant != PLANT_NO_ALL_FOUR) continue;
         m.fattr[f] |= QA_PATTERN;
         for (int i = 0; i < 4; ++i) {
             m.fattr[nbf[i]] |= QA_REMOVE;
@@ -983,11 +1009,12 @@ static void splitNonManifold(Mesh<R>& m, const Grid& g, R iso, int step, int max
     const bool copyDataBack = true;
     int iter = 0;
     while (true) {
-        if (iter >= maxIters) { m.anomalies++; break; }
         const int F = (int)m.face_he.size();
         int n = 0;
         for (int f = 0; f < F; ++f) n += (m.fattr[f] & QA_NM) ? 1 : 0;
         if (n == 0) break;
+        // still non-manifold after the allowed iterations (the reference loops until clean)
+        if (iter >= maxIters) { m.anomalies++; break; }
         const int H = (int)m.he.size();
         const int V = (int)m.pos.size();
         WCHalfedges wh(20 * n, H);
@@ -1338,6 +1365,9 @@ struct Result {
 };
 
 template <class R>
+static void fillStats(const Mesh<R>& m, int flags, Stats& s);
+
+template <class R>
 static Result<R>* run(const float* data, const int dims[3], const float origin[3], const float delta[3], float isoF, int flags, int refinement, int step, int maxIters) {
     auto* res = new Result<R>();
     res->grid = prepareGrid(data, dims, origin, delta, refinement, (flags & F_PAD1) != 0);
@@ -1356,7 +1386,12 @@ static Result<R>* run(const float* data, const int dims[3], const float origin[3
         m.vflags.resize(m.pos.size());
         m.vert_he.resize(m.pos.size());
     }
-    Stats& s = res->st;
+    fillStats(m, flags, res->st);
+    return res;
+}
+
+template <class R>
+static void fillStats(const Mesh<R>& m, int flags, Stats& s) {
     std::memset(&s, 0, sizeof s);
     s.n_vertices = (int64_t)m.pos.size();
     s.n_halfedges = (int64_t)m.he.size();
@@ -1385,6 +1420,33 @@ static Result<R>* run(const float* data, const int dims[3], const float origin[3
     }
     s.split_iterations = m.splitIterations;
     s.anomalies = m.anomalies;
+}
+
+// The simplifications on a caller-built quad mesh (no grid): HalfedgeMesh::initialize,
+// then simplify3X3Y / simplify3X3YOld / simplify3333 in the order runPostProcesses
+// calls them (DualMarchingCubes.cu:335-380).  quads[4q+i] is corner i of quad q;
+// attr[q] is its QuadrilateralAttribute word (colour in bits 0-4).
+template <class R>
+static Result<R>* runOnQuads(int nV, const float* pos, const float* nrm, int nQ, const int* quads, const uint16_t* attr, int flags) {
+    auto* res = new Result<R>();
+    Mesh<R>& m = res->mesh;
+    m.pos.resize(nV);
+    m.nrm.resize(nV);
+    for (int v = 0; v < nV; ++v) {
+        m.pos[v] = V3<R>{R(pos[3 * v]), R(pos[3 * v + 1]), R(pos[3 * v + 2])};
+        m.nrm[v] = V3<R>{R(nrm[3 * v]), R(nrm[3 * v + 1]), R(nrm[3 * v + 2])};
+    }
+    m.quad.assign(quads, quads + 4 * (size_t)nQ);
+    m.qattr.assign(attr, attr + nQ);
+    m.dmcVertices = nV;
+    m.dmcQuads = nQ;
+    if (nV > 0 && nQ > 0) {
+        halfedgeInitialize(m);
+        if (flags & F_P3X3Y_COLOR) simplify3X3YColor(m);
+        if (flags & F_P3X3Y_OLD) simplify3X3YOld(m);
+        if (flags & F_P3333) simplify3333(m);
+    }
+    fillStats(m, flags & ~F_DETECT_NM_VERTS, res->st);
     return res;
 }
 
@@ -1405,6 +1467,23 @@ struct orc_handle {
     int fp64;
     void* r;
 };
+
+// hand-built quad meshes through the simplifications (see runOnQuads)
+void* orc_simplify_quads(int32_t nV, const float* pos, const float* nrm, int32_t nQ, const int32_t* quads, const uint16_t* attr, int32_t flags, orc::Stats* st) {
+    auto* h = new orc_handle();
+    h->fp64 = 0;
+    auto* r = orc::runOnQuads<float>(nV, pos, nrm, nQ, quads, attr, flags);
+    h->r = r;
+    if (st) *st = r->st;
+    return h;
+}
+
+// test hook: one planted mistake (orc::PLANT_*) or 0; returns the previous value
+int32_t orc_set_plant(int32_t which) {
+    const int old = orc::g_plant;
+    orc::g_plant = which;
+    return old;
+}
 
 void* orc_extract(const float* data, const int32_t dims[3], const float origin[3], const float delta[3], float iso, const orc_options* o, int32_t fp64, orc::Stats* st) {
     const int d[3] = {dims[0], dims[1], dims[2]};

@@ -54,6 +54,11 @@ def lib():
         L.orc_get_positions_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_get_grid.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
         L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_simplify_quads.restype = ctypes.c_void_p
+        L.orc_simplify_quads.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(OrcStats)]
+        L.orc_set_plant.restype = ctypes.c_int32
+        L.orc_set_plant.argtypes = [ctypes.c_int32]
         L.orc_cell_polygon.restype = ctypes.c_int32
         L.orc_cell_polygon.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
@@ -114,6 +119,58 @@ def extract(data, origin, delta, iso, flags=SURF_ITER | PAD1 | SPLIT_NM | DETECT
         return out
     finally:
         L.orc_free(h)
+
+
+def _mesh_from_handle(L, h, st):
+    V, H, F = st.n_vertices, st.n_halfedges, st.n_faces
+    pos = np.zeros((V, 3), np.float32)
+    nrm = np.zeros((V, 3), np.float32)
+    he = np.zeros((H, 4), np.int32)
+    face_he = np.zeros(F, np.int32)
+    face_attr = np.zeros(F, np.uint16)
+    vflags = np.zeros(V, np.uint8)
+    vhe = np.zeros(V, np.int32)
+    L.orc_get_mesh(h, _ptr(pos), _ptr(nrm), _ptr(he), _ptr(face_he), _ptr(face_attr), _ptr(vflags), _ptr(vhe))
+    return OracleMesh(pos=pos, nrm=nrm, he=he, face_he=face_he, face_attr=face_attr, vflags=vflags, vhe=vhe,
+                      stats={k: (list(getattr(st, k)) if k == "quads_with_nm" else getattr(st, k)) for k, _ in OrcStats._fields_})
+
+
+def simplify_quads(pos, nrm, quads, attr, flags):
+    """The oracle's halfedge initialize plus the simplifications selected in flags
+    (P3X3Y_COLOR, P3X3Y_OLD, P3333) on a caller-built quad mesh."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+    quads = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
+    attr = np.ascontiguousarray(attr, np.uint16).reshape(-1)
+    assert len(pos) == len(nrm) and len(attr) == len(quads)
+    assert quads.size == 0 or (quads.min() >= 0 and quads.max() < len(pos))
+    L = lib()
+    st = OrcStats()
+    h = L.orc_simplify_quads(len(pos), _ptr(pos), _ptr(nrm), len(quads), _ptr(quads), _ptr(attr), flags, ctypes.byref(st))
+    try:
+        return _mesh_from_handle(L, h, st)
+    finally:
+        L.orc_free(h)
+
+
+# planted mistakes of the oracle (orc::PLANT_* in oracle/mdmc_oracle.cpp)
+PLANT_SWAP_SURVIVOR, PLANT_COLOR_LT, PLANT_NO_RENORM, PLANT_FAR_CORNER = 1, 2, 3, 4
+PLANT_NO_ALL_FOUR, PLANT_NO_NEIGHBOR_NM, PLANT_OLD_NO_COUNT, PLANT_MID_XZ = 5, 6, 7, 8
+
+
+class planted:
+    """with planted(oa.PLANT_...): the oracle runs with that one mistake."""
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        self.old = lib().orc_set_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(self.old)
+        return False
 
 
 def cell_polygon(values, iso, use_decider):

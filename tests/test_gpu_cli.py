@@ -40,9 +40,9 @@ def test_cli_obj_matches_reference_and_python(tmp_path, setting, problem):
     assert r.returncode == 0, r.stderr
     st = json.loads(r.stdout)
     # against the reference's .obj: positions to its 6 printed digits, faces as oriented cycles
-    rv, _, rf = oa.read_obj(out)
-    res = fixtures.compare_with_obj(rv, rf, fixtures.obj_path(setting, problem))
-    assert res == {"n_vertices": 0, "n_faces": 0, "bad_position": 0, "face_mismatch": 0, "unreferenced_delta": 0}, res
+    rv, rn, rf = oa.read_obj(out)
+    res = fixtures.compare_with_obj(rv, rf, fixtures.obj_path(setting, problem), nrm=rn)
+    assert res == {"n_vertices": 0, "n_faces": 0, "bad_position": 0, "bad_normal": 0, "face_mismatch": 0, "unreferenced_delta": 0}, res
     assert st["vertices"] == len(rv) and st["faces"] == len(rf)
     # against the Python binding on the same input
     vals, origin, deltas, _ = fixtures.load_problem(problem)

@@ -52,8 +52,8 @@ def test_fixture_cases_match_oracle(setting, problem):
     vals, origin, deltas, iso = fixtures.load_problem(problem)
     kw = fixtures.settings_for(setting, problem)
     g, _ = assert_same(vals, origin, deltas, iso, kw["flags"], split_step=kw["split_step"], label=setting + "/" + problem)
-    res = fixtures.compare_with_obj(g.positions, g.faces(), fixtures.obj_path(setting, problem))
-    assert res == {"n_vertices": 0, "n_faces": 0, "bad_position": 0, "face_mismatch": 0, "unreferenced_delta": 0}, res
+    res = fixtures.compare_with_obj(g.positions, g.faces(), fixtures.obj_path(setting, problem), nrm=g.normals)
+    assert res == {"n_vertices": 0, "n_faces": 0, "bad_position": 0, "bad_normal": 0, "face_mismatch": 0, "unreferenced_delta": 0}, res
 
 
 @pytest.mark.parametrize("step", [0, 1, 2, 3, 4, 5, 6])

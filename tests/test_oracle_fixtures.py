@@ -9,8 +9,8 @@ import oracle_api as oa
 def test_oracle_reproduces_reference_mesh(setting, problem):
     vals, origin, deltas, iso = fixtures.load_problem(problem)
     m = oa.extract(vals, origin, deltas, iso, **fixtures.settings_for(setting, problem))
-    res = fixtures.compare_with_obj(m.pos, fixtures.faces_from(m.he, m.face_he), fixtures.obj_path(setting, problem))
-    assert res == {"n_vertices": 0, "n_faces": 0, "bad_position": 0, "face_mismatch": 0, "unreferenced_delta": 0}, res
+    res = fixtures.compare_with_obj(m.pos, fixtures.faces_from(m.he, m.face_he), fixtures.obj_path(setting, problem), nrm=m.nrm)
+    assert res == {"n_vertices": 0, "n_faces": 0, "bad_position": 0, "bad_normal": 0, "face_mismatch": 0, "unreferenced_delta": 0}, res
     assert m.stats["anomalies"] == 0
 
 
