@@ -27,7 +27,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--cpu-planes", type=int, default=96, help="z-planes of the field in the CPU baseline sample")
+    ap.add_argument("--cpu-planes", type=int, default=16, help="z-planes per slab in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -70,21 +70,96 @@ def make_field(n):
     return out
 
 
-def cpu_baseline(n, planes, flags):
-    """The CPU oracle (single thread) on a z-slab of the same field."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    """Threads for the all-cores run: the process's CPU share (OMP_NUM_THREADS when set, else the
+    affinity mask), at most 16."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return max(1, min(16, n))
+
+
+def cpu_baseline(n, planes, flags, reps=3):
+    """The CPU oracle (the same algorithm, single-threaded C++, -O3) on z-slabs of the same field:
+    one slab on one thread, and one slab per thread on all of the process's host threads at
+    once (independent extractions, the way a single-threaded extractor uses several cores).
+    Best of `reps` for each."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as oa
+
+    from manifolddmc_amd import grids
+
+    threads = _host_threads()
+    slabs = [grids.gyroid(n, z0=k * planes, z1=(k + 1) * planes) for k in range(threads)]
+    cells = (n + 1) * (n + 1) * (planes + 1)
+
+    def one(k):
+        return len(oa.extract(slabs[k], (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags=flags).face_he)
+
+    t1 = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        faces = one(0)
+        t1.append(time.perf_counter() - t0)
+    tn = []
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            list(ex.map(one, range(threads)))
+            tn.append(time.perf_counter() - t0)
+    single = cells / min(t1) / 1e6
+    multi = threads * cells / min(tn) / 1e6
+    return {"value": round(multi, 3), "unit": "Mcells/s", "cores": threads, "kind": "port",
+            "single_thread_value": round(single, 3), "cpu_model": _cpu_model(),
+            "sample": "z-slabs of %d planes of the same %d^3 gyroid (%dx%dx%d each, padded to %d cells), same flags; "
+                      "1 thread: one slab, best of %d = %.2f s (%d faces); %d threads: one slab each at once, best of %d = %.2f s"
+                      % (planes, n, n, n, planes, cells, reps, min(t1), faces, threads, reps, min(tn))}
+
+
+def fp64_check(n, planes, flags):
+    """FP32 positions of the GPU against the oracle evaluated in FP64 (checker use, in the CPU
+    leg) on z-planes 0..planes-1 of the bench field: per stage the largest relative position
+    error |p32 - p64|inf / max(|p64|inf, 1) and the largest error in cells (the cell size is 1)."""
+    import numpy as np
+
+    import manifolddmc_amd as mdmc
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as oa
 
     from manifolddmc_amd import grids
 
     slab = grids.gyroid(n, z0=0, z1=planes)
-    t0 = time.perf_counter()
-    m = oa.extract(slab, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags=flags)
-    dt = time.perf_counter() - t0
-    cells = (n + 1) * (n + 1) * (planes + 1)
-    return {"value": cells / dt / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
-            "sample": "z-planes 0..%d of the same %d^3 gyroid (%dx%dx%d, padded to %d cells), same flags, %.1f s, %d faces"
-                      % (planes - 1, n, n, n, planes, cells, dt, len(m.face_he))}
+    out = {"workload": "z-planes 0..%d of the %d^3 gyroid" % (planes - 1, n)}
+    d = mdmc.DualMarchingCubes(device=0)
+    try:
+        for label, fl in (("dual_vertices", mdmc.PAD1 | mdmc.SURF_ITER),
+                          ("simplified", mdmc.PAD1 | mdmc.SURF_ITER | mdmc.P3X3Y_COLOR | mdmc.P3X3Y_OLD | mdmc.P3333), ("all_post", flags)):
+            d.extract_raw(slab.ctypes.data, (n, n, planes), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, fl, 0, False)
+            g = d.mesh_numpy()
+            o = oa.extract(slab, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags=fl, fp64=True)
+            if not np.array_equal(g.halfedges, o.he):
+                out[label] = {"topology_equal": False}
+                continue
+            e = np.abs(g.positions.astype(np.float64) - o.pos64)
+            rel = e.max(axis=1) / np.maximum(np.abs(o.pos64).max(axis=1), 1.0)
+            out[label] = {"topology_equal": True, "vertices": int(len(g.positions)), "max_rel": float(rel.max()),
+                          "max_cells": float(e.max()), "normal_max_abs": float(np.abs(g.normals.astype(np.float64) - o.nrm64).max())}
+    finally:
+        d.close()
+    return out
 
 
 def batch_fields(n_grids=256, isos_per_grid=4, seed=5):
@@ -305,6 +380,8 @@ def main():
     if not args.no_cpu:
         print("cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_planes, flags)
+        print("fp64 check", file=sys.stderr, flush=True)
+        out["fp64_error"] = fp64_check(n, 32, flags)
     print(json.dumps(out), flush=True)
 
 

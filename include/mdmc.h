@@ -176,6 +176,58 @@ mdmc_status mdmc_backward(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield,
  * indices uint32[6*nF], device or host. */
 mdmc_status mdmc_mesh_triangles(const mdmc_ctx* ctx, uint32_t* indices, int32_t dst_on_device);
 
+/* Render vertex buffer: HalfedgeMesh insertVertices (HalfedgeMesh.cu:716-760) into the GL
+ * vertex layout of DMC::Vertex (Mesh.h:15-19): per vertex 10 floats = position xyz, normal
+ * xyz, colour rgba.  The colour is HSV(hue, 0.8, 0.8) -> RGB (LazyEngine Color::HSVtoRGB)
+ * with hue 180 (cyan) by default, 0 on a non-manifold edge, 60 on an edge of 3 faces, 270 for
+ * a vertex with several fans, 300 for non-manifold + 3-face, 240 for non-manifold + several
+ * fans.  A vertex without an outgoing halfedge (left unreferenced by a merge) stays all
+ * zeros, as the reference's buffer and the .obj files written from it (Mesh.cu:446-506) show.
+ * vertices float[10*nV], device or host. */
+mdmc_status mdmc_mesh_render_vertices(const mdmc_ctx* ctx, float* vertices, int32_t dst_on_device);
+
+/* 5-colouring of the faces (OriginalDMC FaceColoring::colorFaces, FaceColoring.cu:5-141): the
+ * faces' grid-edge colours (0..23, face_attr bits 0-4) with 5..23 replaced class by class by the
+ * first of 0..4 no neighbour has, then colour-4 faces moved to a free colour of 0..3.  Faces
+ * without a grid colour (31, e.g. after a simplification) keep 31.  The resident mesh is not
+ * changed.  colors uint8[nF], device or host. */
+mdmc_status mdmc_mesh_face_colors(const mdmc_ctx* ctx, uint8_t* colors, int32_t dst_on_device);
+/* Quad -> triangle split by the MaxMin angle criterion with the mean-ratio quality of every
+ * triangle (OriginalDMC DualMarchingCubes.cu:51-107, Triangles.h:80-90, QualityMeasure.h:37-70):
+ * per face (v0..v3) either (v1,v3,v0),(v1,v2,v3) or (v0,v1,v2),(v0,v2,v3), whichever cut has
+ * the larger smallest angle; quality[2f + k] = 2 sqrt(3) |cross| / (sum of squared edge lengths)
+ * of triangle k (1 = equilateral).  indices uint32[6*nF], quality float[2*nF] (may be NULL). */
+mdmc_status mdmc_mesh_triangulate_maxmin(const mdmc_ctx* ctx, uint32_t* indices, float* quality, int32_t dst_on_device);
+
+/* ---- synthetic grids (UniformGridHost::generateVolume / generateEdgeCase) ---- */
+/* SurfaceCase, UniformGrid.h:26-37; the surfaces of UniformGrid.inl:25-127 */
+#define MDMC_SURFACE_SPHERE 0
+#define MDMC_SURFACE_TORUS 1
+#define MDMC_SURFACE_TWO_HOLED_TORUS 2
+#define MDMC_SURFACE_FOUR_HOLED_TORUS 3
+#define MDMC_SURFACE_GENUS_TWO 4
+#define MDMC_SURFACE_IWP 5
+#define MDMC_SURFACE_PW_HYBRID 6
+#define MDMC_SURFACE_NEOVIUS 7
+#define MDMC_SURFACE_GOURSAT 8
+#define MDMC_SURFACE_STEINER_ROMAN 9
+/* EdgeCase, UniformGrid.h:39-42 */
+#define MDMC_EDGE_NM_VERTEX0 0
+#define MDMC_EDGE_NM_VERTEX1 1
+
+/* Replaces UniformGridHost::generateVolume(dims, surfaceCase) (UniformGrid.inl:463-486,
+ * fillVolume :137-189): dims[0]*dims[1]*dims[2] values (x fastest) of the implicit surface at
+ * the points origin + index * delta of [-1, 1]^3, delta = 2 / (dims - 1).  Iso-value 0 gives
+ * the surface.  values: device or host float array; origin / delta (may be NULL) receive the
+ * grid geometry.  Synchronises the context's stream. */
+mdmc_status mdmc_generate_surface(mdmc_ctx* ctx, int32_t surface_case, const int32_t dims[3], float* values, int32_t dst_on_device,
+                                  float origin[3], float delta[3]);
+/* Replaces UniformGridHost::generateEdgeCase (UniformGrid.inl:489-509, fillEdgeCase :191-269):
+ * the 4x4x3 grid (48 values, 850 / 950) with a non-manifold vertex at iso 900.  dims / origin /
+ * delta (may be NULL) receive {4,4,3}, -1, 2 / (dims - 1). */
+mdmc_status mdmc_generate_edge_case(mdmc_ctx* ctx, int32_t edge_case, float* values, int32_t dst_on_device, int32_t dims[3], float origin[3],
+                                    float delta[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,16 @@ TILE_SHAPES = {1: "1024 cells, XCD-chunked", 2: "2048 cells, XCD-chunked", 3: "4
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "CAPACITY", 4: "HASH_FULL", 5: "HIP", 6: "NO_RESULT", 7: "INTERNAL"}
 
 
+class SurfaceCase:
+    """UniformGrid.h:26-37: the implicit surfaces of UniformGrid.inl:25-127 (iso-value 0)."""
+    Sphere, Torus, TwoHoledTorus, FourHoledTorus, GenusTwo, iWP, pwHybrid, neovius, Goursat, SteinerRoman = range(10)
+
+
+class EdgeCase:
+    """UniformGrid.h:39-42: 4x4x3 grids with a non-manifold vertex at iso-value 900."""
+    NonManifoldVertex0, NonManifoldVertex1 = 0, 1
+
+
 class MdmcError(RuntimeError):
     def __init__(self, status, message):
         super().__init__("mdmc %s: %s" % (STATUS.get(status, status), message))
@@ -90,8 +100,9 @@ class _Info(ctypes.Structure):
 _lib = None
 
 EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", "mdmc_extract",
-           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_backward",
-           "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape"]
+           "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_mesh_render_vertices", "mdmc_backward",
+           "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape",
+           "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin"]
 
 
 def lib():
@@ -127,6 +138,16 @@ def lib():
         L.mdmc_mesh_copy.restype = i32
         L.mdmc_mesh_triangles.argtypes = [vp, vp, i32]
         L.mdmc_mesh_triangles.restype = i32
+        L.mdmc_mesh_render_vertices.argtypes = [vp, vp, i32]
+        L.mdmc_mesh_face_colors.argtypes = [vp, vp, i32]
+        L.mdmc_mesh_face_colors.restype = i32
+        L.mdmc_mesh_triangulate_maxmin.argtypes = [vp, vp, vp, i32]
+        L.mdmc_mesh_triangulate_maxmin.restype = i32
+        L.mdmc_generate_surface.argtypes = [vp, i32, vp, vp, i32, vp, vp]
+        L.mdmc_generate_surface.restype = i32
+        L.mdmc_generate_edge_case.argtypes = [vp, i32, vp, i32, vp, vp, vp]
+        L.mdmc_generate_edge_case.restype = i32
+        L.mdmc_mesh_render_vertices.restype = i32
         L.mdmc_backward.argtypes = [vp, vp, vp, i32]
         L.mdmc_backward.restype = i32
         L.mdmc_extract_batch.argtypes = [vp, ctypes.POINTER(_Grid), i32, vp, i32, ctypes.POINTER(_Options), ctypes.POINTER(Stats)]
@@ -168,12 +189,27 @@ class HalfedgeMesh:
             h = he[h, 2]
         return out
 
+    def render_positions_normals(self):
+        """Positions and normals as the render vertex buffer holds them (HalfedgeMesh.cu:716-760):
+        zeros for a vertex that no halfedge starts at."""
+        pos = np.asarray(self.positions).copy()
+        nrm = np.asarray(self.normals).copy()
+        used = np.zeros(len(pos), bool)
+        he = np.asarray(self.halfedges)
+        if len(he):
+            used[he[:, 0]] = True
+        pos[~used] = 0
+        nrm[~used] = 0
+        return pos, nrm
+
     def write_obj(self, path):
-        """The reference's .obj layout (Mesh.cu:446-506): v, vn, then f a//a b//b c//c d//d (1-based)."""
+        """The reference's .obj layout (Mesh.cu:446-506), written from the render vertex buffer:
+        v, vn (zeros for unreferenced vertices), then f a//a b//b c//c d//d (1-based)."""
+        pos, nrm = self.render_positions_normals()
         with open(path, "w") as f:
-            for p in np.asarray(self.positions):
+            for p in pos:
                 f.write("v %g %g %g\n" % tuple(p))
-            for n in np.asarray(self.normals):
+            for n in nrm:
                 f.write("vn %g %g %g\n" % tuple(n))
             for q in self.faces() + 1:
                 f.write("f %d//%d %d//%d %d//%d %d//%d\n" % (q[0], q[0], q[1], q[1], q[2], q[2], q[3], q[3]))
@@ -341,10 +377,86 @@ class DualMarchingCubes:
         last extraction, which must have used RECORD_ADJOINT."""
         self._check(lib().mdmc_backward(self._ctx, ctypes.c_void_p(dpos_ptr), ctypes.c_void_p(dfield_ptr), 1 if accumulate else 0))
 
-    def copy_device(self, positions=0, normals=0, halfedges=0, face_halfedge=0):
+    def copy_device(self, positions=0, normals=0, halfedges=0, face_halfedge=0, face_attr=0, vertex_flags=0, vertex_halfedge=0):
         """mdmc_mesh_copy into device buffers given by pointer (0 = skip)."""
         p = lambda a: ctypes.c_void_p(a) if a else None
-        self._check(lib().mdmc_mesh_copy(self._ctx, p(positions), p(normals), p(halfedges), p(face_halfedge), None, None, None, 1))
+        self._check(lib().mdmc_mesh_copy(self._ctx, p(positions), p(normals), p(halfedges), p(face_halfedge), p(face_attr),
+                                         p(vertex_flags), p(vertex_halfedge), 1))
+
+    def mesh_torch(self):
+        """The resident mesh copied into new torch tensors on the context's device (needs torch):
+        dict of positions [V,3] f32, normals [V,3] f32, halfedges [H,4] i32, face_halfedge [F] i32,
+        face_attr [F] i16 (the uint16 bits), vertex_flags [V] u8, vertex_halfedge [V] i32."""
+        import torch
+        V, H, F = self.info()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        t = {"positions": torch.empty((V, 3), dtype=torch.float32, device=dev),
+             "normals": torch.empty((V, 3), dtype=torch.float32, device=dev),
+             "halfedges": torch.empty((H, 4), dtype=torch.int32, device=dev),
+             "face_halfedge": torch.empty(F, dtype=torch.int32, device=dev),
+             "face_attr": torch.empty(F, dtype=torch.int16, device=dev),
+             "vertex_flags": torch.empty(V, dtype=torch.uint8, device=dev),
+             "vertex_halfedge": torch.empty(V, dtype=torch.int32, device=dev)}
+        torch.cuda.synchronize()
+        self.copy_device(**{k: (v.data_ptr() if v.numel() else 0) for k, v in t.items()})
+        return t
+
+    def generate_volume(self, dims, surface_case):
+        """UniformGridHost::generateVolume(dims, surfaceCase) (UniformGrid.inl:463-486): the implicit
+        surface sampled at dims = (nx, ny, nz) points of [-1, 1]^3 on the GPU.  Returns
+        (values float32 [nz, ny, nx], origin, delta)."""
+        nx, ny, nz = (int(d) for d in dims)
+        d3 = np.array([nx, ny, nz], np.int32)
+        vals = np.zeros((nz, ny, nx), np.float32)
+        o = np.zeros(3, np.float32)
+        dl = np.zeros(3, np.float32)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self._check(lib().mdmc_generate_surface(self._ctx, int(surface_case), p(d3), p(vals), 0, p(o), p(dl)))
+        return vals, tuple(float(x) for x in o), tuple(float(x) for x in dl)
+
+    def generate_edge_case(self, edge_case):
+        """UniformGridHost::generateEdgeCase (UniformGrid.inl:489-509): (values [3, 4, 4], origin, delta)."""
+        vals = np.zeros((3, 4, 4), np.float32)
+        d3 = np.zeros(3, np.int32)
+        o = np.zeros(3, np.float32)
+        dl = np.zeros(3, np.float32)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self._check(lib().mdmc_generate_edge_case(self._ctx, int(edge_case), p(vals), 0, p(d3), p(o), p(dl)))
+        assert tuple(d3) == (4, 4, 3)
+        return vals, tuple(float(x) for x in o), tuple(float(x) for x in dl)
+
+    def triangles(self):
+        """mdmc_mesh_triangles: uint32 [F, 6], two triangles (x,y,z),(z,w,x) per face."""
+        F = self.info()[2]
+        idx = np.zeros((F, 6), np.uint32)
+        if F:
+            self._check(lib().mdmc_mesh_triangles(self._ctx, idx.ctypes.data_as(ctypes.c_void_p), 0))
+        return idx
+
+    def face_colors(self):
+        """mdmc_mesh_face_colors: uint8 [F], the 5-colouring of OriginalDMC's FaceColoring."""
+        F = self.info()[2]
+        out = np.zeros(F, np.uint8)
+        if F:
+            self._check(lib().mdmc_mesh_face_colors(self._ctx, out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out
+
+    def triangulate_maxmin(self):
+        """mdmc_mesh_triangulate_maxmin: (uint32 [F, 6] triangles, float32 [F, 2] mean ratios)."""
+        F = self.info()[2]
+        tri = np.zeros((F, 6), np.uint32)
+        q = np.zeros((F, 2), np.float32)
+        if F:
+            self._check(lib().mdmc_mesh_triangulate_maxmin(self._ctx, tri.ctypes.data_as(ctypes.c_void_p), q.ctypes.data_as(ctypes.c_void_p), 0))
+        return tri, q
+
+    def render_vertices(self):
+        """mdmc_mesh_render_vertices: float32 [V, 10] = position, normal, rgba by vertex flags."""
+        V = self.info()[0]
+        out = np.zeros((V, 10), np.float32)
+        if V:
+            self._check(lib().mdmc_mesh_render_vertices(self._ctx, out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out
 
     def info(self):
         i = _Info()

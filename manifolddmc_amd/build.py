@@ -14,7 +14,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdmc.so")
 CLI = os.path.join(HERE, "mdmc_cli")
 ROCM_LIB = "/opt/rocm/lib"
-SOURCES = ["mdmc_scan.hip", "mdmc_dmc.hip", "mdmc_gridpass.hip", "mdmc_pipeline.hip", "mdmc_adjoint.hip", "mdmc_batch.hip", "mdmc_api.hip"]
+SOURCES = ["mdmc_scan.hip", "mdmc_dmc.hip", "mdmc_gridpass.hip", "mdmc_pipeline.hip", "mdmc_adjoint.hip", "mdmc_batch.hip", "mdmc_generate.hip",
+           "mdmc_faces.hip", "mdmc_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MDMC_ARCH", "gfx950")

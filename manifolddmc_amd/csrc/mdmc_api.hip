@@ -475,4 +475,96 @@ mdmc_status mdmc_mesh_triangles(const mdmc_ctx* cctx, uint32_t* indices, int32_t
     }
 }
 
+mdmc_status mdmc_mesh_render_vertices(const mdmc_ctx* cctx, float* vertices, int32_t dst_on_device) {
+    mdmc_ctx* ctx = const_cast<mdmc_ctx*>(cctx);
+    if (!ctx || !vertices) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return fail(ctx, MDMC_ERR_NO_RESULT, "no extraction result");
+    DeviceGuard dg(ctx->device);
+    try {
+        ctx->p.renderVertices(vertices, dst_on_device != 0);
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_generate_surface(mdmc_ctx* ctx, int32_t surface_case, const int32_t dims[3], float* values, int32_t dst_on_device,
+                                  float origin[3], float delta[3]) {
+    if (!ctx || !dims || !values) return MDMC_ERR_INVALID_ARG;
+    if (surface_case < MDMC_SURFACE_SPHERE || surface_case > MDMC_SURFACE_STEINER_ROMAN)
+        return fail(ctx, MDMC_ERR_INVALID_ARG, "unknown surface case");
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2) return fail(ctx, MDMC_ERR_INVALID_ARG, "grid dimensions below 2");
+    DeviceGuard dg(ctx->device);
+    try {
+        // generateVolume: the domain [-1, 1]^3, delta = (max - min) / (dims - 1), origin = min
+        float o[3], d[3];
+        for (int k = 0; k < 3; ++k) {
+            o[k] = -1.f;
+            d[k] = (1.f - (-1.f)) / ((float)dims[k] - 1.f);
+        }
+        const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+        float* dst = dst_on_device ? values : ctx->p.arena.get<float>("gen_values", n);
+        mdmc::launchFillSurface(dst, dims, surface_case, o, d, ctx->p.stream);
+        MDMC_HIP(hipGetLastError());
+        if (!dst_on_device) MDMC_HIP(hipMemcpyAsync(values, dst, n * sizeof(float), hipMemcpyDeviceToHost, ctx->p.stream));
+        MDMC_HIP(hipStreamSynchronize(ctx->p.stream));
+        for (int k = 0; k < 3; ++k) {
+            if (origin) origin[k] = o[k];
+            if (delta) delta[k] = d[k];
+        }
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_generate_edge_case(mdmc_ctx* ctx, int32_t edge_case, float* values, int32_t dst_on_device, int32_t dims[3], float origin[3],
+                                    float delta[3]) {
+    if (!ctx || !values) return MDMC_ERR_INVALID_ARG;
+    if (edge_case != MDMC_EDGE_NM_VERTEX0 && edge_case != MDMC_EDGE_NM_VERTEX1) return fail(ctx, MDMC_ERR_INVALID_ARG, "unknown edge case");
+    DeviceGuard dg(ctx->device);
+    try {
+        const int dm[3] = {4, 4, 3};
+        float* dst = dst_on_device ? values : ctx->p.arena.get<float>("gen_values", 48);
+        mdmc::launchFillEdgeCase(dst, edge_case, ctx->p.stream);
+        MDMC_HIP(hipGetLastError());
+        if (!dst_on_device) MDMC_HIP(hipMemcpyAsync(values, dst, 48 * sizeof(float), hipMemcpyDeviceToHost, ctx->p.stream));
+        MDMC_HIP(hipStreamSynchronize(ctx->p.stream));
+        for (int k = 0; k < 3; ++k) {
+            if (dims) dims[k] = dm[k];
+            if (origin) origin[k] = -1.f;
+            if (delta) delta[k] = (1.f - (-1.f)) / ((float)dm[k] - 1.f);
+        }
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_mesh_face_colors(const mdmc_ctx* cctx, uint8_t* colors, int32_t dst_on_device) {
+    mdmc_ctx* ctx = const_cast<mdmc_ctx*>(cctx);
+    if (!ctx || !colors) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return fail(ctx, MDMC_ERR_NO_RESULT, "no extraction result");
+    DeviceGuard dg(ctx->device);
+    try {
+        ctx->p.faceColours(colors, dst_on_device != 0);
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
+mdmc_status mdmc_mesh_triangulate_maxmin(const mdmc_ctx* cctx, uint32_t* indices, float* quality, int32_t dst_on_device) {
+    mdmc_ctx* ctx = const_cast<mdmc_ctx*>(cctx);
+    if (!ctx || !indices) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return fail(ctx, MDMC_ERR_NO_RESULT, "no extraction result");
+    DeviceGuard dg(ctx->device);
+    try {
+        ctx->p.triangulateMaxMin(indices, quality, dst_on_device != 0);
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
+}
+
 }  // extern "C"

@@ -57,12 +57,15 @@ bool writeBin(const std::string& path, const Grid& g) {
     return std::fclose(f) == 0 && ok;
 }
 
-bool writeObj(const std::string& path, const std::vector<float>& pos, const std::vector<float>& nrm, const std::vector<uint32_t>& tri) {
+// Mesh::writeToFile (Mesh.cu:446-506): the render vertex buffer (10 floats per vertex:
+// position, normal, colour; all zero for a vertex no halfedge starts at), then one quad per
+// six triangle indices (x, y, z, w = indices 0, 1, 2, 4)
+bool writeObj(const std::string& path, const std::vector<float>& vtx, const std::vector<uint32_t>& tri) {
     FILE* f = std::fopen(path.c_str(), "w");
     if (!f) return false;
-    const size_t nv = pos.size() / 3;
-    for (size_t i = 0; i < nv; ++i) std::fprintf(f, "v %g %g %g\n", pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-    for (size_t i = 0; i < nv; ++i) std::fprintf(f, "vn %g %g %g\n", nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    const size_t nv = vtx.size() / 10;
+    for (size_t i = 0; i < nv; ++i) std::fprintf(f, "v %g %g %g\n", vtx[10 * i], vtx[10 * i + 1], vtx[10 * i + 2]);
+    for (size_t i = 0; i < nv; ++i) std::fprintf(f, "vn %g %g %g\n", vtx[10 * i + 3], vtx[10 * i + 4], vtx[10 * i + 5]);
     for (size_t i = 0; i + 6 <= tri.size(); i += 6) {
         const uint32_t a = tri[i] + 1, b = tri[i + 1] + 1, c = tri[i + 2] + 1, d = tri[i + 4] + 1;
         std::fprintf(f, "f %u//%u %u//%u %u//%u %u//%u\n", a, a, b, b, c, c, d, d);
@@ -188,13 +191,13 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "mdmc_cli: %s\n", mdmc_last_error(ctx));
         rc = 1;
     } else {
-        std::vector<float> pos(3 * (size_t)info.n_vertices), nrm(3 * (size_t)info.n_vertices);
+        std::vector<float> vtx(10 * (size_t)info.n_vertices);
         std::vector<uint32_t> tri(6 * (size_t)info.n_faces);
-        if (mdmc_mesh_copy(ctx, pos.data(), nrm.data(), nullptr, nullptr, nullptr, nullptr, nullptr, 0) != MDMC_OK ||
+        if ((info.n_vertices > 0 && mdmc_mesh_render_vertices(ctx, vtx.data(), 0) != MDMC_OK) ||
             (info.n_faces > 0 && mdmc_mesh_triangles(ctx, tri.data(), 0) != MDMC_OK)) {
             std::fprintf(stderr, "mdmc_cli: %s\n", mdmc_last_error(ctx));
             rc = 1;
-        } else if (!writeObj(out, pos, nrm, tri)) {
+        } else if (!writeObj(out, vtx, tri)) {
             std::fprintf(stderr, "mdmc_cli: cannot write %s\n", out.c_str());
             rc = 1;
         } else if (!quiet) {

@@ -86,6 +86,54 @@ __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restric
     o[3] = v[2]; o[4] = v[3]; o[5] = v[0];
 }
 
+// LazyEngine::Color::HSVtoRGB (LazyEngine/Renderer/Materials/Colors.h:15-56)
+__device__ __forceinline__ f3 hsvToRgb(float h, float sat, float val) {
+    float f = h / 60.0f;
+    const float hi = floorf(f);
+    f = f - hi;
+    const float p = val * (1 - sat);
+    const float q = val * (1 - sat * f);
+    const float t = val * (1 - sat * (1 - f));
+    if (hi == 0.0f || hi == 6.0f) return mk3(val, t, p);
+    if (hi == 1.0f) return mk3(q, val, p);
+    if (hi == 2.0f) return mk3(p, val, t);
+    if (hi == 3.0f) return mk3(p, q, val);
+    if (hi == 4.0f) return mk3(t, p, val);
+    return mk3(val, p, q);
+}
+
+// HalfedgeMesh insertVertices (HalfedgeMesh.cu:716-760): every halfedge writes the render
+// record of its origin vertex -- position, normal, colour by the vertex flags -- into the GL
+// vertex layout {vec3 position, vec3 normal, vec4 colour} (Mesh.h:15-19).  All writers of a
+// vertex store the same record; a vertex without outgoing halfedges keeps the zeros the
+// buffer was filled with.
+__global__ void kRenderVertices(const int4* __restrict__ he, int64_t H, const float* __restrict__ pos, const float* __restrict__ nrm,
+                                const uint32_t* __restrict__ vflags, float* __restrict__ out) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const int v = he[h].x;
+    const uint32_t fl = vflags[v];
+    const bool nm = fl & VF_NM, weird = fl & VF_WEIRD, fans = fl & VF_FANS;
+    float hue = 180.f;  // cyan
+    if (nm) hue = 0.f;            // on a non-manifold edge: red
+    if (weird) hue = 60.f;        // on an edge with 3 faces: yellow
+    if (fans) hue = 270.f;        // non-manifold vertex: purple
+    if (weird && nm) hue = 300.f;  // magenta
+    if (nm && fans) hue = 240.f;   // dark blue
+    const f3 c = hsvToRgb(hue, 0.8f, 0.8f);
+    float* o = out + 10 * (int64_t)v;
+    o[0] = pos[3 * (int64_t)v];
+    o[1] = pos[3 * (int64_t)v + 1];
+    o[2] = pos[3 * (int64_t)v + 2];
+    o[3] = nrm[3 * (int64_t)v];
+    o[4] = nrm[3 * (int64_t)v + 1];
+    o[5] = nrm[3 * (int64_t)v + 2];
+    o[6] = c.x;
+    o[7] = c.y;
+    o[8] = c.z;
+    o[9] = 1.f;
+}
+
 }  // namespace
 
 int64_t prepareBlocks(int64_t points) { return (points + kBlock - 1) / kBlock; }
@@ -98,6 +146,10 @@ void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int r
 
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s) {
     if (F > 0) kTriangles<<<gridFor(F), kBlock, 0, s>>>(he, face_he, F, idx);
+}
+
+void launchRenderVertices(const int4* he, int64_t H, const float* pos, const float* nrm, const uint32_t* vflags, float* out, hipStream_t s) {
+    if (H > 0) kRenderVertices<<<gridFor(H), kBlock, 0, s>>>(he, H, pos, nrm, vflags, out);
 }
 
 }  // namespace mdmc

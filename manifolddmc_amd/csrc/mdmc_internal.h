@@ -27,6 +27,18 @@ void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total
 
 inline unsigned gridFor(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
+// --- the reference's synthetic grids (mdmc_generate.hip): implicit surface sc (SurfaceCase
+// numbering) at origin + index * delta, and the 4x4x3 edge case ec
+void launchFillSurface(float* out, const int dims[3], int surfaceCase, const float origin[3], const float delta[3], hipStream_t s);
+void launchFillEdgeCase(float* out, int edgeCase, hipStream_t s);
+
+// --- face exports of OriginalDMC (mdmc_faces.hip): the 5-colouring of the faces' grid-edge
+// colours (a, b: F bytes each; *result points at the one holding the colours), and the
+// MaxMin-angle triangulation with per-triangle mean ratio (quality may be null)
+void launchFaceColouring(const int4* he, const int32_t* face_he, const uint32_t* fattr, int64_t F, uint8_t* a, uint8_t* b, uint8_t** result,
+                         hipStream_t s);
+void launchTriangulateMaxMin(const int4* he, const int32_t* face_he, int64_t F, const float* pos, uint32_t* tri, float* quality, hipStream_t s);
+
 // --- DMC grid pass (mdmc_dmc.hip)
 // padding / refinement (refineGrid) of several grids in one launch: grid g reads in[inx*iny*inz]
 // and writes out[onx*ony*onz] with the blocks from base on (prepareBlocks(points) of them)
@@ -39,5 +51,8 @@ struct PrepareDesc {
 int64_t prepareBlocks(int64_t points);
 void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, hipStream_t s);
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s);
+// render vertices {position, normal, rgba by vertex flags} of every vertex with an outgoing
+// halfedge into out[10 * V] (the caller zero-fills out first)
+void launchRenderVertices(const int4* he, int64_t H, const float* pos, const float* nrm, const uint32_t* vflags, float* out, hipStream_t s);
 
 }  // namespace mdmc

@@ -83,6 +83,9 @@ class Pipeline {
     void copyOut(float* positions, float* normals, int32_t* halfedges, int32_t* face_halfedge, uint16_t* face_attr, uint8_t* vertex_flags,
                  int32_t* vertex_halfedge, bool dstDevice);
     void triangles(uint32_t* idx, bool dstDevice);
+    void renderVertices(float* out, bool dstDevice);
+    void faceColours(uint8_t* out, bool dstDevice);
+    void triangulateMaxMin(uint32_t* tri, float* quality, bool dstDevice);
 
     int64_t readI64(const int64_t* dev);
     // mapped pinned host words for small read-backs: kernels and copies write them

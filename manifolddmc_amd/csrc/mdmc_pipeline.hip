@@ -2714,6 +2714,38 @@ void Pipeline::copyOut(float* positions, float* normals, int32_t* halfedges, int
     MDMC_HIP(hipStreamSynchronize(stream));
 }
 
+void Pipeline::renderVertices(float* out, bool dstDevice) {
+    if (V == 0) return;
+    const size_t bytes = 10 * (size_t)V * sizeof(float);
+    float* dst = dstDevice ? out : arena.get<float>("out_render", 10 * (size_t)V);
+    zero(dst, bytes);
+    launchRenderVertices(arena.ptr<int4>("he"), H, arena.ptr<float>("pos"), arena.ptr<float>("nrm"), arena.ptr<uint32_t>("vflags"), dst, stream);
+    if (!dstDevice) MDMC_HIP(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+void Pipeline::faceColours(uint8_t* out, bool dstDevice) {
+    if (F == 0) return;
+    uint8_t* a = arena.get<uint8_t>("fcol_a", (size_t)F);
+    uint8_t* b = arena.get<uint8_t>("fcol_b", (size_t)F);
+    uint8_t* res = nullptr;
+    launchFaceColouring(arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"), F, a, b, &res, stream);
+    MDMC_HIP(hipMemcpyAsync(out, res, (size_t)F, dstDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
+void Pipeline::triangulateMaxMin(uint32_t* tri, float* quality, bool dstDevice) {
+    if (F == 0) return;
+    uint32_t* t = dstDevice ? tri : arena.get<uint32_t>("out_tri", 6 * (size_t)F);
+    float* q = quality ? (dstDevice ? quality : arena.get<float>("out_quality", 2 * (size_t)F)) : nullptr;
+    launchTriangulateMaxMin(arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), F, arena.ptr<float>("pos"), t, q, stream);
+    if (!dstDevice) {
+        MDMC_HIP(hipMemcpyAsync(tri, t, 6 * (size_t)F * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        if (quality) MDMC_HIP(hipMemcpyAsync(quality, q, 2 * (size_t)F * sizeof(float), hipMemcpyDeviceToHost, stream));
+    }
+    MDMC_HIP(hipStreamSynchronize(stream));
+}
+
 void Pipeline::triangles(uint32_t* idx, bool dstDevice) {
     if (F == 0) return;
     if (dstDevice) {

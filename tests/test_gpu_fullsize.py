@@ -68,8 +68,10 @@ def test_fullsize_structure_and_statistics(field):
     assert np.array_equal(twin[t], idx[has])
     assert np.array_equal(origin[t], origin[nxt[has]])
     assert np.array_equal(origin[nxt[t]], origin[has])
-    # the iterative split leaves no edge with 4 faces
+    # the iterative split leaves no edge with 4 faces, and nothing the reference leaves
+    # undefined happened (out-of-range ids, fan walks past their guard, oversized buckets)
     assert st["nm_edges4"] == 0
+    assert st["anomalies"] == 0
     assert st["n_edges"] > 0 and st["n_dmc_quads"] > F
 
 
