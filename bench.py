@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--tile-shape", type=int, default=0, help="grid-pass tile shape 1-6 (0: autotuned)")
+    ap.add_argument("--tile-shape", type=int, default=0, help="grid-pass shape 1-8 (0: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers 5 times (reference for the HBM traffic counters) and exit")
     return ap.parse_args()
@@ -50,6 +50,34 @@ def profiled_traffic(n, shape):
             if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == "shape=%d" % shape:
                 return int(parts[2])
     return None
+
+
+def stage_roofline(s, n_in, stats):
+    """Per stage: minimal algorithmic HBM bytes (each input read once, each output written once)
+    from the extraction's own counts, the stage's mean time, and the fraction of the 8 TB/s
+    peak that those bytes over that time reach.  P = points of the padded grid, V/Q = DMC
+    vertices / quads, H/F = final halfedges / faces; the halfedge mesh is 16 B per halfedge
+    record + 8 B per face (first halfedge, attribute) + 1 B per vertex (flags)."""
+    P = (n_in + 2) ** 3
+    V, Q = s["n_dmc_vertices"], s["n_dmc_quads"]
+    H, F, Vf = s["n_halfedges"], s["n_faces"], s["n_vertices"]
+    he_mesh = lambda q, v: 16 * 4 * q + 8 * q + v
+    model = {
+        "ms_grid": ("pad-by-1: read the field, write the padded grid", 4 * n_in ** 3 + 4 * P),
+        "ms_dmc": ("grid pass: read the padded grid, write vertices (pos + normal) and quads", 4 * P + 24 * V + 16 * Q),
+        "ms_halfedge": ("halfedge initialize: read quads, write the halfedge mesh", 16 * Q + he_mesh(Q, V)),
+        "ms_simplify": ("3 simplifications: each reads the halfedge mesh and writes the compacted vertices, quads and "
+                        "their halfedge mesh", 3 * (16 * 4 * Q + 24 * V + 16 * Q + he_mesh(Q, V))),
+        "ms_fix_twins": ("twin repair: read the halfedge records", 16 * H),
+        "ms_split": ("split: per iteration read the face attributes and halfedge records", s["split_iterations"] * (4 * F + 16 * H)),
+        "ms_detect": ("NM-vertex detection: read the halfedge records, write vertex flags", 16 * H + Vf),
+    }
+    out = {}
+    for k, (what, b) in model.items():
+        ms = stats[k]
+        gbs = b / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        out[k[3:]] = {"ms": round(ms, 4), "bytes": int(b), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "model": what}
+    return out
 
 
 def make_field(n):
@@ -370,6 +398,8 @@ def main():
                      "tile_shape": mdmc.TILE_SHAPES.get(shape, shape), "tune_ms": {mdmc.TILE_SHAPES[k]: round(v, 4) for k, v in tune_ms.items()}},
         "stages_ms": {k: round(float(np.mean([x[k] for x in stats])), 3) for k in stage_keys},
     }
+    out["stages_roofline"] = stage_roofline(s, n, {k: float(np.mean([x[k] for x in stats])) for k in stage_keys})
+    out["anomalies"] = s["anomalies"]
     d.close()
     if not args.no_batch:
         print("batched", file=sys.stderr, flush=True)

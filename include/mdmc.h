@@ -109,7 +109,7 @@ const char* mdmc_last_error(const mdmc_ctx* ctx);
 /* Run all following work on this stream (hipStream_t passed as void*); NULL = own stream. */
 mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* hip_stream);
 
-/* Tile shape of the grid pass (no reference counterpart; the reference launches one thread
+/* Shape of the grid pass (no reference counterpart; the reference launches one thread
  * per grid point, DualMarchingCubes.cu:38-67).  MDMC_TILE_AUTO (the default): on the first
  * extraction of a workload class (cell count and problem count to the next power of two,
  * move-to-surface on/off) the context times every shape on that input and keeps the
@@ -123,9 +123,14 @@ mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* hip_stream);
 #define MDMC_TILE_1024_LINEAR 4   /* the same tiles in launch order */
 #define MDMC_TILE_2048_LINEAR 5
 #define MDMC_TILE_4096_LINEAR 6
+#define MDMC_TILE_SLAB4 7         /* row slabs: a block owns up to 254 cells of 4 rows and marches through
+                                     a chunk of slices; counts per row segment, one scan, then the emit */
+#define MDMC_TILE_SLAB2 8         /* the same with 2 rows */
+#define MDMC_TILE_SHAPES 8        /* number of shapes */
 mdmc_status mdmc_set_tile_shape(mdmc_ctx* ctx, int32_t shape);
-/* The shape the last extraction used; tune_ms (may be NULL, double[6]) receives the grid-pass
- * times of shapes 1..6 from this context's last tuning (0 where none ran). */
+/* The shape the last extraction used; tune_ms (may be NULL, double[MDMC_TILE_SHAPES]) receives
+ * the grid-pass times of shapes 1..MDMC_TILE_SHAPES from this context's last tuning (0 where none
+ * ran). */
 mdmc_status mdmc_get_tile_shape(const mdmc_ctx* ctx, int32_t* last_shape, double* tune_ms);
 
 /* Replaces DualMarchingCubes::run(iso, mesh, grid) + runPostProcesses

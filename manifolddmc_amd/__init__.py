@@ -43,7 +43,8 @@ class SplittingStep:
 # grid-pass tile shapes, include/mdmc.h (TILE_AUTO: autotuned per workload class)
 TILE_AUTO = 0
 TILE_SHAPES = {1: "1024 cells, XCD-chunked", 2: "2048 cells, XCD-chunked", 3: "4096 cells, XCD-chunked",
-               4: "1024 cells, launch order", 5: "2048 cells, launch order", 6: "4096 cells, launch order"}
+               4: "1024 cells, launch order", 5: "2048 cells, launch order", 6: "4096 cells, launch order",
+               7: "row slabs of 4 rows, XCD-chunked", 8: "row slabs of 2 rows, XCD-chunked"}
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "CAPACITY", 4: "HASH_FULL", 5: "HIP", 6: "NO_RESULT", 7: "INTERNAL"}
 
@@ -288,7 +289,7 @@ class DualMarchingCubes:
     def tile_shape(self):
         """(shape of the last extraction, {shape: grid-pass ms of this context's last tuning})."""
         sh = ctypes.c_int32()
-        ms = (ctypes.c_double * 6)()
+        ms = (ctypes.c_double * len(TILE_SHAPES))()
         self._check(lib().mdmc_get_tile_shape(self._ctx, ctypes.byref(sh), ms))
         return sh.value, {k: ms[k - 1] for k in TILE_SHAPES}
 

@@ -135,12 +135,14 @@ mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* s) {
     return MDMC_OK;
 }
 
-static_assert(MDMC_TILE_AUTO == mdmc::GP_SHAPE_AUTO && MDMC_TILE_1024 == mdmc::GP_SHAPE_FIRST && MDMC_TILE_4096_LINEAR == mdmc::GP_SHAPE_LAST,
+static_assert(MDMC_TILE_AUTO == mdmc::GP_SHAPE_AUTO && MDMC_TILE_1024 == mdmc::GP_SHAPE_FIRST &&
+                  MDMC_TILE_4096_LINEAR == mdmc::GP_SHAPE_LAST_TILE && MDMC_TILE_SLAB4 == mdmc::GP_SHAPE_SLAB4 &&
+                  MDMC_TILE_SLAB2 == mdmc::GP_SHAPE_SLAB2 && MDMC_TILE_SHAPES == mdmc::GP_SHAPE_LAST,
               "tile shape numbering");
 
 mdmc_status mdmc_set_tile_shape(mdmc_ctx* ctx, int32_t shape) {
     if (!ctx) return MDMC_ERR_INVALID_ARG;
-    if (shape < MDMC_TILE_AUTO || shape > MDMC_TILE_4096_LINEAR) return fail(ctx, MDMC_ERR_INVALID_ARG, "unknown tile shape");
+    if (shape < MDMC_TILE_AUTO || shape > MDMC_TILE_SHAPES) return fail(ctx, MDMC_ERR_INVALID_ARG, "unknown tile shape");
     ctx->p.tileShape = shape;
     return MDMC_OK;
 }
@@ -149,7 +151,7 @@ mdmc_status mdmc_get_tile_shape(const mdmc_ctx* ctx, int32_t* last_shape, double
     if (!ctx || !last_shape) return MDMC_ERR_INVALID_ARG;
     *last_shape = ctx->p.lastShape;
     if (tune_ms)
-        for (int i = MDMC_TILE_1024; i <= MDMC_TILE_4096_LINEAR; ++i) tune_ms[i - MDMC_TILE_1024] = ctx->p.tuneMs[i];
+        for (int i = MDMC_TILE_1024; i <= MDMC_TILE_SHAPES; ++i) tune_ms[i - MDMC_TILE_1024] = ctx->p.tuneMs[i];
     return MDMC_OK;
 }
 

@@ -8,12 +8,20 @@
 
 namespace mdmc {
 
-// Tile shapes of the grid pass: cells per tile and the block -> tile order.  Every shape
-// gives the same mesh; the autotuner picks the fastest per workload class.
-//   1..3: 1024 / 2048 / 4096 cells per tile, tiles dealt to the 8 XCDs in contiguous chunks
+// Shapes of the grid pass.  Every shape gives the same mesh; the autotuner picks the fastest
+// per workload class.
+//   1..3: tiles of 1024 / 2048 / 4096 consecutive cells, dealt to the 8 XCDs in contiguous
+//         chunks (classify, scan, emit)
 //   4..6: the same tiles in launch order (block b -> tile b)
-enum : int { GP_SHAPE_AUTO = 0, GP_SHAPE_FIRST = 1, GP_SHAPE_LAST = 6, GP_SHAPE_DEFAULT = 2 };
+//   7, 8: row slabs of 4 / 2 rows: a block owns a band of up to 254 cells of its rows and
+//         marches through a chunk of slices (count, scan over row segments, emit)
+enum : int { GP_SHAPE_AUTO = 0, GP_SHAPE_FIRST = 1, GP_SHAPE_LAST_TILE = 6, GP_SHAPE_SLAB4 = 7, GP_SHAPE_SLAB2 = 8, GP_SHAPE_LAST = 8,
+             GP_SHAPE_DEFAULT = 2 };
 int gridPassTile(int shape);
+inline bool gridPassIsSlab(int shape) { return shape >= GP_SHAPE_SLAB4; }
+inline int gridPassSlabRows(int shape) { return shape == GP_SHAPE_SLAB4 ? 4 : 2; }
+constexpr int SLAB_COLS = 256;            // threads of a slab block, one LDS column each
+constexpr int SLAB_XMAX = SLAB_COLS - 2;  // cells of a band row (plus a halo cell and the x + 1 points)
 const char* gridPassShapeName(int shape);
 
 // One (grid, iso-value) problem of an extraction.  A batch concatenates the problems'
@@ -25,6 +33,11 @@ struct ProblemDev {
     int64_t cells;     // CX * CY * CZ
     int64_t cellBase;  // first cell in the concatenation
     int64_t tileBase;  // first grid-pass tile
+    // row slabs: bands of bandW cells per row, rowBlocks blocks of rows, zChunks chunks of zChunk
+    // slices; the problem's blocks start at blockBase, its row segments (slice, row, band) at
+    // segBase, segment (z, y, b) = segBase + (z * CY + y) * bands + b
+    int bands, bandW, rowBlocks, zChunk, zChunks;
+    int64_t blockBase, segBase;
 };
 
 struct GridPassArgs {
@@ -49,15 +62,23 @@ struct GridPassArgs {
     float* nrm;
     int4* quads;
     uint32_t* qattr;
-    // per vertex: its cell inside its problem and its contour in that cell (emit -> vertices)
+    // per vertex: its cell inside its problem and its contour's edges (emit -> vertices):
+    // edge i of the contour in bits 4i..4i+3, the contour size in bits 60..63
     int32_t* vcell;
-    uint8_t* vcontour;
+    uint64_t* vcw;
     int64_t V;
     // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1
     // (single-problem extractions only)
     int32_t* adjMap;
     // per vertex its problem, when not null
     int32_t* vpid;
+    // row slabs: blocks of the launch, vertex / quad count per row segment ([S + 1], last 0)
+    // and their exclusive prefixes
+    int64_t numBlocks, numSegs;
+    int32_t* segV;
+    int32_t* segQ;
+    const int64_t* segVOff;
+    const int64_t* segQOff;
 };
 
 // uploads the polygon table of the classify kernel; once per device, with that device current
@@ -65,5 +86,9 @@ void initGridPassTables();
 void launchGridClassify(int shape, const GridPassArgs& a, hipStream_t s);
 // emit (quads, vertex cells) and the dual vertices
 void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s);
+// row slabs: per row segment vertex / quad counts; then (after the scan of the counts) quads,
+// vertex work items and the dual vertices
+void launchSlabCount(int shape, const GridPassArgs& a, hipStream_t s);
+void launchSlabEmit(int shape, const GridPassArgs& a, hipStream_t s);
 
 }  // namespace mdmc

@@ -169,13 +169,21 @@ __device__ __forceinline__ void neighbourSamples(const GridDev& g, int x, int y,
     }
 }
 
-__device__ __forceinline__ f3 edgeMean(const float s[8], float iso, uint64_t w, int t) {
+// contour t of polygon word w as a contour word: its edges from nibble 0, its size in the top nibble
+__device__ __forceinline__ uint64_t contourWord(uint64_t w, int t) {
     int j = 0;
     for (int c = 0; c < t; ++c) j += polySize(w, c);
     const int sz = polySize(w, t);
+    const uint64_t edges = (w >> (16 + 4 * j)) & ((1ULL << (4 * sz)) - 1ULL);
+    return edges | ((uint64_t)sz << 60);
+}
+
+// the mean of the contour's edge crossings in cell-local coordinates (CellIntersection.inl:391-406)
+__device__ __forceinline__ f3 edgeMean(const float s[8], float iso, uint64_t cw) {
+    const int sz = (int)(cw >> 60);
     f3 p = mk3(0.f, 0.f, 0.f);
-    for (int i = 0; i < sz; ++i, ++j) {
-        const int e = polyEdge(w, j);
+    for (int i = 0; i < sz; ++i) {
+        const int e = (int)((cw >> (4 * i)) & 0xF);
         const int a = edgeA(e), b = edgeB(e), ax = edgeAxis(e);
         const float tt = (iso - s[a]) / (s[b] - s[a]);
         f3 off;
@@ -460,7 +468,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
         const int ai = upperIndex<TILE>(s_vpre, NA, i);
         const int64_t v = voff + i;
         a.vcell[v] = (int)(c0 + s_cell[ai]);
-        a.vcontour[v] = (uint8_t)(i - s_vpre[ai]);
+        a.vcw[v] = contourWord(s_poly[ai], i - s_vpre[ai]);
         if (a.vpid) a.vpid[v] = tc.prob;
     }
 
@@ -522,6 +530,289 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     }
 }
 
+// ---- row slabs
+// A block owns bandW (<= SLAB_XMAX) cells of R consecutive rows and a chunk of slices, one LDS
+// column per thread: column c is the cell x0 - 1 + c (c = 0 is the last cell of the band to
+// the left), row r the cell row y0 - 1 + r (r = 0 is the last row of the block above).  The
+// block marches z through its chunk with the two point planes of the current slice in LDS.
+// Each row segment (slice, row, band) is a contiguous run of the linear cell order, so the
+// count pass gives every segment its vertex / quad counts, one scan over the segments gives
+// each its first vertex and quad, and the emit pass numbers its cells from those offsets:
+// the same ids, in the same (cell, contour) / (edge key) order, as the tile shapes.  The emit
+// pass also classifies the halo cells, whose ids it needs for its quads, and keeps the
+// previous slice's ids and contour maps on chip for the quads' z - 1 corners.
+struct SlabCtx {
+    int prob;
+    int band, rb, zc;
+    int x0, xn, y0, yn, z0, z1;
+};
+
+template <int R>
+__device__ __forceinline__ SlabCtx slabCtx(const GridPassArgs& a, int chunked) {
+    SlabCtx c;
+    const int64_t L = tileOfBlock(blockIdx.x, a.numBlocks, chunked != 0);
+    int lo = 0, hi = a.nProblems;  // probs[lo].blockBase <= L
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.probs[mid].blockBase <= L) lo = mid;
+        else hi = mid;
+    }
+    const ProblemDev& P = a.probs[lo];
+    c.prob = lo;
+    const int64_t rel = L - P.blockBase;
+    c.band = (int)(rel % P.bands);
+    const int64_t rest = rel / P.bands;
+    c.rb = (int)(rest % P.rowBlocks);
+    c.zc = (int)(rest / P.rowBlocks);
+    const int CZ = P.g.nz - 1;
+    c.x0 = c.band * P.bandW;
+    c.xn = min(P.bandW, P.CX - c.x0);
+    c.y0 = c.rb * R;
+    c.yn = min(R, P.CY - c.y0);
+    c.z0 = c.zc * P.zChunk;
+    c.z1 = min(CZ, c.z0 + P.zChunk);
+    return c;
+}
+
+__device__ __forceinline__ int64_t segIndex(const ProblemDev& P, int z, int y, int band) {
+    return P.segBase + ((int64_t)z * P.CY + y) * P.bands + band;
+}
+
+// EMIT = false: vertex and quad counts per row segment.  EMIT = true: ids, quads and the
+// vertex work items (cell, contour word) of the block's cells.
+template <int R, bool EMIT>
+__global__ void __launch_bounds__(SLAB_COLS) kSlab(GridPassArgs a, int chunked) {
+    constexpr int NR = R + 1;             // cell rows: the halo row and the R own rows
+    constexpr int PR = R + 2;             // point rows y0 - 1 .. y0 + R
+    constexpr int NC = NR * SLAB_COLS;    // cell slots per slice: row * SLAB_COLS + column
+    constexpr int NW = SLAB_COLS / 64;
+    constexpr int RING = EMIT ? 2 : 1;
+    __shared__ float s_pl[2][PR][SLAB_COLS];  // point planes, plane z in s_pl[z & 1]
+    __shared__ uint8_t s_mc[NC];
+    __shared__ uint16_t s_act[NC];            // active cell slots, row-major
+    __shared__ uint64_t s_poly[NC];           // polygon word by active index
+    __shared__ uint32_t s_cnt[NC];            // vertices | quads << 8 | owned bits (x, y, z edge, edgy) << 16
+    __shared__ int32_t s_id[RING][NC];        // first vertex id of the cell, -1 without one; ring by z parity
+    __shared__ uint32_t s_cm[RING][NC];       // contour of each crossed edge, 2 bits per edge
+    __shared__ int32_t s_qb[EMIT ? NC : 1];   // first quad id of an own cell
+    __shared__ int s_wn[NR][NW];              // per row and wave: active cells, then their base
+    __shared__ uint32_t s_wsum[NR][NW];       // per row and wave: vertices | quads << 16
+    __shared__ int64_t s_rowV[NR], s_rowQ[NR];
+    __shared__ int s_nact;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const SlabCtx bc = slabCtx<R>(a, chunked);
+    const ProblemDev& P = a.probs[bc.prob];
+    const GridDev g = P.g;
+    const float iso = P.iso;
+    const int CX = P.CX, CY = P.CY;
+    const int64_t NX = g.nx, NXY = (int64_t)g.nx * g.ny;
+    // this thread's column: cell x = x0 - 1 + tid, point x the same
+    const int x = bc.x0 - 1 + tid;
+    const bool colPoint = tid <= bc.xn + 1 && x >= 0 && x < g.nx;
+    const bool colCell = EMIT ? (tid <= bc.xn && x >= 0) : (tid >= 1 && tid <= bc.xn);
+    const int zs = EMIT ? max(bc.z0 - 1, 0) : bc.z0;
+
+    auto loadPlane = [&](int zp) {
+        float* dst = &s_pl[zp & 1][0][0];
+#pragma unroll
+        for (int k = 0; k < PR; ++k) {
+            const int py = bc.y0 - 1 + k;
+            float v = 0.f;
+            if (colPoint && py >= 0 && py < g.ny && zp < g.nz) v = g.v[(int64_t)zp * NXY + (int64_t)py * NX + x];
+            dst[k * SLAB_COLS + tid] = v;
+        }
+    };
+    if (bc.z0 < bc.z1) loadPlane(zs);
+
+    for (int z = zs; z < bc.z1; ++z) {
+        const bool out = EMIT ? (z >= bc.z0) : true;  // the slice before the chunk only feeds the quads
+        const int slot = z & 1;
+        const int ring = EMIT ? slot : 0;
+        loadPlane(z + 1);
+        __syncthreads();
+        const float* lo = &s_pl[z & 1][0][0];
+        const float* hi = &s_pl[(z + 1) & 1][0][0];
+
+        // classify this thread's column (one cell per row) and ballot the active cells
+        uint64_t bal[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int y = bc.y0 - 1 + r;
+            const bool cell = colCell && y >= 0 && r <= bc.yn && (EMIT || r >= 1);
+            int mc = 0;
+            if (cell) {
+                const int k0 = r * SLAB_COLS + tid, k1 = (r + 1) * SLAB_COLS + tid;
+                const float c8[8] = {lo[k0], lo[k0 + 1], lo[k1], lo[k1 + 1], hi[k0], hi[k0 + 1], hi[k1], hi[k1 + 1]};
+#pragma unroll
+                for (int q = 0; q < 8; ++q) mc |= (c8[q] >= iso ? 1 : 0) << q;
+            }
+            const bool act = cell && mc != 0 && mc != 255;
+            bal[r] = __ballot(act);
+            s_mc[r * SLAB_COLS + tid] = (uint8_t)mc;
+            s_cnt[r * SLAB_COLS + tid] = 0;
+            if (lane == 0) s_wn[r][wid] = __popcll(bal[r]);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int run = 0;
+            for (int r = 0; r < NR; ++r)
+                for (int w = 0; w < NW; ++w) {
+                    const int n = s_wn[r][w];
+                    s_wn[r][w] = run;
+                    run += n;
+                }
+            s_nact = run;
+        }
+        __syncthreads();
+        const uint64_t below = (1ULL << lane) - 1ULL;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if ((bal[r] >> lane) & 1ULL) s_act[s_wn[r][wid] + __popcll(bal[r] & below)] = (uint16_t)(r * SLAB_COLS + tid);
+        __syncthreads();
+        const int nact = s_nact;
+
+        // polygons of the active cells, every lane busy
+        for (int ai = tid; ai < nact; ai += SLAB_COLS) {
+            const int c = s_act[ai];
+            const int r = c / SLAB_COLS, col = c % SLAB_COLS;
+            const int k0 = r * SLAB_COLS + col, k1 = (r + 1) * SLAB_COLS + col;
+            const float sv[8] = {lo[k0], lo[k0 + 1], lo[k1], lo[k1 + 1], hi[k0], hi[k0 + 1], hi[k1], hi[k1 + 1]};
+            const uint64_t w = cellPolygonTab(s_mc[c], sv, iso);
+            int own = 0;
+            if (r >= 1 && col >= 1) own = ownedBits(sv, iso, bc.x0 - 1 + col, bc.y0 - 1 + r, z);
+            s_cnt[c] = (uint32_t)polyCount(w) | ((uint32_t)__popc(own & 7) << 8) | ((uint32_t)own << 16);
+            s_poly[ai] = w;
+            if (EMIT) s_cm[ring][c] = contourMap(w);
+        }
+        __syncthreads();
+
+        // per row: exclusive prefix over the columns of vertices and quads (wave scan, then waves)
+        uint32_t pre[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const uint32_t cnt = s_cnt[r * SLAB_COLS + tid];
+            const uint32_t val = (cnt & 0xFFu) | (((cnt >> 8) & 0xFFu) << 16);
+            uint32_t xs = val;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(xs, d, 64);
+                if (lane >= d) xs += y;
+            }
+            pre[r] = xs - val;
+            if (lane == 63) s_wsum[r][wid] = xs;
+        }
+        if (EMIT && tid < NR) {
+            // the row's first vertex: the row segment's offset, less the halo cell's vertices (the
+            // last ones of the band to the left); its first quad: the segment's offset
+            const int y = bc.y0 - 1 + tid;
+            if (y >= 0 && tid <= bc.yn) {
+                const int64_t sg = segIndex(P, z, y, bc.band);
+                s_rowV[tid] = a.segVOff[sg] - (int64_t)(s_cnt[tid * SLAB_COLS] & 0xFFu);
+                s_rowQ[tid] = a.segQOff[sg];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            uint32_t add = 0, tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t v = s_wsum[r][w];
+                if (w < wid) add += v;
+                tot += v;
+            }
+            pre[r] += add;
+            if (!EMIT && tid == 0 && r >= 1 && r <= bc.yn && out) {
+                const int64_t sg = segIndex(P, z, bc.y0 - 1 + r, bc.band);
+                a.segV[sg] = (int32_t)(tot & 0xFFFFu);
+                a.segQ[sg] = (int32_t)(tot >> 16);
+            }
+        }
+        if (EMIT) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int c = r * SLAB_COLS + tid;
+                const uint32_t cnt = s_cnt[c];
+                const int y = bc.y0 - 1 + r;
+                const bool rowOk = y >= 0 && r <= bc.yn;
+                s_id[ring][c] = (rowOk && (cnt & 0xFFu)) ? (int32_t)(s_rowV[r] + (int64_t)(pre[r] & 0xFFFFu)) : -1;
+                if (rowOk && r >= 1) s_qb[c] = (int32_t)(s_rowQ[r] + (int64_t)(pre[r] >> 16));
+                // for the adjoint: cell -> (first vertex << 2 | contours - 1), -1 for empty cells
+                if (a.adjMap && out && colCell && tid >= 1 && r >= 1 && r <= bc.yn) {
+                    const int64_t cidx = ((int64_t)z * CY + y) * CX + x;
+                    a.adjMap[cidx] = (cnt & 0xFFu) ? (int32_t)(((int64_t)s_id[ring][c] << 2) | (int64_t)((cnt & 0xFFu) - 1)) : -1;
+                }
+            }
+        }
+        __syncthreads();
+
+        if (EMIT && out) {
+            // the vertex work items and the quads of the active own cells
+            for (int ai = tid; ai < nact; ai += SLAB_COLS) {
+                const int c = s_act[ai];
+                const int r = c / SLAB_COLS, col = c % SLAB_COLS;
+                if (r < 1 || col < 1) continue;
+                const uint32_t cnt = s_cnt[c];
+                const int cx = bc.x0 - 1 + col, cy = bc.y0 - 1 + r;
+                const int id = s_id[ring][c];
+                const uint64_t w = s_poly[ai];
+                const int nv = (int)(cnt & 0xFFu);
+                const int32_t cidx = (int32_t)(((int64_t)z * CY + cy) * CX + cx);
+                for (int t = 0; t < nv; ++t) {
+                    const int64_t v = (int64_t)id + t;
+                    a.vcell[v] = cidx;
+                    a.vcw[v] = contourWord(w, t);
+                    if (a.vpid) a.vpid[v] = bc.prob;
+                }
+                const int own = (int)(cnt >> 16);
+                if (!(own & 7)) continue;
+                const int edgy = (own >> 3) & 1;
+                const uint32_t cm = s_cm[ring][c];
+                int q = s_qb[c];
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (!((own >> axis) & 1)) continue;
+                    // own edge, then (dx, dy, dz, edge) of the three other cells around it, 4 bits
+                    // each: x: (y-1: e2) (z-1: e4) (y-1,z-1: e6); y: (x-1: e1) (z-1: e7) (x-1,z-1: e5);
+                    // z: (x-1: e9) (y-1: e11) (x-1,y-1: e10)
+                    const int ownE = axis == 0 ? 0 : (axis == 1 ? 3 : 8);
+                    const uint32_t nbTable = axis == 0 ? 0x664422u : (axis == 1 ? 0x557411u : 0xA3B291u);
+                    int s0 = -1, s1 = -1, s2 = -1, s3 = -1;
+                    {
+                        const int sl = edgeSlot(ownE, edgy);
+                        const int vid = id + (int)((cm >> (2 * ownE)) & 3u);
+                        s0 = sl == 0 ? vid : s0;
+                        s1 = sl == 1 ? vid : s1;
+                        s2 = sl == 2 ? vid : s2;
+                        s3 = sl == 3 ? vid : s3;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const uint32_t ent = (nbTable >> (8 * k)) & 0xFFu;
+                        const int d = (int)(ent & 0xF), e = (int)(ent >> 4);
+                        const int nc = c - (d & 1) - ((d >> 1) & 1) * SLAB_COLS;
+                        const int nring = EMIT ? (((d >> 2) & 1) ? ring ^ 1 : ring) : 0;
+                        const int vid = s_id[nring][nc] + (int)((s_cm[nring][nc] >> (2 * e)) & 3u);
+                        const int sl = edgeSlot(e, edgy);
+                        s0 = sl == 0 ? vid : s0;
+                        s1 = sl == 1 ? vid : s1;
+                        s2 = sl == 2 ? vid : s2;
+                        s3 = sl == 3 ? vid : s3;
+                    }
+                    a.quads[q] = make_int4(s0, s1, s2, s3);
+                    a.qattr[q] = (uint32_t)(3 * ((cx & 1) | ((cy & 1) << 1) | ((z & 1) << 2)) + axis);
+                    ++q;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!EMIT && tid == 0 && blockIdx.x == 0) {
+        a.segV[a.numSegs] = 0;  // the entry past the last segment: the scans' totals
+        a.segQ[a.numSegs] = 0;
+    }
+}
+
 // ---- dual vertices (sliceP), one per thread over all vertices of the pass
 // (blocks XCD-chunked like the tiles when `chunked`: consecutive vertices share rows and
 // slices of the field, which then stay in one XCD's L2).  The move to the surface takes
@@ -556,10 +847,7 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
         y = (int)(r % (unsigned)P.CY);
         z = (int)(r / (unsigned)P.CY);
         loadCorners(g, x, y, z, s);
-        int mc = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) mc |= (s[k] >= iso ? 1 : 0) << k;
-        p = edgeMean(s, iso, cellPolygonTab(mc, s, iso), a.vcontour[v]);
+        p = edgeMean(s, iso, a.vcw[v]);
         if (a.surfIter) {
             moverInit(m, s, p);
             for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);
@@ -664,7 +952,8 @@ void initGridPassTables() {
 
 const char* gridPassShapeName(int shape) {
     static const char* names[] = {"auto", "1024 cells, XCD-chunked", "2048 cells, XCD-chunked", "4096 cells, XCD-chunked",
-                                  "1024 cells, launch order", "2048 cells, launch order", "4096 cells, launch order"};
+                                  "1024 cells, launch order", "2048 cells, launch order", "4096 cells, launch order",
+                                  "row slabs of 4 rows, XCD-chunked", "row slabs of 2 rows, XCD-chunked"};
     return (shape >= 0 && shape <= GP_SHAPE_LAST) ? names[shape] : "?";
 }
 
@@ -677,6 +966,19 @@ void launchGridClassify(int shape, const GridPassArgs& a, hipStream_t s) {
         case 2048: kClassify<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
         default: kClassify<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
     }
+}
+
+void launchSlabCount(int shape, const GridPassArgs& a, hipStream_t s) {
+    if (a.numBlocks <= 0) return;
+    if (gridPassSlabRows(shape) == 4) kSlab<4, false><<<(unsigned)a.numBlocks, SLAB_COLS, 0, s>>>(a, 1);
+    else kSlab<2, false><<<(unsigned)a.numBlocks, SLAB_COLS, 0, s>>>(a, 1);
+}
+
+void launchSlabEmit(int shape, const GridPassArgs& a, hipStream_t s) {
+    if (a.numBlocks <= 0) return;
+    if (gridPassSlabRows(shape) == 4) kSlab<4, true><<<(unsigned)a.numBlocks, SLAB_COLS, 0, s>>>(a, 1);
+    else kSlab<2, true><<<(unsigned)a.numBlocks, SLAB_COLS, 0, s>>>(a, 1);
+    if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a, 1);
 }
 
 void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {

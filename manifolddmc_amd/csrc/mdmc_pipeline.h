@@ -120,6 +120,7 @@ class Pipeline {
     void fillNeg(void* p, size_t bytes);
     void launchFill(void* p, uint8_t byte, size_t bytes);
     void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
+    void slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
     void buildBuckets(const int32_t* list, int64_t n);
     int32_t* bucketPassValence(const int64_t* off, int64_t nV);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);

@@ -2043,6 +2043,74 @@ void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t))
 int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
 
 // ---- grid pass
+// Row slabs: bands of at most SLAB_XMAX cells per row, R rows per block, and chunks of slices
+// sized so that the launch has about 4096 blocks (a problem gets its share by its cell count;
+// at least 8 slices per chunk, each chunk re-reads one slice before it for its quads).
+void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint) {
+    const int R = gridPassSlabRows(shape);
+    int64_t blocks = 0, segs = 0;
+    for (ProblemDev& pr : problems) {
+        const int CZ = pr.g.nz - 1;
+        pr.bands = (pr.CX + SLAB_XMAX - 1) / SLAB_XMAX;
+        pr.bandW = (pr.CX + pr.bands - 1) / pr.bands;
+        pr.rowBlocks = (pr.CY + R - 1) / R;
+        const int64_t cols = (int64_t)pr.bands * pr.rowBlocks;
+        const int64_t share = C > 0 ? (int64_t)(4096.0 * (double)pr.cells / (double)C / (double)cols) : 1;
+        const int nzc = (int)std::max<int64_t>(1, std::min<int64_t>((CZ + 7) / 8, share));
+        pr.zChunk = (CZ + nzc - 1) / nzc;
+        pr.zChunks = (CZ + pr.zChunk - 1) / pr.zChunk;
+        pr.blockBase = blocks;
+        blocks += cols * pr.zChunks;
+        pr.segBase = segs;
+        segs += (int64_t)CZ * pr.CY * pr.bands;
+    }
+    if (blocks >= (int64_t)INT32_MAX || segs >= (int64_t)INT32_MAX) throw HipError("grid pass: too many row segments", -3);
+    probsDev = arena.get<ProblemDev>("gp_probs", problems.size());
+    upload(probsDev, problems.data(), problems.size() * sizeof(ProblemDev), 0);
+    GridPassArgs a{};
+    a.probs = probsDev;
+    a.nProblems = nProblems;
+    a.surfIter = surfIter ? 1 : 0;
+    a.numBlocks = blocks;
+    a.numSegs = segs;
+    a.segV = arena.get<int32_t>("gp_segv", (size_t)segs + 1);
+    a.segQ = arena.get<int32_t>("gp_segq", (size_t)segs + 1);
+    int64_t* svo = arena.get<int64_t>("gp_segvo", (size_t)segs + 1);
+    int64_t* sqo = arena.get<int64_t>("gp_segqo", (size_t)segs + 1);
+    a.segVOff = svo;
+    a.segQOff = sqo;
+    void* scratchV = arena.get<char>("scan_scratch", scanScratchBytes(segs + 1));
+    void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(segs + 1));
+    int64_t* h = pinHost();
+    int64_t* pd = pinDev();
+    h[0] = h[1] = h[2] = 0;
+    MDMC_HIP(hipEventRecord(kev[0], stream));
+    launchSlabCount(shape, a, stream);
+    scanExclusiveI32(a.segV, svo, segs + 1, pd + 1, scratchV, stream, pd);
+    scanExclusiveI32(a.segQ, sqo, segs + 1, pd + 2, scratchQ, stream, pd);
+    MDMC_HIP(hipEventRecord(kev[1], stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    V = h[1];
+    Q = h[2];
+    if (V >= (int64_t)INT32_MAX / 2 || 4 * Q >= (int64_t)INT32_MAX) throw HipError("mesh exceeds the int32 id range", -3);
+    if (recordAdjoint && (V >= ((int64_t)1 << 29) || C >= (int64_t)INT32_MAX))
+        throw HipError("adjoint recording needs fewer than 2^29 vertices and 2^31 cells", -3);
+    a.pos = arena.get<float>("pos", 3 * (size_t)std::max<int64_t>(V, 1));
+    a.nrm = arena.get<float>("nrm", 3 * (size_t)std::max<int64_t>(V, 1));
+    a.quads = arena.get<int4>("quads", (size_t)std::max<int64_t>(Q, 1));
+    a.qattr = arena.get<uint32_t>("qattr", (size_t)std::max<int64_t>(Q, 1));
+    a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
+    a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
+    a.vcw = arena.get<uint64_t>("gp_vcw", (size_t)std::max<int64_t>(V, 1));
+    a.V = V;
+    a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
+    MDMC_HIP(hipEventRecord(kev[2], stream));
+    launchSlabEmit(shape, a, stream);
+    MDMC_HIP(hipEventRecord(kev[3], stream));
+    dmcV = V;
+    dmcQ = Q;
+}
+
 void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint) {
     vertHeV = vertHeQ = 0;
     batchVertHe = false;
@@ -2071,8 +2139,8 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
         if (it != tuned.end()) {
             lastShape = it->second;
         } else {
-            // the XCD-chunked shapes (1-3) move less data out of L2 (about half, measured at
-            // 512^3); a launch-order shape must beat them by 3% to be chosen
+            // the XCD-chunked shapes (1-3, 7, 8) move less data out of L2 (about half for the
+            // tiles, measured at 512^3); a launch-order shape must beat them by 3% to be chosen
             double best = 1e30;
             for (int sh = GP_SHAPE_FIRST; sh <= GP_SHAPE_LAST; ++sh) {
                 gridPass(problems, C, sh, surfIter, recordAdjoint);
@@ -2080,7 +2148,7 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
                 double ms = 0.0;
                 kernelTimes(&ms);
                 tuneMs[sh] = ms;
-                const double cost = sh <= 3 ? ms : ms * 1.03;
+                const double cost = (sh <= 3 || gridPassIsSlab(sh)) ? ms : ms * 1.03;
                 if (cost < best) {
                     best = cost;
                     lastShape = sh;
@@ -2095,6 +2163,10 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
 
 // classify -> scan of the tile counts -> exact output sizes -> emit
 void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint) {
+    if (gridPassIsSlab(shape)) {
+        slabPass(problems, C, shape, surfIter, recordAdjoint);
+        return;
+    }
     const int T = gridPassTile(shape);
     int64_t numTiles = 0;
     for (ProblemDev& pr : problems) {
@@ -2143,7 +2215,7 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     a.qattr = arena.get<uint32_t>("qattr", (size_t)std::max<int64_t>(Q, 1));
     a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
     a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
-    a.vcontour = arena.get<uint8_t>("gp_vcontour", (size_t)std::max<int64_t>(V, 1));
+    a.vcw = arena.get<uint64_t>("gp_vcw", (size_t)std::max<int64_t>(V, 1));
     a.V = V;
     a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
     MDMC_HIP(hipEventRecord(kev[2], stream));

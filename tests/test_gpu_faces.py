@@ -142,7 +142,10 @@ def test_maxmin_triangulation_and_quality(name):
     chosen = np.where(is13, m13, m02)
     other = np.where(is13, m02, m13)
     good = np.isfinite(chosen) & np.isfinite(other)
-    assert (chosen[good] >= other[good] - 1e-4).all(), (chosen - other)[good].min()
+    # the kernel's angles are acosf of a float32 cosine: an error of a few ulp in the cosine
+    # moves an angle t by about ulp / sin(t), large for thin triangles
+    tol = 1e-5 + 2e-6 / np.maximum(np.sin(np.minimum(chosen, other)), 1e-6)
+    assert (chosen[good] >= other[good] - tol[good]).all(), (chosen - other + tol)[good].min()
     T = tri.reshape(-1, 3).astype(np.int64)
     expect = mean_ratio(P[T[:, 0]], P[T[:, 1]], P[T[:, 2]])
     ok = np.isfinite(expect)

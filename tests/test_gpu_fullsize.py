@@ -76,10 +76,13 @@ def test_fullsize_structure_and_statistics(field):
 
 
 def test_fullsize_deterministic_and_independent_of_tile_shape(field):
-    print("gyroid 512^3: shapes 1, 1, 5", flush=True)
+    print("gyroid 512^3: shapes 1, 1, 5, 7, 8", flush=True)
     a, _ = extract(field, 1)
     b, _ = extract(field, 1)
     c, _ = extract(field, 5)
-    da, db, dc = digest(a), digest(b), digest(c)
+    d, _ = extract(field, 7)
+    e, _ = extract(field, 8)
+    da, db, dc, dd, de = digest(a), digest(b), digest(c), digest(d), digest(e)
     assert da == db
     assert da == dc
+    assert da == dd and da == de
