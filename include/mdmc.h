@@ -177,6 +177,16 @@ mdmc_status mdmc_batch_stats(const mdmc_ctx* ctx, mdmc_stats* per_problem);
  * to dL_dfield instead of overwriting it.  Asynchronous on the context's stream; deterministic. */
 mdmc_status mdmc_backward(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield, int32_t accumulate);
 
+/* No reference counterpart: mdmc_backward for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same result bit for bit; it calls no hipMalloc / hipFree and no allocating
+ * routine.  Its scratch (8 floats per dual vertex, plus one per point of the padded / refined
+ * grid when padding or refinement was used) is a fixed 32 Mi-float (128 MiB) array per device,
+ * held until the process ends; larger problems get MDMC_ERR_CAPACITY.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several streams or contexts at once)
+ * race, so serialise them.  Asynchronous on the context's stream. */
+mdmc_status mdmc_backward_noalloc(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield, int32_t accumulate);
+
 /* Triangle index buffer, two triangles per face (x,y,z),(z,w,x): HalfedgeMesh.cu:690-714.
  * indices uint32[6*nF], device or host. */
 mdmc_status mdmc_mesh_triangles(const mdmc_ctx* ctx, uint32_t* indices, int32_t dst_on_device);

@@ -103,7 +103,8 @@ _lib = None
 EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", "mdmc_extract",
            "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_mesh_render_vertices", "mdmc_backward",
            "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape",
-           "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin"]
+           "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin",
+           "mdmc_backward_noalloc"]
 
 
 def lib():
@@ -151,6 +152,8 @@ def lib():
         L.mdmc_mesh_render_vertices.restype = i32
         L.mdmc_backward.argtypes = [vp, vp, vp, i32]
         L.mdmc_backward.restype = i32
+        L.mdmc_backward_noalloc.argtypes = [vp, vp, vp, i32]
+        L.mdmc_backward_noalloc.restype = i32
         L.mdmc_extract_batch.argtypes = [vp, ctypes.POINTER(_Grid), i32, vp, i32, ctypes.POINTER(_Options), ctypes.POINTER(Stats)]
         L.mdmc_extract_batch.restype = i32
         L.mdmc_batch_offsets.argtypes = [vp, vp, vp, vp]
@@ -377,6 +380,11 @@ class DualMarchingCubes:
         """mdmc_backward: device dL/d(vertex position) [V,3] -> device dL/d(field) of the
         last extraction, which must have used RECORD_ADJOINT."""
         self._check(lib().mdmc_backward(self._ctx, ctypes.c_void_p(dpos_ptr), ctypes.c_void_p(dfield_ptr), 1 if accumulate else 0))
+
+    def backward_noalloc(self, dpos_ptr, dfield_ptr, accumulate=False):
+        """mdmc_backward_noalloc: mdmc_backward with its scratch in the library's fixed per-device
+        array (no device allocation; MdmcError with status MDMC_ERR_CAPACITY past its size)."""
+        self._check(lib().mdmc_backward_noalloc(self._ctx, ctypes.c_void_p(dpos_ptr), ctypes.c_void_p(dfield_ptr), 1 if accumulate else 0))
 
     def copy_device(self, positions=0, normals=0, halfedges=0, face_halfedge=0, face_attr=0, vertex_flags=0, vertex_halfedge=0):
         """mdmc_mesh_copy into device buffers given by pointer (0 = skip)."""

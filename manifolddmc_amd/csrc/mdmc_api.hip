@@ -41,6 +41,16 @@ struct mdmc_ctx {
 
 namespace {
 
+// Scratch of mdmc_backward_noalloc (the 8 corner contributions of every dual vertex, then the
+// gradient on the padded / refined grid): at most kNoAllocFloats floats (128 MiB) per device,
+// held from the library's load until the process ends.  Host code never passes `scratch`
+// itself to a kernel: its address on each device comes from hipGetSymbolAddress in
+// mdmc_create (with that device current) and is kept in g_noallocScratch.
+constexpr size_t kNoAllocFloats = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+float* g_noallocScratch[kMaxDevices] = {};
+
 mdmc_status fail(mdmc_ctx* ctx, mdmc_status st, const std::string& msg) {
     if (ctx) ctx->err = msg;
     return st;
@@ -80,10 +90,11 @@ mdmc_status mdmc_create(mdmc_ctx** out, int32_t device) {
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return MDMC_ERR_HIP;
     int dev = device;
     if (dev < 0) (void)hipGetDevice(&dev);
-    if (dev >= count) return MDMC_ERR_INVALID_ARG;
+    if (dev >= count || dev >= kMaxDevices) return MDMC_ERR_INVALID_ARG;
     DeviceGuard g(dev);
     {
-        // per-device constant tables of the grid pass, uploaded once per process
+        // per device, once per process: the grid pass's constant tables, and the address of
+        // this device's copy of the no-allocation scratch array
         static std::mutex mu;
         static std::vector<char> ready;
         std::lock_guard<std::mutex> lock(mu);
@@ -94,6 +105,9 @@ mdmc_status mdmc_create(mdmc_ctx** out, int32_t device) {
             } catch (const std::exception&) {
                 return MDMC_ERR_HIP;
             }
+            float* ptr = nullptr;
+            if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MDMC_ERR_HIP;
+            g_noallocScratch[dev] = ptr;
             ready[dev] = 1;
         }
     }
@@ -407,6 +421,41 @@ mdmc_status mdmc_batch_stats(const mdmc_ctx* ctx, mdmc_stats* per_problem) {
         st.anomalies = -1;
         st.cells = -1;
     }
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_backward_noalloc(mdmc_ctx* ctx, const float* dL_dpos, float* dL_dfield, int32_t accumulate) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult || !ctx->adj.valid)
+        return fail(ctx, MDMC_ERR_NO_RESULT, "mdmc_backward_noalloc needs a preceding mdmc_extract with MDMC_RECORD_ADJOINT");
+    const AdjointState& a = ctx->adj;
+    if (a.V > 0 && !dL_dpos) return fail(ctx, MDMC_ERR_INVALID_ARG, "null dL_dpos");
+    if (!dL_dfield) return fail(ctx, MDMC_ERR_INVALID_ARG, "null dL_dfield");
+    float* base = g_noallocScratch[ctx->device];
+    if (!base) return fail(ctx, MDMC_ERR_INTERNAL, "no-allocation scratch not set up for this device");
+    const bool direct = a.pad == 0 && a.refinement == 0;
+    const int64_t P = (int64_t)a.outDims[0] * a.outDims[1] * a.outDims[2];
+    const int64_t need = 8 * a.V + (direct ? 0 : P);
+    if (need > (int64_t)kNoAllocFloats)
+        return fail(ctx, MDMC_ERR_CAPACITY, "mdmc_backward_noalloc: 8 floats per dual vertex plus one per padded / refined grid point exceed "
+                                            "its scratch of 32 Mi floats (128 MiB); use mdmc_backward");
+    DeviceGuard dg(ctx->device);
+    mdmc::Pipeline& p = ctx->p;
+    hipStream_t s = p.stream;
+    const int32_t* map = p.arena.ptr<int32_t>("adj_map");
+    const int32_t* cell = p.arena.ptr<int32_t>("gp_vcell");
+    // contributions at the start of the scratch, the processed-grid gradient after them (both
+    // written in full by their kernels, so nothing from an earlier call needs clearing)
+    float* contrib = base;
+    mdmc::launchAdjointVertex(a.g, a.iso, map, cell, a.V, dL_dpos, contrib, s);
+    if (direct) {
+        mdmc::launchAdjointGather(a.g, map, contrib, dL_dfield, accumulate ? 1 : 0, s);
+    } else {
+        float* dproc = base + 8 * a.V;
+        mdmc::launchAdjointGather(a.g, map, contrib, dproc, 0, s);
+        mdmc::launchPrepareAdjoint(a.inDims, a.outDims, a.refinement, a.pad, dproc, dL_dfield, accumulate ? 1 : 0, s);
+    }
+    if (hipGetLastError() != hipSuccess) return fail(ctx, MDMC_ERR_HIP, "mdmc_backward_noalloc: kernel launch failed");
     return MDMC_OK;
 }
 
