@@ -161,6 +161,17 @@ mdmc_status mdmc_mesh_copy(const mdmc_ctx* ctx, float* positions, float* normals
  * one pipeline.  stats (may be NULL) holds totals.  Synchronises the stream. */
 mdmc_status mdmc_extract_batch(mdmc_ctx* ctx, const mdmc_grid* grids, int32_t n_grids, const float* isos, int32_t isos_per_grid,
                                const mdmc_options* opt, mdmc_stats* stats);
+/* Graph replay of the batched form (no reference counterpart).  For a batch whose grids are
+ * all on the device, once its tile shape is chosen, the part of the pipeline that the grid
+ * shapes alone determine -- the descriptor uploads, padding / refinement, classification and
+ * the tile-count scans -- is captured into a HIP graph on the second call with the same
+ * shapes, options and buffers, and replayed (with the new calls' grid pointers and iso-values)
+ * on every later one; the stages after it size their buffers from the counts and run eagerly.
+ * Results are the same with or without.  enable = 0 turns it off and releases the graph
+ * (default on).  mdmc_get_graph_replays: how many extractions of this context ran the graph. */
+mdmc_status mdmc_set_graphs(mdmc_ctx* ctx, int32_t enable);
+mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays);
+
 /* Per-problem offsets of the resident result (host arrays of n_problems + 1; any may be NULL).
  * After mdmc_extract (one problem) they are {0, n}. */
 mdmc_status mdmc_batch_offsets(const mdmc_ctx* ctx, int64_t* vertex_offsets, int64_t* halfedge_offsets, int64_t* face_offsets);

@@ -104,7 +104,7 @@ EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", 
            "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_mesh_render_vertices", "mdmc_backward",
            "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape",
            "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin",
-           "mdmc_backward_noalloc"]
+           "mdmc_backward_noalloc", "mdmc_set_graphs", "mdmc_get_graph_replays"]
 
 
 def lib():
@@ -153,6 +153,10 @@ def lib():
         L.mdmc_backward.argtypes = [vp, vp, vp, i32]
         L.mdmc_backward.restype = i32
         L.mdmc_backward_noalloc.argtypes = [vp, vp, vp, i32]
+        L.mdmc_set_graphs.argtypes = [vp, i32]
+        L.mdmc_set_graphs.restype = i32
+        L.mdmc_get_graph_replays.argtypes = [vp, ctypes.POINTER(i64)]
+        L.mdmc_get_graph_replays.restype = i32
         L.mdmc_backward_noalloc.restype = i32
         L.mdmc_extract_batch.argtypes = [vp, ctypes.POINTER(_Grid), i32, vp, i32, ctypes.POINTER(_Options), ctypes.POINTER(Stats)]
         L.mdmc_extract_batch.restype = i32
@@ -295,6 +299,15 @@ class DualMarchingCubes:
         ms = (ctypes.c_double * len(TILE_SHAPES))()
         self._check(lib().mdmc_get_tile_shape(self._ctx, ctypes.byref(sh), ms))
         return sh.value, {k: ms[k - 1] for k in TILE_SHAPES}
+
+    def set_graphs(self, enable):
+        """mdmc_set_graphs: replay the batched form's shape-determined front from a HIP graph."""
+        self._check(lib().mdmc_set_graphs(self._ctx, 1 if enable else 0))
+
+    def graph_replays(self):
+        n = ctypes.c_int64()
+        self._check(lib().mdmc_get_graph_replays(self._ctx, ctypes.byref(n)))
+        return n.value
 
     def set_stream(self, stream_ptr):
         st = lib().mdmc_set_stream(self._ctx, ctypes.c_void_p(stream_ptr))

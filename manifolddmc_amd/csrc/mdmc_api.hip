@@ -267,14 +267,32 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
                 problems.push_back(pr);
             }
         }
-        {
-            mdmc::PrepareDesc* pd = p.arena.get<mdmc::PrepareDesc>("prep_desc", (size_t)G);
+        const bool surf = (flags & MDMC_SURF_ITER) != 0;
+        mdmc::PrepareDesc* pd = p.arena.get<mdmc::PrepareDesc>("prep_desc", (size_t)G);
+        // A batch of grids already on the device, with a tile shape chosen: the shape-determined
+        // front (padding, classification, tile-count scans) goes through a graph captured once
+        // for these shapes and replayed (stage times then count it under the grid pass)
+        const int64_t C = p.dmcSetup(problems, record);
+        const int shapeNow = p.resolveShape(C, surf);
+        const bool graphable = p.useGraphs && B > 1 && PinTot == 0 && !record && shapeNow != mdmc::GP_SHAPE_AUTO && !mdmc::gridPassIsSlab(shapeNow);
+        std::vector<int64_t> shapeSig;
+        if (graphable) {
+            shapeSig = {G, I, r, pad, surf ? 1 : 0, shapeNow, prepBlocks};
+            for (int gi = 0; gi < G; ++gi)
+                for (int k = 0; k < 3; ++k) shapeSig.push_back(grids[gi].dims[k]);
+        }
+        if (graphable && p.runFrontGraph(shapeSig, prep.data(), (size_t)G * sizeof(mdmc::PrepareDesc), pd, G, prepBlocks, r, pad, problems, C,
+                                         shapeNow, surf)) {
+            MDMC_HIP(hipEventRecord(ctx->ev[1], s));
+            p.gridPassBack(C, shapeNow, record);
+        } else {
             p.upload(pd, prep.data(), (size_t)G * sizeof(mdmc::PrepareDesc), 1);
             mdmc::launchPrepareGrids(pd, G, prepBlocks, r, pad, s);
+            MDMC_HIP(hipEventRecord(ctx->ev[1], s));
+            // ---- DMC
+            p.dmc(problems, surf, record);
+            if (graphable) p.noteEagerFront(shapeSig);
         }
-        MDMC_HIP(hipEventRecord(ctx->ev[1], s));
-        // ---- DMC
-        p.dmc(problems, (flags & MDMC_SURF_ITER) != 0, record);
         if (B > 1) p.problemDmcCounts(B);
         MDMC_HIP(hipEventRecord(ctx->ev[2], s));
         int64_t st9[9] = {0};
@@ -616,6 +634,22 @@ mdmc_status mdmc_mesh_triangulate_maxmin(const mdmc_ctx* cctx, uint32_t* indices
     } catch (const std::exception& e) {
         return fromException(ctx, e);
     }
+}
+
+mdmc_status mdmc_set_graphs(mdmc_ctx* ctx, int32_t enable) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    ctx->p.useGraphs = enable != 0;
+    if (!enable) {
+        DeviceGuard dg(ctx->device);
+        ctx->p.releaseGraphs();
+    }
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays) {
+    if (!ctx || !replays) return MDMC_ERR_INVALID_ARG;
+    *replays = ctx->p.graphReplays;
+    return MDMC_OK;
 }
 
 }  // extern "C"

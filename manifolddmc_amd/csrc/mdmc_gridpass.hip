@@ -336,16 +336,32 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
     const int CX = P.CX, CY = P.CY;
     const int64_t C = P.cells;
 
-    // MC case of every cell; a ballot per run of 64 cells marks the active ones
+    // MC case of every cell; a ballot per run of 64 cells marks the active ones.  This thread's
+    // cells are GP_THREADS apart: their coordinates advance from the first one's, with no
+    // division per cell.
+    int cx, cy, cz;
+    cellXYZ(tc.tb, tid, CX, CY, cx, cy, cz);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int l = k * GP_THREADS + tid;
         int mc = 0;
+        if (k > 0) {
+            if (CX >= GP_THREADS) {  // at most one row boundary per step
+                cx += GP_THREADS;
+                if (cx >= CX) {
+                    cx -= CX;
+                    if (++cy == CY) {
+                        cy = 0;
+                        ++cz;
+                    }
+                }
+            } else {
+                cellXYZ(tc.tb, l, CX, CY, cx, cy, cz);
+            }
+        }
         if (tc.c0 + l < C) {
-            int x, y, z;
-            cellXYZ(tc.tb, l, CX, CY, x, y, z);
             float s[8];
-            loadCorners(g, x, y, z, s);
+            loadCorners(g, cx, cy, cz, s);
 #pragma unroll
             for (int q = 0; q < 8; ++q) mc |= (s[q] >= iso ? 1 : 0) << q;
         }
@@ -820,16 +836,32 @@ __global__ void __launch_bounds__(SLAB_COLS) kSlab(GridPassArgs a, int chunked) 
 // gyroid), so every lane runs at most kMoveSteps steps; the block's unfinished lanes leave
 // their state in LDS, consecutive lanes finish them on full waves, and each vertex then
 // computes its normal from its final point.  Each vertex's arithmetic is unchanged.
+//
+// The edge crossings t = (iso - s_a) / (s_b - s_a) of the block's contours are computed as a
+// flat list of (vertex, edge) items on every lane (a contour has 3 to 12 edges, and a lane
+// looping over its own would keep a wave busy for its longest contour); each vertex then
+// adds its crossings' offsets in contour order, the same operations in the same order.
 constexpr int kMoveSteps = 2;
 constexpr int kMoveState = 17;  // s[8], iso, i1, v1, point, step
+constexpr int kMaxItems = 12 * GP_THREADS;
 
 __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chunked) {
-    __shared__ float s_st[kMoveState][GP_THREADS];
+    // the crossings (items) and the move-to-surface continuation state are live one after the other
+    __shared__ union {
+        struct {
+            float tt[kMaxItems];
+            uint16_t item[kMaxItems];  // vertex slot | edge << 8
+        } e;
+        float st[kMoveState][GP_THREADS];
+    } s_u;
+    __shared__ float s_cs[8][GP_THREADS];  // corner values of each vertex slot
+    __shared__ float s_iso[GP_THREADS];
     __shared__ int s_wave[GP_WAVES];
+    auto& s_st = s_u.st;
     const int64_t nb = (a.V + GP_THREADS - 1) / GP_THREADS;
     const int64_t v = tileOfBlock(blockIdx.x, nb, chunked != 0) * GP_THREADS + threadIdx.x;
     const bool live = v < a.V;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     GridDev g{};
     float iso = 0.f;
     int x = 0, y = 0, z = 0;
@@ -837,6 +869,7 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
     Mover m{};
     m.done = true;
     f3 p = mk3(0.f, 0.f, 0.f);
+    uint64_t cw = 0;
     if (live) {
         const ProblemDev& P = a.probs[a.vpid ? a.vpid[v] : 0];
         g = P.g;
@@ -847,7 +880,53 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
         y = (int)(r % (unsigned)P.CY);
         z = (int)(r / (unsigned)P.CY);
         loadCorners(g, x, y, z, s);
-        p = edgeMean(s, iso, a.vcw[v]);
+        cw = a.vcw[v];
+    }
+    // the block's items: an exclusive scan of the contour sizes gives each vertex its run
+    const int sz = (int)(uint32_t)(cw >> 60);
+    {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s_cs[k][tid] = s[k];
+        s_iso[tid] = iso;
+        int xs = sz;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y2 = __shfl_up(xs, d, 64);
+            if (lane >= d) xs += y2;
+        }
+        if (lane == 63) s_wave[wid] = xs;
+        __syncthreads();
+        int base = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < GP_WAVES; ++w) {
+            if (w < wid) base += s_wave[w];
+            total += s_wave[w];
+        }
+        const int off = base + xs - sz;
+        for (int i = 0; i < sz; ++i) s_u.e.item[off + i] = (uint16_t)(tid | (int)(((cw >> (4 * i)) & 0xF) << 8));
+        __syncthreads();
+        for (int it = tid; it < total; it += GP_THREADS) {
+            const int itm = s_u.e.item[it];
+            const int vs = itm & 0xFF, e = itm >> 8;
+            const float sa = s_cs[edgeA(e)][vs], sb = s_cs[edgeB(e)][vs];
+            s_u.e.tt[it] = (s_iso[vs] - sa) / (sb - sa);
+        }
+        __syncthreads();
+        // p = the mean of the crossings' cell-local points: component k of edge e's point is
+        // 0 + (t * [axis == k] + corner_k), i.e. 0 + ((axis == k ? t : t * 0) + corner_k)
+        for (int i = 0; i < sz; ++i) {
+            const int e = (int)((cw >> (4 * i)) & 0xF);
+            const int ca = edgeA(e), ax = edgeAxis(e);
+            const float tt = s_u.e.tt[off + i];
+            const float t0 = tt * 0.f;
+            p.x = p.x + (0.f + ((ax == 0 ? tt : t0) + (float)(ca & 1)));
+            p.y = p.y + (0.f + ((ax == 1 ? tt : t0) + (float)((ca >> 1) & 1)));
+            p.z = p.z + (0.f + ((ax == 2 ? tt : t0) + (float)((ca >> 2) & 1)));
+        }
+        if (live) p = divs(p, (float)sz);
+        __syncthreads();  // s_u turns into the continuation state below
+    }
+    if (live) {
         if (a.surfIter) {
             moverInit(m, s, p);
             for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);

@@ -58,6 +58,26 @@ class Pipeline {
     // cell counts are filled in); recordAdjoint keeps the cell -> vertex map ("adj_map") that
     // the adjoint reads with the vertex -> cell array ("gp_vcell") (one problem only)
     void dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint = false);
+    // the grid pass in parts, for the batched form's graph: problem geometry (no GPU work), the
+    // tuned or fixed shape (GP_SHAPE_AUTO while a tuning is due), and the shape-determined front
+    // (padding, classification, tile-count scans) captured once and replayed for later calls
+    // with the same shapes and buffers (false: not replayable yet, run it eagerly), then the rest
+    int64_t dmcSetup(std::vector<ProblemDev>& problems, bool recordAdjoint);
+    int64_t assignTiles(std::vector<ProblemDev>& problems, int shape);
+    void gridPassFront(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, void* stagedProbs, bool events);
+    void gridPassBack(int64_t C, int shape, bool recordAdjoint);
+    int shapeClass(int64_t C, bool surfIter) const;
+    int resolveShape(int64_t C, bool surfIter);
+    bool runFrontGraph(const std::vector<int64_t>& shapeSig, const void* prep, size_t prepBytes, void* prepDev, int G, int64_t prepBlocks,
+                       int refinement, int pad, std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter);
+    void noteEagerFront(const std::vector<int64_t>& shapeSig) {
+        fgLastEager_ = shapeSig;
+        const std::vector<int64_t> b = frontBuffers();
+        fgLastEager_.insert(fgLastEager_.end(), b.begin(), b.end());
+    }
+    void releaseGraphs();
+    bool useGraphs = true;
+    int64_t graphReplays = 0;
     int nProblems = 1;
     ProblemDev* probsDev = nullptr;
     // per vertex its problem (several problems), else nullptr
@@ -121,6 +141,18 @@ class Pipeline {
     void launchFill(void* p, uint8_t byte, size_t bytes);
     void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
     void slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
+    GridPassArgs ga_{};
+    // the captured front of a batched extraction: the graph, the shapes / options / buffer
+    // addresses it was captured with, its pinned descriptor staging, its kernel arguments
+    struct FrontGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<int64_t> sig;
+        void* stage = nullptr;
+        size_t stageCap = 0;
+    } fg_;
+    GridPassArgs fgArgs_{};
+    std::vector<int64_t> fgLastEager_;
+    std::vector<int64_t> frontBuffers();
     void buildBuckets(const int32_t* list, int64_t n);
     int32_t* bucketPassValence(const int64_t* off, int64_t nV);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);

@@ -1952,6 +1952,7 @@ __global__ void kNarrow8(const uint32_t* __restrict__ in, int64_t n, uint8_t* __
 
 Pipeline::Pipeline() {}
 Pipeline::~Pipeline() {
+    releaseGraphs();
     for (auto& e : kev)
         if (e) (void)hipEventDestroy(e);
     if (pinH_) (void)hipHostFree(pinH_);
@@ -2111,7 +2112,7 @@ void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     dmcQ = Q;
 }
 
-void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint) {
+int64_t Pipeline::dmcSetup(std::vector<ProblemDev>& problems, bool recordAdjoint) {
     vertHeV = vertHeQ = 0;
     batchVertHe = false;
     nProblems = (int)problems.size();
@@ -2127,38 +2128,123 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
     }
     if (!kev[0])
         for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
+    return C;
+}
+
+// the autotuner's workload class: cell count and problem count to the next power of two,
+// move-to-surface on / off
+int Pipeline::shapeClass(int64_t C, bool surfIter) const {
+    int lc = 0, lp = 0;
+    while (((int64_t)1 << lc) < C + 1) ++lc;
+    while ((1 << lp) < nProblems) ++lp;
+    return (lc << 16) | (lp << 1) | (surfIter ? 1 : 0);
+}
+
+int Pipeline::resolveShape(int64_t C, bool surfIter) {
     lastShape = tileShape;
     if (lastShape == GP_SHAPE_AUTO) {
-        // autotuner: on the first extraction of a workload class, time every tile shape on
-        // this very input and keep the fastest for the class
-        int lc = 0, lp = 0;
-        while (((int64_t)1 << lc) < C + 1) ++lc;
-        while ((1 << lp) < nProblems) ++lp;
-        const int key = (lc << 16) | (lp << 1) | (surfIter ? 1 : 0);
-        auto it = tuned.find(key);
-        if (it != tuned.end()) {
-            lastShape = it->second;
-        } else {
-            // the XCD-chunked shapes (1-3, 7, 8) move less data out of L2 (about half for the
-            // tiles, measured at 512^3); a launch-order shape must beat them by 3% to be chosen
-            double best = 1e30;
-            for (int sh = GP_SHAPE_FIRST; sh <= GP_SHAPE_LAST; ++sh) {
-                gridPass(problems, C, sh, surfIter, recordAdjoint);
-                MDMC_HIP(hipEventSynchronize(kev[3]));
-                double ms = 0.0;
-                kernelTimes(&ms);
-                tuneMs[sh] = ms;
-                const double cost = (sh <= 3 || gridPassIsSlab(sh)) ? ms : ms * 1.03;
-                if (cost < best) {
-                    best = cost;
-                    lastShape = sh;
-                }
+        auto it = tuned.find(shapeClass(C, surfIter));
+        if (it != tuned.end()) lastShape = it->second;
+    }
+    return lastShape;
+}
+
+void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool recordAdjoint) {
+    const int64_t C = dmcSetup(problems, recordAdjoint);
+    if (resolveShape(C, surfIter) == GP_SHAPE_AUTO) {
+        // autotuner: on the first extraction of a workload class, time every shape on this
+        // very input and keep the fastest for the class.  The XCD-chunked shapes (1-3, 7, 8)
+        // move less data out of L2 (about half for the tiles, measured at 512^3); a
+        // launch-order shape must beat them by 3% to be chosen.
+        double best = 1e30;
+        for (int sh = GP_SHAPE_FIRST; sh <= GP_SHAPE_LAST; ++sh) {
+            gridPass(problems, C, sh, surfIter, recordAdjoint);
+            MDMC_HIP(hipEventSynchronize(kev[3]));
+            double ms = 0.0;
+            kernelTimes(&ms);
+            tuneMs[sh] = ms;
+            const double cost = (sh <= 3 || gridPassIsSlab(sh)) ? ms : ms * 1.03;
+            if (cost < best) {
+                best = cost;
+                lastShape = sh;
             }
-            tuned[key] = lastShape;
-            if (lastShape == GP_SHAPE_LAST) return;  // the last candidate's output is in place
         }
+        tuned[shapeClass(C, surfIter)] = lastShape;
+        if (lastShape == GP_SHAPE_LAST) return;  // the last candidate's output is in place
     }
     gridPass(problems, C, lastShape, surfIter, recordAdjoint);
+}
+
+// addresses of every device buffer the front of a tile-shape grid pass touches
+std::vector<int64_t> Pipeline::frontBuffers() {
+    std::vector<int64_t> b;
+    for (const char* n : {"grid", "prep_desc", "gp_probs", "gp_rpoly", "gp_rinfo", "gp_tv", "gp_tq", "gp_tna", "gp_map", "gp_tvo", "gp_tqo",
+                          "scan_scratch", "scan_scratch2"})
+        b.push_back((int64_t)(intptr_t)arena.ptr<char>(n));
+    b.push_back((int64_t)(intptr_t)pinDev());
+    return b;
+}
+
+void Pipeline::releaseGraphs() {
+    if (fg_.exec) (void)hipGraphExecDestroy(fg_.exec);
+    fg_.exec = nullptr;
+    fg_.sig.clear();
+    if (fg_.stage) (void)hipHostFree(fg_.stage);
+    fg_.stage = nullptr;
+    fg_.stageCap = 0;
+}
+
+bool Pipeline::runFrontGraph(const std::vector<int64_t>& shapeSig, const void* prep, size_t prepBytes, void* prepDev, int G, int64_t prepBlocks,
+                             int refinement, int pad, std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter) {
+    std::vector<int64_t> sig = shapeSig;
+    const std::vector<int64_t> bufs = frontBuffers();
+    sig.insert(sig.end(), bufs.begin(), bufs.end());
+    const size_t pbytes = problems.size() * sizeof(ProblemDev);
+    const size_t probOff = (prepBytes + 255) & ~(size_t)255;
+    if (fg_.exec && fg_.sig == sig) {
+        // replay: the descriptors of this call into the graph's staging, then the graph
+        assignTiles(problems, shape);
+        std::memcpy(fg_.stage, prep, prepBytes);
+        std::memcpy((char*)fg_.stage + probOff, problems.data(), pbytes);
+        probsDev = arena.ptr<ProblemDev>("gp_probs");
+        ga_ = fgArgs_;
+        MDMC_HIP(hipEventRecord(kev[0], stream));
+        MDMC_HIP(hipGraphLaunch(fg_.exec, stream));
+        MDMC_HIP(hipEventRecord(kev[1], stream));
+        ++graphReplays;
+        return true;
+    }
+    if (sig != fgLastEager_) return false;  // first call with these shapes and buffers: run it eagerly
+    // second call with the same shapes and no buffer moved since: capture the front once
+    releaseGraphs();
+    fg_.stageCap = probOff + pbytes;
+    MDMC_HIP(hipHostMalloc(&fg_.stage, fg_.stageCap, hipHostMallocDefault));
+    std::memcpy(fg_.stage, prep, prepBytes);
+    hipGraph_t graph = nullptr;
+    MDMC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    try {
+        MDMC_HIP(hipMemcpyAsync(prepDev, fg_.stage, prepBytes, hipMemcpyHostToDevice, stream));
+        launchPrepareGrids((const PrepareDesc*)prepDev, G, prepBlocks, refinement, pad, stream);
+        gridPassFront(problems, C, shape, surfIter, (char*)fg_.stage + probOff, false);
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        releaseGraphs();
+        throw;
+    }
+    MDMC_HIP(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MDMC_HIP(e);
+    fg_.exec = exec;
+    fg_.sig = sig;
+    fgArgs_ = ga_;
+    MDMC_HIP(hipEventRecord(kev[0], stream));
+    MDMC_HIP(hipGraphLaunch(fg_.exec, stream));
+    MDMC_HIP(hipEventRecord(kev[1], stream));
+    ++graphReplays;
+    return true;
 }
 
 // classify -> scan of the tile counts -> exact output sizes -> emit
@@ -2167,15 +2253,37 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
         slabPass(problems, C, shape, surfIter, recordAdjoint);
         return;
     }
+    gridPassFront(problems, C, shape, surfIter, nullptr, true);
+    gridPassBack(C, shape, recordAdjoint);
+}
+
+int64_t Pipeline::assignTiles(std::vector<ProblemDev>& problems, int shape) {
     const int T = gridPassTile(shape);
     int64_t numTiles = 0;
     for (ProblemDev& pr : problems) {
         pr.tileBase = numTiles;
         numTiles += (pr.cells + T - 1) / T;
     }
+    return numTiles;
+}
+
+// The part of a tile-shape grid pass that the shapes alone determine: the problems' upload,
+// classification and the tile-count scans, with no host synchronisation.  stagedProbs: copy
+// the problems from that pinned buffer (a graph capture; the caller fills it before every
+// replay) instead of through upload().  events: time the part with kev[0] / kev[1].
+void Pipeline::gridPassFront(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, void* stagedProbs, bool events) {
+    const int T = gridPassTile(shape);
+    const int64_t numTiles = assignTiles(problems, shape);
     probsDev = arena.get<ProblemDev>("gp_probs", problems.size());
-    upload(probsDev, problems.data(), problems.size() * sizeof(ProblemDev), 0);
-    GridPassArgs a;
+    const size_t pbytes = problems.size() * sizeof(ProblemDev);
+    if (stagedProbs) {
+        std::memcpy(stagedProbs, problems.data(), pbytes);
+        MDMC_HIP(hipMemcpyAsync(probsDev, stagedProbs, pbytes, hipMemcpyHostToDevice, stream));
+    } else {
+        upload(probsDev, problems.data(), pbytes, 0);
+    }
+    GridPassArgs& a = ga_;
+    a = GridPassArgs{};
     a.probs = probsDev;
     a.nProblems = nProblems;
     a.surfIter = surfIter ? 1 : 0;
@@ -2193,16 +2301,22 @@ void Pipeline::gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     void* scratchV = arena.get<char>("scan_scratch", scanScratchBytes(numTiles + 1));
     void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(numTiles + 1));
     // totals straight into pinned host memory: [0] direct-sum diagnostic, [1] V, [2] Q
-    int64_t* h = pinHost();
     int64_t* pd = pinDev();
-    h[0] = h[1] = h[2] = 0;
-    MDMC_HIP(hipEventRecord(kev[0], stream));
+    if (events) MDMC_HIP(hipEventRecord(kev[0], stream));
     launchGridClassify(shape, a, stream);
     if (numTiles > 0) {
         scanExclusiveI32(a.tileV, tvo, numTiles + 1, pd + 1, scratchV, stream, pd);
         scanExclusiveI32(a.tileQ, tqo, numTiles + 1, pd + 2, scratchQ, stream, pd);
+    } else {
+        zero(pd + 1, 2 * sizeof(int64_t));
     }
-    MDMC_HIP(hipEventRecord(kev[1], stream));
+    if (events) MDMC_HIP(hipEventRecord(kev[1], stream));
+}
+
+// the rest: read the totals back, size the outputs, emit quads and dual vertices
+void Pipeline::gridPassBack(int64_t C, int shape, bool recordAdjoint) {
+    GridPassArgs& a = ga_;
+    int64_t* h = pinHost();
     MDMC_HIP(hipStreamSynchronize(stream));
     V = h[1];
     Q = h[2];

@@ -95,3 +95,55 @@ def test_batch_with_empty_problems_and_one_grid():
     finally:
         d.close()
         one.close()
+
+
+def test_batch_graph_replays_equal_eager_batches():
+    """The batched form's front (padding, classification, tile scans) is captured on the second
+    call with the same shapes and replayed from the third on; every call, including replays
+    with new field values and iso-values (same shapes) and with other extractions in between,
+    equals the same batch run with graphs off."""
+    import torch
+
+    import manifolddmc_amd as mdmc
+    from manifolddmc_amd import grids
+    rng = np.random.default_rng(5)
+    shapes = [tuple(int(v) for v in rng.integers(12, 40, size=3)) for _ in range(6)]
+
+    def fields(seed):
+        out = []
+        for i, (nz, ny, nx) in enumerate(shapes):
+            f = grids.gyroid(max(nx, ny, nz), periods=2 + i % 3, noise=0.05, seed=seed + i)[:nz, :ny, :nx]
+            out.append(torch.from_numpy(np.ascontiguousarray(f, np.float32)).cuda())
+        return out
+
+    flags = oa.PAD1 | oa.SURF_ITER | oa.ALL_POST
+    g = mdmc.DualMarchingCubes()
+    ref = mdmc.DualMarchingCubes()
+    other = mdmc.DualMarchingCubes()
+    ref.set_graphs(False)
+    try:
+        g.set_tile_shape(1)
+        ref.set_tile_shape(1)
+        sets = [(fields(1), [[-0.2, 0.1]] * 6), (fields(1), [[-0.2, 0.1]] * 6), (fields(1), [[0.0, 0.2]] * 6),
+                (fields(9), [[-0.1, 0.3]] * 6), (fields(9), [[-0.1, 0.3]] * 6)]
+        for k, (fs, isos) in enumerate(sets):
+            torch.cuda.synchronize()
+            gl = [(f.data_ptr(), (f.shape[2], f.shape[1], f.shape[0]), (0, 0, 0), (1, 1, 1), True) for f in fs]
+            print("graph batch call", k, flush=True)
+            g.extract_batch_raw(gl, isos, flags)
+            a = g.mesh_numpy()
+            ref.extract_batch_raw(gl, isos, flags)
+            b = ref.mesh_numpy()
+            for name in ("positions", "normals", "halfedges", "face_halfedge", "face_attr", "vertex_flags", "vertex_halfedge"):
+                e = meshcheck.exact_diff(name, getattr(a, name), getattr(b, name))
+                assert e is None, (k, e)
+            # unrelated eager work between the calls
+            f0 = fs[0]
+            other.extract_raw(f0.data_ptr(), (f0.shape[2], f0.shape[1], f0.shape[0]), (0, 0, 0), (1, 1, 1), 0.0, flags, 0, True)
+            torch.cuda.synchronize()
+        assert g.graph_replays() == 4  # call 0 ran eagerly, call 1 captured the front and ran it, calls 2-4 replayed it
+        assert ref.graph_replays() == 0
+    finally:
+        g.close()
+        ref.close()
+        other.close()
