@@ -254,6 +254,15 @@ mdmc_status mdmc_generate_surface(mdmc_ctx* ctx, int32_t surface_case, const int
 mdmc_status mdmc_generate_edge_case(mdmc_ctx* ctx, int32_t edge_case, float* values, int32_t dst_on_device, int32_t dims[3], float origin[3],
                                     float delta[3]);
 
+/* Replaces reading back the grid that UniformGridHost::readFromFile(file, refinement,
+ * addZeroBoundary) builds (UniformGrid.inl:573-608; getDimensions / getDeltas / getOrigin,
+ * UniformGrid.h:316-345): the padded / refined grid the last extraction ran on, grid
+ * grid_index of a batch.  values (may be NULL to query the geometry only) receives
+ * dims[0]*dims[1]*dims[2] floats, x fastest, on the device or the host; dims / origin / delta
+ * (may be NULL) receive the geometry.  Synchronises the context's stream. */
+mdmc_status mdmc_processed_grid(mdmc_ctx* ctx, int32_t grid_index, float* values, int32_t dst_on_device, int32_t dims[3], float origin[3],
+                                float delta[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,7 +104,7 @@ EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", 
            "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_mesh_render_vertices", "mdmc_backward",
            "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape",
            "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin",
-           "mdmc_backward_noalloc", "mdmc_set_graphs", "mdmc_get_graph_replays"]
+           "mdmc_backward_noalloc", "mdmc_set_graphs", "mdmc_get_graph_replays", "mdmc_processed_grid"]
 
 
 def lib():
@@ -149,6 +149,8 @@ def lib():
         L.mdmc_generate_surface.restype = i32
         L.mdmc_generate_edge_case.argtypes = [vp, i32, vp, i32, vp, vp, vp]
         L.mdmc_generate_edge_case.restype = i32
+        L.mdmc_processed_grid.argtypes = [vp, i32, vp, i32, vp, vp, vp]
+        L.mdmc_processed_grid.restype = i32
         L.mdmc_mesh_render_vertices.restype = i32
         L.mdmc_backward.argtypes = [vp, vp, vp, i32]
         L.mdmc_backward.restype = i32
@@ -445,6 +447,18 @@ class DualMarchingCubes:
         p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         self._check(lib().mdmc_generate_edge_case(self._ctx, int(edge_case), p(vals), 0, p(d3), p(o), p(dl)))
         assert tuple(d3) == (4, 4, 3)
+        return vals, tuple(float(x) for x in o), tuple(float(x) for x in dl)
+
+    def processed_grid(self, index=0):
+        """The padded / refined grid the last extraction ran on (UniformGridHost::readFromFile,
+        UniformGrid.inl:573-608), grid `index` of a batch: (values float32 [nz, ny, nx], origin, delta)."""
+        d3 = np.zeros(3, np.int32)
+        o = np.zeros(3, np.float32)
+        dl = np.zeros(3, np.float32)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self._check(lib().mdmc_processed_grid(self._ctx, int(index), None, 0, p(d3), p(o), p(dl)))
+        vals = np.zeros((int(d3[2]), int(d3[1]), int(d3[0])), np.float32)
+        self._check(lib().mdmc_processed_grid(self._ctx, int(index), p(vals), 0, None, None, None))
         return vals, tuple(float(x) for x in o), tuple(float(x) for x in dl)
 
     def triangles(self):

@@ -37,6 +37,7 @@ struct mdmc_ctx {
     AdjointState adj;
     int nProblems = 0;  // problems of the resident result
     int flags = 0;
+    std::vector<mdmc::GridDev> procGrids;  // the processed grids of the resident result
 };
 
 namespace {
@@ -183,6 +184,7 @@ void processedDims(const mdmc_grid* g, int r, int pad, int od[3]) {
 mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const float* isos, int I, const mdmc_options* opt, mdmc_stats* stats) {
     ctx->hasResult = false;
     ctx->adj.valid = false;
+    ctx->procGrids.clear();
     ctx->nProblems = 0;
     if (!grids || !isos || !opt || G < 1 || I < 1) return fail(ctx, MDMC_ERR_INVALID_ARG, "null grids, iso-values or options, or an empty batch");
     if ((int64_t)G * I > (int64_t)1 << 20) return fail(ctx, MDMC_ERR_INVALID_ARG, "more than 2^20 problems");
@@ -241,7 +243,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             int od[3];
             processedDims(grid, r, pad, od);
             prep[gi] = mdmc::PrepareDesc{in, gp + pOff[gi], grid->dims[0], grid->dims[1], grid->dims[2], od[0], od[1], od[2], prepBlocks};
-            prepBlocks += mdmc::prepareBlocks((int64_t)od[0] * od[1] * od[2]);
+            prepBlocks += mdmc::prepareBlocks(od);
             mdmc::GridDev g;
             g.v = gp + pOff[gi];
             g.nx = od[0];
@@ -260,6 +262,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
                 g.oz -= g.dz;
             }
             if (gi == 0) g0 = g;
+            ctx->procGrids.push_back(g);
             for (int k = 0; k < I; ++k) {
                 mdmc::ProblemDev pr{};
                 pr.g = g;
@@ -287,7 +290,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             p.gridPassBack(C, shapeNow, record);
         } else {
             p.upload(pd, prep.data(), (size_t)G * sizeof(mdmc::PrepareDesc), 1);
-            mdmc::launchPrepareGrids(pd, G, prepBlocks, r, pad, s);
+            mdmc::launchPrepareGrids(pd, G, prepBlocks, r, pad, p.prepSpecial(), s);
             MDMC_HIP(hipEventRecord(ctx->ev[1], s));
             // ---- DMC
             p.dmc(problems, surf, record);
@@ -650,6 +653,32 @@ mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays) {
     if (!ctx || !replays) return MDMC_ERR_INVALID_ARG;
     *replays = ctx->p.graphReplays;
     return MDMC_OK;
+}
+
+mdmc_status mdmc_processed_grid(mdmc_ctx* ctx, int32_t grid_index, float* values, int32_t dst_on_device, int32_t dims[3], float origin[3],
+                                float delta[3]) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    if (!ctx->hasResult) return fail(ctx, MDMC_ERR_NO_RESULT, "no extraction result");
+    if (grid_index < 0 || grid_index >= (int32_t)ctx->procGrids.size()) return fail(ctx, MDMC_ERR_INVALID_ARG, "grid index out of range");
+    const mdmc::GridDev& g = ctx->procGrids[(size_t)grid_index];
+    DeviceGuard dg(ctx->device);
+    try {
+        if (values) {
+            const size_t n = (size_t)g.nx * g.ny * g.nz;
+            MDMC_HIP(hipMemcpyAsync(values, g.v, n * sizeof(float), dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->p.stream));
+            MDMC_HIP(hipStreamSynchronize(ctx->p.stream));
+        }
+        const int dm[3] = {g.nx, g.ny, g.nz};
+        const float o[3] = {g.ox, g.oy, g.oz}, d[3] = {g.dx, g.dy, g.dz};
+        for (int k = 0; k < 3; ++k) {
+            if (dims) dims[k] = dm[k];
+            if (origin) origin[k] = o[k];
+            if (delta) delta[k] = d[k];
+        }
+        return MDMC_OK;
+    } catch (const std::exception& e) {
+        return fromException(ctx, e);
+    }
 }
 
 }  // extern "C"

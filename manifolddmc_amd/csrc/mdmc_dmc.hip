@@ -6,69 +6,102 @@
 #include "mdmc_device.h"
 #include "mdmc_internal.h"
 
+#include <algorithm>
+
 namespace mdmc {
 
 namespace {
 
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
-// one thread per point of every processed grid; grid g owns blocks [d[g].base, d[g + 1].base),
-// so the grid a block works on, and with it every size below, is uniform across the block.
-// The block's first point is split into (x0, y0, z0) once (uniform, scalar); each thread
-// carries x0 + tid into y and z.  Without refinement the sample position is the point
-// itself: x / 1 = x exactly, floor(x) = x and the weights are +0, the same values the
-// general path computes, so the 8 corner loads and the trilinear blend stay (a non-finite
-// neighbour still reaches the result through 0 * s).
-template <bool REFINE>
-__global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int refinement, int pad) {
-    const int64_t blk = blockIdx.x;
-    int lo = 0, hi = G;  // d[lo].base <= blk
+// One block per row (y, z) of every processed grid, its threads striding over x; grid g owns
+// rows [d[g].base, d[g + 1].base), so the grid a block works on, and with it every size
+// below, is uniform across the block.
+//
+// MODE 2 (refinement > 0): the trilinear sample at x / (r + 1).
+// MODE 1 (no refinement, exact): the sample position is the point itself: x / 1 = x exactly,
+//   floor(x) = x and the weights are +0, the same values the general path computes, so the
+//   8 corner loads and the trilinear blend stay (a non-finite neighbour still reaches the
+//   result through 0 * s).  Runs only when *special is set, as a grid-stride loop.
+// MODE 0 (no refinement, plain): when every input value v has |v| < 2^126 and is not -0, each
+//   difference of two of them is finite, so s0 + 0 * (s1 - s0) = s0 and every level of the
+//   zero-weight blend returns its first operand: the result is the first corner, bit for bit.
+//   This mode copies that corner and sets *special when it meets any other value (every input
+//   point is some output point's first corner, so none is missed); MODE 1 then redoes all.
+__device__ __forceinline__ bool plainValue(float v) {
+    const uint32_t b = __float_as_uint(v);
+    return (b & 0x7FFFFFFFu) < 0x7E800000u && b != 0x80000000u;
+}
+
+template <int MODE>
+__device__ __forceinline__ void prepareRow(const PrepareDesc* __restrict__ d, int G, int refinement, int pad, int64_t row, int64_t* special) {
+    int lo = 0, hi = G;  // d[lo].base <= row
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (d[mid].base <= blk) lo = mid;
+        if (d[mid].base <= row) lo = mid;
         else hi = mid;
     }
     const PrepareDesc& p = d[lo];
     const int onx = p.onx, ony = p.ony, onz = p.onz, inx = p.inx, iny = p.iny, inz = p.inz;
-    const int64_t i0 = (blk - p.base) * kBlock;
-    const int64_t i = i0 + threadIdx.x;
-    if (i >= (int64_t)onx * ony * onz) return;
-    const int64_t r0 = i0 / onx;
-    int x = (int)(i0 - r0 * onx) + (int)threadIdx.x;
-    int y = (int)(r0 % ony);
-    int z = (int)(r0 / ony);
-    while (x >= onx) {  // once at most when onx >= kBlock
-        x -= onx;
-        if (++y == ony) {
-            y = 0;
-            ++z;
-        }
-    }
+    const int r = (int)(row - p.base);  // < ony * onz < 2^31
+    int y = r % ony, z = r / ony;
+    float* out = p.out + ((int64_t)z * ony + y) * onx;
     if (pad) {
-        x = clampi(x, 1, onx - 2) - 1;
         y = clampi(y, 1, ony - 2) - 1;
         z = clampi(z, 1, onz - 2) - 1;
     }
-    int mx = x, my = y, mz = z;
-    f3 dd = mk3(0.f, 0.f, 0.f);
-    if (REFINE) {
-        const float rr = (float)(refinement + 1);
-        const float fx = (float)x / rr, fy = (float)y / rr, fz = (float)z / rr;
-        mx = (int)floorf(fx);
+    const int64_t nxy = (int64_t)inx * iny;
+    if (MODE == 0) {
+        const float* in = p.in + (int64_t)z * nxy + (int64_t)y * inx;
+        bool plain = true;
+        for (int x = threadIdx.x; x < onx; x += kBlock) {
+            const float v = in[pad ? clampi(x, 1, onx - 2) - 1 : x];
+            plain = plain && plainValue(v);
+            out[x] = v;
+        }
+        if (!plain) *special = 1;
+        return;
+    }
+    const float rr = (float)(refinement + 1);
+    int my = y, mz = z;
+    float dy = 0.f, dz = 0.f;
+    if (MODE == 2) {
+        const float fy = (float)y / rr, fz = (float)z / rr;
         my = (int)floorf(fy);
         mz = (int)floorf(fz);
-        dd = mk3(fx - (float)mx, fy - (float)my, fz - (float)mz);
+        dy = fy - (float)my;
+        dz = fz - (float)mz;
     }
-    const int x0 = clampi(mx, 0, inx - 1), x1 = clampi(mx + 1, 0, inx - 1);
     const int y0 = clampi(my, 0, iny - 1), y1 = clampi(my + 1, 0, iny - 1);
     const int z0 = clampi(mz, 0, inz - 1), z1 = clampi(mz + 1, 0, inz - 1);
-    const int64_t nxy = (int64_t)inx * iny;
     const float* r00 = p.in + (int64_t)z0 * nxy + (int64_t)y0 * inx;
     const float* r10 = p.in + (int64_t)z0 * nxy + (int64_t)y1 * inx;
     const float* r01 = p.in + (int64_t)z1 * nxy + (int64_t)y0 * inx;
     const float* r11 = p.in + (int64_t)z1 * nxy + (int64_t)y1 * inx;
-    const float s[8] = {r00[x0], r00[x1], r10[x0], r10[x1], r01[x0], r01[x1], r11[x0], r11[x1]};
-    p.out[i] = trilinear(s, dd);
+    for (int x = threadIdx.x; x < onx; x += kBlock) {
+        const int px = pad ? clampi(x, 1, onx - 2) - 1 : x;
+        int mx = px;
+        float dx = 0.f;
+        if (MODE == 2) {
+            const float fx = (float)px / rr;
+            mx = (int)floorf(fx);
+            dx = fx - (float)mx;
+        }
+        const int x0 = clampi(mx, 0, inx - 1), x1 = clampi(mx + 1, 0, inx - 1);
+        const float s[8] = {r00[x0], r00[x1], r10[x0], r10[x1], r01[x0], r01[x1], r11[x0], r11[x1]};
+        out[x] = trilinear(s, mk3(dx, dy, dz));
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int refinement, int pad, int64_t rows,
+                                                        int64_t* special) {
+    if (MODE == 1) {
+        if (*(volatile int64_t*)special == 0) return;
+        for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) prepareRow<1>(d, G, refinement, pad, row, special);
+        return;
+    }
+    prepareRow<MODE>(d, G, refinement, pad, blockIdx.x, special);
 }
 
 __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, uint32_t* __restrict__ idx) {
@@ -136,12 +169,17 @@ __global__ void kRenderVertices(const int4* __restrict__ he, int64_t H, const fl
 
 }  // namespace
 
-int64_t prepareBlocks(int64_t points) { return (points + kBlock - 1) / kBlock; }
+int64_t prepareBlocks(const int od[3]) { return (int64_t)od[1] * od[2]; }
 
-void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, hipStream_t s) {
+void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, int64_t* special, hipStream_t s) {
     if (blocks <= 0) return;
-    if (refinement > 0) kPrepareGrids<true><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad);
-    else kPrepareGrids<false><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad);
+    if (refinement > 0) {
+        kPrepareGrids<2><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad, blocks, special);
+        return;
+    }
+    // *special is zero on entry; the exact pass returns at once unless the copy set it
+    kPrepareGrids<0><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad, blocks, special);
+    kPrepareGrids<1><<<(unsigned)std::min<int64_t>(blocks, 2048), kBlock, 0, s>>>(descDev, G, refinement, pad, blocks, special);
 }
 
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s) {

@@ -45,10 +45,10 @@ struct GridPassArgs {
     int nProblems;
     int surfIter;
     int64_t numTiles;
-    // classify -> emit: per tile its active cells' polygon word and (cell | owned edges << 12),
-    // at t * tile + active index; per tile its vertex / quad / active-cell counts
-    uint64_t* recPoly;
-    uint16_t* recInfo;
+    // classify -> emit: per tile its active cells' 32-bit records (polygon table entry, cell in
+    // the tile, owned edges, contour count) at t * tile + active index; per tile its vertex /
+    // quad / active-cell counts
+    uint32_t* rec;
     int32_t* tileV;   // [numTiles + 1], last entry 0 (scanned in place of a total)
     int32_t* tileQ;   // [numTiles + 1]
     int32_t* tileNA;  // [numTiles]

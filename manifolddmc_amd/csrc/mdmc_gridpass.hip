@@ -69,9 +69,13 @@ __device__ __forceinline__ int64_t tileOfBlock(int64_t b, int64_t numTiles, bool
 constexpr int kPolyTabSize = 656;
 __constant__ uint64_t c_polyTab[kPolyTabSize];
 __constant__ uint32_t c_polyBase[256];
+// per table entry: the contour of each crossed edge, 2 bits per edge (contourMap of the word),
+// and the number of contours - 1 in bits 24-25
+__constant__ uint32_t c_polyCmap[kPolyTabSize];
 
-__device__ __forceinline__ uint64_t cellPolygonTab(int mc, const float s[8], float iso) {
-    const uint32_t b = c_polyBase[mc];
+// the table entry of a cell of case mc: the decider bits of the case's ambiguous faces (only
+// those need the corner values; s is read for them alone)
+__device__ __forceinline__ int deciderIndex(uint32_t b, const float s[8], float iso) {
     const uint32_t mask = b >> 16;
     int k = 0, bit = 0;
 #define MDMC_FACE(F)                                          \
@@ -81,7 +85,40 @@ __device__ __forceinline__ uint64_t cellPolygonTab(int mc, const float s[8], flo
     }
     MDMC_FACE(0) MDMC_FACE(1) MDMC_FACE(2) MDMC_FACE(3) MDMC_FACE(4) MDMC_FACE(5)
 #undef MDMC_FACE
-    return c_polyTab[(b & 0xFFFFu) + k];
+    return (int)(b & 0xFFFFu) + k;
+}
+__device__ __forceinline__ uint64_t cellPolygonTab(int mc, const float s[8], float iso) {
+    return c_polyTab[deciderIndex(c_polyBase[mc], s, iso)];
+}
+
+// An active cell's record (classify -> emit), 28 bits: table entry (bits 0-9), cell in the
+// tile (10-21), owned x / y / z edge crossed with all four cells present (22-24), edgy: the
+// cell's first corner below the iso-value (25), contours - 1 (26-27)
+__device__ __forceinline__ int recIndex(uint32_t r) { return (int)(r & 0x3FFu); }
+__device__ __forceinline__ int recCell(uint32_t r) { return (int)((r >> 10) & 0xFFFu); }
+__device__ __forceinline__ int recOwn(uint32_t r) { return (int)((r >> 22) & 0xFu); }
+__device__ __forceinline__ int recVerts(uint32_t r) { return (int)((r >> 26) & 3u) + 1; }
+__device__ __forceinline__ int recQuads(uint32_t r) { return __popc((r >> 22) & 7u); }
+
+// n / d and n % d for n < 2^24, d < 2^24 (d and its float reciprocal uniform): the float
+// quotient is off by at most one, fixed with the exact remainder
+struct SmallDiv {
+    unsigned d;
+    float rcp;
+};
+__device__ __forceinline__ SmallDiv smallDiv(unsigned d) { return SmallDiv{d, 1.0f / (float)d}; }
+__device__ __forceinline__ unsigned divSmall(const SmallDiv& D, unsigned n, unsigned& rem) {
+    int q = (int)((float)n * D.rcp);
+    int r = (int)n - q * (int)D.d;
+    if (r < 0) {
+        --q;
+        r += (int)D.d;
+    } else if (r >= (int)D.d) {
+        ++q;
+        r -= (int)D.d;
+    }
+    rem = (unsigned)r;
+    return (unsigned)q;
 }
 
 // map entry: bits 0-13 vertex index in the tile, then 2 bits per edge 1,2,4,5,6,7,9,10,11
@@ -95,17 +132,23 @@ __device__ __forceinline__ int mapContour(uint32_t m, int e) {
     return (int)((m >> (14 + 2 * k)) & 3u);
 }
 
+// The field pointer comes from a descriptor in memory, where the compiler cannot see that it
+// addresses global memory; read through the global address space (global, not flat, loads).
+typedef const __attribute__((address_space(1))) float GlobalFloat;
+__device__ __forceinline__ GlobalFloat* fieldOf(const GridDev& g) { return (GlobalFloat*)g.v; }
+
 __device__ __forceinline__ void loadCorners(const GridDev& g, int x, int y, int z, float s[8]) {
     const int64_t nx = g.nx, nxy = (int64_t)g.nx * g.ny;
     const int64_t b = (int64_t)z * nxy + (int64_t)y * nx + x;
-    s[0] = g.v[b];
-    s[1] = g.v[b + 1];
-    s[2] = g.v[b + nx];
-    s[3] = g.v[b + nx + 1];
-    s[4] = g.v[b + nxy];
-    s[5] = g.v[b + nxy + 1];
-    s[6] = g.v[b + nxy + nx];
-    s[7] = g.v[b + nxy + nx + 1];
+    GlobalFloat* v = fieldOf(g);
+    s[0] = v[b];
+    s[1] = v[b + 1];
+    s[2] = v[b + nx];
+    s[3] = v[b + nx + 1];
+    s[4] = v[b + nxy];
+    s[5] = v[b + nxy + 1];
+    s[6] = v[b + nxy + nx];
+    s[7] = v[b + nxy + nx + 1];
 }
 
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
@@ -118,6 +161,14 @@ __device__ __forceinline__ int ownedBits(const float s[8], float iso, int x, int
     if (x >= 1 && z >= 1 && crossed(s[0], s[2], iso)) m |= 2;
     if (x >= 1 && y >= 1 && crossed(s[0], s[4], iso)) m |= 4;
     if (s[0] < iso) m |= 8;
+    return m;
+}
+// the same from the case bits (bit q: s[q] >= iso) and the edgy bit, with no corner values
+__device__ __forceinline__ int ownedBitsCase(int mc, int edgy, int x, int y, int z) {
+    int m = edgy ? 8 : 0;
+    if (y >= 1 && z >= 1 && ((mc ^ (mc >> 1)) & 1)) m |= 1;
+    if (x >= 1 && z >= 1 && ((mc ^ (mc >> 2)) & 1)) m |= 2;
+    if (x >= 1 && y >= 1 && ((mc ^ (mc >> 4)) & 1)) m |= 4;
     return m;
 }
 
@@ -158,7 +209,8 @@ __device__ __forceinline__ bool exactReciprocal(float d) {
 __device__ __forceinline__ void neighbourSamples(const GridDev& g, int x, int y, int z, float nbx[8], float nby[8], float nbz[8]) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int64_t NX = nx, NXY = (int64_t)nx * ny;
-    auto at = [&](int px, int py, int pz) { return g.v[(int64_t)pz * NXY + (int64_t)py * NX + px]; };
+    GlobalFloat* fv = fieldOf(g);
+    auto at = [&](int px, int py, int pz) { return fv[(int64_t)pz * NXY + (int64_t)py * NX + px]; };
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
@@ -245,13 +297,12 @@ __device__ __forceinline__ void cellXYZ(const TileBase& b, int l, int CX, int CY
     z = b.z0 + (int)zz;
 }
 
-// Block scan of the per-active-cell counts (vertices | quads << 16); thread tid owns active
-// indices CPT*tid .. CPT*tid + CPT - 1.  Writes the exclusive prefixes (s_qpre may be null)
-// and returns the tile's totals.  A tile has at most 4*4096 vertices and 3*4096 quads, so
-// both fit 16 bits and the packed sum never carries.
+// Block scan of the per-active-cell counts (vertices | quads << 16, from the records); thread
+// tid owns active indices CPT*tid .. CPT*tid + CPT - 1.  Writes the exclusive prefixes and
+// returns the tile's totals.  A tile has at most 4*4096 vertices and 3*4096 quads, so both fit
+// 16 bits and the packed sum never carries.
 template <int CPT>
-__device__ __forceinline__ void scanCounts(const uint64_t* s_poly, const uint8_t* s_own, int NA, uint16_t* s_vpre, uint16_t* s_qpre, int* s_wave,
-                                           int& VT, int& QT) {
+__device__ __forceinline__ void scanCounts(const uint32_t* s_rec, int NA, uint16_t* s_vpre, uint16_t* s_qpre, int* s_wave, int& VT, int& QT) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int cnt = 0;
     int nvq[CPT];
@@ -259,7 +310,10 @@ __device__ __forceinline__ void scanCounts(const uint64_t* s_poly, const uint8_t
     for (int j = 0; j < CPT; ++j) {
         const int ai = tid * CPT + j;
         nvq[j] = 0;
-        if (ai < NA) nvq[j] = polyCount(s_poly[ai]) | (__popc(s_own[ai] & 7) << 16);
+        if (ai < NA) {
+            const uint32_t r = s_rec[ai];
+            nvq[j] = recVerts(r) | (recQuads(r) << 16);
+        }
         cnt += nvq[j];
     }
     int x = cnt;
@@ -314,18 +368,28 @@ __device__ __forceinline__ TileCtx tileCtx(const GridPassArgs& a, int chunked) {
     return c;
 }
 
-// ---- classify: active cells of one tile, their polygons, counts and map entries
+// coordinates of tile cell l (the tile's first cell at (x0, y0, z0)), with the small divider
+__device__ __forceinline__ void tileCellXYZ(const TileBase& b, int l, const SmallDiv& dx, const SmallDiv& dy, int& x, int& y, int& z) {
+    unsigned rx, ry;
+    const unsigned r = divSmall(dx, (unsigned)(b.x0 + l), rx);
+    const unsigned zz = divSmall(dy, (unsigned)b.y0 + r, ry);
+    x = (int)rx;
+    y = (int)ry;
+    z = b.z0 + (int)zz;
+}
+
+// ---- classify: active cells of one tile, their records, counts and map entries
 template <int CPT>
 __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chunked) {
     constexpr int TILE = GP_THREADS * CPT, RUNS = TILE / 64;
-    __shared__ uint64_t s_poly[TILE];
+    __shared__ uint32_t s_rec[TILE];
+    __shared__ uint32_t s_cmap[TILE];  // per active cell: its table entry's contour map and count
     __shared__ uint16_t s_cell[TILE];
-    __shared__ uint16_t s_vpre[TILE];  // holds the MC case of each tile cell (bytes) until the scan
-    __shared__ uint8_t s_own[TILE];
+    __shared__ uint16_t s_vpre[TILE];
+    __shared__ uint16_t s_mc[TILE];  // MC case of each tile cell | edgy << 8
     __shared__ unsigned long long s_mask[RUNS];
     __shared__ int s_mpre[RUNS + 1];
     __shared__ int s_wave[GP_WAVES];
-    uint8_t* const s_mc = reinterpret_cast<uint8_t*>(s_vpre);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
@@ -338,36 +402,47 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
 
     // MC case of every cell; a ballot per run of 64 cells marks the active ones.  This thread's
     // cells are GP_THREADS apart: their coordinates advance from the first one's, with no
-    // division per cell.
-    int cx, cy, cz;
-    cellXYZ(tc.tb, tid, CX, CY, cx, cy, cz);
+    // division per cell.  Cells past the grid's end (last tile only) read cell 0 and are
+    // dropped, so that every corner load is issued before the first compare.
+    const SmallDiv dX = smallDiv((unsigned)CX), dY = smallDiv((unsigned)CY);
+    int cx[CPT], cy[CPT], cz[CPT];
+    tileCellXYZ(tc.tb, tid, dX, dY, cx[0], cy[0], cz[0]);
+#pragma unroll
+    for (int k = 1; k < CPT; ++k) {
+        cx[k] = cx[k - 1];
+        cy[k] = cy[k - 1];
+        cz[k] = cz[k - 1];
+        if (CX >= GP_THREADS) {  // at most one row boundary per step
+            cx[k] += GP_THREADS;
+            if (cx[k] >= CX) {
+                cx[k] -= CX;
+                if (++cy[k] == CY) {
+                    cy[k] = 0;
+                    ++cz[k];
+                }
+            }
+        } else {
+            tileCellXYZ(tc.tb, k * GP_THREADS + tid, dX, dY, cx[k], cy[k], cz[k]);
+        }
+    }
+    float cs[CPT][8];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const bool in = tc.c0 + k * GP_THREADS + tid < C;
+        loadCorners(g, in ? cx[k] : 0, in ? cy[k] : 0, in ? cz[k] : 0, cs[k]);
+    }
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int l = k * GP_THREADS + tid;
-        int mc = 0;
-        if (k > 0) {
-            if (CX >= GP_THREADS) {  // at most one row boundary per step
-                cx += GP_THREADS;
-                if (cx >= CX) {
-                    cx -= CX;
-                    if (++cy == CY) {
-                        cy = 0;
-                        ++cz;
-                    }
-                }
-            } else {
-                cellXYZ(tc.tb, l, CX, CY, cx, cy, cz);
-            }
-        }
+        int mc = 0, edgy = 0;
         if (tc.c0 + l < C) {
-            float s[8];
-            loadCorners(g, cx, cy, cz, s);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) mc |= (s[q] >= iso ? 1 : 0) << q;
+            for (int q = 0; q < 8; ++q) mc |= (cs[k][q] >= iso ? 1 : 0) << q;
+            edgy = cs[k][0] < iso ? 1 : 0;
         }
         const uint64_t m = __ballot(mc != 0 && mc != 255);
         if (lane == 0) s_mask[k * GP_WAVES + wid] = m;
-        s_mc[l] = (uint8_t)mc;
+        s_mc[l] = (uint16_t)(mc | (edgy << 8));
     }
     __syncthreads();
     if (tid < 64) {
@@ -391,28 +466,35 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
     }
     __syncthreads();
 
-    // polygon and owned edges of the active cells, every lane busy
+    // records of the active cells, every lane busy: the corners are loaded again only for the
+    // cases with an ambiguous face (the decider reads them); owned edges come from the case
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
         const int l = s_cell[ai];
+        const int mce = s_mc[l], mc = mce & 0xFF;
         int x, y, z;
-        cellXYZ(tc.tb, l, CX, CY, x, y, z);
-        float s[8];
-        loadCorners(g, x, y, z, s);
-        s_poly[ai] = cellPolygonTab(s_mc[l], s, iso);
-        s_own[ai] = (uint8_t)ownedBits(s, iso, x, y, z);
+        tileCellXYZ(tc.tb, l, dX, dY, x, y, z);
+        const uint32_t b = c_polyBase[mc];
+        int idx = (int)(b & 0xFFFFu);
+        if (b >> 16) {
+            float s[8];
+            loadCorners(g, x, y, z, s);
+            idx = deciderIndex(b, s, iso);
+        }
+        const int own = ownedBitsCase(mc, mce >> 8, x, y, z);
+        const uint32_t meta = c_polyCmap[idx];
+        s_cmap[ai] = meta;
+        s_rec[ai] = (uint32_t)idx | ((uint32_t)l << 10) | ((uint32_t)own << 22) | ((meta >> 24) << 26);
     }
     __syncthreads();
     int VT, QT;
-    scanCounts<CPT>(s_poly, s_own, NA, s_vpre, nullptr, s_wave, VT, QT);
+    scanCounts<CPT>(s_rec, NA, s_vpre, nullptr, s_wave, VT, QT);
     __syncthreads();
 
     const int64_t rb = tc.t * TILE;
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
-        const uint64_t w = s_poly[ai];
-        const int l = s_cell[ai];
-        a.recPoly[rb + ai] = w;
-        a.recInfo[rb + ai] = (uint16_t)(l | (s_own[ai] << 12));
-        a.cellMap[tc.gc0 + l] = mapEntry(s_vpre[ai], contourMap(w));
+        const uint32_t r = s_rec[ai];
+        a.rec[rb + ai] = r;
+        a.cellMap[tc.gc0 + recCell(r)] = mapEntry(s_vpre[ai], s_cmap[ai]);
     }
     if (tid == 0) {
         a.tileV[tc.t] = VT;
@@ -425,24 +507,20 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
     }
 }
 
-// ---- emit: the tile's dual vertices and the quads its cells own
+// ---- emit: the tile's dual vertices and the quads its cells own, one active cell per thread
 template <int CPT>
 __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked) {
     constexpr int TILE = GP_THREADS * CPT;
-    __shared__ uint64_t s_poly[TILE];
-    __shared__ uint16_t s_cell[TILE];
+    __shared__ uint32_t s_rec[TILE];
+    __shared__ uint32_t s_cmap[TILE];  // per active cell: contour of each crossed edge, 2 bits per edge
     __shared__ uint16_t s_vpre[TILE];
     __shared__ uint16_t s_qpre[TILE];
-    __shared__ uint8_t s_own[TILE];
     __shared__ uint16_t s_aidx[TILE];  // per tile cell: its active index (0xFFFF: inactive, adjoint map only)
-    __shared__ uint32_t s_cmap[TILE];  // per active cell: contour of each crossed edge, 2 bits per edge
     __shared__ int s_wave[GP_WAVES];
 
     const int tid = threadIdx.x;
     const TileCtx tc = tileCtx<TILE>(a, chunked);
     const ProblemDev& P = a.probs[tc.prob];
-    const GridDev g = P.g;
-    const float iso = P.iso;
     const int CX = P.CX, CY = P.CY;
     const int64_t C = P.cells;
     const int64_t c0 = tc.c0;
@@ -453,18 +531,14 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
         __syncthreads();
     }
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
-        const uint64_t w = a.recPoly[rb + ai];
-        s_poly[ai] = w;
-        s_cmap[ai] = contourMap(w);
-        const int info = a.recInfo[rb + ai];
-        const int l = info & 0xFFF;
-        s_cell[ai] = (uint16_t)l;
-        s_own[ai] = (uint8_t)(info >> 12);
-        s_aidx[l] = (uint16_t)ai;
+        const uint32_t r = a.rec[rb + ai];
+        s_rec[ai] = r;
+        s_cmap[ai] = c_polyCmap[recIndex(r)];
+        s_aidx[recCell(r)] = (uint16_t)ai;
     }
     __syncthreads();
     int VT, QT;
-    scanCounts<CPT>(s_poly, s_own, NA, s_vpre, s_qpre, s_wave, VT, QT);
+    scanCounts<CPT>(s_rec, NA, s_vpre, s_qpre, s_wave, VT, QT);
     __syncthreads();
     const int64_t voff = a.tileVOff[tc.t], qoff = a.tileQOff[tc.t];
 
@@ -474,75 +548,72 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
             const int64_t c = c0 + l;
             if (c < C) {
                 const int ai = s_aidx[l];
-                a.adjMap[c] = ai == 0xFFFF ? -1 : (int)(((voff + s_vpre[ai]) << 2) | (int64_t)(polyCount(s_poly[ai]) - 1));
+                a.adjMap[c] = ai == 0xFFFF ? -1 : (int)(((voff + s_vpre[ai]) << 2) | (int64_t)(recVerts(s_rec[ai]) - 1));
             }
         }
     }
 
-    // per vertex: its cell and contour; kVertices computes position and normal
-    for (int i = tid; i < VT; i += GP_THREADS) {
-        const int ai = upperIndex<TILE>(s_vpre, NA, i);
-        const int64_t v = voff + i;
-        a.vcell[v] = (int)(c0 + s_cell[ai]);
-        a.vcw[v] = contourWord(s_poly[ai], i - s_vpre[ai]);
-        if (a.vpid) a.vpid[v] = tc.prob;
-    }
-
-    // quads whose edge starts at a cell of this tile.  Their other cells lie 1, CX, CX+1,
-    // CX*CY, CX*CY+1 or CX*CY+CX cells back: in this tile (LDS) or an earlier one (map).
-    for (int jq = tid; jq < QT; jq += GP_THREADS) {
-        const int ai = upperIndex<TILE>(s_qpre, NA, jq);
-        int r = jq - s_qpre[ai];
-        const int own = s_own[ai];
-        int axis = 0;
-        for (int b = 0; b < 3; ++b) {
-            if ((own >> b) & 1) {
-                if (r == 0) {
-                    axis = b;
-                    break;
-                }
-                --r;
-            }
+    const SmallDiv dX = smallDiv((unsigned)CX), dY = smallDiv((unsigned)CY);
+    for (int ai = tid; ai < NA; ai += GP_THREADS) {
+        const uint32_t r = s_rec[ai];
+        const int l = recCell(r);
+        const uint32_t cmap = s_cmap[ai];
+        // per vertex: its cell and contour; kVertices computes position and normal
+        const uint64_t w = c_polyTab[recIndex(r)];
+        const int nv = recVerts(r);
+        const int64_t v0 = voff + s_vpre[ai];
+        for (int t = 0; t < nv; ++t) {
+            a.vcell[v0 + t] = (int)(c0 + l);
+            a.vcw[v0 + t] = contourWord(w, t);
+            if (a.vpid) a.vpid[v0 + t] = tc.prob;
         }
-        const int edgy = (own >> 3) & 1;
+        // quads whose edge starts at this cell.  Their other cells lie 1, CX, CX+1, CX*CY,
+        // CX*CY+1 or CX*CY+CX cells back: in this tile (LDS) or an earlier one (map).
+        const int own = recOwn(r);
+        if ((own & 7) == 0) continue;
         int x, y, z;
-        cellXYZ(tc.tb, s_cell[ai], CX, CY, x, y, z);
-        // own edge, then (dx, dy, dz, edge) of the three other cells around it, packed
-        // 4 bits each (dx | dy<<1 | dz<<2, edge) per cell: x: (y-1: e2) (z-1: e4) (y-1,z-1: e6);
-        // y: (x-1: e1) (z-1: e7) (x-1,z-1: e5); z: (x-1: e9) (y-1: e11) (x-1,y-1: e10)
-        const int ownE = axis == 0 ? 0 : (axis == 1 ? 3 : 8);
-        const uint32_t nbTable = axis == 0 ? 0x664422u : (axis == 1 ? 0x557411u : 0xA3B291u);
-        int s0 = -1, s1 = -1, s2 = -1, s3 = -1;
-        {
-            const int sl = edgeSlot(ownE, edgy);
-            const int vid = (int)(voff + s_vpre[ai] + (int)((s_cmap[ai] >> (2 * ownE)) & 3u));
-            s0 = sl == 0 ? vid : s0;
-            s1 = sl == 1 ? vid : s1;
-            s2 = sl == 2 ? vid : s2;
-            s3 = sl == 3 ? vid : s3;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t ent = (nbTable >> (8 * k)) & 0xFFu;
-            const int d = (int)(ent & 0xF), e = (int)(ent >> 4);
-            const int64_t n = (int64_t)(z - ((d >> 2) & 1)) * CX * CY + (int64_t)(y - ((d >> 1) & 1)) * CX + (x - (d & 1));
-            int vid;
-            if (n >= c0) {
-                const int an = s_aidx[n - c0];
-                vid = (int)(voff + s_vpre[an] + (int)((s_cmap[an] >> (2 * e)) & 3u));
-            } else {
-                const uint32_t m = a.cellMap[P.cellBase + n];
-                vid = (int)(a.tileVOff[P.tileBase + n / TILE] + (int64_t)(m & 0x3FFFu) + mapContour(m, e));
+        tileCellXYZ(tc.tb, l, dX, dY, x, y, z);
+        const int edgy = (own >> 3) & 1;
+        int64_t q = qoff + s_qpre[ai];
+        for (int axis = 0; axis < 3; ++axis) {
+            if (!((own >> axis) & 1)) continue;
+            // own edge, then (dx, dy, dz, edge) of the three other cells around it, packed
+            // 4 bits each (dx | dy<<1 | dz<<2, edge) per cell: x: (y-1: e2) (z-1: e4) (y-1,z-1: e6);
+            // y: (x-1: e1) (z-1: e7) (x-1,z-1: e5); z: (x-1: e9) (y-1: e11) (x-1,y-1: e10)
+            const int ownE = axis == 0 ? 0 : (axis == 1 ? 3 : 8);
+            const uint32_t nbTable = axis == 0 ? 0x664422u : (axis == 1 ? 0x557411u : 0xA3B291u);
+            int s0 = -1, s1 = -1, s2 = -1, s3 = -1;
+            {
+                const int sl = edgeSlot(ownE, edgy);
+                const int vid = (int)(v0 + (int)((cmap >> (2 * ownE)) & 3u));
+                s0 = sl == 0 ? vid : s0;
+                s1 = sl == 1 ? vid : s1;
+                s2 = sl == 2 ? vid : s2;
+                s3 = sl == 3 ? vid : s3;
             }
-            const int sl = edgeSlot(e, edgy);
-            s0 = sl == 0 ? vid : s0;
-            s1 = sl == 1 ? vid : s1;
-            s2 = sl == 2 ? vid : s2;
-            s3 = sl == 3 ? vid : s3;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t ent = (nbTable >> (8 * k)) & 0xFFu;
+                const int d = (int)(ent & 0xF), e = (int)(ent >> 4);
+                const int64_t n = (int64_t)(z - ((d >> 2) & 1)) * CX * CY + (int64_t)(y - ((d >> 1) & 1)) * CX + (x - (d & 1));
+                int vid;
+                if (n >= c0) {
+                    const int an = s_aidx[n - c0];
+                    vid = (int)(voff + s_vpre[an] + (int)((s_cmap[an] >> (2 * e)) & 3u));
+                } else {
+                    const uint32_t m = a.cellMap[P.cellBase + n];
+                    vid = (int)(a.tileVOff[P.tileBase + n / TILE] + (int64_t)(m & 0x3FFFu) + mapContour(m, e));
+                }
+                const int sl = edgeSlot(e, edgy);
+                s0 = sl == 0 ? vid : s0;
+                s1 = sl == 1 ? vid : s1;
+                s2 = sl == 2 ? vid : s2;
+                s3 = sl == 3 ? vid : s3;
+            }
+            a.quads[q] = make_int4(s0, s1, s2, s3);
+            a.qattr[q] = (uint32_t)(3 * ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)) + axis);
+            ++q;
         }
-        const int64_t q = qoff + jq;
-        a.quads[q] = make_int4(s0, s1, s2, s3);
-        a.qattr[q] = (uint32_t)(3 * ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)) + axis);
     }
 }
 
@@ -836,32 +907,16 @@ __global__ void __launch_bounds__(SLAB_COLS) kSlab(GridPassArgs a, int chunked) 
 // gyroid), so every lane runs at most kMoveSteps steps; the block's unfinished lanes leave
 // their state in LDS, consecutive lanes finish them on full waves, and each vertex then
 // computes its normal from its final point.  Each vertex's arithmetic is unchanged.
-//
-// The edge crossings t = (iso - s_a) / (s_b - s_a) of the block's contours are computed as a
-// flat list of (vertex, edge) items on every lane (a contour has 3 to 12 edges, and a lane
-// looping over its own would keep a wave busy for its longest contour); each vertex then
-// adds its crossings' offsets in contour order, the same operations in the same order.
 constexpr int kMoveSteps = 2;
 constexpr int kMoveState = 17;  // s[8], iso, i1, v1, point, step
-constexpr int kMaxItems = 12 * GP_THREADS;
 
 __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chunked) {
-    // the crossings (items) and the move-to-surface continuation state are live one after the other
-    __shared__ union {
-        struct {
-            float tt[kMaxItems];
-            uint16_t item[kMaxItems];  // vertex slot | edge << 8
-        } e;
-        float st[kMoveState][GP_THREADS];
-    } s_u;
-    __shared__ float s_cs[8][GP_THREADS];  // corner values of each vertex slot
-    __shared__ float s_iso[GP_THREADS];
+    __shared__ float s_st[kMoveState][GP_THREADS];
     __shared__ int s_wave[GP_WAVES];
-    auto& s_st = s_u.st;
     const int64_t nb = (a.V + GP_THREADS - 1) / GP_THREADS;
     const int64_t v = tileOfBlock(blockIdx.x, nb, chunked != 0) * GP_THREADS + threadIdx.x;
     const bool live = v < a.V;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     GridDev g{};
     float iso = 0.f;
     int x = 0, y = 0, z = 0;
@@ -869,7 +924,6 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
     Mover m{};
     m.done = true;
     f3 p = mk3(0.f, 0.f, 0.f);
-    uint64_t cw = 0;
     if (live) {
         const ProblemDev& P = a.probs[a.vpid ? a.vpid[v] : 0];
         g = P.g;
@@ -880,53 +934,7 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
         y = (int)(r % (unsigned)P.CY);
         z = (int)(r / (unsigned)P.CY);
         loadCorners(g, x, y, z, s);
-        cw = a.vcw[v];
-    }
-    // the block's items: an exclusive scan of the contour sizes gives each vertex its run
-    const int sz = (int)(uint32_t)(cw >> 60);
-    {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) s_cs[k][tid] = s[k];
-        s_iso[tid] = iso;
-        int xs = sz;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y2 = __shfl_up(xs, d, 64);
-            if (lane >= d) xs += y2;
-        }
-        if (lane == 63) s_wave[wid] = xs;
-        __syncthreads();
-        int base = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < GP_WAVES; ++w) {
-            if (w < wid) base += s_wave[w];
-            total += s_wave[w];
-        }
-        const int off = base + xs - sz;
-        for (int i = 0; i < sz; ++i) s_u.e.item[off + i] = (uint16_t)(tid | (int)(((cw >> (4 * i)) & 0xF) << 8));
-        __syncthreads();
-        for (int it = tid; it < total; it += GP_THREADS) {
-            const int itm = s_u.e.item[it];
-            const int vs = itm & 0xFF, e = itm >> 8;
-            const float sa = s_cs[edgeA(e)][vs], sb = s_cs[edgeB(e)][vs];
-            s_u.e.tt[it] = (s_iso[vs] - sa) / (sb - sa);
-        }
-        __syncthreads();
-        // p = the mean of the crossings' cell-local points: component k of edge e's point is
-        // 0 + (t * [axis == k] + corner_k), i.e. 0 + ((axis == k ? t : t * 0) + corner_k)
-        for (int i = 0; i < sz; ++i) {
-            const int e = (int)((cw >> (4 * i)) & 0xF);
-            const int ca = edgeA(e), ax = edgeAxis(e);
-            const float tt = s_u.e.tt[off + i];
-            const float t0 = tt * 0.f;
-            p.x = p.x + (0.f + ((ax == 0 ? tt : t0) + (float)(ca & 1)));
-            p.y = p.y + (0.f + ((ax == 1 ? tt : t0) + (float)((ca >> 1) & 1)));
-            p.z = p.z + (0.f + ((ax == 2 ? tt : t0) + (float)((ca >> 2) & 1)));
-        }
-        if (live) p = divs(p, (float)sz);
-        __syncthreads();  // s_u turns into the continuation state below
-    }
-    if (live) {
+        p = edgeMean(s, iso, a.vcw[v]);
         if (a.surfIter) {
             moverInit(m, s, p);
             for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);
@@ -1025,8 +1033,21 @@ void initGridPassTables() {
         }
     }
     if (n != kPolyTabSize) throw HipError("grid pass: polygon table size mismatch", -4);
+    // per entry: the contour (0..3) of each of its edges, 2 bits per edge
+    uint32_t cmap[kPolyTabSize];
+    for (int i = 0; i < n; ++i) {
+        uint32_t m = 0;
+        int j = 0;
+        for (int c = 0; c < 4; ++c) {
+            const int sz = (int)((tab[i] >> (4 * c)) & 0xF);
+            for (int k = 0; k < sz; ++k, ++j) m |= (uint32_t)c << (2 * (int)((tab[i] >> (16 + 4 * j)) & 0xF));
+            if (sz > 0) m = (m & 0xFFFFFFu) | ((uint32_t)c << 24);
+        }
+        cmap[i] = m;
+    }
     MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyTab), tab, sizeof tab));
     MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyBase), base, sizeof base));
+    MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyCmap), cmap, sizeof cmap));
 }
 
 const char* gridPassShapeName(int shape) {

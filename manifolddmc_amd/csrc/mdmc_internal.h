@@ -41,15 +41,17 @@ void launchTriangulateMaxMin(const int4* he, const int32_t* face_he, int64_t F, 
 
 // --- DMC grid pass (mdmc_dmc.hip)
 // padding / refinement (refineGrid) of several grids in one launch: grid g reads in[inx*iny*inz]
-// and writes out[onx*ony*onz] with the blocks from base on (prepareBlocks(points) of them)
+// and writes out[onx*ony*onz] with the blocks from base on (prepareBlocks(dims) of them)
 struct PrepareDesc {
     const float* in;
     float* out;
     int inx, iny, inz, onx, ony, onz;
     int64_t base;
 };
-int64_t prepareBlocks(int64_t points);
-void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, hipStream_t s);
+// blocks of the padding / refinement pass over one processed grid of od points: one per row
+int64_t prepareBlocks(const int od[3]);
+// special: a device word that is zero on entry (set when an input value needs the exact blend)
+void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int refinement, int pad, int64_t* special, hipStream_t s);
 void launchDmcTrianglesFromFaces(const int4* he, const int32_t* face_he, int64_t F, uint32_t* idx, hipStream_t s);
 // render vertices {position, normal, rgba by vertex flags} of every vertex with an outgoing
 // halfedge into out[10 * V] (the caller zero-fills out first)

@@ -53,6 +53,8 @@ class Pipeline {
 
     void resetCounters();
     int64_t anomalies();
+    // the padding pass's flag word (one of the counters, zeroed by resetCounters)
+    int64_t* prepSpecial();
 
     // the grid pass over one or more (grid, iso) problems (their cellBase / tileBase and
     // cell counts are filled in); recordAdjoint keeps the cell -> vertex map ("adj_map") that

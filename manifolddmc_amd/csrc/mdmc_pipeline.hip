@@ -36,7 +36,7 @@ constexpr int kMaxBucket = 16;
 constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
-enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_NCOUNTERS = 8 };
+enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_NCOUNTERS = 8 };
 
 __device__ __forceinline__ void anomaly(int64_t* ctr) { atomicAdd((unsigned long long*)&ctr[A_ANOMALY], 1ULL); }
 
@@ -2043,6 +2043,8 @@ void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t))
 
 int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
 
+int64_t* Pipeline::prepSpecial() { return counters() + A_PREP_SPECIAL; }
+
 // ---- grid pass
 // Row slabs: bands of at most SLAB_XMAX cells per row, R rows per block, and chunks of slices
 // sized so that the launch has about 4096 blocks (a problem gets its share by its cell count;
@@ -2125,6 +2127,8 @@ int64_t Pipeline::dmcSetup(std::vector<ProblemDev>& problems, bool recordAdjoint
         pr.cellBase = C;
         C += pr.cells;
         if (pr.cells >= (int64_t)INT32_MAX) throw HipError("a grid has 2^31 cells or more", -3);
+        // the grid pass splits tile cell offsets with a float-reciprocal division (exact below 2^24)
+        if (pr.CX >= (1 << 23) || pr.CY >= (1 << 23)) throw HipError("a grid has 2^23 cells or more along x or y", -3);
     }
     if (!kev[0])
         for (auto& e : kev) MDMC_HIP(hipEventCreate(&e));
@@ -2178,7 +2182,7 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
 // addresses of every device buffer the front of a tile-shape grid pass touches
 std::vector<int64_t> Pipeline::frontBuffers() {
     std::vector<int64_t> b;
-    for (const char* n : {"grid", "prep_desc", "gp_probs", "gp_rpoly", "gp_rinfo", "gp_tv", "gp_tq", "gp_tna", "gp_map", "gp_tvo", "gp_tqo",
+    for (const char* n : {"counters", "grid", "prep_desc", "gp_probs", "gp_rec", "gp_tv", "gp_tq", "gp_tna", "gp_map", "gp_tvo", "gp_tqo",
                           "scan_scratch", "scan_scratch2"})
         b.push_back((int64_t)(intptr_t)arena.ptr<char>(n));
     b.push_back((int64_t)(intptr_t)pinDev());
@@ -2224,7 +2228,7 @@ bool Pipeline::runFrontGraph(const std::vector<int64_t>& shapeSig, const void* p
     MDMC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
     try {
         MDMC_HIP(hipMemcpyAsync(prepDev, fg_.stage, prepBytes, hipMemcpyHostToDevice, stream));
-        launchPrepareGrids((const PrepareDesc*)prepDev, G, prepBlocks, refinement, pad, stream);
+        launchPrepareGrids((const PrepareDesc*)prepDev, G, prepBlocks, refinement, pad, prepSpecial(), stream);
         gridPassFront(problems, C, shape, surfIter, (char*)fg_.stage + probOff, false);
     } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
@@ -2288,8 +2292,7 @@ void Pipeline::gridPassFront(std::vector<ProblemDev>& problems, int64_t C, int s
     a.nProblems = nProblems;
     a.surfIter = surfIter ? 1 : 0;
     a.numTiles = numTiles;
-    a.recPoly = arena.get<uint64_t>("gp_rpoly", (size_t)std::max<int64_t>(numTiles * T, 1));
-    a.recInfo = arena.get<uint16_t>("gp_rinfo", (size_t)std::max<int64_t>(numTiles * T, 1));
+    a.rec = arena.get<uint32_t>("gp_rec", (size_t)std::max<int64_t>(numTiles * T, 1));
     a.tileV = arena.get<int32_t>("gp_tv", (size_t)numTiles + 1);
     a.tileQ = arena.get<int32_t>("gp_tq", (size_t)numTiles + 1);
     a.tileNA = arena.get<int32_t>("gp_tna", (size_t)numTiles + 1);

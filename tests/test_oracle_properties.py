@@ -130,3 +130,31 @@ def test_empty_surface():
     v = np.zeros((6, 6, 6), np.float32)
     m = oa.extract(v, (0, 0, 0), (1, 1, 1), 1.0)
     assert len(m.pos) == 0 and len(m.face_he) == 0
+
+
+def _zero_weight_blend(s):
+    """trilinear(s, (0, 0, 0)) in float32, in the operation order of the refinement blend
+    (Interpolation.inl:48-56): lerp in x, then y, then z, each a + w * (b - a) / (1 - w) * a + w * b."""
+    f = np.float32
+    z, one = f(0), f(1)
+    lx = [s[i] + z * (s[i + 1] - s[i]) for i in (0, 2, 4, 6)]
+    ly = [(one - z) * lx[0] + z * lx[1], (one - z) * lx[2] + z * lx[3]]
+    return (one - z) * ly[0] + z * ly[1]
+
+
+def test_zero_weight_blend_of_plain_values_is_the_first_corner():
+    # the padding pass without refinement copies the first corner unless some input value is
+    # not plain (|v| >= 2^126 or -0); this is the arithmetic fact it relies on
+    rng = np.random.default_rng(4)
+    scales = np.float32([1e-42, 1e-30, 1.0, 1e20, 8.4e37])  # a subnormal up to just under 2^126
+    with np.errstate(over="ignore", invalid="ignore"):
+        for _ in range(4000):
+            s = (rng.uniform(-1, 1, 8) * rng.choice(scales, 8)).astype(np.float32)
+            s[rng.random(8) < 0.1] = np.float32(0.0)
+            assert np.abs(s).max() < np.float32(2.0 ** 126)
+            assert _zero_weight_blend(s).view(np.uint32) == s[0].view(np.uint32)
+        # and what is not plain can change the result: -0 first corner, overflowing differences
+        s = np.float32([-0.0, 1, 1, 1, 1, 1, 1, 1])
+        assert _zero_weight_blend(s).view(np.uint32) == 0  # +0, not -0
+        s = np.float32([3e38, -3e38, 0, 0, 0, 0, 0, 0])
+        assert np.isnan(_zero_weight_blend(s))
