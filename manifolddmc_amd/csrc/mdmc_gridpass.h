@@ -62,10 +62,10 @@ struct GridPassArgs {
     float* nrm;
     int4* quads;
     uint32_t* qattr;
-    // per vertex: its cell inside its problem and its contour's edges (emit -> vertices):
-    // edge i of the contour in bits 4i..4i+3, the contour size in bits 60..63
+    // per vertex: its cell inside its problem and its contour (emit -> vertices): the cell's
+    // polygon table entry | contour index << 10
     int32_t* vcell;
-    uint64_t* vcw;
+    uint16_t* vct;
     int64_t V;
     // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1
     // (single-problem extractions only)

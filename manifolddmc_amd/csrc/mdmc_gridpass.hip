@@ -383,7 +383,8 @@ template <int CPT>
 __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chunked) {
     constexpr int TILE = GP_THREADS * CPT, RUNS = TILE / 64;
     __shared__ uint32_t s_rec[TILE];
-    __shared__ uint32_t s_cmap[TILE];  // per active cell: its table entry's contour map and count
+    __shared__ uint32_t s_base[256];          // c_polyBase
+    __shared__ uint32_t s_tab[kPolyTabSize];  // c_polyCmap
     __shared__ uint16_t s_cell[TILE];
     __shared__ uint16_t s_vpre[TILE];
     __shared__ uint16_t s_mc[TILE];  // MC case of each tile cell | edgy << 8
@@ -399,6 +400,9 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
     const float iso = P.iso;
     const int CX = P.CX, CY = P.CY;
     const int64_t C = P.cells;
+    // the two small tables in LDS (read after the first barrier below)
+    s_base[tid] = c_polyBase[tid];
+    for (int i = tid; i < kPolyTabSize; i += GP_THREADS) s_tab[i] = c_polyCmap[i];
 
     // MC case of every cell; a ballot per run of 64 cells marks the active ones.  This thread's
     // cells are GP_THREADS apart: their coordinates advance from the first one's, with no
@@ -473,7 +477,7 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
         const int mce = s_mc[l], mc = mce & 0xFF;
         int x, y, z;
         tileCellXYZ(tc.tb, l, dX, dY, x, y, z);
-        const uint32_t b = c_polyBase[mc];
+        const uint32_t b = s_base[mc];
         int idx = (int)(b & 0xFFFFu);
         if (b >> 16) {
             float s[8];
@@ -481,9 +485,7 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
             idx = deciderIndex(b, s, iso);
         }
         const int own = ownedBitsCase(mc, mce >> 8, x, y, z);
-        const uint32_t meta = c_polyCmap[idx];
-        s_cmap[ai] = meta;
-        s_rec[ai] = (uint32_t)idx | ((uint32_t)l << 10) | ((uint32_t)own << 22) | ((meta >> 24) << 26);
+        s_rec[ai] = (uint32_t)idx | ((uint32_t)l << 10) | ((uint32_t)own << 22) | ((s_tab[idx] >> 24) << 26);
     }
     __syncthreads();
     int VT, QT;
@@ -494,7 +496,7 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
         const uint32_t r = s_rec[ai];
         a.rec[rb + ai] = r;
-        a.cellMap[tc.gc0 + recCell(r)] = mapEntry(s_vpre[ai], s_cmap[ai]);
+        a.cellMap[tc.gc0 + recCell(r)] = mapEntry(s_vpre[ai], s_tab[recIndex(r)]);
     }
     if (tid == 0) {
         a.tileV[tc.t] = VT;
@@ -512,7 +514,7 @@ template <int CPT>
 __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked) {
     constexpr int TILE = GP_THREADS * CPT;
     __shared__ uint32_t s_rec[TILE];
-    __shared__ uint32_t s_cmap[TILE];  // per active cell: contour of each crossed edge, 2 bits per edge
+    __shared__ uint32_t s_tab[kPolyTabSize];  // c_polyCmap: per table entry the contour of each crossed edge
     __shared__ uint16_t s_vpre[TILE];
     __shared__ uint16_t s_qpre[TILE];
     __shared__ uint16_t s_aidx[TILE];  // per tile cell: its active index (0xFFFF: inactive, adjoint map only)
@@ -526,6 +528,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     const int64_t c0 = tc.c0;
     const int NA = a.tileNA[tc.t];
     const int64_t rb = tc.t * TILE;
+    for (int i = tid; i < kPolyTabSize; i += GP_THREADS) s_tab[i] = c_polyCmap[i];
     if (a.adjMap) {
         for (int l = tid; l < TILE; l += GP_THREADS) s_aidx[l] = 0xFFFF;
         __syncthreads();
@@ -533,7 +536,6 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
         const uint32_t r = a.rec[rb + ai];
         s_rec[ai] = r;
-        s_cmap[ai] = c_polyCmap[recIndex(r)];
         s_aidx[recCell(r)] = (uint16_t)ai;
     }
     __syncthreads();
@@ -557,14 +559,13 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     for (int ai = tid; ai < NA; ai += GP_THREADS) {
         const uint32_t r = s_rec[ai];
         const int l = recCell(r);
-        const uint32_t cmap = s_cmap[ai];
-        // per vertex: its cell and contour; kVertices computes position and normal
-        const uint64_t w = c_polyTab[recIndex(r)];
+        const uint32_t cmap = s_tab[recIndex(r)];
+        // per vertex: its cell, table entry and contour; kVertices computes position and normal
         const int nv = recVerts(r);
         const int64_t v0 = voff + s_vpre[ai];
         for (int t = 0; t < nv; ++t) {
             a.vcell[v0 + t] = (int)(c0 + l);
-            a.vcw[v0 + t] = contourWord(w, t);
+            a.vct[v0 + t] = (uint16_t)(recIndex(r) | (t << 10));
             if (a.vpid) a.vpid[v0 + t] = tc.prob;
         }
         // quads whose edge starts at this cell.  Their other cells lie 1, CX, CX+1, CX*CY,
@@ -599,7 +600,7 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
                 int vid;
                 if (n >= c0) {
                     const int an = s_aidx[n - c0];
-                    vid = (int)(voff + s_vpre[an] + (int)((s_cmap[an] >> (2 * e)) & 3u));
+                    vid = (int)(voff + s_vpre[an] + (int)((s_tab[recIndex(s_rec[an])] >> (2 * e)) & 3u));
                 } else {
                     const uint32_t m = a.cellMap[P.cellBase + n];
                     vid = (int)(a.tileVOff[P.tileBase + n / TILE] + (int64_t)(m & 0x3FFFu) + mapContour(m, e));
@@ -677,7 +678,7 @@ __global__ void __launch_bounds__(SLAB_COLS) kSlab(GridPassArgs a, int chunked) 
     __shared__ float s_pl[2][PR][SLAB_COLS];  // point planes, plane z in s_pl[z & 1]
     __shared__ uint8_t s_mc[NC];
     __shared__ uint16_t s_act[NC];            // active cell slots, row-major
-    __shared__ uint64_t s_poly[NC];           // polygon word by active index
+    __shared__ uint16_t s_poly[NC];           // polygon table entry by active index
     __shared__ uint32_t s_cnt[NC];            // vertices | quads << 8 | owned bits (x, y, z edge, edgy) << 16
     __shared__ int32_t s_id[RING][NC];        // first vertex id of the cell, -1 without one; ring by z parity
     __shared__ uint32_t s_cm[RING][NC];       // contour of each crossed edge, 2 bits per edge
@@ -765,11 +766,12 @@ __global__ void __launch_bounds__(SLAB_COLS) kSlab(GridPassArgs a, int chunked) 
             const int r = c / SLAB_COLS, col = c % SLAB_COLS;
             const int k0 = r * SLAB_COLS + col, k1 = (r + 1) * SLAB_COLS + col;
             const float sv[8] = {lo[k0], lo[k0 + 1], lo[k1], lo[k1 + 1], hi[k0], hi[k0 + 1], hi[k1], hi[k1 + 1]};
-            const uint64_t w = cellPolygonTab(s_mc[c], sv, iso);
+            const int idx = deciderIndex(c_polyBase[s_mc[c]], sv, iso);
+            const uint64_t w = c_polyTab[idx];
             int own = 0;
             if (r >= 1 && col >= 1) own = ownedBits(sv, iso, bc.x0 - 1 + col, bc.y0 - 1 + r, z);
             s_cnt[c] = (uint32_t)polyCount(w) | ((uint32_t)__popc(own & 7) << 8) | ((uint32_t)own << 16);
-            s_poly[ai] = w;
+            s_poly[ai] = (uint16_t)idx;
             if (EMIT) s_cm[ring][c] = contourMap(w);
         }
         __syncthreads();
@@ -843,13 +845,13 @@ __global__ void __launch_bounds__(SLAB_COLS) kSlab(GridPassArgs a, int chunked) 
                 const uint32_t cnt = s_cnt[c];
                 const int cx = bc.x0 - 1 + col, cy = bc.y0 - 1 + r;
                 const int id = s_id[ring][c];
-                const uint64_t w = s_poly[ai];
+                const int pidx = s_poly[ai];
                 const int nv = (int)(cnt & 0xFFu);
                 const int32_t cidx = (int32_t)(((int64_t)z * CY + cy) * CX + cx);
                 for (int t = 0; t < nv; ++t) {
                     const int64_t v = (int64_t)id + t;
                     a.vcell[v] = cidx;
-                    a.vcw[v] = contourWord(w, t);
+                    a.vct[v] = (uint16_t)(pidx | (t << 10));
                     if (a.vpid) a.vpid[v] = bc.prob;
                 }
                 const int own = (int)(cnt >> 16);
@@ -934,7 +936,8 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
         y = (int)(r % (unsigned)P.CY);
         z = (int)(r / (unsigned)P.CY);
         loadCorners(g, x, y, z, s);
-        p = edgeMean(s, iso, a.vcw[v]);
+        const int ct = a.vct[v];
+        p = edgeMean(s, iso, contourWord(c_polyTab[ct & 0x3FF], ct >> 10));
         if (a.surfIter) {
             moverInit(m, s, p);
             for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);

@@ -2104,7 +2104,7 @@ void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     a.qattr = arena.get<uint32_t>("qattr", (size_t)std::max<int64_t>(Q, 1));
     a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
     a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
-    a.vcw = arena.get<uint64_t>("gp_vcw", (size_t)std::max<int64_t>(V, 1));
+    a.vct = arena.get<uint16_t>("gp_vct", (size_t)std::max<int64_t>(V, 1));
     a.V = V;
     a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
     MDMC_HIP(hipEventRecord(kev[2], stream));
@@ -2332,7 +2332,7 @@ void Pipeline::gridPassBack(int64_t C, int shape, bool recordAdjoint) {
     a.qattr = arena.get<uint32_t>("qattr", (size_t)std::max<int64_t>(Q, 1));
     a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
     a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
-    a.vcw = arena.get<uint64_t>("gp_vcw", (size_t)std::max<int64_t>(V, 1));
+    a.vct = arena.get<uint16_t>("gp_vct", (size_t)std::max<int64_t>(V, 1));
     a.V = V;
     a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
     MDMC_HIP(hipEventRecord(kev[2], stream));
