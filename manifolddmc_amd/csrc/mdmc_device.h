@@ -267,8 +267,10 @@ __device__ __forceinline__ void movePointToSurface(const float* s, float iso, f3
     point = moverResult(m);
 }
 
-// QuadrilateralAttribute bits (DeviceVectors.h:334-450) and vertex Flags bits (Halfedge.h:14-111)
-enum : uint32_t { QA_COLOR = 0x1F, QA_NM = 0x20, QA_PATTERN = 0x40, QA_REMOVE = 0x80, QA_BOUNDARY = 0x100 };
+// QuadrilateralAttribute bits (DeviceVectors.h:334-450) and vertex Flags bits (Halfedge.h:14-111).
+// QA_DEAD has no reference counterpart: a quad removed by an earlier simplification of a chain
+// whose compaction is deferred to the chain's last step (never in an output array).
+enum : uint32_t { QA_COLOR = 0x1F, QA_NM = 0x20, QA_PATTERN = 0x40, QA_REMOVE = 0x80, QA_BOUNDARY = 0x100, QA_DEAD = 0x200 };
 enum : uint32_t { VF_NM = 1, VF_BOUNDARY = 2, VF_WEIRD = 4, VF_FANS = 8, VF_REMOVE = 16 };
 
 }  // namespace mdmc

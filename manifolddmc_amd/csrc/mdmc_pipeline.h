@@ -34,6 +34,8 @@ class Pipeline {
     // vertex valence they start from
     int simplifyQueue = 0;
     bool valenceReady = false;
+    // simplifications run since the initialize without compaction (the last of a chain compacts)
+    int deferredSteps = 0;
     // vertices the outgoing lists of the last halfedge build describe (-1: none), and
     // whether a split since then keeps the per-vertex "touched" flags
     int64_t listsV = -1;
@@ -161,6 +163,7 @@ class Pipeline {
     void checkValid(const char* what);
     void simplifyInit(int32_t** valence, int32_t** count);
     void simplifyCompactAndRebuild(int dropType, bool p3333);
+    void simplifyDeferredStep(int dropType, bool p3333);
     void buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag);
     int32_t* stableSortByKey(const int32_t* key, int64_t n, int B, const std::string& tag);
     void countSorted(const int32_t* key, const int32_t* perm, int64_t n, int B, unsigned long long* count, int stride, int slot);

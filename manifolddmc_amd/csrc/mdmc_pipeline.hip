@@ -566,6 +566,23 @@ __global__ void kMaterialize(const int4* __restrict__ quads, int64_t H, const in
 // onto either endpoint and no member lay in a removed quad), renumbered in the same order,
 // and its halfedges keep size, rank and (renumbered) twin; the others are recomputed from
 // the new outgoing lists.  qnew[f] is the new id of old quad f (-1: removed).
+//
+// Source chains (by old vertex id): the outgoing halfedges of vertex t are the entries of the
+// lists of t and of every vertex on its chain head[t], next[head[t]], ...  A dropped vertex u
+// joins its target's chain together with its own chain (vertices dropped into u by earlier
+// steps of a deferred chain of simplifications).  Entries of removed or dead quads stay in the
+// lists and are skipped by their quad.
+__device__ __forceinline__ void spliceChain(int u, int t, int32_t* __restrict__ head, int32_t* __restrict__ next) {
+    int last = u;
+    const int h0 = head[u];
+    if (h0 >= 0) {
+        next[u] = h0;
+        last = h0;
+        while (next[last] >= 0) last = next[last];
+    }
+    next[last] = atomicExch(&head[t], u);
+}
+
 __global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, const int32_t* __restrict__ address, int64_t V,
                          int dropType, uint8_t* __restrict__ touched, int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext) {
     const int64_t u0 = firstOfFour();
@@ -588,22 +605,36 @@ __global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __res
         const int tn = address[t];
         if (tn < 0) continue;  // the relabelled quads then hold -1 and fail validation
         touched[tn] = 1;
-        srcNext[u] = atomicExch(&srcHead[tn], (int)u);
+        spliceChain((int)u, t, srcHead, srcNext);
     }
 }
 
-// new vertex v': the halfedges of kept quads in the old lists of its own vertex and of the
-// dropped vertices relabelled to it (a chain through srcNext), counted (list lengths less
-// the halfedges of removed quads), then written with their new ids (the bucket pass checks
-// every entry it reads against the quad's corner)
-__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ removedAt,
-                          const int32_t* __restrict__ srcHead, const int32_t* __restrict__ srcNext, int32_t* __restrict__ cnt) {
+// new vertex v': the halfedges of kept quads in the lists of its old vertex and of the
+// vertices on that vertex's chain, counted (with removedAt: list lengths less the halfedges of
+// removed quads, when every list is by current corner; without it: entries whose quad is
+// kept), then written with their new ids (the bucket pass checks every entry it reads
+// against the quad's corner)
+__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                          const int32_t* __restrict__ removedAt, const int32_t* __restrict__ qnew, const int32_t* __restrict__ srcHead,
+                          const int32_t* __restrict__ srcNext, int32_t* __restrict__ cnt) {
     const int64_t vn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vn >= Vn) return;
     int64_t c = 0;
-    int u = oldV[vn], s = srcHead[vn];
+    int u = oldV[vn];
+    int s = srcHead[u];
     for (;;) {
-        c += off[u + 1] - off[u] - removedAt[u];
+        if (removedAt) {
+            c += off[u + 1] - off[u] - removedAt[u];
+        } else {
+            const int64_t k1 = off[u + 1];
+            for (int64_t kb = off[u]; kb < k1; kb += 4) {
+                int xs[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) xs[i] = kb + i < k1 ? out[kb + i] : -4;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) c += (xs[i] >= 0 && qnew[xs[i] >> 2] >= 0) ? 1 : 0;
+            }
+        }
         if (s < 0) break;
         u = s;
         s = srcNext[s];
@@ -618,7 +649,8 @@ __global__ void kOutFill(const int32_t* __restrict__ oldV, int64_t Vn, const int
     const int64_t vn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vn >= Vn) return;
     int64_t w = off2[vn];
-    int u = oldV[vn], s = srcHead[vn];
+    int u = oldV[vn];
+    int s = srcHead[u];
     for (;;) {
         // groups of four list entries, their new quad ids loaded side by side
         const int64_t k1 = off[u + 1];
@@ -694,6 +726,212 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
         const int4 q = quads2[h >> 2];
         const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
         storeHalfedge(h, a, b, bucketFromOut(h, a, b, quads2, H2, off2, out2, ctr), twin, bpack, vflags, fattr, qattr, valence);
+    }
+}
+
+// ---- a chain of simplifications with its compaction deferred to the chain's last step.
+// Between the steps every id stays as the halfedge initialize numbered it: a removed quad
+// becomes dead (QA_DEAD in qattr and fattr, skipped by every later kernel), a dropped vertex
+// becomes dead (vdead) after splicing itself and its chain onto its target's chain, and the
+// surviving quads have their corners relabelled in place.  Twins and bucket words are
+// recomputed in place for the halfedges with a touched endpoint (the targets) only; the
+// others keep theirs (the same argument as for the compacting rebuild above).  The chain's
+// last step compacts vertices and quads once, renumbering everything, which gives the arrays
+// the step-by-step compactions give (stable compactions compose, and bucket ranks and the
+// twin rules only compare ids, whose order every compaction keeps).
+
+// out-degree of every vertex from the initialize's lists
+__global__ void kOutdegInit(const int64_t* __restrict__ off, int64_t V, int32_t* __restrict__ outdeg) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    outdeg[v] = (int32_t)(off[v + 1] - off[v]);
+}
+
+// removeQuadrilateralsP3X3Y / removeQuadsP3333 in place: a removed quad dies (its corners each
+// lose an outgoing halfedge); a kept quad gets the compaction's attributes (colour unset,
+// NM and boundary kept) and its dropped corners (P3X3Y) or all its corners (P3333 pattern
+// face) relabelled to their targets
+// (attrSrc is qattr or fattr: no __restrict__ on the three attribute arrays)
+__global__ void kApplyRemovals(int4* __restrict__ quads, const uint32_t* attrSrc, int64_t Q, const int32_t* __restrict__ vtype,
+                               const int32_t* __restrict__ vtarget, int dropType, int p3333, uint32_t* qattr, uint32_t* fattr,
+                               int32_t* __restrict__ outdeg, int64_t V, int64_t* ctr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const uint32_t a = attrSrc[q];
+    if (a & QA_DEAD) return;
+    const int4 v4 = quads[q];
+    const int v[4] = {v4.x, v4.y, v4.z, v4.w};
+    if (a & QA_REMOVE) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (v[i] >= 0 && v[i] < V) atomicSub(&outdeg[v[i]], 1);
+        qattr[q] = QA_DEAD;
+        fattr[q] = QA_DEAD;
+        return;
+    }
+    const bool pat = p3333 && (a & QA_PATTERN) != 0;
+    int o[4];
+    bool bad = false, moved = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const bool in = v[i] >= 0 && v[i] < V;
+        o[i] = in ? v[i] : -1;
+        if (in && (pat || vtype[v[i]] == dropType)) {
+            // P3333: a dropped corner outside a pattern face has no new vertex (invalid, as in
+            // the compacting path); P3X3Y: the target, which must itself survive
+            const int src = vtarget[v[i]];
+            o[i] = (src >= 0 && src < V && (pat || !p3333) && vtype[src] != dropType) ? src : -1;
+            moved = true;
+        }
+        bad = bad || o[i] < 0;
+    }
+    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+    if (moved) quads[q] = make_int4(o[0], o[1], o[2], o[3]);
+    uint32_t na = QA_COLOR;
+    if (a & QA_NM) na |= QA_NM;
+    if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
+    qattr[q] = na;
+    fattr[q] = na;
+}
+
+// a dropped vertex dies: it and its chain join its target's chain, its out-degree goes to the
+// target (after kApplyRemovals took the removed quads' halfedges off), the target is touched
+__global__ void kDropVertices(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V, int dropType,
+                              uint8_t* __restrict__ vdead, int32_t* __restrict__ head, int32_t* __restrict__ next, int32_t* __restrict__ outdeg,
+                              uint8_t* __restrict__ touched) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= V || vtype[u] != dropType) return;
+    const int t = vtarget[u];
+    if (t < 0 || t >= V || vtype[t] == dropType) return;  // the relabelled quads then hold -1 and fail validation
+    vdead[u] = 1;
+    touched[t] = 1;
+    spliceChain((int)u, t, head, next);
+    atomicAdd(&outdeg[t], outdeg[u]);
+    outdeg[u] = 0;
+}
+
+// valence = out-degree (incoming boundary halfedges added by the rebuild); flags cleared
+__global__ void kValenceInit(const int32_t* __restrict__ outdeg, int64_t V, int32_t* __restrict__ valence, uint32_t* __restrict__ vflags) {
+    const int64_t v0 = firstOfFour();
+    if (v0 >= V) return;
+    if (vec4(v0, V, outdeg, valence) && vec4(v0, V, vflags, vflags)) {
+        reinterpret_cast<int4*>(valence)[v0 >> 2] = reinterpret_cast<const int4*>(outdeg)[v0 >> 2];
+        reinterpret_cast<uint4*>(vflags)[v0 >> 2] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        for (int64_t v = v0; v < v0 + 4 && v < V; ++v) {
+            valence[v] = outdeg[v];
+            vflags[v] = 0u;
+        }
+    }
+}
+
+// bucketFromOut over the chain lists: the entries of m's list and of its chain's lists, those
+// of dead quads skipped
+__device__ BucketOf bucketFromChains(int h, int a, int b, const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t H,
+                                     const int64_t* __restrict__ off, const int32_t* __restrict__ out, const int32_t* __restrict__ head,
+                                     const int32_t* __restrict__ next, int64_t* ctr) {
+    const int m = min(a, b), o = max(a, b);
+    int mem[4], dir[4];
+    int n = 0, rk = 0, partner = -1;
+    bool self = false;
+    auto add = [&](int id, int d) {
+        if (n < 4) {
+            mem[n] = id;
+            dir[n] = d;
+        }
+        ++n;
+        if (id < h) ++rk;
+        if (id != h) partner = id;
+        else self = true;
+    };
+    int u = m, s = head[m];
+    for (;;) {
+        const int64_t k0 = off[u], k1 = off[u + 1];
+        for (int64_t kb = k0; kb < k1; kb += 4) {
+            int xs[4];
+            int4 qs[4];
+            uint32_t at[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xs[i] = kb + i < k1 ? out[kb + i] : -1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool ok = xs[i] >= 0 && xs[i] < H;
+                qs[i] = ok ? quads[xs[i] >> 2] : make_int4(0, 0, 0, 0);
+                at[i] = ok ? qattr[xs[i] >> 2] : QA_DEAD;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (kb + i >= k1) continue;
+                const int x = xs[i];
+                if (x < 0 || x >= H) {
+                    anomaly(ctr);
+                    continue;
+                }
+                if (at[i] & QA_DEAD) continue;
+                const int4 q = qs[i];
+                const int j = x & 3;
+                if (quadCorner(q, j) != m) anomaly(ctr);
+                if (quadCorner(q, (j + 1) & 3) == o) add(x, 0);
+                if (o != m && quadCorner(q, (j + 3) & 3) == o) add((x & ~3) | ((j + 3) & 3), 1);
+            }
+        }
+        if (s < 0) break;
+        u = s;
+        s = next[s];
+    }
+    if (!self) anomaly(ctr);
+    return BucketOf{n, rk, bucketTwin(n, rk, partner, mem, dir)};
+}
+
+// the rebuild of a deferred step, in place: every live halfedge re-applies its bucket's flags,
+// boundary valence and NM face bits; those with a touched endpoint are first recomputed (on
+// full waves, gathered per block in LDS) and store twin and bucket word
+__global__ void __launch_bounds__(kBlock) kDeferRebuild(const int4* __restrict__ quads, int64_t H, const uint8_t* __restrict__ touched,
+                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                                                        const int32_t* __restrict__ head, const int32_t* __restrict__ next,
+                                                        int32_t* __restrict__ twin, uint32_t* __restrict__ bpack, uint32_t* __restrict__ vflags,
+                                                        uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence,
+                                                        int64_t* ctr) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool later = false;
+    if (hh < H && !(qattr[hh >> 2] & QA_DEAD)) {
+        const int h = (int)hh;
+        const int4 q = quads[h >> 2];
+        const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+        const int tt = twin[h];
+        const uint32_t bo = bpack[h];
+        if (touched[a] | touched[b]) {
+            later = true;
+        } else {
+            // twin and bucket word unchanged: the flags alone
+            const int n = (int)(bo & 0xFFFFu);
+            if (tt == -1) atomicAdd(&valence[b], 1);
+            const uint32_t vf = bucketVertexFlag(n);
+            if (vf) atomicOr(&vflags[a], vf);
+            if (n == 4) {
+                atomicOr(&fattr[h >> 2], (uint32_t)QA_NM);
+                atomicOr(&qattr[h >> 2], (uint32_t)QA_NM);
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t m = __ballot(later);
+    if (lane == 0) s_wave[wid] = __popcll(m);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
+    }
+    if (later) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)hh;
+    __syncthreads();
+    if ((int)threadIdx.x < total) {
+        const int h = s_list[threadIdx.x];
+        const int4 q = quads[h >> 2];
+        const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+        storeHalfedge(h, a, b, bucketFromChains(h, a, b, quads, qattr, H, off, out, head, next, ctr), twin, bpack, vflags, fattr, qattr, valence);
     }
 }
 
@@ -826,7 +1064,7 @@ __global__ void kMark3x3y(const int4* __restrict__ quads, uint32_t* __restrict__
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     uint32_t a = qattr[q];
-    if ((a & QA_NM) || (a & QA_BOUNDARY)) return;
+    if ((a & QA_NM) || (a & QA_BOUNDARY) || (a & QA_DEAD)) return;
     a &= 0x3F;
     const int4 v = quads[q];
     const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
@@ -838,7 +1076,9 @@ __global__ void kCount3x3yOld(const int4* __restrict__ quads, uint32_t* __restri
                               int32_t* __restrict__ count) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
-    uint32_t a = qattr[q] & 0x3F;
+    const uint32_t a0 = qattr[q];
+    if (a0 & QA_DEAD) return;
+    uint32_t a = a0 & 0x3F;
     const int4 v = quads[q];
     const int val[4] = {valence[v.x], valence[v.y], valence[v.z], valence[v.w]};
     if (pattern1(val)) {
@@ -925,7 +1165,7 @@ __global__ void kClear3333(uint32_t* __restrict__ fattr, int64_t F) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const uint32_t a = fattr[f];
-    if ((a & QA_NM) || (a & QA_BOUNDARY)) return;
+    if ((a & QA_NM) || (a & QA_BOUNDARY) || (a & QA_DEAD)) return;
     fattr[f] = a & 0x3F;
 }
 
@@ -935,7 +1175,7 @@ __global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restr
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const uint32_t a0 = fattr[f];
-    if ((a0 & QA_NM) || (a0 & QA_BOUNDARY)) return;
+    if ((a0 & QA_NM) || (a0 & QA_BOUNDARY) || (a0 & QA_DEAD)) return;
     const int4 qv = quads[f];
     const int org[4] = {qv.x, qv.y, qv.z, qv.w};
     // the four valences (and below the four neighbours) loaded side by side before any test
@@ -973,14 +1213,15 @@ __global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restr
     }
 }
 
-__global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int dropType, int32_t* __restrict__ keep) {
+// keep = not dropped by this simplification nor (vdead, when given) by an earlier one of the chain
+__global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int dropType, const uint8_t* __restrict__ vdead, int32_t* __restrict__ keep) {
     const int64_t v0 = firstOfFour();
     if (v0 >= V) return;
-    if (vec4(v0, V, vtype, keep)) {
+    if (vec4(v0, V, vtype, keep) && !vdead) {
         const int4 t = reinterpret_cast<const int4*>(vtype)[v0 >> 2];
         reinterpret_cast<int4*>(keep)[v0 >> 2] = make_int4(t.x != dropType, t.y != dropType, t.z != dropType, t.w != dropType);
     } else {
-        for (int64_t v = v0; v < v0 + 4 && v < V; ++v) keep[v] = vtype[v] == dropType ? 0 : 1;
+        for (int64_t v = v0; v < v0 + 4 && v < V; ++v) keep[v] = (vtype[v] == dropType || (vdead && vdead[v])) ? 0 : 1;
     }
 }
 
@@ -1019,9 +1260,10 @@ __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t
     if (q0 >= Q) return;
     if (vec4(q0, Q, attr, keep)) {
         const uint4 a = reinterpret_cast<const uint4*>(attr)[q0 >> 2];
-        reinterpret_cast<int4*>(keep)[q0 >> 2] = make_int4(!(a.x & QA_REMOVE), !(a.y & QA_REMOVE), !(a.z & QA_REMOVE), !(a.w & QA_REMOVE));
+        constexpr uint32_t gone = QA_REMOVE | QA_DEAD;
+        reinterpret_cast<int4*>(keep)[q0 >> 2] = make_int4(!(a.x & gone), !(a.y & gone), !(a.z & gone), !(a.w & gone));
     } else {
-        for (int64_t q = q0; q < q0 + 4 && q < Q; ++q) keep[q] = (attr[q] & QA_REMOVE) ? 0 : 1;
+        for (int64_t q = q0; q < q0 + 4 && q < Q; ++q) keep[q] = (attr[q] & (QA_REMOVE | QA_DEAD)) ? 0 : 1;
     }
 }
 
@@ -1033,13 +1275,16 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     if (!keep[q]) {
-        // the outgoing halfedges this quad takes from its corners' lists
+        // the outgoing halfedges this quad takes from its corners' lists (when the lists are
+        // by current corner; a quad dead since an earlier step of the chain takes none)
         qnew[q] = -1;
-        const int4 r = quads[q];
-        atomicAdd(&removedAt[r.x], 1);
-        atomicAdd(&removedAt[r.y], 1);
-        atomicAdd(&removedAt[r.z], 1);
-        atomicAdd(&removedAt[r.w], 1);
+        if (removedAt && !(attr[q] & QA_DEAD)) {
+            const int4 r = quads[q];
+            atomicAdd(&removedAt[r.x], 1);
+            atomicAdd(&removedAt[r.y], 1);
+            atomicAdd(&removedAt[r.z], 1);
+            atomicAdd(&removedAt[r.w], 1);
+        }
         return;
     }
     qnew[q] = (int32_t)qaddr[q];
@@ -2414,6 +2659,7 @@ void Pipeline::halfedgeInitialize() {
     if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, ends, twin, bpack, vflags, fattr, qattr, valence, counters());
     heMaterialized = false;
     posSlotted = false;  // the grid pass wrote positions by vertex
+    deferredSteps = 0;
     listsV = V;
     splitTouched = false;
     canonTracked = false;
@@ -2453,7 +2699,53 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
 }
 
 // ---- simplifications
+// a simplification followed by another one of the chain: removals and relabelling in place,
+// no compaction (the ids stay the initialize's; the chain's last step compacts)
+void Pipeline::simplifyDeferredStep(int dropType, bool p3333) {
+    int32_t* vtype = arena.ptr<int32_t>("vtype");
+    int32_t* vtarget = arena.ptr<int32_t>("vtarget");
+    uint32_t* qattr = arena.ptr<uint32_t>("qattr");
+    uint32_t* fattr = arena.ptr<uint32_t>("fattr");
+    int4* quads = arena.ptr<int4>("quads");
+    const int64_t* off = arena.ptr<int64_t>("out_off");
+    const int32_t* out = arena.ptr<int32_t>("out_list");
+    const size_t nv = (size_t)std::max<int64_t>(V, 1);
+    int32_t* outdeg = arena.get<int32_t>("outdeg", nv);
+    uint8_t* vdead = arena.get<uint8_t>("vdead", nv);
+    int32_t* head = arena.get<int32_t>("src_head", nv);
+    int32_t* next = arena.get<int32_t>("src_next", nv);
+    if (deferredSteps == 0) {
+        if (V > 0) kOutdegInit<<<gridFor(V), kBlock, 0, stream>>>(off, V, outdeg);
+        zero(vdead, (size_t)V);
+        fillNeg(head, (size_t)V * sizeof(int32_t));
+    }
+    uint8_t* touched = arena.get<uint8_t>("touched", nv);
+    zero(touched, (size_t)V);
+    if (Q > 0)
+        kApplyRemovals<<<gridFor(Q), kBlock, 0, stream>>>(quads, p3333 ? fattr : qattr, Q, vtype, vtarget, dropType, p3333 ? 1 : 0, qattr, fattr,
+                                                          outdeg, V, counters());
+    if (V > 0) kDropVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, vdead, head, next, outdeg, touched);
+    int32_t* valence = arena.get<int32_t>("valence", nv);
+    uint32_t* vflags = arena.ptr<uint32_t>("vflags");
+    if (V > 0) kValenceInit<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(outdeg, V, valence, vflags);
+    if (H > 0)
+        kDeferRebuild<<<gridFor(H), kBlock, 0, stream>>>(quads, H, touched, off, out, head, next, arena.ptr<int32_t>("twin"), arena.ptr<uint32_t>("bpack"),
+                                                         vflags, fattr, qattr, valence, counters());
+    valenceReady = true;
+    ++deferredSteps;
+    heMaterialized = false;
+    splitTouched = false;
+    canonTracked = false;
+}
+
 void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
+    if (simplifyQueue > 0) {  // another simplification follows
+        simplifyDeferredStep(dropType, p3333);
+        return;
+    }
+    // the chain's last step: this step's removals and the dead elements of the earlier ones
+    const bool chained = deferredSteps > 0;
+    deferredSteps = 0;
     int32_t* vtype = arena.ptr<int32_t>("vtype");
     int32_t* vtarget = arena.ptr<int32_t>("vtarget");
     const uint32_t* attr = p3333 ? arena.ptr<uint32_t>("fattr") : arena.ptr<uint32_t>("qattr");
@@ -2464,7 +2756,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int64_t* qaddr = arena.get<int64_t>("qaddr", (size_t)Q + 1);
     void* sv = arena.get<char>("scan_scratch_v", scanScratchBytes(V + 1));
     void* sq = arena.get<char>("scan_scratch_q", scanScratchBytes(Q + 1));
-    if (V > 0) kKeepVertices<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, V, dropType, keep);
+    if (V > 0) kKeepVertices<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, V, dropType, chained ? arena.ptr<uint8_t>("vdead") : nullptr, keep);
     if (Q > 0) kKeepQuads<<<gridFor((Q + 3) / 4), kBlock, 0, stream>>>(attr, Q, keepq);
     // totals straight into pinned host memory: [0] direct-sum diagnostic, [1] V, [2] Q
     int64_t* h = pinHost();
@@ -2489,8 +2781,10 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     uint32_t* nattr = arena.get<uint32_t>("qattr2", capQ);
     int32_t* oldQ = arena.get<int32_t>("oldQ", (size_t)std::max<int64_t>(nQ, 1));
     int32_t* qnew = arena.get<int32_t>("qnew", (size_t)std::max<int64_t>(Q, 1));
-    int32_t* removedAt = arena.get<int32_t>("removed_at", (size_t)std::max<int64_t>(V, 1));
-    zero(removedAt, (size_t)V * sizeof(int32_t));
+    // with every list still by current corner the removed quads' entries are counted per
+    // corner; after deferred steps the lists are the initialize's and kOutCount checks entries
+    int32_t* removedAt = chained ? nullptr : arena.get<int32_t>("removed_at", (size_t)std::max<int64_t>(V, 1));
+    if (removedAt) zero(removedAt, (size_t)V * sizeof(int32_t));
     if (Q > 0)
         kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ,
                                                          qnew, removedAt, V, counters());
@@ -2501,16 +2795,17 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     // renumbered; halfedges with untouched endpoints keep their bucket results
     const int64_t nH = 4 * nQ;
     uint8_t* touched = arena.get<uint8_t>("touched", (size_t)std::max<int64_t>(nV, 1));
-    int32_t* srcHead = arena.get<int32_t>("src_head", (size_t)std::max<int64_t>(nV, 1));
+    // source chains by old vertex id (kept from the deferred steps of the chain)
+    int32_t* srcHead = arena.get<int32_t>("src_head", (size_t)std::max<int64_t>(V, 1));
     int32_t* srcNext = arena.get<int32_t>("src_next", (size_t)std::max<int64_t>(V, 1));
     zero(touched, (size_t)nV);
-    fillNeg(srcHead, (size_t)nV * sizeof(int32_t));
+    if (!chained) fillNeg(srcHead, (size_t)V * sizeof(int32_t));
     if (V > 0) kSources<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext);
     const int64_t* off = arena.ptr<int64_t>("out_off");
     const int32_t* out = arena.ptr<int32_t>("out_list");
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
     zero(cnt + nV, sizeof(int32_t));
-    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, removedAt, srcHead, srcNext, cnt);
+    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, removedAt, qnew, srcHead, srcNext, cnt);
     int64_t* off2 = arena.get<int64_t>("out_off2", (size_t)std::max<int64_t>(nV, dmcV) + 1);
     scanExclusiveI32(cnt, off2, nV + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(nV + 1)), stream);
     int32_t* out2 = arena.get<int32_t>("out_list2", 4 * capQ);
