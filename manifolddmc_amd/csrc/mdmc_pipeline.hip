@@ -584,7 +584,8 @@ __device__ __forceinline__ void spliceChain(int u, int t, int32_t* __restrict__ 
 }
 
 __global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, const int32_t* __restrict__ address, int64_t V,
-                         int dropType, uint8_t* __restrict__ touched, int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext) {
+                         int dropType, uint8_t* __restrict__ touched, int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext,
+                         int32_t* __restrict__ outdeg) {
     const int64_t u0 = firstOfFour();
     if (u0 >= V) return;
     int ty[4];
@@ -606,35 +607,32 @@ __global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __res
         if (tn < 0) continue;  // the relabelled quads then hold -1 and fail validation
         touched[tn] = 1;
         spliceChain((int)u, t, srcHead, srcNext);
+        if (outdeg) {
+            atomicAdd(&outdeg[t], outdeg[u]);
+            outdeg[u] = 0;
+        }
     }
 }
 
 // new vertex v': the halfedges of kept quads in the lists of its old vertex and of the
-// vertices on that vertex's chain, counted (with removedAt: list lengths less the halfedges of
-// removed quads, when every list is by current corner; without it: entries whose quad is
-// kept), then written with their new ids (the bucket pass checks every entry it reads
-// against the quad's corner)
-__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                          const int32_t* __restrict__ removedAt, const int32_t* __restrict__ qnew, const int32_t* __restrict__ srcHead,
-                          const int32_t* __restrict__ srcNext, int32_t* __restrict__ cnt) {
+// vertices on that vertex's chain, counted (every list by current corner: list lengths less the
+// halfedges of removed quads, removedAt; after deferred steps: the out-degree kept through the
+// chain's removals and drops), then written with their new ids (the bucket pass checks every
+// entry it reads against the quad's corner)
+__global__ void kOutCount(const int32_t* __restrict__ oldV, int64_t Vn, const int64_t* __restrict__ off, const int32_t* __restrict__ removedAt,
+                          const int32_t* __restrict__ srcHead, const int32_t* __restrict__ srcNext, const int32_t* __restrict__ outdeg,
+                          int32_t* __restrict__ cnt) {
     const int64_t vn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vn >= Vn) return;
-    int64_t c = 0;
     int u = oldV[vn];
+    if (outdeg) {
+        cnt[vn] = outdeg[u];
+        return;
+    }
+    int64_t c = 0;
     int s = srcHead[u];
     for (;;) {
-        if (removedAt) {
-            c += off[u + 1] - off[u] - removedAt[u];
-        } else {
-            const int64_t k1 = off[u + 1];
-            for (int64_t kb = off[u]; kb < k1; kb += 4) {
-                int xs[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) xs[i] = kb + i < k1 ? out[kb + i] : -4;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) c += (xs[i] >= 0 && qnew[xs[i] >> 2] >= 0) ? 1 : 0;
-            }
-        }
+        c += off[u + 1] - off[u] - removedAt[u];
         if (s < 0) break;
         u = s;
         s = srcNext[s];
@@ -748,45 +746,27 @@ __global__ void kOutdegInit(const int64_t* __restrict__ off, int64_t V, int32_t*
 }
 
 // removeQuadrilateralsP3X3Y / removeQuadsP3333 in place: a removed quad dies (its corners each
-// lose an outgoing halfedge); a kept quad gets the compaction's attributes (colour unset,
-// NM and boundary kept) and its dropped corners (P3X3Y) or all its corners (P3333 pattern
-// face) relabelled to their targets
+// lose an outgoing halfedge; its corners become -1, so that list walks see it dead from the
+// quad alone); a kept quad gets the compaction's attributes (colour unset, NM and boundary
+// kept).  Corners are relabelled by kDropVertices.
 // (attrSrc is qattr or fattr: no __restrict__ on the three attribute arrays)
-__global__ void kApplyRemovals(int4* __restrict__ quads, const uint32_t* attrSrc, int64_t Q, const int32_t* __restrict__ vtype,
-                               const int32_t* __restrict__ vtarget, int dropType, int p3333, uint32_t* qattr, uint32_t* fattr,
-                               int32_t* __restrict__ outdeg, int64_t V, int64_t* ctr) {
+__global__ void kApplyRemovals(int4* __restrict__ quads, const uint32_t* attrSrc, int64_t Q, uint32_t* qattr, uint32_t* fattr,
+                               int32_t* __restrict__ outdeg, int64_t V) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const uint32_t a = attrSrc[q];
     if (a & QA_DEAD) return;
-    const int4 v4 = quads[q];
-    const int v[4] = {v4.x, v4.y, v4.z, v4.w};
     if (a & QA_REMOVE) {
+        const int4 v4 = quads[q];
+        const int v[4] = {v4.x, v4.y, v4.z, v4.w};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if (v[i] >= 0 && v[i] < V) atomicSub(&outdeg[v[i]], 1);
+        quads[q] = make_int4(-1, -1, -1, -1);
         qattr[q] = QA_DEAD;
         fattr[q] = QA_DEAD;
         return;
     }
-    const bool pat = p3333 && (a & QA_PATTERN) != 0;
-    int o[4];
-    bool bad = false, moved = false;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const bool in = v[i] >= 0 && v[i] < V;
-        o[i] = in ? v[i] : -1;
-        if (in && (pat || vtype[v[i]] == dropType)) {
-            // P3333: a dropped corner outside a pattern face has no new vertex (invalid, as in
-            // the compacting path); P3X3Y: the target, which must itself survive
-            const int src = vtarget[v[i]];
-            o[i] = (src >= 0 && src < V && (pat || !p3333) && vtype[src] != dropType) ? src : -1;
-            moved = true;
-        }
-        bad = bad || o[i] < 0;
-    }
-    if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-    if (moved) quads[q] = make_int4(o[0], o[1], o[2], o[3]);
     uint32_t na = QA_COLOR;
     if (a & QA_NM) na |= QA_NM;
     if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
@@ -794,18 +774,38 @@ __global__ void kApplyRemovals(int4* __restrict__ quads, const uint32_t* attrSrc
     fattr[q] = na;
 }
 
-// a dropped vertex dies: it and its chain join its target's chain, its out-degree goes to the
-// target (after kApplyRemovals took the removed quads' halfedges off), the target is touched
+// a dropped vertex u (P3X3Y: vtype == dropType) dies: the corners its live outgoing
+// halfedges start at -- one per quad around it, found through its lists -- become its
+// target; it and its chain join the target's chain; its out-degree (after kApplyRemovals took
+// the removed quads' halfedges off) goes to the target, which is touched.  A target that is
+// out of range or dropped itself is invalid, as in the compacting path.
 __global__ void kDropVertices(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V, int dropType,
+                              int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
                               uint8_t* __restrict__ vdead, int32_t* __restrict__ head, int32_t* __restrict__ next, int32_t* __restrict__ outdeg,
-                              uint8_t* __restrict__ touched) {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= V || vtype[u] != dropType) return;
+                              uint8_t* __restrict__ touched, int64_t* ctr) {
+    const int64_t uu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (uu >= V || vtype[uu] != dropType) return;
+    const int u = (int)uu;
     const int t = vtarget[u];
-    if (t < 0 || t >= V || vtype[t] == dropType) return;  // the relabelled quads then hold -1 and fail validation
+    if (t < 0 || t >= V || vtype[t] == dropType) {
+        atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+        return;
+    }
+    int* qc = reinterpret_cast<int*>(quads);
+    int w = u, s = head[u];
+    for (;;) {
+        for (int64_t k = off[w]; k < off[w + 1]; ++k) {
+            const int x = out[k];
+            if (x < 0 || x >= H) continue;
+            if (qc[x] == u) qc[x] = t;  // corner x & 3 of quad x >> 2 (dead quads hold -1)
+        }
+        if (s < 0) break;
+        w = s;
+        s = next[s];
+    }
     vdead[u] = 1;
     touched[t] = 1;
-    spliceChain((int)u, t, head, next);
+    spliceChain(u, t, head, next);
     atomicAdd(&outdeg[t], outdeg[u]);
     outdeg[u] = 0;
 }
@@ -826,8 +826,8 @@ __global__ void kValenceInit(const int32_t* __restrict__ outdeg, int64_t V, int3
 }
 
 // bucketFromOut over the chain lists: the entries of m's list and of its chain's lists, those
-// of dead quads skipped
-__device__ BucketOf bucketFromChains(int h, int a, int b, const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t H,
+// of dead quads (corners -1) skipped
+__device__ BucketOf bucketFromChains(int h, int a, int b, const int4* __restrict__ quads, int64_t H,
                                      const int64_t* __restrict__ off, const int32_t* __restrict__ out, const int32_t* __restrict__ head,
                                      const int32_t* __restrict__ next, int64_t* ctr) {
     const int m = min(a, b), o = max(a, b);
@@ -850,15 +850,10 @@ __device__ BucketOf bucketFromChains(int h, int a, int b, const int4* __restrict
         for (int64_t kb = k0; kb < k1; kb += 4) {
             int xs[4];
             int4 qs[4];
-            uint32_t at[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) xs[i] = kb + i < k1 ? out[kb + i] : -1;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const bool ok = xs[i] >= 0 && xs[i] < H;
-                qs[i] = ok ? quads[xs[i] >> 2] : make_int4(0, 0, 0, 0);
-                at[i] = ok ? qattr[xs[i] >> 2] : QA_DEAD;
-            }
+            for (int i = 0; i < 4; ++i) qs[i] = (xs[i] >= 0 && xs[i] < H) ? quads[xs[i] >> 2] : make_int4(0, 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (kb + i >= k1) continue;
@@ -867,9 +862,9 @@ __device__ BucketOf bucketFromChains(int h, int a, int b, const int4* __restrict
                     anomaly(ctr);
                     continue;
                 }
-                if (at[i] & QA_DEAD) continue;
                 const int4 q = qs[i];
                 const int j = x & 3;
+                if (quadCorner(q, j) == -1) continue;  // a dead quad
                 if (quadCorner(q, j) != m) anomaly(ctr);
                 if (quadCorner(q, (j + 1) & 3) == o) add(x, 0);
                 if (o != m && quadCorner(q, (j + 3) & 3) == o) add((x & ~3) | ((j + 3) & 3), 1);
@@ -896,10 +891,10 @@ __global__ void __launch_bounds__(kBlock) kDeferRebuild(const int4* __restrict__
     __shared__ int s_wave[kBlock / 64];
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool later = false;
-    if (hh < H && !(qattr[hh >> 2] & QA_DEAD)) {
+    const int4 q0 = hh < H ? quads[hh >> 2] : make_int4(-1, -1, -1, -1);
+    if (q0.x != -1) {  // live (a dead quad's corners are -1)
         const int h = (int)hh;
-        const int4 q = quads[h >> 2];
-        const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+        const int a = quadCorner(q0, h & 3), b = quadCorner(q0, (h + 1) & 3);
         const int tt = twin[h];
         const uint32_t bo = bpack[h];
         if (touched[a] | touched[b]) {
@@ -931,7 +926,7 @@ __global__ void __launch_bounds__(kBlock) kDeferRebuild(const int4* __restrict__
         const int h = s_list[threadIdx.x];
         const int4 q = quads[h >> 2];
         const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-        storeHalfedge(h, a, b, bucketFromChains(h, a, b, quads, qattr, H, off, out, head, next, ctr), twin, bpack, vflags, fattr, qattr, valence);
+        storeHalfedge(h, a, b, bucketFromChains(h, a, b, quads, H, off, out, head, next, ctr), twin, bpack, vflags, fattr, qattr, valence);
     }
 }
 
@@ -1271,19 +1266,28 @@ __global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t
 __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __restrict__ attr, int64_t Q, const int32_t* __restrict__ keep,
                               const int64_t* __restrict__ qaddr, const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
                               const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr,
-                              int32_t* __restrict__ oldQ, int32_t* __restrict__ qnew, int32_t* __restrict__ removedAt, int64_t V, int64_t* ctr) {
+                              int32_t* __restrict__ oldQ, int32_t* __restrict__ qnew, int32_t* __restrict__ removedAt, int32_t* __restrict__ outdeg,
+                              int64_t V, int64_t* ctr) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     if (!keep[q]) {
         // the outgoing halfedges this quad takes from its corners' lists (when the lists are
         // by current corner; a quad dead since an earlier step of the chain takes none)
         qnew[q] = -1;
-        if (removedAt && !(attr[q] & QA_DEAD)) {
+        if (!(attr[q] & QA_DEAD)) {
             const int4 r = quads[q];
-            atomicAdd(&removedAt[r.x], 1);
-            atomicAdd(&removedAt[r.y], 1);
-            atomicAdd(&removedAt[r.z], 1);
-            atomicAdd(&removedAt[r.w], 1);
+            if (removedAt) {
+                atomicAdd(&removedAt[r.x], 1);
+                atomicAdd(&removedAt[r.y], 1);
+                atomicAdd(&removedAt[r.z], 1);
+                atomicAdd(&removedAt[r.w], 1);
+            }
+            if (outdeg) {  // after deferred steps: the out-degrees kOutCount reads
+                atomicSub(&outdeg[r.x], 1);
+                atomicSub(&outdeg[r.y], 1);
+                atomicSub(&outdeg[r.z], 1);
+                atomicSub(&outdeg[r.w], 1);
+            }
         }
         return;
     }
@@ -2721,10 +2725,11 @@ void Pipeline::simplifyDeferredStep(int dropType, bool p3333) {
     }
     uint8_t* touched = arena.get<uint8_t>("touched", nv);
     zero(touched, (size_t)V);
-    if (Q > 0)
-        kApplyRemovals<<<gridFor(Q), kBlock, 0, stream>>>(quads, p3333 ? fattr : qattr, Q, vtype, vtarget, dropType, p3333 ? 1 : 0, qattr, fattr,
-                                                          outdeg, V, counters());
-    if (V > 0) kDropVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, vdead, head, next, outdeg, touched);
+    // (P3333 ends a chain, so a deferred step is a P3X3Y one: it drops single vertices)
+    if (p3333) throw HipError("deferred P3333 step", -4);
+    if (Q > 0) kApplyRemovals<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, qattr, fattr, outdeg, V);
+    if (V > 0)
+        kDropVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, quads, H, off, out, vdead, head, next, outdeg, touched, counters());
     int32_t* valence = arena.get<int32_t>("valence", nv);
     uint32_t* vflags = arena.ptr<uint32_t>("vflags");
     if (V > 0) kValenceInit<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(outdeg, V, valence, vflags);
@@ -2785,9 +2790,10 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     // corner; after deferred steps the lists are the initialize's and kOutCount checks entries
     int32_t* removedAt = chained ? nullptr : arena.get<int32_t>("removed_at", (size_t)std::max<int64_t>(V, 1));
     if (removedAt) zero(removedAt, (size_t)V * sizeof(int32_t));
+    int32_t* outdeg = chained ? arena.ptr<int32_t>("outdeg") : nullptr;
     if (Q > 0)
         kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ,
-                                                         qnew, removedAt, V, counters());
+                                                         qnew, removedAt, outdeg, V, counters());
     checkValid("quad vertex ids");
 
     // HalfedgeMesh::initialize of the compacted quads, from the previous build: the new
@@ -2800,12 +2806,12 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* srcNext = arena.get<int32_t>("src_next", (size_t)std::max<int64_t>(V, 1));
     zero(touched, (size_t)nV);
     if (!chained) fillNeg(srcHead, (size_t)V * sizeof(int32_t));
-    if (V > 0) kSources<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext);
+    if (V > 0) kSources<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext, outdeg);
     const int64_t* off = arena.ptr<int64_t>("out_off");
     const int32_t* out = arena.ptr<int32_t>("out_list");
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
     zero(cnt + nV, sizeof(int32_t));
-    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, removedAt, qnew, srcHead, srcNext, cnt);
+    if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, removedAt, srcHead, srcNext, outdeg, cnt);
     int64_t* off2 = arena.get<int64_t>("out_off2", (size_t)std::max<int64_t>(nV, dmcV) + 1);
     scanExclusiveI32(cnt, off2, nV + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(nV + 1)), stream);
     int32_t* out2 = arena.get<int32_t>("out_list2", 4 * capQ);
