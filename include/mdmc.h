@@ -171,6 +171,11 @@ mdmc_status mdmc_extract_batch(mdmc_ctx* ctx, const mdmc_grid* grids, int32_t n_
  * (default on).  mdmc_get_graph_replays: how many extractions of this context ran the graph. */
 mdmc_status mdmc_set_graphs(mdmc_ctx* ctx, int32_t enable);
 mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays);
+/* Local twin update in a chain of simplifications (no reference counterpart).  While another
+ * simplification follows, a P3X3Y merge whose neighbourhood is a plain patch rewrites its two
+ * new twin pairs directly instead of having its region's buckets rebuilt; the mesh is the same
+ * either way.  enable = 0 sends every merge through the region rebuild (default on). */
+mdmc_status mdmc_set_local_merges(mdmc_ctx* ctx, int32_t enable);
 
 /* Per-problem offsets of the resident result (host arrays of n_problems + 1; any may be NULL).
  * After mdmc_extract (one problem) they are {0, n}. */

@@ -104,7 +104,8 @@ EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", 
            "mdmc_mesh_get_info", "mdmc_mesh_copy", "mdmc_mesh_triangles", "mdmc_mesh_render_vertices", "mdmc_backward",
            "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape",
            "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin",
-           "mdmc_backward_noalloc", "mdmc_set_graphs", "mdmc_get_graph_replays", "mdmc_processed_grid"]
+           "mdmc_backward_noalloc", "mdmc_set_graphs", "mdmc_get_graph_replays", "mdmc_processed_grid",
+           "mdmc_set_local_merges"]
 
 
 def lib():
@@ -157,6 +158,8 @@ def lib():
         L.mdmc_backward_noalloc.argtypes = [vp, vp, vp, i32]
         L.mdmc_set_graphs.argtypes = [vp, i32]
         L.mdmc_set_graphs.restype = i32
+        L.mdmc_set_local_merges.argtypes = [vp, i32]
+        L.mdmc_set_local_merges.restype = i32
         L.mdmc_get_graph_replays.argtypes = [vp, ctypes.POINTER(i64)]
         L.mdmc_get_graph_replays.restype = i32
         L.mdmc_backward_noalloc.restype = i32
@@ -305,6 +308,11 @@ class DualMarchingCubes:
     def set_graphs(self, enable):
         """mdmc_set_graphs: replay the batched form's shape-determined front from a HIP graph."""
         self._check(lib().mdmc_set_graphs(self._ctx, 1 if enable else 0))
+
+    def set_local_merges(self, enable):
+        """mdmc_set_local_merges: plain-patch P3X3Y merges of a simplification chain update their
+        twins directly (False: every merge's region is rebuilt; the mesh is the same)."""
+        self._check(lib().mdmc_set_local_merges(self._ctx, 1 if enable else 0))
 
     def graph_replays(self):
         n = ctypes.c_int64()

@@ -649,6 +649,12 @@ mdmc_status mdmc_set_graphs(mdmc_ctx* ctx, int32_t enable) {
     return MDMC_OK;
 }
 
+mdmc_status mdmc_set_local_merges(mdmc_ctx* ctx, int32_t enable) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    ctx->p.localMerges = enable != 0;
+    return MDMC_OK;
+}
+
 mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays) {
     if (!ctx || !replays) return MDMC_ERR_INVALID_ARG;
     *replays = ctx->p.graphReplays;

@@ -81,6 +81,9 @@ class Pipeline {
     }
     void releaseGraphs();
     bool useGraphs = true;
+    // P3X3Y merges of a deferred chain step with a plain neighbourhood update their twins
+    // directly (else every merge's region is rebuilt)
+    bool localMerges = true;
     int64_t graphReplays = 0;
     int nProblems = 1;
     ProblemDev* probsDev = nullptr;

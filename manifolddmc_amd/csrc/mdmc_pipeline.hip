@@ -36,7 +36,7 @@ constexpr int kMaxBucket = 16;
 constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
-enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_NCOUNTERS = 8 };
+enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_NCOUNTERS = 8 };
 
 __device__ __forceinline__ void anomaly(int64_t* ctr) { atomicAdd((unsigned long long*)&ctr[A_ANOMALY], 1ULL); }
 
@@ -583,34 +583,36 @@ __device__ __forceinline__ void spliceChain(int u, int t, int32_t* __restrict__ 
     next[last] = atomicExch(&head[t], u);
 }
 
-__global__ void kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, const int32_t* __restrict__ address, int64_t V,
-                         int dropType, uint8_t* __restrict__ touched, int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext,
-                         int32_t* __restrict__ outdeg) {
-    const int64_t u0 = firstOfFour();
-    if (u0 >= V) return;
-    int ty[4];
-    if (vec4(u0, V, vtype, vtype)) {
-        const int4 t4 = reinterpret_cast<const int4*>(vtype)[u0 >> 2];
-        ty[0] = t4.x;
-        ty[1] = t4.y;
-        ty[2] = t4.z;
-        ty[3] = t4.w;
-    } else {
-        for (int j = 0; j < 4; ++j) ty[j] = u0 + j < V ? vtype[u0 + j] : dropType + 1;
+__global__ void __launch_bounds__(kBlock) kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
+                                                   const int32_t* __restrict__ address, int64_t V, int dropType, uint8_t* __restrict__ touched,
+                                                   int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext, int32_t* __restrict__ outdeg) {
+    // the block's dropped vertices gathered in LDS and handled on consecutive lanes
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int64_t uu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool drop = uu < V && vtype[uu] == dropType;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t m = __ballot(drop);
+    if (lane == 0) s_wave[wid] = __popcll(m);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
     }
-    for (int j = 0; j < 4; ++j) {
-        if (ty[j] != dropType) continue;
-        const int64_t u = u0 + j;
-        const int t = vtarget[u];
-        if (t < 0 || t >= V) continue;
-        const int tn = address[t];
-        if (tn < 0) continue;  // the relabelled quads then hold -1 and fail validation
-        touched[tn] = 1;
-        spliceChain((int)u, t, srcHead, srcNext);
-        if (outdeg) {
-            atomicAdd(&outdeg[t], outdeg[u]);
-            outdeg[u] = 0;
-        }
+    if (drop) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)uu;
+    __syncthreads();
+    if ((int)threadIdx.x >= total) return;
+    const int u = s_list[threadIdx.x];
+    const int t = vtarget[u];
+    if (t < 0 || t >= V) return;
+    const int tn = address[t];
+    if (tn < 0) return;  // the relabelled quads then hold -1 and fail validation
+    touched[tn] = 1;
+    spliceChain(u, t, srcHead, srcNext);
+    if (outdeg) {
+        atomicAdd(&outdeg[t], outdeg[u]);
+        outdeg[u] = 0;
     }
 }
 
@@ -727,6 +729,10 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
     }
 }
 
+// the P3X3Y valence patterns (markElementsP3X3Y, HalfedgeMesh.cu:317-339)
+__device__ __forceinline__ bool pattern1(const int v[4]) { return v[0] == 3 && v[1] >= 5 && v[2] == 3 && v[3] >= 5; }
+__device__ __forceinline__ bool pattern2(const int v[4]) { return v[0] >= 5 && v[1] == 3 && v[2] >= 5 && v[3] == 3; }
+
 // ---- a chain of simplifications with its compaction deferred to the chain's last step.
 // Between the steps every id stays as the halfedge initialize numbered it: a removed quad
 // becomes dead (QA_DEAD in qattr and fattr, skipped by every later kernel), a dropped vertex
@@ -738,91 +744,199 @@ __global__ void __launch_bounds__(kBlock) kRemapRebuild(const int4* __restrict__
 // the step-by-step compactions give (stable compactions compose, and bucket ranks and the
 // twin rules only compare ids, whose order every compaction keeps).
 
-// out-degree of every vertex from the initialize's lists
-__global__ void kOutdegInit(const int64_t* __restrict__ off, int64_t V, int32_t* __restrict__ outdeg) {
+// out-degree of every vertex from the initialize's lists, and its incoming boundary halfedges
+// (the rest of its valence)
+__global__ void kOutdegInit(const int64_t* __restrict__ off, int64_t V, const int32_t* __restrict__ valence, int32_t* __restrict__ outdeg,
+                            int32_t* __restrict__ bdin) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    outdeg[v] = (int32_t)(off[v + 1] - off[v]);
+    const int32_t d = (int32_t)(off[v + 1] - off[v]);
+    outdeg[v] = d;
+    bdin[v] = valence[v] - d;
 }
 
-// removeQuadrilateralsP3X3Y / removeQuadsP3333 in place: a removed quad dies (its corners each
-// lose an outgoing halfedge; its corners become -1, so that list walks see it dead from the
-// quad alone); a kept quad gets the compaction's attributes (colour unset, NM and boundary
-// kept).  Corners are relabelled by kDropVertices.
-// (attrSrc is qattr or fattr: no __restrict__ on the three attribute arrays)
-__global__ void kApplyRemovals(int4* __restrict__ quads, const uint32_t* attrSrc, int64_t Q, uint32_t* qattr, uint32_t* fattr,
-                               int32_t* __restrict__ outdeg, int64_t V) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    const uint32_t a = attrSrc[q];
-    if (a & QA_DEAD) return;
-    if (a & QA_REMOVE) {
+// P3X3Y merge (x, a1, z, a3) -> x whose neighbourhood is a plain patch: the four edges of the
+// removed quad Q are pairs, x and z each have a closed fan of three quads (Q, N_X, N_B and Q,
+// N_A, N_Z) with third neighbours d and c, and after every merge of the step is applied (a
+// dropped vertex stands for its target, rep) rep(c) and rep(d) differ from each other and
+// from the quad's corners.  Then x's and z's edges to a1 and a3 become two new pairs -- the
+// halfedge across a1-x with the one across z-a1, the one across a3-z with the one across x-a3
+// -- every other bucket keeps its members (the edge z-c is renamed x-rep(c), x-d is x-rep(d),
+// neither meeting another), and no flag or boundary count moves, so the four twins and bucket
+// words are written here.  (Q's neighbours are not removed: two merges never share a quad,
+// and no other merge writes or reads the halfedges read or written here.)  Any other merge
+// marks its target touched for the region rebuild below and is counted in ctr[A_FALLBACK].  The block's
+// merges are gathered in LDS first: each is a chain of dependent loads.
+// The same launch applies removeQuadrilateralsP3X3Y in place: every kept quad gets the
+// compaction's attributes (colour unset, NM and boundary kept: no REMOVE or DEAD bit before or
+// after, so the neighbour tests read the same either way), and the lane of each removed quad,
+// after reading it, takes one outgoing halfedge off each of its corners, sets its corners to
+// -1 and marks it dead.  For a local merge the two corners holding z in N_A and N_Z are
+// recorded (zslot) for kDropVertices; a fallback merge's target relabels through z's lists.
+__global__ void __launch_bounds__(kBlock) kLocalMerge(int4* __restrict__ quads, uint32_t* qattr, uint32_t* fattr, int64_t Q,
+                                                      const int32_t* __restrict__ valence, const int32_t* __restrict__ vtype,
+                                                      const int32_t* __restrict__ vtarget, int32_t* __restrict__ twin, uint32_t* __restrict__ bpack,
+                                                      int32_t* __restrict__ outdeg, int2* __restrict__ zslot, int64_t V, uint8_t* __restrict__ touched,
+                                                      int allowLocal, int64_t* ctr) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int64_t qq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool merging = false;
+    if (qq < Q) {
+        const uint32_t a = qattr[qq];
+        merging = !(a & QA_DEAD) && (a & QA_REMOVE);
+        if (!(a & QA_DEAD) && !merging) {
+            uint32_t na = QA_COLOR;
+            if (a & QA_NM) na |= QA_NM;
+            if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
+            qattr[qq] = na;
+            fattr[qq] = na;
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t mb = __ballot(merging);
+    if (lane == 0) s_wave[wid] = __popcll(mb);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
+    }
+    if (merging) s_list[base + __popcll(mb & ((1ULL << lane) - 1ULL))] = (int)qq;
+    __syncthreads();
+    bool fallback = false;
+    if ((int)threadIdx.x < total) {
+        const int q = s_list[threadIdx.x];
         const int4 v4 = quads[q];
         const int v[4] = {v4.x, v4.y, v4.z, v4.w};
+        const int val[4] = {valence[v[0]], valence[v[1]], valence[v[2]], valence[v[3]]};
+        const int X = pattern1(val) ? 0 : 1;  // the merge used pattern 1 (x = corner 0) or 2 (x = corner 1)
+        const int Z = X + 2;
+        const int x = v[X], a1 = v[X + 1], z = v[Z], a3 = v[(X + 3) & 3];
+        const int4 tw4 = reinterpret_cast<const int4*>(twin)[q];
+        const int t[4] = {tw4.x, tw4.y, tw4.z, tw4.w};
+        const int tX = t[X], tA = t[X + 1], tZ = t[Z], tB = t[(X + 3) & 3];  // across x-a1, a1-z, z-a3, a3-x
+        const uint4 bp = reinterpret_cast<const uint4*>(bpack)[q];
+        bool ok = allowLocal && x != a1 && x != z && x != a3 && a1 != z && a1 != a3 && z != a3;
+        ok = ok && (bp.x & 0xFFFFu) == 2 && (bp.y & 0xFFFFu) == 2 && (bp.z & 0xFFFFu) == 2 && (bp.w & 0xFFFFu) == 2;
+        ok = ok && tX >= 0 && tA >= 0 && tZ >= 0 && tB >= 0;
+        if (ok) {
+            const int nX = tX >> 2, nA = tA >> 2, nZ = tZ >> 2, nB = tB >> 2;
+            ok = nX != q && nA != q && nZ != q && nB != q;
+            constexpr uint32_t gone = QA_REMOVE | QA_DEAD;
+            ok = ok && !((qattr[nX] | qattr[nA] | qattr[nZ] | qattr[nB]) & gone);
+            if (ok) {
+                const int4 NX = quads[nX], NA = quads[nA], NZ = quads[nZ], NB = quads[nB];
+                // x's fan: after tX (a1 -> x) in N_X comes x -> d1; before tB (x -> a3) in N_B comes d2 -> x
+                const int hXn = (tX & ~3) | ((tX + 1) & 3), hBp = (tB & ~3) | ((tB + 3) & 3);
+                const int d1 = quadCorner(NX, (tX + 2) & 3), d2 = quadCorner(NB, (tB + 3) & 3);
+                // z's fan: before tA (z -> a1) in N_A comes c1 -> z; after tZ (a3 -> z) in N_Z comes z -> c2
+                const int hAp = (tA & ~3) | ((tA + 3) & 3), hZn = (tZ & ~3) | ((tZ + 1) & 3);
+                const int c1 = quadCorner(NA, (tA + 3) & 3), c2 = quadCorner(NZ, (tZ + 2) & 3);
+                ok = quadCorner(NX, tX & 3) == a1 && quadCorner(NA, tA & 3) == z && quadCorner(NZ, tZ & 3) == a3 && quadCorner(NB, tB & 3) == x;
+                ok = ok && d1 == d2 && c1 == c2 && d1 >= 0 && c1 >= 0;
+                ok = ok && twin[hXn] == hBp && twin[hBp] == hXn && twin[hAp] == hZn && twin[hZn] == hAp;
+                ok = ok && (bpack[hXn] & 0xFFFFu) == 2 && (bpack[hAp] & 0xFFFFu) == 2;
+                if (ok) {
+                    const int rc = vtype[c1] == 3 ? vtarget[c1] : c1, rd = vtype[d1] == 3 ? vtarget[d1] : d1;
+                    ok = rc >= 0 && rd >= 0 && rc != rd && rc != x && rc != z && rc != a1 && rc != a3 && rd != x && rd != z && rd != a1 &&
+                         rd != a3;
+                }
+            }
+        }
+        if (ok) {
+            twin[tX] = tA;
+            twin[tA] = tX;
+            bpack[tX] = packBucket(2, tX < tA ? 0 : 1);
+            bpack[tA] = packBucket(2, tA < tX ? 0 : 1);
+            twin[tZ] = tB;
+            twin[tB] = tZ;
+            bpack[tZ] = packBucket(2, tZ < tB ? 0 : 1);
+            bpack[tB] = packBucket(2, tB < tZ ? 0 : 1);
+            // z is the origin of tA (corner tA & 3 of N_A) and the destination of tZ
+            if (z >= 0 && z < V) zslot[z] = make_int2(tA, (tZ & ~3) | ((tZ + 1) & 3));
+        } else {
+            touched[x] = 1;
+            fallback = true;
+            if (z >= 0 && z < V) zslot[z] = make_int2(-1, -1);
+        }
+        // the removal
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if (v[i] >= 0 && v[i] < V) atomicSub(&outdeg[v[i]], 1);
         quads[q] = make_int4(-1, -1, -1, -1);
         qattr[q] = QA_DEAD;
         fattr[q] = QA_DEAD;
-        return;
     }
-    uint32_t na = QA_COLOR;
-    if (a & QA_NM) na |= QA_NM;
-    if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
-    qattr[q] = na;
-    fattr[q] = na;
+    const uint64_t fb = __ballot(fallback);
+    if (lane == 0 && fb) atomicAdd((unsigned long long*)&ctr[A_FALLBACK], (unsigned long long)__popcll(fb));
+}
+
+// valence = out-degree + incoming boundary halfedges (a step with only local merges: no flag
+// or boundary count changed)
+__global__ void kValenceFromBdin(const int32_t* __restrict__ outdeg, const int32_t* __restrict__ bdin, int64_t V, int32_t* __restrict__ valence) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    valence[v] = outdeg[v] + bdin[v];
 }
 
 // a dropped vertex u (P3X3Y: vtype == dropType) dies: the corners its live outgoing
-// halfedges start at -- one per quad around it, found through its lists -- become its
-// target; it and its chain join the target's chain; its out-degree (after kApplyRemovals took
-// the removed quads' halfedges off) goes to the target, which is touched.  A target that is
-// out of range or dropped itself is invalid, as in the compacting path.
-__global__ void kDropVertices(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V, int dropType,
-                              int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                              uint8_t* __restrict__ vdead, int32_t* __restrict__ head, int32_t* __restrict__ next, int32_t* __restrict__ outdeg,
-                              uint8_t* __restrict__ touched, int64_t* ctr) {
+// halfedges start at become its target (the two recorded by kLocalMerge, or -- after a
+// fallback -- one per quad around it, found through its lists); it and its chain join the
+// target's chain; its out-degree (after kLocalMerge took the removed quads' halfedges off)
+// goes to the target.  A target that is out of range or dropped itself is invalid, as in the
+// compacting path.
+// (the dropped vertices of a block are gathered in LDS and handled on consecutive lanes: each
+// one's work is a chain of dependent loads, which sparse lanes would spread over every wave)
+__global__ void __launch_bounds__(kBlock) kDropVertices(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V,
+                                                        int dropType, int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ out, uint8_t* __restrict__ vdead, int32_t* __restrict__ head,
+                                                        int32_t* __restrict__ next, int32_t* __restrict__ outdeg, const int2* __restrict__ zslot,
+                                                        int64_t* ctr) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
     const int64_t uu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (uu >= V || vtype[uu] != dropType) return;
-    const int u = (int)uu;
+    const bool drop = uu < V && vtype[uu] == dropType;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t m = __ballot(drop);
+    if (lane == 0) s_wave[wid] = __popcll(m);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
+    }
+    if (drop) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)uu;
+    __syncthreads();
+    if ((int)threadIdx.x >= total) return;
+    const int u = s_list[threadIdx.x];
     const int t = vtarget[u];
     if (t < 0 || t >= V || vtype[t] == dropType) {
         atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
         return;
     }
     int* qc = reinterpret_cast<int*>(quads);
-    int w = u, s = head[u];
-    for (;;) {
-        for (int64_t k = off[w]; k < off[w + 1]; ++k) {
-            const int x = out[k];
-            if (x < 0 || x >= H) continue;
-            if (qc[x] == u) qc[x] = t;  // corner x & 3 of quad x >> 2 (dead quads hold -1)
+    const int2 zs = zslot[u];
+    if (zs.x >= 0) {  // a local merge: u's two live quads and its corners in them are known
+        qc[zs.x] = t;
+        qc[zs.y] = t;
+    } else {
+        int w = u, sn = head[u];
+        for (;;) {
+            for (int64_t k = off[w]; k < off[w + 1]; ++k) {
+                const int x = out[k];
+                if (x < 0 || x >= H) continue;
+                if (qc[x] == u) qc[x] = t;  // corner x & 3 of quad x >> 2 (dead quads hold -1)
+            }
+            if (sn < 0) break;
+            w = sn;
+            sn = next[sn];
         }
-        if (s < 0) break;
-        w = s;
-        s = next[s];
     }
     vdead[u] = 1;
-    touched[t] = 1;
     spliceChain(u, t, head, next);
     atomicAdd(&outdeg[t], outdeg[u]);
     outdeg[u] = 0;
-}
-
-// valence = out-degree (incoming boundary halfedges added by the rebuild); flags cleared
-__global__ void kValenceInit(const int32_t* __restrict__ outdeg, int64_t V, int32_t* __restrict__ valence, uint32_t* __restrict__ vflags) {
-    const int64_t v0 = firstOfFour();
-    if (v0 >= V) return;
-    if (vec4(v0, V, outdeg, valence) && vec4(v0, V, vflags, vflags)) {
-        reinterpret_cast<int4*>(valence)[v0 >> 2] = reinterpret_cast<const int4*>(outdeg)[v0 >> 2];
-        reinterpret_cast<uint4*>(vflags)[v0 >> 2] = make_uint4(0u, 0u, 0u, 0u);
-    } else {
-        for (int64_t v = v0; v < v0 + 4 && v < V; ++v) {
-            valence[v] = outdeg[v];
-            vflags[v] = 0u;
-        }
-    }
 }
 
 // bucketFromOut over the chain lists: the entries of m's list and of its chain's lists, those
@@ -878,41 +992,17 @@ __device__ BucketOf bucketFromChains(int h, int a, int b, const int4* __restrict
     return BucketOf{n, rk, bucketTwin(n, rk, partner, mem, dir)};
 }
 
-// the rebuild of a deferred step, in place: every live halfedge re-applies its bucket's flags,
-// boundary valence and NM face bits; those with a touched endpoint are first recomputed (on
-// full waves, gathered per block in LDS) and store twin and bucket word
-__global__ void __launch_bounds__(kBlock) kDeferRebuild(const int4* __restrict__ quads, int64_t H, const uint8_t* __restrict__ touched,
-                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ out,
-                                                        const int32_t* __restrict__ head, const int32_t* __restrict__ next,
-                                                        int32_t* __restrict__ twin, uint32_t* __restrict__ bpack, uint32_t* __restrict__ vflags,
-                                                        uint32_t* __restrict__ fattr, uint32_t* __restrict__ qattr, int32_t* __restrict__ valence,
-                                                        int64_t* ctr) {
-    __shared__ int s_list[kBlock];
-    __shared__ int s_wave[kBlock / 64];
-    const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool later = false;
-    const int4 q0 = hh < H ? quads[hh >> 2] : make_int4(-1, -1, -1, -1);
-    if (q0.x != -1) {  // live (a dead quad's corners are -1)
-        const int h = (int)hh;
-        const int a = quadCorner(q0, h & 3), b = quadCorner(q0, (h + 1) & 3);
-        const int tt = twin[h];
-        const uint32_t bo = bpack[h];
-        if (touched[a] | touched[b]) {
-            later = true;
-        } else {
-            // twin and bucket word unchanged: the flags alone
-            const int n = (int)(bo & 0xFFFFu);
-            if (tt == -1) atomicAdd(&valence[b], 1);
-            const uint32_t vf = bucketVertexFlag(n);
-            if (vf) atomicOr(&vflags[a], vf);
-            if (n == 4) {
-                atomicOr(&fattr[h >> 2], (uint32_t)QA_NM);
-                atomicOr(&qattr[h >> 2], (uint32_t)QA_NM);
-            }
-        }
-    }
+// ---- the fallback of a deferred step, limited to the region of the merges kLocalMerge left:
+// the halfedges at a touched vertex (its live outgoing ones and the ones before them in their
+// quads, which end at it) are marked dirty, their endpoints affected; the dirty ones get their
+// buckets recomputed; the affected vertices their flags and incoming boundary count (a
+// vertex's flags come from the buckets of its outgoing halfedges, its boundary count from the
+// twinless halfedges ending at it: both are read off its chain lists).
+__device__ __forceinline__ int gatherFlagged(const uint8_t* __restrict__ flag, int64_t n, int* s_list, int* s_wave) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = i < n && flag[i];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint64_t m = __ballot(later);
+    const uint64_t m = __ballot(on);
     if (lane == 0) s_wave[wid] = __popcll(m);
     __syncthreads();
     int base = 0, total = 0;
@@ -920,14 +1010,89 @@ __global__ void __launch_bounds__(kBlock) kDeferRebuild(const int4* __restrict__
         if (w < wid) base += s_wave[w];
         total += s_wave[w];
     }
-    if (later) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)hh;
+    if (on) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)i;
     __syncthreads();
-    if ((int)threadIdx.x < total) {
-        const int h = s_list[threadIdx.x];
-        const int4 q = quads[h >> 2];
-        const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
-        storeHalfedge(h, a, b, bucketFromChains(h, a, b, quads, H, off, out, head, next, ctr), twin, bpack, vflags, fattr, qattr, valence);
+    return total;
+}
+
+__global__ void __launch_bounds__(kBlock) kMarkDirtyRegion(const uint8_t* __restrict__ touched, int64_t V, const int4* __restrict__ quads, int64_t H,
+                                                           const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                                                           const int32_t* __restrict__ head, const int32_t* __restrict__ next,
+                                                           uint8_t* __restrict__ hdirty, uint8_t* __restrict__ aff) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int total = gatherFlagged(touched, V, s_list, s_wave);
+    if ((int)threadIdx.x >= total) return;
+    const int t = s_list[threadIdx.x];
+    const int* qc = reinterpret_cast<const int*>(quads);
+    aff[t] = 1;
+    int w = t, sn = head[t];
+    for (;;) {
+        for (int64_t k = off[w]; k < off[w + 1]; ++k) {
+            const int h = out[k];
+            if (h < 0 || h >= H || qc[h] != t) continue;  // dead, or not t's any more
+            const int pr = (h & ~3) | ((h + 3) & 3);
+            hdirty[h] = 1;
+            hdirty[pr] = 1;
+            aff[qc[(h & ~3) | ((h + 1) & 3)]] = 1;
+            aff[qc[pr]] = 1;
+        }
+        if (sn < 0) break;
+        w = sn;
+        sn = next[sn];
     }
+}
+
+// the dirty halfedges' buckets (gathered per block in LDS): twin, bucket word and NM face bits
+__global__ void __launch_bounds__(kBlock) kRebuildDirty(const int4* __restrict__ quads, int64_t H, const uint8_t* __restrict__ hdirty,
+                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                                                        const int32_t* __restrict__ head, const int32_t* __restrict__ next,
+                                                        int32_t* __restrict__ twin, uint32_t* __restrict__ bpack, uint32_t* __restrict__ fattr,
+                                                        uint32_t* __restrict__ qattr, int64_t* ctr) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int total = gatherFlagged(hdirty, H, s_list, s_wave);
+    if ((int)threadIdx.x >= total) return;
+    const int h = s_list[threadIdx.x];
+    const int4 q = quads[h >> 2];
+    const int a = quadCorner(q, h & 3), b = quadCorner(q, (h + 1) & 3);
+    const BucketOf bk = bucketFromChains(h, a, b, quads, H, off, out, head, next, ctr);
+    twin[h] = bk.twin;
+    bpack[h] = packBucket(bk.n, bk.rank);
+    if (bk.n == 4) {
+        atomicOr(&fattr[h >> 2], (uint32_t)QA_NM);
+        atomicOr(&qattr[h >> 2], (uint32_t)QA_NM);
+    }
+}
+
+// an affected vertex's flags and incoming boundary count, from its chain lists
+__global__ void __launch_bounds__(kBlock) kAffectedFlags(const uint8_t* __restrict__ aff, int64_t V, const int4* __restrict__ quads, int64_t H,
+                                                         const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                                                         const int32_t* __restrict__ head, const int32_t* __restrict__ next,
+                                                         const int32_t* __restrict__ twin, const uint32_t* __restrict__ bpack,
+                                                         uint32_t* __restrict__ vflags, int32_t* __restrict__ bdin) {
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int total = gatherFlagged(aff, V, s_list, s_wave);
+    if ((int)threadIdx.x >= total) return;
+    const int v = s_list[threadIdx.x];
+    const int* qc = reinterpret_cast<const int*>(quads);
+    uint32_t f = 0;
+    int bd = 0;
+    int w = v, sn = head[v];
+    for (;;) {
+        for (int64_t k = off[w]; k < off[w + 1]; ++k) {
+            const int h = out[k];
+            if (h < 0 || h >= H || qc[h] != v) continue;
+            f |= bucketVertexFlag((int)(bpack[h] & 0xFFFFu));
+            if (twin[(h & ~3) | ((h + 3) & 3)] == -1) ++bd;
+        }
+        if (sn < 0) break;
+        w = sn;
+        sn = next[sn];
+    }
+    vflags[v] = f;
+    bdin[v] = bd;
 }
 
 // list lengths of the fan detection's slow vertices: their out-degree (the halfedges
@@ -1052,8 +1217,6 @@ __global__ void kValence(const int4* __restrict__ quads, const int32_t* __restri
     if (twin[h] == -1) atomicAdd(&valence[quadCorner(q, (int)((h + 1) & 3))], 1);
 }
 
-__device__ __forceinline__ bool pattern1(const int v[4]) { return v[0] == 3 && v[1] >= 5 && v[2] == 3 && v[3] >= 5; }
-__device__ __forceinline__ bool pattern2(const int v[4]) { return v[0] >= 5 && v[1] == 3 && v[2] >= 5 && v[3] == 3; }
 
 __global__ void kMark3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ valence) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1090,7 +1253,10 @@ __global__ void kCount3x3yOld(const int4* __restrict__ quads, uint32_t* __restri
 
 // HalfedgeMesh.cu:368-388.  Positions and normals stay where the grid pass wrote them while
 // the simplifications run: vertex v's are at slot vslot[v] (v itself when vslot is null).
-__device__ __forceinline__ void handleMerge(float* pos, float* nrm, const int32_t* vslot, int32_t* vtype, int32_t* vtarget, int x, int y, int z, int w) {
+// (markTarget: the target gets vtype kTargetType, for the deferred steps' local twin update)
+constexpr int kTargetType = 4;
+__device__ __forceinline__ void handleMerge(float* pos, float* nrm, const int32_t* vslot, int32_t* vtype, int32_t* vtarget, int x, int y, int z, int w,
+                                            int markTarget) {
     const int64_t sx = vslot ? vslot[x] : x, sy = vslot ? vslot[y] : y, sw = vslot ? vslot[w] : w;
     f3 p = mk3(pos[3 * sy], pos[3 * sy + 1], pos[3 * sy + 2]);
     const f3 pw = mk3(pos[3 * sw], pos[3 * sw + 1], pos[3 * sw + 2]);
@@ -1103,6 +1269,7 @@ __device__ __forceinline__ void handleMerge(float* pos, float* nrm, const int32_
     nrm[3 * sx] = n.x; nrm[3 * sx + 1] = n.y; nrm[3 * sx + 2] = n.z;
     vtype[z] = 3;  // VertexType::PatternRemovable
     vtarget[z] = x;
+    if (markTarget) vtype[x] = kTargetType;
 }
 
 // The simplifications run right after a halfedge build, whose faces are the quads in order:
@@ -1126,7 +1293,8 @@ __device__ __forceinline__ bool neighborFaces(const int32_t* __restrict__ twin, 
 // mergeVerticesP3X3YColor (colour = 1) and mergeVerticesP3X3Y (colour = 0), HalfedgeMesh.cu:391-498
 __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ twin,
                            const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
-                           float* __restrict__ pos, float* __restrict__ nrm, const int32_t* __restrict__ vslot, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour) {
+                           float* __restrict__ pos, float* __restrict__ nrm, const int32_t* __restrict__ vslot, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour,
+                           int markTarget) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const uint32_t a = qattr[q];
@@ -1147,10 +1315,10 @@ __global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict_
             if ((na[i] & QA_PATTERN) && ((na[i] & QA_COLOR) <= c)) return;
     }
     if (pattern1(val) && (colour || (count[v.x] == 1 && count[v.z] == 1))) {
-        handleMerge(pos, nrm, vslot, vtype, vtarget, v.x, v.y, v.z, v.w);
+        handleMerge(pos, nrm, vslot, vtype, vtarget, v.x, v.y, v.z, v.w, markTarget);
         atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
     } else if (pattern2(val) && (colour || (count[v.y] == 1 && count[v.w] == 1))) {
-        handleMerge(pos, nrm, vslot, vtype, vtarget, v.y, v.x, v.w, v.z);
+        handleMerge(pos, nrm, vslot, vtype, vtarget, v.y, v.x, v.w, v.z, markTarget);
         atomicOr(&qattr[q], (uint32_t)QA_REMOVE);
     }
 }
@@ -2718,8 +2886,12 @@ void Pipeline::simplifyDeferredStep(int dropType, bool p3333) {
     uint8_t* vdead = arena.get<uint8_t>("vdead", nv);
     int32_t* head = arena.get<int32_t>("src_head", nv);
     int32_t* next = arena.get<int32_t>("src_next", nv);
+    int32_t* bdin = arena.get<int32_t>("bdin", nv);
+    int32_t* valence = arena.get<int32_t>("valence", nv);
+    int32_t* twin = arena.ptr<int32_t>("twin");
+    uint32_t* bpack = arena.ptr<uint32_t>("bpack");
     if (deferredSteps == 0) {
-        if (V > 0) kOutdegInit<<<gridFor(V), kBlock, 0, stream>>>(off, V, outdeg);
+        if (V > 0) kOutdegInit<<<gridFor(V), kBlock, 0, stream>>>(off, V, valence, outdeg, bdin);
         zero(vdead, (size_t)V);
         fillNeg(head, (size_t)V * sizeof(int32_t));
     }
@@ -2727,15 +2899,26 @@ void Pipeline::simplifyDeferredStep(int dropType, bool p3333) {
     zero(touched, (size_t)V);
     // (P3333 ends a chain, so a deferred step is a P3X3Y one: it drops single vertices)
     if (p3333) throw HipError("deferred P3333 step", -4);
-    if (Q > 0) kApplyRemovals<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, qattr, fattr, outdeg, V);
+    zero(counters() + A_FALLBACK, sizeof(int64_t));
+    int2* zslot = arena.get<int2>("zslot", nv);
+    if (Q > 0)
+        kLocalMerge<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, fattr, Q, valence, vtype, vtarget, twin, bpack, outdeg, zslot, V, touched,
+                                                       localMerges ? 1 : 0, counters());
     if (V > 0)
-        kDropVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, quads, H, off, out, vdead, head, next, outdeg, touched, counters());
-    int32_t* valence = arena.get<int32_t>("valence", nv);
-    uint32_t* vflags = arena.ptr<uint32_t>("vflags");
-    if (V > 0) kValenceInit<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(outdeg, V, valence, vflags);
-    if (H > 0)
-        kDeferRebuild<<<gridFor(H), kBlock, 0, stream>>>(quads, H, touched, off, out, head, next, arena.ptr<int32_t>("twin"), arena.ptr<uint32_t>("bpack"),
-                                                         vflags, fattr, qattr, valence, counters());
+        kDropVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, quads, H, off, out, vdead, head, next, outdeg, zslot, counters());
+    // the merges the local update could not take: their region's buckets, flags and boundary
+    // counts recomputed
+    if (readI64(counters() + A_FALLBACK) > 0) {
+        uint8_t* hdirty = arena.get<uint8_t>("hdirty", (size_t)std::max<int64_t>(H, 1));
+        uint8_t* aff = arena.get<uint8_t>("vaffected", nv);
+        zero(hdirty, (size_t)H);
+        zero(aff, (size_t)V);
+        uint32_t* vflags = arena.ptr<uint32_t>("vflags");
+        if (V > 0) kMarkDirtyRegion<<<gridFor(V), kBlock, 0, stream>>>(touched, V, quads, H, off, out, head, next, hdirty, aff);
+        if (H > 0) kRebuildDirty<<<gridFor(H), kBlock, 0, stream>>>(quads, H, hdirty, off, out, head, next, twin, bpack, fattr, qattr, counters());
+        if (V > 0) kAffectedFlags<<<gridFor(V), kBlock, 0, stream>>>(aff, V, quads, H, off, out, head, next, twin, bpack, vflags, bdin);
+    }
+    if (V > 0) kValenceFromBdin<<<gridFor(V), kBlock, 0, stream>>>(outdeg, bdin, V, valence);
     valenceReady = true;
     ++deferredSteps;
     heMaterialized = false;
@@ -2806,7 +2989,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* srcNext = arena.get<int32_t>("src_next", (size_t)std::max<int64_t>(V, 1));
     zero(touched, (size_t)nV);
     if (!chained) fillNeg(srcHead, (size_t)V * sizeof(int32_t));
-    if (V > 0) kSources<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext, outdeg);
+    if (V > 0) kSources<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext, outdeg);
     const int64_t* off = arena.ptr<int64_t>("out_off");
     const int32_t* out = arena.ptr<int32_t>("out_list");
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
@@ -2878,7 +3061,7 @@ void Pipeline::simplify3X3Y(bool colour) {
         else kCount3x3yOld<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, valence, count);
         kMerge3x3y<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, arena.ptr<int32_t>("twin"), valence, count,
                                                       arena.ptr<float>("pos"), arena.ptr<float>("nrm"), vertexSlots(), arena.ptr<int32_t>("vtype"),
-                                                      arena.ptr<int32_t>("vtarget"), colour ? 1 : 0);
+                                                      arena.ptr<int32_t>("vtarget"), colour ? 1 : 0, simplifyQueue > 0 ? 1 : 0);
     }
     simplifyCompactAndRebuild(3, false);
 }
