@@ -1291,15 +1291,34 @@ __device__ __forceinline__ bool neighborFaces(const int32_t* __restrict__ twin, 
 }
 
 // mergeVerticesP3X3YColor (colour = 1) and mergeVerticesP3X3Y (colour = 0), HalfedgeMesh.cu:391-498
-__global__ void kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ twin,
+__global__ void __launch_bounds__(kBlock) kMerge3x3y(const int4* __restrict__ quads, uint32_t* __restrict__ qattr, int64_t Q, const int32_t* __restrict__ twin,
                            const int32_t* __restrict__ valence, const int32_t* __restrict__ count,
                            float* __restrict__ pos, float* __restrict__ nrm, const int32_t* __restrict__ vslot, int32_t* __restrict__ vtype, int32_t* __restrict__ vtarget, int colour,
                            int markTarget) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
+    // the block's pattern quads gathered in LDS and handled on consecutive lanes (each is a
+    // chain of dependent loads; a few per wave would spread it over every wave)
+    __shared__ int s_list[kBlock];
+    __shared__ int s_wave[kBlock / 64];
+    const int64_t qq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool pat = false;
+    if (qq < Q) {
+        const uint32_t a0 = qattr[qq];
+        pat = (a0 & QA_PATTERN) && (colour || !((a0 & QA_NM) || (a0 & QA_BOUNDARY)));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t mb = __ballot(pat);
+    if (lane == 0) s_wave[wid] = __popcll(mb);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) base += s_wave[w];
+        total += s_wave[w];
+    }
+    if (pat) s_list[base + __popcll(mb & ((1ULL << lane) - 1ULL))] = (int)qq;
+    __syncthreads();
+    if ((int)threadIdx.x >= total) return;
+    const int64_t q = s_list[threadIdx.x];
     const uint32_t a = qattr[q];
-    if (!colour && ((a & QA_NM) || (a & QA_BOUNDARY))) return;
-    if (!(a & QA_PATTERN)) return;
     const int4 v = quads[q];
     int nb[4];
     if (!neighborFaces(twin, (int)q, nb)) return;
