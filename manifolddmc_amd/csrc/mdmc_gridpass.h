@@ -67,6 +67,8 @@ struct GridPassArgs {
     int32_t* vcell;
     uint16_t* vct;
     int64_t V;
+    // per vertex its out-degree in the halfedge mesh (interior edges of its contour), when not null
+    int32_t* outdeg;
     // recorded for the adjoint when not null: per cell (first vertex << 2 | contours - 1) or -1
     // (single-problem extractions only)
     int32_t* adjMap;
@@ -90,5 +92,9 @@ void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s);
 // vertex work items and the dual vertices
 void launchSlabCount(int shape, const GridPassArgs& a, hipStream_t s);
 void launchSlabEmit(int shape, const GridPassArgs& a, hipStream_t s);
+// outgoing-list ranks of the grid pass's quads (from the corner vertices' contours), the
+// face -> halfedge array and the face attributes, for the halfedge initialize
+void launchQuadRanks(const int4* quads, const uint32_t* qattr, int64_t Q, int64_t V, const int32_t* vcell, const uint16_t* vct, const int32_t* vpid,
+                     const ProblemDev* probs, int32_t* rank, int32_t* face_he, uint32_t* fattr, int64_t* invalid, hipStream_t s);
 
 }  // namespace mdmc

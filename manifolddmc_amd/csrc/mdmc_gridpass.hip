@@ -230,6 +230,26 @@ __device__ __forceinline__ uint64_t contourWord(uint64_t w, int t) {
     return edges | ((uint64_t)sz << 60);
 }
 
+// The number of interior edges (all four cells around the edge inside the grid of CX x CY x CZ
+// cells) among the contour's edges before edge `stop` (12: all of them) of cell (x, y, z).  A
+// DMC vertex is in one quad per interior edge of its contour, so the count over all edges is its
+// out-degree, and up to a quad's edge its rank in its outgoing list.
+__device__ __forceinline__ int interiorEdgesBefore(uint64_t cw, int stop, int x, int y, int z, int CX, int CY, int CZ) {
+    const int sz = (int)(cw >> 60);
+    int n = 0;
+    for (int i = 0; i < sz; ++i) {
+        const int e = (int)((cw >> (4 * i)) & 0xF);
+        if (e == stop) break;
+        const int ca = edgeA(e), ax = edgeAxis(e);
+        const int px = x + (ca & 1), py = y + ((ca >> 1) & 1), pz = z + ((ca >> 2) & 1);
+        const bool inx = ax == 0 || (px >= 1 && px <= CX - 1);
+        const bool iny = ax == 1 || (py >= 1 && py <= CY - 1);
+        const bool inz = ax == 2 || (pz >= 1 && pz <= CZ - 1);
+        n += (inx && iny && inz) ? 1 : 0;
+    }
+    return n;
+}
+
 // the mean of the contour's edge crossings in cell-local coordinates (CellIntersection.inl:391-406)
 __device__ __forceinline__ f3 edgeMean(const float s[8], float iso, uint64_t cw) {
     const int sz = (int)(cw >> 60);
@@ -937,7 +957,10 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
         z = (int)(r / (unsigned)P.CY);
         loadCorners(g, x, y, z, s);
         const int ct = a.vct[v];
-        p = edgeMean(s, iso, contourWord(c_polyTab[ct & 0x3FF], ct >> 10));
+        const uint64_t cw = contourWord(c_polyTab[ct & 0x3FF], ct >> 10);
+        p = edgeMean(s, iso, cw);
+        // out-degree of the vertex in the halfedge mesh: one quad per interior edge of its contour
+        if (a.outdeg) a.outdeg[v] = interiorEdgesBefore(cw, 12, x, y, z, P.CX, P.CY, g.nz - 1);
         if (a.surfIter) {
             moverInit(m, s, p);
             for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);
@@ -998,6 +1021,63 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
     a.nrm[3 * v + 0] = nn.x;
     a.nrm[3 * v + 1] = nn.y;
     a.nrm[3 * v + 2] = nn.z;
+}
+
+// Outgoing-list rank of every halfedge 4q + i (the initialize's list of its origin, corner i):
+// the interior edges of the corner vertex's contour that come before the quad's edge.  The
+// quad's owner cell (the one whose corner 0 starts the edge) is the largest of its corners'
+// cells, and the other cells lie 1, CX or CX*CY cells back in two of the axes (kEmit's
+// neighbour table), which gives each corner's local edge.  Quads with a vertex id outside
+// [0, V) are counted as invalid (the caller then stops).  Also the face -> halfedge array and
+// the face attributes (collectHalfedgeElements, HalfedgeMesh.cu:14-58).
+__global__ void kQuadRanks(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int64_t V,
+                           const int32_t* __restrict__ vcell, const uint16_t* __restrict__ vct, const int32_t* __restrict__ vpid,
+                           const ProblemDev* __restrict__ probs, int32_t* __restrict__ rank, int32_t* __restrict__ face_he,
+                           uint32_t* __restrict__ fattr, int64_t* invalid) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int4 v4 = quads[q];
+    const int v[4] = {v4.x, v4.y, v4.z, v4.w};
+    const uint32_t at = qattr[q];
+    face_he[q] = (int32_t)(4 * q);
+    fattr[q] = at;
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bad = bad || v[i] < 0 || v[i] >= V;
+    if (bad) {
+        atomicAdd((unsigned long long*)invalid, 1ULL);
+        return;
+    }
+    const ProblemDev& P = probs[vpid ? vpid[v[0]] : 0];
+    const int CX = P.CX, CY = P.CY, CZ = P.g.nz - 1;
+    const int64_t CXY = (int64_t)CX * CY;
+    int c[4], t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        c[i] = vcell[v[i]];
+        t[i] = vct[v[i]];
+    }
+    const int owner = max(max(c[0], c[1]), max(c[2], c[3]));
+    const unsigned r = (unsigned)owner / (unsigned)CX;
+    const int ox = (int)((unsigned)owner - r * (unsigned)CX), oy = (int)(r % (unsigned)CY), oz = (int)(r / (unsigned)CY);
+    const int axis = (int)((at & QA_COLOR) % 3u);
+    int4 rk;
+    int* rr = &rk.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int64_t d = (int64_t)owner - c[i];
+        const int dz = d >= CXY ? 1 : 0;
+        d -= dz * CXY;
+        const int dy = d >= CX ? 1 : 0;
+        d -= dy * CX;
+        const int dx = (int)d;
+        // local edge of the corner cell: x axis (dy, dz) 0 2 4 6; y axis (dx, dz) 3 1 7 5; z axis (dx, dy) 8 9 11 10
+        const int k = axis == 0 ? (dy | (dz << 1)) : (axis == 1 ? (dx | (dz << 1)) : (dx | (dy << 1)));
+        const int e = (int)((axis == 0 ? 0x6420u : (axis == 1 ? 0x5713u : 0xAB98u)) >> (4 * k)) & 0xF;
+        const uint64_t cw = contourWord(c_polyTab[t[i] & 0x3FF], t[i] >> 10);
+        rr[i] = interiorEdgesBefore(cw, e, ox - dx, oy - dy, oz - dz, CX, CY, CZ);
+    }
+    reinterpret_cast<int4*>(rank)[q] = rk;
 }
 
 }  // namespace
@@ -1082,6 +1162,11 @@ void launchSlabEmit(int shape, const GridPassArgs& a, hipStream_t s) {
     if (gridPassSlabRows(shape) == 4) kSlab<4, true><<<(unsigned)a.numBlocks, SLAB_COLS, 0, s>>>(a, 1);
     else kSlab<2, true><<<(unsigned)a.numBlocks, SLAB_COLS, 0, s>>>(a, 1);
     if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a, 1);
+}
+
+void launchQuadRanks(const int4* quads, const uint32_t* qattr, int64_t Q, int64_t V, const int32_t* vcell, const uint16_t* vct, const int32_t* vpid,
+                     const ProblemDev* probs, int32_t* rank, int32_t* face_he, uint32_t* fattr, int64_t* invalid, hipStream_t s) {
+    if (Q > 0) kQuadRanks<<<(unsigned)((Q + 255) / 256), 256, 0, s>>>(quads, qattr, Q, V, vcell, vct, vpid, probs, rank, face_he, fattr, invalid);
 }
 
 void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {

@@ -473,30 +473,6 @@ __device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketO
     }
 }
 
-// first list pass over the quads: the rank of every halfedge in its origin's list, the
-// face -> halfedge array and the face attributes (collectHalfedgeElements, HalfedgeMesh.cu:14-58)
-// (a quad with a vertex id outside [0, V) is counted as invalid and takes no list slot; the
-// caller then stops)
-__global__ void kCollectOut(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int64_t V, int32_t* __restrict__ cnt,
-                            int32_t* __restrict__ rank, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int64_t* ctr) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    const int4 v = quads[q];
-    if (v.x < 0 || v.x >= V || v.y < 0 || v.y >= V || v.z < 0 || v.z >= V || v.w < 0 || v.w >= V) {
-        atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-        return;
-    }
-    const int b = (int)(4 * q);
-    int4 r;
-    r.x = atomicAdd(&cnt[v.x], 1);
-    r.y = atomicAdd(&cnt[v.y], 1);
-    r.z = atomicAdd(&cnt[v.z], 1);
-    r.w = atomicAdd(&cnt[v.w], 1);
-    reinterpret_cast<int4*>(rank)[q] = r;
-    face_he[q] = b;
-    fattr[q] = qattr[q];
-}
-
 // the entry of h in its origin's list, and beside it h's destination and the origin of the
 // halfedge before h in its quad
 __global__ void kFillOut(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ rank,
@@ -2541,6 +2517,7 @@ void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
     a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
     a.vct = arena.get<uint16_t>("gp_vct", (size_t)std::max<int64_t>(V, 1));
+    a.outdeg = arena.get<int32_t>("out_cnt", (size_t)V + 1);  // the halfedge initialize's list lengths
     a.V = V;
     a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
     MDMC_HIP(hipEventRecord(kev[2], stream));
@@ -2769,6 +2746,7 @@ void Pipeline::gridPassBack(int64_t C, int shape, bool recordAdjoint) {
     a.adjMap = recordAdjoint ? arena.get<int32_t>("adj_map", (size_t)std::max<int64_t>(C, 1)) : nullptr;
     a.vcell = arena.get<int32_t>("gp_vcell", (size_t)std::max<int64_t>(V, 1));
     a.vct = arena.get<uint16_t>("gp_vct", (size_t)std::max<int64_t>(V, 1));
+    a.outdeg = arena.get<int32_t>("out_cnt", (size_t)V + 1);  // the halfedge initialize's list lengths
     a.V = V;
     a.vpid = nProblems > 1 ? arena.get<int32_t>("pid", (size_t)std::max<int64_t>(V, 1)) : nullptr;
     MDMC_HIP(hipEventRecord(kev[2], stream));
@@ -2820,8 +2798,8 @@ int32_t* Pipeline::bucketPassValence(const int64_t* off, int64_t nV) {
     return valence;
 }
 
-// HalfedgeMesh::initialize from the quad list alone (the grid pass's output): outgoing lists
-// by counting, then every halfedge from its bucket
+// HalfedgeMesh::initialize from the grid pass's output: outgoing lists (lengths and ranks from
+// the vertices' contours), then every halfedge from its bucket
 void Pipeline::halfedgeInitialize() {
     H = 4 * Q;
     F = Q;
@@ -2837,8 +2815,11 @@ void Pipeline::halfedgeInitialize() {
     // the vertex -> halfedge array is derived from these quads when it is copied out
     vertHeV = V;
     vertHeQ = Q;
-    zero(cnt, ((size_t)V + 1) * sizeof(int32_t));
-    if (Q > 0) kCollectOut<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, Q, V, cnt, rank, face_he, fattr, counters());
+    // the list lengths (out-degrees) come from the grid pass's vertex kernel, the ranks from the
+    // corner vertices' contours: no counting atomics
+    zero(cnt + V, sizeof(int32_t));
+    launchQuadRanks(quads, qattr, Q, V, arena.ptr<int32_t>("gp_vcell"), arena.ptr<uint16_t>("gp_vct"), pid(), probsDev, rank, face_he, fattr,
+                    counters() + A_INVALID, stream);
     checkValid("quad vertex ids");
     int64_t* off = arena.get<int64_t>("out_off", (size_t)V + 1);
     scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream);
