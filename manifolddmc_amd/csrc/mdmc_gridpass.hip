@@ -72,6 +72,8 @@ __constant__ uint32_t c_polyBase[256];
 // per table entry: the contour of each crossed edge, 2 bits per edge (contourMap of the word),
 // and the number of contours - 1 in bits 24-25
 __constant__ uint32_t c_polyCmap[kPolyTabSize];
+// per table entry and contour t: the contour's edges as a 12-bit mask, at 4 * entry + t
+__constant__ uint16_t c_contourEdges[4 * kPolyTabSize];
 
 // the table entry of a cell of case mc: the decider bits of the case's ambiguous faces (only
 // those need the corner values; s is read for them alone)
@@ -230,24 +232,22 @@ __device__ __forceinline__ uint64_t contourWord(uint64_t w, int t) {
     return edges | ((uint64_t)sz << 60);
 }
 
-// The number of interior edges (all four cells around the edge inside the grid of CX x CY x CZ
-// cells) among the contour's edges before edge `stop` (12: all of them) of cell (x, y, z).  A
-// DMC vertex is in one quad per interior edge of its contour, so the count over all edges is its
-// out-degree, and up to a quad's edge its rank in its outgoing list.
-__device__ __forceinline__ int interiorEdgesBefore(uint64_t cw, int stop, int x, int y, int z, int CX, int CY, int CZ) {
-    const int sz = (int)(cw >> 60);
-    int n = 0;
-    for (int i = 0; i < sz; ++i) {
-        const int e = (int)((cw >> (4 * i)) & 0xF);
-        if (e == stop) break;
-        const int ca = edgeA(e), ax = edgeAxis(e);
-        const int px = x + (ca & 1), py = y + ((ca >> 1) & 1), pz = z + ((ca >> 2) & 1);
-        const bool inx = ax == 0 || (px >= 1 && px <= CX - 1);
-        const bool iny = ax == 1 || (py >= 1 && py <= CY - 1);
-        const bool inz = ax == 2 || (pz >= 1 && pz <= CZ - 1);
-        n += (inx && iny && inz) ? 1 : 0;
-    }
-    return n;
+// Interior edges of cell (x, y, z) of a grid of CX x CY x CZ cells as a 12-bit mask: an edge is
+// interior when all four cells around it are inside the grid, i.e. each coordinate across the
+// edge lies in [1, C - 1] at the edge's corner, which is x >= 1 for an edge on the cell's
+// low-x face and x <= CX - 2 for one on its high-x face (likewise y, z).  Edge e's faces come
+// from its first corner and its axis (EdgeVertices, CellIntersection.inl:383): low x: edges 3,
+// 7, 8, 11; high x: 1, 5, 9, 10; low y: 0, 4, 8, 9; high y: 2, 6, 10, 11; low z: 0, 1, 2, 3; high
+// z: 4, 5, 6, 7.  A DMC vertex is in one quad per interior edge of its contour.
+__device__ __forceinline__ uint32_t interiorEdgeMask(int x, int y, int z, int CX, int CY, int CZ) {
+    uint32_t out = 0;
+    if (x < 1) out |= 0x988u;
+    if (x > CX - 2) out |= 0x622u;
+    if (y < 1) out |= 0x311u;
+    if (y > CY - 2) out |= 0xC44u;
+    if (z < 1) out |= 0x00Fu;
+    if (z > CZ - 2) out |= 0x0F0u;
+    return ~out & 0xFFFu;
 }
 
 // the mean of the contour's edge crossings in cell-local coordinates (CellIntersection.inl:391-406)
@@ -960,7 +960,7 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
         const uint64_t cw = contourWord(c_polyTab[ct & 0x3FF], ct >> 10);
         p = edgeMean(s, iso, cw);
         // out-degree of the vertex in the halfedge mesh: one quad per interior edge of its contour
-        if (a.outdeg) a.outdeg[v] = interiorEdgesBefore(cw, 12, x, y, z, P.CX, P.CY, g.nz - 1);
+        if (a.outdeg) a.outdeg[v] = __popc(interiorEdgeMask(x, y, z, P.CX, P.CY, g.nz - 1) & c_contourEdges[4 * (ct & 0x3FF) + (ct >> 10)]);
         if (a.surfIter) {
             moverInit(m, s, p);
             for (int k = 0; k < kMoveSteps && !m.done; ++k) moverStep(m, s, iso);
@@ -1024,7 +1024,9 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
 }
 
 // Outgoing-list rank of every halfedge 4q + i (the initialize's list of its origin, corner i):
-// the interior edges of the corner vertex's contour that come before the quad's edge.  The
+// the interior edges of the corner vertex's contour with a lower edge number than the quad's
+// edge (any order that numbers a vertex's halfedges 0 .. out-degree - 1 will do: a list's
+// order never matters, buckets compare ids).  The
 // quad's owner cell (the one whose corner 0 starts the edge) is the largest of its corners'
 // cells, and the other cells lie 1, CX or CX*CY cells back in two of the axes (kEmit's
 // neighbour table), which gives each corner's local edge.  Quads with a vertex id outside
@@ -1074,8 +1076,8 @@ __global__ void kQuadRanks(const int4* __restrict__ quads, const uint32_t* __res
         // local edge of the corner cell: x axis (dy, dz) 0 2 4 6; y axis (dx, dz) 3 1 7 5; z axis (dx, dy) 8 9 11 10
         const int k = axis == 0 ? (dy | (dz << 1)) : (axis == 1 ? (dx | (dz << 1)) : (dx | (dy << 1)));
         const int e = (int)((axis == 0 ? 0x6420u : (axis == 1 ? 0x5713u : 0xAB98u)) >> (4 * k)) & 0xF;
-        const uint64_t cw = contourWord(c_polyTab[t[i] & 0x3FF], t[i] >> 10);
-        rr[i] = interiorEdgesBefore(cw, e, ox - dx, oy - dy, oz - dz, CX, CY, CZ);
+        const uint32_t m = interiorEdgeMask(ox - dx, oy - dy, oz - dz, CX, CY, CZ) & c_contourEdges[4 * (t[i] & 0x3FF) + (t[i] >> 10)];
+        rr[i] = __popc(m & ((1u << e) - 1u));
     }
     reinterpret_cast<int4*>(rank)[q] = rk;
 }
@@ -1131,6 +1133,16 @@ void initGridPassTables() {
     MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyTab), tab, sizeof tab));
     MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyBase), base, sizeof base));
     MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polyCmap), cmap, sizeof cmap));
+    // per entry and contour: the contour's edges as a mask
+    uint16_t cedges[4 * kPolyTabSize] = {};
+    for (int i = 0; i < n; ++i) {
+        int j = 0;
+        for (int c = 0; c < 4; ++c) {
+            const int sz = (int)((tab[i] >> (4 * c)) & 0xF);
+            for (int k = 0; k < sz; ++k, ++j) cedges[4 * i + c] |= (uint16_t)(1u << (int)((tab[i] >> (16 + 4 * j)) & 0xF));
+        }
+    }
+    MDMC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_contourEdges), cedges, sizeof cedges));
 }
 
 const char* gridPassShapeName(int shape) {
