@@ -92,9 +92,11 @@ void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s);
 // vertex work items and the dual vertices
 void launchSlabCount(int shape, const GridPassArgs& a, hipStream_t s);
 void launchSlabEmit(int shape, const GridPassArgs& a, hipStream_t s);
-// outgoing-list ranks of the grid pass's quads (from the corner vertices' contours), the
-// face -> halfedge array and the face attributes, for the halfedge initialize
-void launchQuadRanks(const int4* quads, const uint32_t* qattr, int64_t Q, int64_t V, const int32_t* vcell, const uint16_t* vct, const int32_t* vpid,
-                     const ProblemDev* probs, int32_t* rank, int32_t* face_he, uint32_t* fattr, int64_t* invalid, hipStream_t s);
+// the halfedge initialize's outgoing lists from the grid pass's quads: entry off[v] + rank (the
+// rank from v's contour) of halfedge 4q + i from corner v = i holds 4q + i (out) and corners
+// i + 1 and i - 1 (ends); also the face -> halfedge array and the face attributes
+void launchQuadLists(const int4* quads, const uint32_t* qattr, int64_t Q, int64_t V, const int32_t* vcell, const uint16_t* vct, const int32_t* vpid,
+                     const ProblemDev* probs, const int64_t* off, int32_t* out, int2* ends, int32_t* face_he, uint32_t* fattr, int64_t* invalid,
+                     hipStream_t s);
 
 }  // namespace mdmc

@@ -1023,19 +1023,21 @@ __global__ void __launch_bounds__(GP_THREADS) kVertices(GridPassArgs a, int chun
     a.nrm[3 * v + 2] = nn.z;
 }
 
-// Outgoing-list rank of every halfedge 4q + i (the initialize's list of its origin, corner i):
-// the interior edges of the corner vertex's contour with a lower edge number than the quad's
-// edge (any order that numbers a vertex's halfedges 0 .. out-degree - 1 will do: a list's
-// order never matters, buckets compare ids).  The
+// The initialize's outgoing lists from the grid pass's quads: halfedge 4q + i (from corner i
+// to corner i + 1) goes to entry off[v] + rank of its origin v = corner i: out holds 4q + i and
+// ends its destination (corner i + 1) and the origin of the halfedge before it (corner i - 1).  The rank is the number of interior edges of v's contour with
+// a lower edge number than the quad's edge (any order that numbers a vertex's halfedges
+// 0 .. out-degree - 1 will do: a list's order never matters, buckets compare ids).  The
 // quad's owner cell (the one whose corner 0 starts the edge) is the largest of its corners'
 // cells, and the other cells lie 1, CX or CX*CY cells back in two of the axes (kEmit's
-// neighbour table), which gives each corner's local edge.  Quads with a vertex id outside
-// [0, V) are counted as invalid (the caller then stops).  Also the face -> halfedge array and
-// the face attributes (collectHalfedgeElements, HalfedgeMesh.cu:14-58).
-__global__ void kQuadRanks(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int64_t V,
+// neighbour table), which gives each corner's local edge.  A quad with a vertex id outside
+// [0, V), or whose edge is not an interior edge of a corner's contour or ranks past the
+// corner's list, is counted as invalid and writes no entry for it (the caller then stops).  Also the face -> halfedge array and the
+// face attributes (collectHalfedgeElements, HalfedgeMesh.cu:14-58).
+__global__ void kQuadLists(const int4* __restrict__ quads, const uint32_t* __restrict__ qattr, int64_t Q, int64_t V,
                            const int32_t* __restrict__ vcell, const uint16_t* __restrict__ vct, const int32_t* __restrict__ vpid,
-                           const ProblemDev* __restrict__ probs, int32_t* __restrict__ rank, int32_t* __restrict__ face_he,
-                           uint32_t* __restrict__ fattr, int64_t* invalid) {
+                           const ProblemDev* __restrict__ probs, const int64_t* __restrict__ off, int32_t* __restrict__ out,
+                           int2* __restrict__ ends, int32_t* __restrict__ face_he, uint32_t* __restrict__ fattr, int64_t* invalid) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Q) return;
     const int4 v4 = quads[q];
@@ -1054,17 +1056,19 @@ __global__ void kQuadRanks(const int4* __restrict__ quads, const uint32_t* __res
     const int CX = P.CX, CY = P.CY, CZ = P.g.nz - 1;
     const int64_t CXY = (int64_t)CX * CY;
     int c[4], t[4];
+    int64_t o[4], o1[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         c[i] = vcell[v[i]];
         t[i] = vct[v[i]];
+        o[i] = off[v[i]];
+        o1[i] = off[v[i] + 1];
     }
     const int owner = max(max(c[0], c[1]), max(c[2], c[3]));
     const unsigned r = (unsigned)owner / (unsigned)CX;
     const int ox = (int)((unsigned)owner - r * (unsigned)CX), oy = (int)(r % (unsigned)CY), oz = (int)(r / (unsigned)CY);
     const int axis = (int)((at & QA_COLOR) % 3u);
-    int4 rk;
-    int* rr = &rk.x;
+    int nbad = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         int64_t d = (int64_t)owner - c[i];
@@ -1077,9 +1081,15 @@ __global__ void kQuadRanks(const int4* __restrict__ quads, const uint32_t* __res
         const int k = axis == 0 ? (dy | (dz << 1)) : (axis == 1 ? (dx | (dz << 1)) : (dx | (dy << 1)));
         const int e = (int)((axis == 0 ? 0x6420u : (axis == 1 ? 0x5713u : 0xAB98u)) >> (4 * k)) & 0xF;
         const uint32_t m = interiorEdgeMask(ox - dx, oy - dy, oz - dz, CX, CY, CZ) & c_contourEdges[4 * (t[i] & 0x3FF) + (t[i] >> 10)];
-        rr[i] = __popc(m & ((1u << e) - 1u));
+        const int64_t pos = o[i] + __popc(m & ((1u << e) - 1u));
+        if (!((m >> e) & 1u) || pos >= o1[i] || pos < 0 || pos >= 4 * Q) {
+            ++nbad;
+            continue;
+        }
+        out[pos] = (int)(4 * q + i);
+        ends[pos] = make_int2(v[(i + 1) & 3], v[(i + 3) & 3]);
     }
-    reinterpret_cast<int4*>(rank)[q] = rk;
+    if (nbad) atomicAdd((unsigned long long*)invalid, (unsigned long long)nbad);
 }
 
 }  // namespace
@@ -1176,9 +1186,10 @@ void launchSlabEmit(int shape, const GridPassArgs& a, hipStream_t s) {
     if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a, 1);
 }
 
-void launchQuadRanks(const int4* quads, const uint32_t* qattr, int64_t Q, int64_t V, const int32_t* vcell, const uint16_t* vct, const int32_t* vpid,
-                     const ProblemDev* probs, int32_t* rank, int32_t* face_he, uint32_t* fattr, int64_t* invalid, hipStream_t s) {
-    if (Q > 0) kQuadRanks<<<(unsigned)((Q + 255) / 256), 256, 0, s>>>(quads, qattr, Q, V, vcell, vct, vpid, probs, rank, face_he, fattr, invalid);
+void launchQuadLists(const int4* quads, const uint32_t* qattr, int64_t Q, int64_t V, const int32_t* vcell, const uint16_t* vct, const int32_t* vpid,
+                     const ProblemDev* probs, const int64_t* off, int32_t* out, int2* ends, int32_t* face_he, uint32_t* fattr, int64_t* invalid,
+                     hipStream_t s) {
+    if (Q > 0) kQuadLists<<<(unsigned)((Q + 255) / 256), 256, 0, s>>>(quads, qattr, Q, V, vcell, vct, vpid, probs, off, out, ends, face_he, fattr, invalid);
 }
 
 void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {

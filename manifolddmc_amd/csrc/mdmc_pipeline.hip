@@ -393,7 +393,7 @@ __device__ BucketOf bucketFromOutEnds(int h, int a, int b, int64_t H, const int6
         if (id != h) partner = id;
         else self = true;
     };
-    const int64_t k0 = off[m], k1 = off[m + 1];
+    const int64_t k0 = off[m], k1 = min(off[m + 1], H);
     for (int64_t kb = k0; kb < k1; kb += 4) {
         int xs[4];
         int2 es[4];
@@ -471,19 +471,6 @@ __device__ __forceinline__ void storeHalfedge(int h, int a, int b, const BucketO
         atomicOr(&fattr[f], (uint32_t)QA_NM);
         atomicOr(&qattr[f], (uint32_t)QA_NM);
     }
-}
-
-// the entry of h in its origin's list, and beside it h's destination and the origin of the
-// halfedge before h in its quad
-__global__ void kFillOut(const int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ rank,
-                         int32_t* __restrict__ out, int2* __restrict__ ends) {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= H) return;
-    const int4 q = quads[h >> 2];
-    const int j = (int)(h & 3);
-    const int64_t k = off[quadCorner(q, j)] + rank[h];
-    out[k] = (int32_t)h;
-    ends[k] = make_int2(quadCorner(q, (j + 1) & 3), quadCorner(q, (j + 3) & 3));
 }
 
 // valence = outgoing halfedges (the list length); incoming boundary halfedges are added by
@@ -2808,7 +2795,6 @@ void Pipeline::halfedgeInitialize() {
     uint32_t* fattr = arena.get<uint32_t>("fattr", (size_t)F);
     uint32_t* vflags = arena.get<uint32_t>("vflags", (size_t)V);
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)V + 1);
-    int32_t* rank = arena.get<int32_t>("bl_rank", (size_t)H);
     uint32_t* bpack = arena.get<uint32_t>("bpack", (size_t)std::max<int64_t>(H, 1));
     const int4* quads = arena.ptr<int4>("quads");
     uint32_t* qattr = arena.ptr<uint32_t>("qattr");
@@ -2818,14 +2804,13 @@ void Pipeline::halfedgeInitialize() {
     // the list lengths (out-degrees) come from the grid pass's vertex kernel, the ranks from the
     // corner vertices' contours: no counting atomics
     zero(cnt + V, sizeof(int32_t));
-    launchQuadRanks(quads, qattr, Q, V, arena.ptr<int32_t>("gp_vcell"), arena.ptr<uint16_t>("gp_vct"), pid(), probsDev, rank, face_he, fattr,
-                    counters() + A_INVALID, stream);
-    checkValid("quad vertex ids");
     int64_t* off = arena.get<int64_t>("out_off", (size_t)V + 1);
     scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream);
     int32_t* out = arena.get<int32_t>("out_list", (size_t)std::max<int64_t>(H, 1));
     int2* ends = arena.get<int2>("out_ends", (size_t)std::max<int64_t>(H, 1));
-    if (H > 0) kFillOut<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, rank, out, ends);
+    launchQuadLists(quads, qattr, Q, V, arena.ptr<int32_t>("gp_vcell"), arena.ptr<uint16_t>("gp_vct"), pid(), probsDev, off, out, ends, face_he, fattr,
+                    counters() + A_INVALID, stream);
+    checkValid("quad vertex ids");
     zero(vflags, (size_t)V * sizeof(uint32_t));
     int32_t* valence = bucketPassValence(off, V);
     if (H > 0) kBucketsAll<<<gridFor(H), kBlock, 0, stream>>>(quads, H, off, out, ends, twin, bpack, vflags, fattr, qattr, valence, counters());
