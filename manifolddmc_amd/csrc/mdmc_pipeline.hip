@@ -2324,7 +2324,12 @@ __global__ void __launch_bounds__(kBlock) kStats(const int4* __restrict__ he, in
                 h = nextOf(he, canon, h);
             }
         }
-        loc[3 + k] += 1;
+        // (constant indices only: a computed index would put loc in scratch memory)
+        loc[3] += k == 0;
+        loc[4] += k == 1;
+        loc[5] += k == 2;
+        loc[6] += k == 3;
+        loc[7] += k == 4;
     }
     for (int64_t v = t0; v < V; v += stride) loc[8] += (vflags[v] & VF_FANS) ? 1u : 0u;
     for (int k = 0; k < 9; ++k) {
