@@ -66,8 +66,9 @@ def stage_roofline(s, n_in, stats):
         "ms_grid": ("pad-by-1: read the field, write the padded grid", 4 * n_in ** 3 + 4 * P),
         "ms_dmc": ("grid pass: read the padded grid, write vertices (pos + normal) and quads", 4 * P + 24 * V + 16 * Q),
         "ms_halfedge": ("halfedge initialize: read quads, write the halfedge mesh", 16 * Q + he_mesh(Q, V)),
-        "ms_simplify": ("3 simplifications: each reads the halfedge mesh and writes the compacted vertices, quads and "
-                        "their halfedge mesh", 3 * (16 * 4 * Q + 24 * V + 16 * Q + he_mesh(Q, V))),
+        "ms_simplify": ("chain of 3 simplifications compacted once: each step reads the quads, their attributes and the "
+                        "corner valences; the chain writes the compacted vertices, quads and their halfedge mesh",
+                        3 * (20 * Q + 4 * V) + 16 * Q + he_mesh(Q, V) + 24 * V),
         "ms_fix_twins": ("twin repair: read the halfedge records", 16 * H),
         "ms_split": ("split: per iteration read the face attributes and halfedge records", s["split_iterations"] * (4 * F + 16 * H)),
         "ms_detect": ("NM-vertex detection: read the halfedge records, write vertex flags", 16 * H + Vf),
