@@ -24,6 +24,10 @@ struct GridDev {
 size_t scanScratchBytes(int64_t n);
 // total_dev and directOut may point into mapped pinned host memory
 void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s, int64_t* directOut = nullptr);
+// stream compaction in one pass: the i < n with (attr[i] & mask) != 0, ascending, into
+// idx (capacity n); the count through *total_dev; scratch sized by scanScratchBytes(n)
+void compactFlagged(const uint32_t* attr, int64_t n, uint32_t mask, int32_t* idx, int64_t* total_dev, void* scratch, hipStream_t s,
+                    int64_t* directOut = nullptr);
 
 inline unsigned gridFor(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 

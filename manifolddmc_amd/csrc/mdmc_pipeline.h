@@ -143,6 +143,7 @@ class Pipeline {
     size_t upCap_[2] = {0, 0};
     int64_t* counters();
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
+    int64_t compactNM(int32_t* nm);
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
     void launchFill(void* p, uint8_t byte, size_t bytes);

@@ -1635,23 +1635,6 @@ __global__ void __launch_bounds__(kBlock) kFixTwins(int4* __restrict__ he, int64
 }
 
 // ---------------------------------------------------------------- ManifoldSplitter
-__global__ void kNMFacePred(const uint32_t* __restrict__ fattr, int64_t F, int32_t* __restrict__ pred) {
-    const int64_t f0 = firstOfFour();
-    if (f0 >= F) return;
-    if (vec4(f0, F, fattr, pred)) {
-        const uint4 a = reinterpret_cast<const uint4*>(fattr)[f0 >> 2];
-        reinterpret_cast<int4*>(pred)[f0 >> 2] = make_int4((a.x & QA_NM) != 0, (a.y & QA_NM) != 0, (a.z & QA_NM) != 0, (a.w & QA_NM) != 0);
-    } else {
-        for (int64_t f = f0; f < f0 + 4 && f < F; ++f) pred[f] = (fattr[f] & QA_NM) ? 1 : 0;
-    }
-}
-
-__global__ void kScatterIndex(const int32_t* __restrict__ pred, const int64_t* __restrict__ addr, int64_t n, int32_t* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (pred[i]) out[addr[i]] = (int32_t)i;
-}
-
 struct WC {
     // vertices (capacity 8n)
     float *pos, *nrm, *pos0, *nrm0;
@@ -2432,6 +2415,19 @@ int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outNam
     return h[1];
 }
 
+// the faces with the NM bit, ascending, into nm (capacity F); returns their number
+int64_t Pipeline::compactNM(int32_t* nm) {
+    void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(F + 1));
+    int64_t* h = pinHost();
+    int64_t* d = pinDev();
+    h[0] = 0;
+    compactFlagged(arena.ptr<uint32_t>("fattr"), F, (uint32_t)QA_NM, nm, d + 1, scratch, stream, d);
+    MDMC_HIP(hipStreamSynchronize(stream));
+    const int64_t n = h[1];
+    if (n < 0 || n > F) throw HipError("inconsistent NM-face count (" + std::to_string(n) + ")", -4);
+    return n;
+}
+
 // fills with one launch (the runtime's memset launches a body and a tail kernel)
 void Pipeline::zero(void* p, size_t bytes) { launchFill(p, 0x00, bytes); }
 void Pipeline::fillNeg(void* p, size_t bytes) { launchFill(p, 0xFF, bytes); }
@@ -3109,11 +3105,9 @@ void Pipeline::split(int step, int maxIters) {
     splitTouched = true;
     while (true) {
         const int64_t iterH = H;
-        // countNonManifoldFaces
-        int32_t* pred = arena.get<int32_t>("nmpred", (size_t)F);
-        if (F > 0) kNMFacePred<<<gridFor((F + 3) / 4), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F, pred);
-        int64_t* faddr = nullptr;
-        const int64_t n64 = scan32(pred, F, "nmaddr", &faddr);
+        // countNonManifoldFaces: the NM faces' ids, ascending, in one compaction pass
+        int32_t* nm = arena.get<int32_t>("nmlist", (size_t)std::max<int64_t>(F, 1));
+        const int64_t n64 = compactNM(nm);
         if (n64 == 0) break;
         // still non-manifold after the allowed iterations (the reference loops until clean)
         if (iter >= maxIters) {
@@ -3123,8 +3117,6 @@ void Pipeline::split(int step, int maxIters) {
         pinHost()[kPinMergeTimeout] = 0;  // direct-sum diagnostic of the merge-list scans
         if (20 * n64 >= (int64_t)INT32_MAX / 2) throw HipError("working copy exceeds the int32 id range", -3);
         const int n = (int)n64;
-        int32_t* nm = arena.get<int32_t>("nmlist", (size_t)n);
-        kScatterIndex<<<gridFor(F), kBlock, 0, stream>>>(pred, faddr, F, nm);
 
         WC w;
         w.n = n;
