@@ -14,6 +14,17 @@
 
 namespace mdmc {
 
+// up to kFillSetMax byte fills for one launch (kFillMany): per range its pointer, the unaligned
+// head and tail bytes, the 16-byte aligned body and the range's first block
+constexpr int kFillSetMax = 12;
+struct FillSet {
+    uint8_t* p[kFillSetMax];
+    size_t head[kFillSetMax], body[kFillSetMax], n[kFillSetMax];
+    unsigned start[kFillSetMax];
+    uint8_t byte[kFillSetMax];
+    int count = 0;
+};
+
 class Pipeline {
    public:
     Pipeline();
@@ -147,6 +158,26 @@ class Pipeline {
     void zero(void* p, size_t bytes);
     void fillNeg(void* p, size_t bytes);
     void launchFill(void* p, uint8_t byte, size_t bytes);
+    // fills collected and launched together (at flush or at the end of the scope)
+    class FillBatch {
+       public:
+        explicit FillBatch(Pipeline& p) : pl(p) {}
+        ~FillBatch() { flush(); }
+        void add(void* p, uint8_t byte, size_t bytes);
+        void flush();
+
+       private:
+        Pipeline& pl;
+        FillSet fs;
+        unsigned blocks = 0;
+    };
+    // the splitter's old -> copy maps: -1 on [0, clean) between steps (each step resets the
+    // entries it set); a map the arena reallocated, or one a failed step left dirty, is filled
+    struct WcMap {
+        void* ptr = nullptr;
+        int64_t clean = 0, cap = 0;
+    } wcMaps[3];
+    bool wcMapsDirty = true;
     void gridPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
     void slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape, bool surfIter, bool recordAdjoint);
     GridPassArgs ga_{};

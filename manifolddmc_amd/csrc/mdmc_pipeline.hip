@@ -65,6 +65,24 @@ __global__ void kFillBytes(uint8_t* __restrict__ p, uint8_t byte, size_t head, s
     }
 }
 
+// several fills in one launch: block b works on the range j with start[j] <= b < start[j + 1],
+// as kFillBytes does on one range
+__global__ void kFillMany(FillSet fs) {
+    int j = 0;
+    while (j + 1 < fs.count && (unsigned)blockIdx.x >= fs.start[j + 1]) ++j;
+    const size_t t = (size_t)(blockIdx.x - fs.start[j]) * blockDim.x + threadIdx.x;
+    uint8_t* p = fs.p[j];
+    const uint8_t byte = fs.byte[j];
+    if (16 * t < fs.body[j]) {
+        const uint32_t w = 0x01010101u * byte;
+        *reinterpret_cast<uint4*>(p + fs.head[j] + 16 * t) = make_uint4(w, w, w, w);
+    }
+    if (t < 16) {
+        if (t < fs.head[j]) p[t] = byte;
+        if (fs.head[j] + fs.body[j] + t < fs.n[j]) p[fs.head[j] + fs.body[j] + t] = byte;
+    }
+}
+
 // Elementwise kernels below take four elements per thread: one 16-byte access per array when
 // all four are in range and the arrays are 16-byte aligned (the arena's are), else one by one.
 __device__ __forceinline__ int64_t firstOfFour() { return 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
@@ -1650,6 +1668,9 @@ struct WC {
     uint32_t* fflags;
     int32_t *f_c2o, *f_o2c;
     int32_t* pid;  // per vertex its problem (several problems), else nullptr
+    // the old vertex and halfedge of each corner slot as kWcCopy took them (4n): the entries
+    // of the old -> copy maps that the step sets, reset to -1 by kWcUndo at the step's end
+    int32_t *v_orig, *h_orig;
     int n;
 };
 
@@ -1675,10 +1696,12 @@ __global__ void kWcCopy(const int32_t* __restrict__ nm, int n, const int4* __res
         w.nrm[3 * c + k] = nrm[3 * (int64_t)v + k];
     }
     w.v_c2o[c] = v;
+    w.v_orig[c] = v;
     if (w.pid) w.pid[c] = pid[v];
     atomicMax(&w.v_o2c[v], c);  // racy "last writer" in ascending order
     w.he[c] = e;
     w.h_c2o[c] = h;
+    w.h_orig[c] = h;
     w.h_o2c[h] = c;
     if (i == 0) {
         w.f_first[r] = 4 * r;
@@ -1686,6 +1709,16 @@ __global__ void kWcCopy(const int32_t* __restrict__ nm, int n, const int4* __res
         w.f_c2o[r] = f;
         w.f_o2c[f] = r;
     }
+}
+
+// the old -> copy map entries the step set (kWcCopy's, which kSubdivide only re-points) back
+// to -1, so the maps are all -1 again for the next step without a full fill
+__global__ void kWcUndo(const int32_t* __restrict__ nm, WC w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 4 * w.n) return;
+    w.v_o2c[w.v_orig[c]] = -1;
+    w.h_o2c[w.h_orig[c]] = -1;
+    if ((c & 3) == 0) w.f_o2c[nm[c >> 2]] = -1;
 }
 
 // updateWorkingCopyHalfedges, ManifoldSplitter.cu:463-505
@@ -2439,6 +2472,28 @@ void Pipeline::launchFill(void* p, uint8_t byte, size_t bytes) {
     kFillBytes<<<(unsigned)std::max<size_t>((vec + kBlock - 1) / kBlock, 1), kBlock, 0, stream>>>((uint8_t*)p, byte, head, body, bytes);
 }
 
+void Pipeline::FillBatch::add(void* p, uint8_t byte, size_t bytes) {
+    if (bytes == 0) return;
+    if (fs.count == kFillSetMax) flush();
+    const int j = fs.count++;
+    const size_t head = std::min<size_t>((16 - ((uintptr_t)p & 15)) & 15, bytes);
+    const size_t body = (bytes - head) & ~(size_t)15;
+    const size_t vec = body / 16;
+    fs.p[j] = (uint8_t*)p;
+    fs.byte[j] = byte;
+    fs.head[j] = head;
+    fs.body[j] = body;
+    fs.n[j] = bytes;
+    fs.start[j] = blocks;
+    blocks += (unsigned)std::max<size_t>((vec + kBlock - 1) / kBlock, 1);
+}
+void Pipeline::FillBatch::flush() {
+    if (fs.count == 0) return;
+    kFillMany<<<blocks, kBlock, 0, pl.stream>>>(fs);
+    fs.count = 0;
+    blocks = 0;
+}
+
 void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t)); }
 
 int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
@@ -3127,6 +3182,10 @@ void Pipeline::split(int step, int maxIters) {
         w.nrm0 = arena.get<float>("wc_nrm0", 3 * (size_t)nv);
         w.vflags = arena.get<uint32_t>("wc_vflags", (size_t)nv);
         w.v_c2o = arena.get<int32_t>("wc_v_c2o", (size_t)nv);
+        // a map the arena reallocates holds no -1s: its capacity before the get tells
+        wcMaps[0].cap = (int64_t)arena.capacity<int32_t>("wc_v_o2c");
+        wcMaps[1].cap = (int64_t)arena.capacity<int32_t>("wc_h_o2c");
+        wcMaps[2].cap = (int64_t)arena.capacity<int32_t>("wc_f_o2c");
         w.v_o2c = arena.get<int32_t>("wc_v_o2c", (size_t)(V + nv));
         w.he = arena.get<int4>("wc_he", (size_t)nh);
         w.hflags = arena.get<uint32_t>("wc_hflags", (size_t)nh);
@@ -3138,18 +3197,36 @@ void Pipeline::split(int step, int maxIters) {
         w.f_c2o = arena.get<int32_t>("wc_f_c2o", (size_t)nf);
         w.f_o2c = arena.get<int32_t>("wc_f_o2c", (size_t)(F + nf));
         w.pid = pid() ? arena.get<int32_t>("wc_pid", (size_t)nv) : nullptr;
+        w.v_orig = arena.get<int32_t>("wc_v_orig", 4 * (size_t)n);
+        w.h_orig = arena.get<int32_t>("wc_h_orig", 4 * (size_t)n);
         // positions, normals, halfedge records, first halfedges and face attributes of the
         // working copy are written in full by kWcCopy ([0, 4n) / [0, n)) and kSubdivide (the
-        // rest) before any read, so only the flags and the id maps are initialised here
-        zero(w.vflags, (size_t)nv * sizeof(uint32_t));  // WorkingCopy ignores its initialFlags argument: flags start at 0
-        fillNeg(w.v_c2o, (size_t)nv * sizeof(int32_t));
-        fillNeg(w.v_o2c, (size_t)(V + nv) * sizeof(int32_t));
-        zero(w.hflags, (size_t)nh * sizeof(uint32_t));
-        fillNeg(w.h_c2o, (size_t)nh * sizeof(int32_t));
-        fillNeg(w.h_o2c, (size_t)(H + nh) * sizeof(int32_t));
-        zero(w.fflags, (size_t)nf * sizeof(uint32_t));
-        fillNeg(w.f_c2o, (size_t)nf * sizeof(int32_t));
-        fillNeg(w.f_o2c, (size_t)(F + nf) * sizeof(int32_t));
+        // rest) before any read, so only the flags and the id maps are initialised here (one
+        // launch); the old -> copy maps are all -1 between steps (kWcUndo), so only a part no
+        // step has used yet is filled
+        {
+            FillBatch fb(*this);
+            fb.add(w.vflags, 0x00, (size_t)nv * sizeof(uint32_t));  // WorkingCopy ignores its initialFlags argument: flags start at 0
+            fb.add(w.v_c2o, 0xFF, (size_t)nv * sizeof(int32_t));
+            fb.add(w.hflags, 0x00, (size_t)nh * sizeof(uint32_t));
+            fb.add(w.h_c2o, 0xFF, (size_t)nh * sizeof(int32_t));
+            fb.add(w.fflags, 0x00, (size_t)nf * sizeof(uint32_t));
+            fb.add(w.f_c2o, 0xFF, (size_t)nf * sizeof(int32_t));
+            const int64_t need[3] = {V + nv, H + nh, F + nf};
+            int32_t* maps[3] = {w.v_o2c, w.h_o2c, w.f_o2c};
+            for (int k = 0; k < 3; ++k) {
+                WcMap& m = wcMaps[k];
+                if (maps[k] != m.ptr || wcMapsDirty || m.cap < need[k]) {
+                    fb.add(maps[k], 0xFF, (size_t)need[k] * sizeof(int32_t));
+                    m.ptr = maps[k];
+                    m.clean = need[k];
+                } else if (need[k] > m.clean) {
+                    fb.add(maps[k] + m.clean, 0xFF, (size_t)(need[k] - m.clean) * sizeof(int32_t));
+                    m.clean = need[k];
+                }
+            }
+        }
+        wcMapsDirty = true;  // until kWcUndo has run
 
         kWcCopy<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, n, arena.ptr<int4>("he"), arena.ptr<int32_t>("face_he"), arena.ptr<uint32_t>("fattr"),
                                                    arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), w);
@@ -3257,6 +3334,8 @@ void Pipeline::split(int step, int maxIters) {
             H = H0 + addH;
             F = F0 + addF;
         }
+        kWcUndo<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, w);
+        wcMapsDirty = false;
         uint8_t* dirtyV = arena.get<uint8_t>("dirty_v", (size_t)std::max<int64_t>(V, 1));
         zero(dirtyV, (size_t)std::max<int64_t>(V, 1));
         if (V > touchedV) {
