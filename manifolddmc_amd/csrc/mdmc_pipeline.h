@@ -196,7 +196,7 @@ class Pipeline {
     int32_t* bucketPassValence(const int64_t* off, int64_t nV);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);
     void checkValid(const char* what);
-    void simplifyInit(int32_t** valence, int32_t** count);
+    void simplifyInit(int32_t** valence, int32_t** count, bool needCount);
     void simplifyCompactAndRebuild(int dropType, bool p3333);
     void simplifyDeferredStep(int dropType, bool p3333);
     void buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag);

@@ -2896,8 +2896,11 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     arena.grow<int32_t>("bsize", (size_t)H, (size_t)std::min(oldH, H), stream);
     arena.grow<int32_t>("brank", (size_t)H, (size_t)std::min(oldH, H), stream);
     int32_t* dlist = arena.get<int32_t>("dirty_list", (size_t)std::max<int64_t>(H, 1));
-    zero(degM, ((size_t)V + 1) * sizeof(int32_t));
-    zero(counters() + A_DIRTY, sizeof(int64_t));
+    {
+        FillBatch fb(*this);
+        fb.add(degM, 0x00, ((size_t)V + 1) * sizeof(int32_t));
+        fb.add(counters() + A_DIRTY, 0x00, sizeof(int64_t));
+    }
     if (F > 0) kFaceClear<<<gridFor((F + 3) / 4), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
     const int64_t per = (int64_t)kBlock * kDirtyPer;
     if (H > 0) kDirtyRecords<<<(unsigned)((H + per - 1) / per), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dlist, counters());
@@ -2931,16 +2934,19 @@ void Pipeline::simplifyDeferredStep(int dropType, bool p3333) {
     int32_t* valence = arena.get<int32_t>("valence", nv);
     int32_t* twin = arena.ptr<int32_t>("twin");
     uint32_t* bpack = arena.ptr<uint32_t>("bpack");
-    if (deferredSteps == 0) {
-        if (V > 0) kOutdegInit<<<gridFor(V), kBlock, 0, stream>>>(off, V, valence, outdeg, bdin);
-        zero(vdead, (size_t)V);
-        fillNeg(head, (size_t)V * sizeof(int32_t));
-    }
     uint8_t* touched = arena.get<uint8_t>("touched", nv);
-    zero(touched, (size_t)V);
     // (P3333 ends a chain, so a deferred step is a P3X3Y one: it drops single vertices)
     if (p3333) throw HipError("deferred P3333 step", -4);
-    zero(counters() + A_FALLBACK, sizeof(int64_t));
+    {
+        FillBatch fb(*this);
+        if (deferredSteps == 0) {
+            fb.add(vdead, 0x00, (size_t)V);
+            fb.add(head, 0xFF, (size_t)V * sizeof(int32_t));
+        }
+        fb.add(touched, 0x00, (size_t)V);
+        fb.add(counters() + A_FALLBACK, 0x00, sizeof(int64_t));
+    }
+    if (deferredSteps == 0 && V > 0) kOutdegInit<<<gridFor(V), kBlock, 0, stream>>>(off, V, valence, outdeg, bdin);
     int2* zslot = arena.get<int2>("zslot", nv);
     if (Q > 0)
         kLocalMerge<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, fattr, Q, valence, vtype, vtarget, twin, bpack, outdeg, zslot, V, touched,
@@ -3075,15 +3081,19 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     canonTracked = false;
 }
 
-void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
+void Pipeline::simplifyInit(int32_t** valence, int32_t** count, bool needCount) {
     if (simplifyQueue > 0) --simplifyQueue;
     *valence = arena.get<int32_t>("valence", (size_t)V);
     *count = arena.get<int32_t>("vcount", (size_t)V);
     int32_t* vtype = arena.get<int32_t>("vtype", (size_t)V);
     int32_t* vtarget = arena.get<int32_t>("vtarget", (size_t)V);
-    zero(*count, (size_t)V * sizeof(int32_t));
-    zero(vtype, (size_t)V * sizeof(int32_t));
-    fillNeg(vtarget, (size_t)V * sizeof(int32_t));
+    {
+        // (the pattern counts only serve the old P3X3Y rule)
+        FillBatch fb(*this);
+        if (needCount) fb.add(*count, 0x00, (size_t)V * sizeof(int32_t));
+        fb.add(vtype, 0x00, (size_t)V * sizeof(int32_t));
+        fb.add(vtarget, 0xFF, (size_t)V * sizeof(int32_t));
+    }
     if (!valenceReady) {  // not computed by the last bucket pass
         zero(*valence, (size_t)V * sizeof(int32_t));
         if (H > 0) kValence<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<int32_t>("twin"), H, *valence);
@@ -3094,7 +3104,7 @@ void Pipeline::simplifyInit(int32_t** valence, int32_t** count) {
 void Pipeline::simplify3X3Y(bool colour) {
     if (V == 0) return;
     int32_t *valence, *count;
-    simplifyInit(&valence, &count);
+    simplifyInit(&valence, &count, !colour);
     int4* quads = arena.ptr<int4>("quads");
     uint32_t* qattr = arena.ptr<uint32_t>("qattr");
     if (Q > 0) {
@@ -3110,7 +3120,7 @@ void Pipeline::simplify3X3Y(bool colour) {
 void Pipeline::simplify3333() {
     if (V == 0) return;
     int32_t *valence, *count;
-    simplifyInit(&valence, &count);
+    simplifyInit(&valence, &count, false);
     if (F > 0) {
         kClear3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<uint32_t>("fattr"), F);
         kMark3333<<<gridFor(F), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), arena.ptr<int32_t>("twin"), arena.ptr<uint32_t>("fattr"), F, valence,
@@ -3337,11 +3347,14 @@ void Pipeline::split(int step, int maxIters) {
         kWcUndo<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, w);
         wcMapsDirty = false;
         uint8_t* dirtyV = arena.get<uint8_t>("dirty_v", (size_t)std::max<int64_t>(V, 1));
-        zero(dirtyV, (size_t)std::max<int64_t>(V, 1));
-        if (V > touchedV) {
-            touched = arena.grow<uint8_t>("split_touched", (size_t)V, (size_t)touchedV, stream);
-            zero(touched + touchedV, (size_t)(V - touchedV));
-            touchedV = V;
+        {
+            FillBatch fb(*this);
+            fb.add(dirtyV, 0x00, (size_t)std::max<int64_t>(V, 1));
+            if (V > touchedV) {
+                touched = arena.grow<uint8_t>("split_touched", (size_t)V, (size_t)touchedV, stream);
+                fb.add(touched + touchedV, 0x00, (size_t)(V - touchedV));
+                touchedV = V;
+            }
         }
         if (vsize > 0) kMarkWcVertices<<<gridFor(vsize), kBlock, 0, stream>>>(w, vsize, dirtyV, touched);
         refreshHashTable(dirtyV, iterH);
