@@ -26,6 +26,7 @@
 #include "mdmc_device.h"
 #include "mdmc_gridpass.h"
 #include "mdmc_internal.h"
+#include "mdmc_lookback.h"
 #include "mdmc_pipeline.h"
 
 namespace mdmc {
@@ -1377,33 +1378,60 @@ __global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restr
 }
 
 // keep = not dropped by this simplification nor (vdead, when given) by an earlier one of the chain
-__global__ void kKeepVertices(const int32_t* __restrict__ vtype, int64_t V, int dropType, const uint8_t* __restrict__ vdead, int32_t* __restrict__ keep) {
-    const int64_t v0 = firstOfFour();
-    if (v0 >= V) return;
-    if (vec4(v0, V, vtype, keep) && !vdead) {
-        const int4 t = reinterpret_cast<const int4*>(vtype)[v0 >> 2];
-        reinterpret_cast<int4*>(keep)[v0 >> 2] = make_int4(t.x != dropType, t.y != dropType, t.z != dropType, t.w != dropType);
-    } else {
-        for (int64_t v = v0; v < v0 + 4 && v < V; ++v) keep[v] = (vtype[v] == dropType || (vdead && vdead[v])) ? 0 : 1;
+// The chain's vertex compaction in one pass: a vertex is kept unless it is dropped by this
+// step (vtype == dropType) or died in an earlier step of the chain; kept vertices are numbered
+// in order (tile counts, block ranks, look-back), every vertex gets its new address (-1 when
+// dropped) and the kept ones their old id, slot and problem.  Positions and normals are not
+// moved: the new vertex keeps the slot of its old vertex.
+template <int R>
+__global__ void __launch_bounds__(kLbBlock) kCompactVertices(const int32_t* __restrict__ vtype, int64_t V, int dropType, const uint8_t* __restrict__ vdead,
+                                                             const int32_t* __restrict__ pid, const int32_t* __restrict__ vslot, int32_t* __restrict__ npid,
+                                                             int32_t* __restrict__ address, int32_t* __restrict__ oldV, int32_t* __restrict__ nvslot,
+                                                             int64_t numTiles, unsigned long long* __restrict__ status, int64_t* __restrict__ total,
+                                                             int64_t* __restrict__ directOut) {
+    __shared__ int s_cnt[R * (kLbBlock / 64)], s_pre[R * (kLbBlock / 64) + 1];
+    __shared__ long long s_off;
+    const int64_t t = blockIdx.x;
+    if (t >= numTiles) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t base = t * (int64_t)(R * kLbBlock);
+    auto keepOf = [&](int64_t e) { return e < V && vtype[e] != dropType && !(vdead && vdead[e]); };
+    uint64_t m[R];
+    uint32_t kb = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool k = keepOf(base + r * kLbBlock + threadIdx.x);
+        m[r] = __ballot(k);
+        kb |= (k ? 1u : 0u) << r;
     }
-}
-
-// (positions and normals are not moved: the new vertex o keeps the slot of its old vertex)
-__global__ void kCompactVertices(const int32_t* __restrict__ pid, const int32_t* __restrict__ keep, const int64_t* __restrict__ addr, int64_t V,
-                                 int32_t* __restrict__ npid, int32_t* __restrict__ address, int32_t* __restrict__ oldV, const int32_t* __restrict__ vslot,
-                                 int32_t* __restrict__ nvslot) {
-    // every vertex gets its new address, -1 when dropped
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    if (!keep[v]) {
-        address[v] = -1;
-        return;
+    const int tileSum = lbBlockRanks<R>(m, s_cnt, s_pre);
+    if (threadIdx.x < 64)
+        lbTileOffset(
+            status, t, tileSum,
+            [&](int l) {
+                int64_t c = 0;
+                for (int64_t e = l; e < base; e += 64) c += keepOf(e) ? 1 : 0;
+                return c;
+            },
+            directOut, &s_off);
+    __syncthreads();
+    const int64_t off = s_off;
+    const uint64_t below = (1ULL << lane) - 1ULL;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t v = base + r * kLbBlock + threadIdx.x;
+        if (v >= V) continue;
+        if (!((kb >> r) & 1u)) {
+            address[v] = -1;
+            continue;
+        }
+        const int64_t o = off + s_pre[r * (kLbBlock / 64) + wid] + __popcll(m[r] & below);
+        address[v] = (int32_t)o;
+        oldV[o] = (int32_t)v;
+        nvslot[o] = vslot ? vslot[v] : (int32_t)v;
+        if (pid) npid[o] = pid[v];
     }
-    const int64_t o = addr[v];
-    address[v] = (int32_t)o;
-    oldV[o] = (int32_t)v;
-    nvslot[o] = vslot ? vslot[v] : (int32_t)v;
-    if (pid) npid[o] = pid[v];
+    if (t == numTiles - 1 && threadIdx.x == 0) *total = off + tileSum;
 }
 
 // positions and normals of the current vertices, gathered from their slots
@@ -1418,30 +1446,15 @@ __global__ void kGatherVertices(const float* __restrict__ pos, const float* __re
     }
 }
 
-__global__ void kKeepQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t* __restrict__ keep) {
-    const int64_t q0 = firstOfFour();
-    if (q0 >= Q) return;
-    if (vec4(q0, Q, attr, keep)) {
-        const uint4 a = reinterpret_cast<const uint4*>(attr)[q0 >> 2];
-        constexpr uint32_t gone = QA_REMOVE | QA_DEAD;
-        reinterpret_cast<int4*>(keep)[q0 >> 2] = make_int4(!(a.x & gone), !(a.y & gone), !(a.z & gone), !(a.w & gone));
-    } else {
-        for (int64_t q = q0; q < q0 + 4 && q < Q; ++q) keep[q] = (attr[q] & (QA_REMOVE | QA_DEAD)) ? 0 : 1;
-    }
-}
-
 // removeQuadrilateralsP3X3Y (p3333 = 0) / removeQuadsP3333 (p3333 = 1), HalfedgeMesh.cu:513-538, :657-683
-__global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __restrict__ attr, int64_t Q, const int32_t* __restrict__ keep,
-                              const int64_t* __restrict__ qaddr, const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
-                              const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr,
-                              int32_t* __restrict__ oldQ, int32_t* __restrict__ qnew, int32_t* __restrict__ removedAt, int32_t* __restrict__ outdeg,
-                              int64_t V, int64_t* ctr) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Q) return;
-    if (!keep[q]) {
+// quad q, kept (new id d) or removed (a quad is removed when it has the remove or the dead bit)
+__device__ __forceinline__ void compactQuad(int64_t q, bool kept, int64_t d, const int4* __restrict__ quads, const uint32_t* __restrict__ attr,
+                                            const int32_t* __restrict__ vtarget, const int32_t* __restrict__ address, int p3333,
+                                            int4* __restrict__ nquads, uint32_t* __restrict__ nattr, int32_t* __restrict__ oldQ,
+                                            int32_t* __restrict__ removedAt, int32_t* __restrict__ outdeg, int64_t V, int64_t* ctr) {
+    if (!kept) {
         // the outgoing halfedges this quad takes from its corners' lists (when the lists are
         // by current corner; a quad dead since an earlier step of the chain takes none)
-        qnew[q] = -1;
         if (!(attr[q] & QA_DEAD)) {
             const int4 r = quads[q];
             if (removedAt) {
@@ -1459,7 +1472,6 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
         }
         return;
     }
-    qnew[q] = (int32_t)qaddr[q];
     const uint32_t a = attr[q];
     const int4 v4 = quads[q];
     const int v[4] = {v4.x, v4.y, v4.z, v4.w};
@@ -1481,13 +1493,66 @@ __global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __
         bad = bad || o[i] < 0;  // a dropped vertex or a missing target: the quad is invalid
     }
     if (bad) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
-    const int64_t d = qaddr[q];
     nquads[d] = make_int4(o[0], o[1], o[2], o[3]);
     uint32_t na = QA_COLOR;
     if (a & QA_NM) na |= QA_NM;
     if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
     nattr[d] = na;
     oldQ[d] = (int32_t)q;
+}
+
+// the quad compaction's numbering in one pass: kept quads (no remove or dead bit) numbered in
+// order (tile counts, block ranks, look-back) into qnew, -1 for removed ones
+template <int R>
+__global__ void __launch_bounds__(kLbBlock) kNumberQuads(const uint32_t* __restrict__ attr, int64_t Q, int32_t* __restrict__ qnew, int64_t numTiles,
+                                                         unsigned long long* __restrict__ status, int64_t* __restrict__ total,
+                                                         int64_t* __restrict__ directOut) {
+    __shared__ int s_cnt[R * (kLbBlock / 64)], s_pre[R * (kLbBlock / 64) + 1];
+    __shared__ long long s_off;
+    const int64_t t = blockIdx.x;
+    if (t >= numTiles) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t base = t * (int64_t)(R * kLbBlock);
+    constexpr uint32_t gone = QA_REMOVE | QA_DEAD;
+    auto keepOf = [&](int64_t e) { return e < Q && !(attr[e] & gone); };
+    uint64_t m[R];
+    uint32_t kb = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool k = keepOf(base + r * kLbBlock + threadIdx.x);
+        m[r] = __ballot(k);
+        kb |= (k ? 1u : 0u) << r;
+    }
+    const int tileSum = lbBlockRanks<R>(m, s_cnt, s_pre);
+    if (threadIdx.x < 64)
+        lbTileOffset(
+            status, t, tileSum,
+            [&](int l) {
+                int64_t c = 0;
+                for (int64_t e = l; e < base; e += 64) c += keepOf(e) ? 1 : 0;
+                return c;
+            },
+            directOut, &s_off);
+    __syncthreads();
+    const int64_t off = s_off;
+    const uint64_t below = (1ULL << lane) - 1ULL;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t q = base + r * kLbBlock + threadIdx.x;
+        if (q < Q) qnew[q] = ((kb >> r) & 1u) ? (int32_t)(off + s_pre[r * (kLbBlock / 64) + wid] + __popcll(m[r] & below)) : -1;
+    }
+    if (t == numTiles - 1 && threadIdx.x == 0) *total = off + tileSum;
+}
+
+// removeQuadrilateralsP3X3Y / removeQuadsP3333 from the numbering, one thread per quad
+__global__ void kCompactQuads(const int4* __restrict__ quads, const uint32_t* __restrict__ attr, int64_t Q, const int32_t* __restrict__ vtarget,
+                              const int32_t* __restrict__ address, int p3333, int4* __restrict__ nquads, uint32_t* __restrict__ nattr,
+                              int32_t* __restrict__ oldQ, const int32_t* __restrict__ qnew, int32_t* __restrict__ removedAt,
+                              int32_t* __restrict__ outdeg, int64_t V, int64_t* ctr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    const int32_t d = qnew[q];
+    compactQuad(q, d >= 0, d, quads, attr, vtarget, address, p3333, nquads, nattr, oldQ, removedAt, outdeg, V, ctr);
 }
 
 // ---------------------------------------------------------------- fixManifoldTwins
@@ -2984,46 +3049,50 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* vtype = arena.ptr<int32_t>("vtype");
     int32_t* vtarget = arena.ptr<int32_t>("vtarget");
     const uint32_t* attr = p3333 ? arena.ptr<uint32_t>("fattr") : arena.ptr<uint32_t>("qattr");
-    // keep flags and their prefix sums for vertices and quads, with one read-back of both totals
-    int32_t* keep = arena.get<int32_t>("keepV", (size_t)std::max<int64_t>(V, 1));
-    int32_t* keepq = arena.get<int32_t>("keepQ", (size_t)std::max<int64_t>(Q, 1));
-    int64_t* vaddr = arena.get<int64_t>("vaddr", (size_t)V + 1);
-    int64_t* qaddr = arena.get<int64_t>("qaddr", (size_t)Q + 1);
-    void* sv = arena.get<char>("scan_scratch_v", scanScratchBytes(V + 1));
-    void* sq = arena.get<char>("scan_scratch_q", scanScratchBytes(Q + 1));
-    if (V > 0) kKeepVertices<<<gridFor((V + 3) / 4), kBlock, 0, stream>>>(vtype, V, dropType, chained ? arena.ptr<uint8_t>("vdead") : nullptr, keep);
-    if (Q > 0) kKeepQuads<<<gridFor((Q + 3) / 4), kBlock, 0, stream>>>(attr, Q, keepq);
-    // totals straight into pinned host memory: [0] direct-sum diagnostic, [1] V, [2] Q
-    int64_t* h = pinHost();
-    int64_t* pd = pinDev();
-    h[0] = h[1] = h[2] = 0;
-    scanExclusiveI32(keep, vaddr, V, pd + 1, sv, stream, pd);
-    scanExclusiveI32(keepq, qaddr, Q, pd + 2, sq, stream, pd);
-    MDMC_HIP(hipStreamSynchronize(stream));
-    const int64_t nV = h[1], nQ = h[2];
+    // vertex and quad compaction, each one pass (look-back over tiles), with one read-back of
+    // both totals; the outputs are sized for the inputs (at most as many are kept)
+    constexpr int RV = 16, RQ = 16;  // rounds per tile (tiles of 4096 elements)
+    const int64_t tV = (V + RV * kLbBlock - 1) / (RV * kLbBlock), tQ = (Q + RQ * kLbBlock - 1) / (RQ * kLbBlock);
+    unsigned long long* sv = arena.get<unsigned long long>("compact_lb_v", (size_t)std::max<int64_t>(tV, 1));
+    unsigned long long* sq = arena.get<unsigned long long>("compact_lb_q", (size_t)std::max<int64_t>(tQ, 1));
     int32_t* address = arena.get<int32_t>("vaddress", (size_t)std::max<int64_t>(V, 1));  // written for every vertex below
     // the compacted arrays go to the second buffers, which then swap with the primary ones;
     // both are sized for the grid pass's output so that later extractions of the same
     // workload reallocate neither
-    const size_t capV = (size_t)std::max(nV, dmcV), capQ = (size_t)std::max(nQ, dmcQ);
+    const size_t capV = (size_t)std::max(V, dmcV), capQ = (size_t)std::max(Q, dmcQ);
     int32_t* npid = pid() ? arena.get<int32_t>("pid2", capV) : nullptr;
-    int32_t* oldV = arena.get<int32_t>("oldV", (size_t)std::max<int64_t>(nV, 1));
+    int32_t* oldV = arena.get<int32_t>("oldV", (size_t)std::max<int64_t>(V, 1));
     int32_t* nvslot = arena.get<int32_t>("vslot2", capV);
-    if (V > 0)
-        kCompactVertices<<<gridFor(V), kBlock, 0, stream>>>(pid(), keep, vaddr, V, npid, address, oldV, vertexSlots(), nvslot);
     int4* quads = arena.ptr<int4>("quads");
     int4* nquads = arena.get<int4>("quads2", capQ);
     uint32_t* nattr = arena.get<uint32_t>("qattr2", capQ);
-    int32_t* oldQ = arena.get<int32_t>("oldQ", (size_t)std::max<int64_t>(nQ, 1));
+    int32_t* oldQ = arena.get<int32_t>("oldQ", (size_t)std::max<int64_t>(Q, 1));
     int32_t* qnew = arena.get<int32_t>("qnew", (size_t)std::max<int64_t>(Q, 1));
     // with every list still by current corner the removed quads' entries are counted per
     // corner; after deferred steps the lists are the initialize's and kOutCount checks entries
     int32_t* removedAt = chained ? nullptr : arena.get<int32_t>("removed_at", (size_t)std::max<int64_t>(V, 1));
-    if (removedAt) zero(removedAt, (size_t)V * sizeof(int32_t));
     int32_t* outdeg = chained ? arena.ptr<int32_t>("outdeg") : nullptr;
-    if (Q > 0)
-        kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, keepq, qaddr, vtype, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ,
-                                                         qnew, removedAt, outdeg, V, counters());
+    {
+        FillBatch fb(*this);
+        fb.add(sv, 0x00, (size_t)tV * sizeof(unsigned long long));
+        fb.add(sq, 0x00, (size_t)tQ * sizeof(unsigned long long));
+        if (removedAt) fb.add(removedAt, 0x00, (size_t)V * sizeof(int32_t));
+    }
+    // totals straight into pinned host memory: [0] direct-count diagnostic, [1] V, [2] Q
+    int64_t* h = pinHost();
+    int64_t* pd = pinDev();
+    h[0] = h[1] = h[2] = 0;
+    if (V > 0)
+        kCompactVertices<RV><<<(unsigned)tV, kLbBlock, 0, stream>>>(vtype, V, dropType, chained ? arena.ptr<uint8_t>("vdead") : nullptr, pid(), vertexSlots(),
+                                                                npid, address, oldV, nvslot, tV, sv, pd + 1, pd);
+    if (Q > 0) {
+        kNumberQuads<RQ><<<(unsigned)tQ, kLbBlock, 0, stream>>>(attr, Q, qnew, tQ, sq, pd + 2, pd);
+        kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ, qnew, removedAt, outdeg,
+                                                         V, counters());
+    }
+    MDMC_HIP(hipStreamSynchronize(stream));
+    const int64_t nV = h[1], nQ = h[2];
+    if (nV < 0 || nV > V || nQ < 0 || nQ > Q) throw HipError("inconsistent compaction counts", -4);
     checkValid("quad vertex ids");
 
     // HalfedgeMesh::initialize of the compacted quads, from the previous build: the new
