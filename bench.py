@@ -411,6 +411,7 @@ def main():
     }
     out["stages_roofline"] = stage_roofline(s, n, {k: float(np.mean([x[k] for x in stats])) for k in stage_keys})
     out["anomalies"] = s["anomalies"]
+    out["split_sequential_merges"] = s["split_sequential_merges"]
     d.close()
     if not args.no_batch:
         print("batched", file=sys.stderr, flush=True)

@@ -90,6 +90,9 @@ typedef struct {
     double ms_total, ms_grid, ms_dmc, ms_halfedge, ms_simplify, ms_fix_twins, ms_split, ms_detect;
     /* the grid-pass kernel alone (classification, dual vertices, quads) */
     double ms_grid_pass_kernel;
+    /* split iterations whose merge phase took the one-thread ascending pass (the parallel merge
+     * found a schedule conflict, or mdmc_set_sequential_merges forced it); -1 for batches */
+    int64_t split_sequential_merges;
 } mdmc_stats;
 
 /* Sizes of the resident result. */
@@ -176,6 +179,12 @@ mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays);
  * new twin pairs directly instead of having its region's buckets rebuilt; the mesh is the same
  * either way.  enable = 0 sends every merge through the region rebuild (default on). */
 mdmc_status mdmc_set_local_merges(mdmc_ctx* ctx, int32_t enable);
+/* Verification switch for the splitter's merge phase (no reference counterpart).  The merges
+ * of ManifoldSplitter::mergeHalfedges (ManifoldSplitter.cu:1103-1263) run in parallel and fall
+ * back to a one-thread pass in ascending order only when two of them conflict; enable = 1
+ * makes every iteration take that pass, so tests can compare it with the parallel one and
+ * with the reference's results (default 0).  The mesh is the same either way. */
+mdmc_status mdmc_set_sequential_merges(mdmc_ctx* ctx, int32_t enable);
 
 /* Per-problem offsets of the resident result (host arrays of n_problems + 1; any may be NULL).
  * After mdmc_extract (one problem) they are {0, n}. */

@@ -84,7 +84,7 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_grid", ctypes.c_double), ("ms_dmc", ctypes.c_double),
                 ("ms_halfedge", ctypes.c_double), ("ms_simplify", ctypes.c_double), ("ms_fix_twins", ctypes.c_double),
                 ("ms_split", ctypes.c_double), ("ms_detect", ctypes.c_double),
-                ("ms_grid_pass_kernel", ctypes.c_double)]
+                ("ms_grid_pass_kernel", ctypes.c_double), ("split_sequential_merges", ctypes.c_int64)]
 
     def as_dict(self):
         out = {}
@@ -105,7 +105,7 @@ EXPORTS = ["mdmc_create", "mdmc_destroy", "mdmc_last_error", "mdmc_set_stream", 
            "mdmc_extract_batch", "mdmc_batch_offsets", "mdmc_batch_stats", "mdmc_set_tile_shape", "mdmc_get_tile_shape",
            "mdmc_generate_surface", "mdmc_generate_edge_case", "mdmc_mesh_face_colors", "mdmc_mesh_triangulate_maxmin",
            "mdmc_backward_noalloc", "mdmc_set_graphs", "mdmc_get_graph_replays", "mdmc_processed_grid",
-           "mdmc_set_local_merges"]
+           "mdmc_set_local_merges", "mdmc_set_sequential_merges"]
 
 
 def lib():
@@ -160,6 +160,8 @@ def lib():
         L.mdmc_set_graphs.restype = i32
         L.mdmc_set_local_merges.argtypes = [vp, i32]
         L.mdmc_set_local_merges.restype = i32
+        L.mdmc_set_sequential_merges.argtypes = [vp, i32]
+        L.mdmc_set_sequential_merges.restype = i32
         L.mdmc_get_graph_replays.argtypes = [vp, ctypes.POINTER(i64)]
         L.mdmc_get_graph_replays.restype = i32
         L.mdmc_backward_noalloc.restype = i32
@@ -313,6 +315,11 @@ class DualMarchingCubes:
         """mdmc_set_local_merges: plain-patch P3X3Y merges of a simplification chain update their
         twins directly (False: every merge's region is rebuilt; the mesh is the same)."""
         self._check(lib().mdmc_set_local_merges(self._ctx, 1 if enable else 0))
+
+    def set_sequential_merges(self, enable):
+        """mdmc_set_sequential_merges: the splitter's merge phase always takes the one-thread
+        ascending pass it otherwise takes only on schedule conflicts (the mesh is the same)."""
+        self._check(lib().mdmc_set_sequential_merges(self._ctx, 1 if enable else 0))
 
     def graph_replays(self):
         n = ctypes.c_int64()

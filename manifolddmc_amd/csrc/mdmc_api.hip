@@ -365,6 +365,7 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             stats->nm_vertices = (flags & MDMC_DETECT_NM_VERTS) ? st9[8] : -1;
             stats->split_iterations = p.splitIterations;
             stats->anomalies = p.anomalies() + p.extraAnomalies;
+            stats->split_sequential_merges = p.sequentialMergeIterations();
             stats->cells = cells;
             float ms = 0.f;
             double* outs[8] = {&stats->ms_total, &stats->ms_grid, &stats->ms_dmc, &stats->ms_halfedge,
@@ -441,6 +442,7 @@ mdmc_status mdmc_batch_stats(const mdmc_ctx* ctx, mdmc_stats* per_problem) {
         st.split_iterations = p.splitIterations;
         st.anomalies = -1;
         st.cells = -1;
+        st.split_sequential_merges = -1;
     }
     return MDMC_OK;
 }
@@ -652,6 +654,12 @@ mdmc_status mdmc_set_graphs(mdmc_ctx* ctx, int32_t enable) {
 mdmc_status mdmc_set_local_merges(mdmc_ctx* ctx, int32_t enable) {
     if (!ctx) return MDMC_ERR_INVALID_ARG;
     ctx->p.localMerges = enable != 0;
+    return MDMC_OK;
+}
+
+mdmc_status mdmc_set_sequential_merges(mdmc_ctx* ctx, int32_t enable) {
+    if (!ctx) return MDMC_ERR_INVALID_ARG;
+    ctx->p.sequentialMerges = enable != 0;
     return MDMC_OK;
 }
 

@@ -66,6 +66,7 @@ class Pipeline {
 
     void resetCounters();
     int64_t anomalies();
+    int64_t sequentialMergeIterations();
     // the padding pass's flag word (one of the counters, zeroed by resetCounters)
     int64_t* prepSpecial();
 
@@ -95,6 +96,8 @@ class Pipeline {
     // P3X3Y merges of a deferred chain step with a plain neighbourhood update their twins
     // directly (else every merge's region is rebuilt)
     bool localMerges = true;
+    // the splitter's merge phase always in the one-thread ascending pass (a verification switch)
+    bool sequentialMerges = false;
     int64_t graphReplays = 0;
     int nProblems = 1;
     ProblemDev* probsDev = nullptr;

@@ -37,7 +37,7 @@ constexpr int kMaxBucket = 16;
 constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
-enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_NCOUNTERS = 8 };
+enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_MERGE_SEQ = 6, A_NCOUNTERS = 8 };
 
 __device__ __forceinline__ void anomaly(int64_t* ctr) { atomicAdd((unsigned long long*)&ctr[A_ANOMALY], 1ULL); }
 
@@ -2128,6 +2128,7 @@ __global__ void kMergeSequential(WC w, int count, const int32_t* __restrict__ nu
                                  int64_t* ctr) {
     if (ctr[A_CONFLICT] == 0) return;
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ++ctr[A_MERGE_SEQ];  // the iterations that took this pass (one thread: a plain increment)
     for (int h = 0; h < count; ++h) {
         if (w.hflags[h] & VF_REMOVE) continue;
         int P[12], np;
@@ -2634,6 +2635,7 @@ void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     dmcV = V;
     dmcQ = Q;
 }
+int64_t Pipeline::sequentialMergeIterations() { return readI64(counters() + A_MERGE_SEQ); }
 
 int64_t Pipeline::dmcSetup(std::vector<ProblemDev>& problems, bool recordAdjoint) {
     vertHeV = vertHeQ = 0;
@@ -3348,6 +3350,7 @@ void Pipeline::split(int step, int maxIters) {
                 zero(counters() + A_CONFLICT, sizeof(int64_t));
                 kMergePlan<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, valid, fw, counters());
                 kMergeCheck<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, fw, counters());
+                if (sequentialMerges) launchFill(counters() + A_CONFLICT, 0x01, 1);  // as if a conflict was found
                 kMergeApply<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize, valid, counters());
                 kMergeSequential<<<1, 1, 0, stream>>>(w, hsize, mlhN, mlhX, mlhY, counters());
             }
