@@ -156,6 +156,8 @@ class Pipeline {
     hipEvent_t upEv_[2] = {nullptr, nullptr};
     size_t upCap_[2] = {0, 0};
     int64_t* counters();
+    int64_t* nmSlotParts();
+    int64_t* nmFoundParts();
     int64_t scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out);
     int64_t compactNM(int32_t* nm);
     void zero(void* p, size_t bytes);

@@ -38,6 +38,33 @@ constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
 enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_MERGE_SEQ = 6, A_NCOUNTERS = 8 };
+// the slotted non-manifold halfedges the split's working copy marks (ManifoldSplitter.cu:
+// 415-455): every halfedge of a bucket of 4, and of a larger bucket the ones at ranks 1, 2, 3
+// and size - 1
+__device__ __forceinline__ bool slottedNM(int n, int r) { return n == 4 || (n > 4 && (r == 1 || r == 2 || r == 3 || r == n - 1)); }
+// A count summed over a whole launch without one hot address: a block adds its total to one of
+// kPartSlots partial sums (by block index, 128 bytes apart), which a later one-wave kernel adds
+// up.  Every thread of the block must call it.
+constexpr int kPartSlots = 64, kPartStride = 16;
+__device__ __forceinline__ void blockAdd(int64_t* parts, int c) {
+    __shared__ int s_part[kBlock / 64];
+    int x = c;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_down(x, d, 64);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += s_part[w];
+        if (t) atomicAdd((unsigned long long*)&parts[(blockIdx.x % kPartSlots) * kPartStride], (unsigned long long)t);
+    }
+}
+__device__ __forceinline__ int64_t sumParts(const int64_t* parts, int lane) {
+    int64_t v = parts[lane * kPartStride];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    return v;
+}
 
 __device__ __forceinline__ void anomaly(int64_t* ctr) { atomicAdd((unsigned long long*)&ctr[A_ANOMALY], 1ULL); }
 
@@ -289,20 +316,25 @@ __device__ __forceinline__ void sizes4(const int32_t* __restrict__ bsize, int64_
         for (int j = 0; j < 4; ++j) n[j] = h0 + j < H ? bsize[h0 + j] : 2;
     }
 }
-__global__ void kFlagsFromBuckets(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ bsize, uint32_t* __restrict__ vflags,
-                                  uint32_t* __restrict__ fattr) {
+__global__ void __launch_bounds__(kBlock) kFlagsFromBuckets(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ bsize,
+                                                            const int32_t* __restrict__ brank, uint32_t* __restrict__ vflags,
+                                                            uint32_t* __restrict__ fattr, int64_t* nmParts) {
     const int64_t h0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (h0 >= H) return;
-    int n[4];
-    sizes4(bsize, H, h0, n);
+    int slotted = 0;
+    if (h0 < H) {
+        int n[4];
+        sizes4(bsize, H, h0, n);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint32_t vf = bucketVertexFlag(n[j]);
-        if (!vf) continue;
-        const int4 e = he[h0 + j];
-        atomicOr(&vflags[e.x], vf);
-        if (n[j] == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+        for (int j = 0; j < 4; ++j) {
+            if (n[j] >= 4 && slottedNM(n[j], brank[h0 + j])) ++slotted;
+            const uint32_t vf = bucketVertexFlag(n[j]);
+            if (!vf) continue;
+            const int4 e = he[h0 + j];
+            atomicOr(&vflags[e.x], vf);
+            if (n[j] == 4) atomicOr(&fattr[e.y], (uint32_t)QA_NM);
+        }
     }
+    blockAdd(nmParts, slotted);
 }
 
 // ---------------------------------------------------------------- halfedge build from outgoing lists
@@ -528,15 +560,21 @@ __global__ void kBucketsAll(const int4* __restrict__ quads, int64_t H, const int
 }
 
 // the halfedge records of the quads, from the ids and the twins, and the bucket sizes / ranks
-__global__ void kMaterialize(const int4* __restrict__ quads, int64_t H, const int32_t* __restrict__ twin, const uint32_t* __restrict__ bpack,
-                             int4* __restrict__ he, int32_t* __restrict__ bsize, int32_t* __restrict__ brank) {
+__global__ void __launch_bounds__(kBlock) kMaterialize(const int4* __restrict__ quads, int64_t H, const int32_t* __restrict__ twin,
+                                                       const uint32_t* __restrict__ bpack, int4* __restrict__ he, int32_t* __restrict__ bsize,
+                                                       int32_t* __restrict__ brank, int64_t* nmParts) {
     const int64_t hh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (hh >= H) return;
-    const int h = (int)hh;
-    he[h] = make_int4(quadCorner(quads[h >> 2], h & 3), h >> 2, quadNext(h, 1), twin[h]);
-    const uint32_t b = bpack[h];
-    bsize[h] = (int32_t)(b & 0xFFFFu);
-    brank[h] = (int32_t)(b >> 16);
+    int slotted = 0;
+    if (hh < H) {
+        const int h = (int)hh;
+        he[h] = make_int4(quadCorner(quads[h >> 2], h & 3), h >> 2, quadNext(h, 1), twin[h]);
+        const uint32_t b = bpack[h];
+        const int n = (int)(b & 0xFFFFu), r = (int)(b >> 16);
+        bsize[h] = n;
+        brank[h] = r;
+        slotted = slottedNM(n, r) ? 1 : 0;
+    }
+    blockAdd(nmParts, slotted);
 }
 
 // ---- rebuild after a simplification's compaction.  New vertex v' = address[v]; a dropped
@@ -1802,27 +1840,33 @@ __global__ void kWcUpdate(WC w, int count) {
 }
 
 // markHalfedgesAsNonManifold, ManifoldSplitter.cu:527-555: the four slotted halfedges of every full bucket
-__global__ void kWcNMFlags(int64_t H, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, WC w, int64_t* ctr) {
-    const int64_t h0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (h0 >= H) return;
-    int ns[4];
-    sizes4(bsize, H, h0, ns);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int n = ns[j];
-        if (n < 4) continue;
-        const int64_t hh = h0 + j;
-        const int r = brank[hh];
-        const bool slotted = (n == 4) ? true : (r == 1 || r == 2 || r == 3 || r == n - 1);
-        if (!slotted) continue;
-        const int c = w.h_o2c[hh];
-        if (c < 0) {
-            anomaly(ctr);
-            continue;
+// markNonManifoldHalfedges of the working copy: the copied halfedges that are slotted
+// non-manifold get the NM flag, as does their origin; their count goes to foundParts
+// (kNMSlotCheck compares it with the count of all of them)
+__global__ void __launch_bounds__(kBlock) kWcNMFlags(int n4, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, WC w,
+                                                     int64_t* foundParts) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int found = 0;
+    if (c < n4) {
+        const int hh = w.h_orig[c];
+        const int n = bsize[hh];
+        if (n >= 4 && slottedNM(n, brank[hh])) {
+            w.hflags[c] |= VF_NM;
+            atomicOr(&w.vflags[w.he[c].x], (uint32_t)VF_NM);
+            found = 1;
         }
-        w.hflags[c] |= VF_NM;
-        atomicOr(&w.vflags[w.he[c].x], (uint32_t)VF_NM);
     }
+    blockAdd(foundParts, found);
+}
+
+// a slotted non-manifold halfedge outside the working copy (its face not marked NM) is an
+// anomaly, one per such halfedge and step, as the reference's working copy would not hold it
+// (one wave: the partial sums of both counts; the found ones are cleared for the next step)
+__global__ void kNMSlotCheck(const int64_t* nmParts, int64_t* foundParts, int64_t* ctr) {
+    const int lane = threadIdx.x;
+    const int64_t total = sumParts(nmParts, lane), found = sumParts(foundParts, lane);
+    foundParts[lane * kPartStride] = 0;
+    if (lane == 0 && total > found) ctr[A_ANOMALY] += total - found;
 }
 
 // performSubdivideQuads, ManifoldSplitter.cu:570-731.  Face r appends 4 vertices at 4n+4r,
@@ -2497,6 +2541,16 @@ int64_t Pipeline::readI64(const int64_t* dev) {
 }
 
 int64_t* Pipeline::counters() { return arena.get<int64_t>("counters", A_NCOUNTERS); }
+// partial sums (kPartSlots, kPartStride apart) of the slotted non-manifold halfedges of the
+// current buckets, counted where the bucket sizes are written, and of those a split step's
+// working copy holds
+int64_t* Pipeline::nmSlotParts() { return arena.get<int64_t>("nm_slot_parts", kPartSlots * kPartStride); }
+int64_t* Pipeline::nmFoundParts() {
+    const bool fresh = arena.capacity<int64_t>("nm_found_parts") < (size_t)(kPartSlots * kPartStride);
+    int64_t* p = arena.get<int64_t>("nm_found_parts", kPartSlots * kPartStride);
+    if (fresh) zero(p, kPartSlots * kPartStride * sizeof(int64_t));  // kNMSlotCheck clears it after each use
+    return p;
+}
 
 const uint8_t* Pipeline::canonFlags() { return canonTracked ? arena.ptr<uint8_t>("he_canon") : nullptr; }
 
@@ -2894,7 +2948,8 @@ void Pipeline::buildBuckets(const int32_t* list, int64_t n) {
         kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, rank, keys, A);
         kBuckets<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, keys, A, bsize, brank);
     }
-    if (H > 0) kFlagsFromBuckets<<<gridFor((H + 3) / 4), kBlock, 0, stream>>>(he, H, bsize, vflags, fattr);
+    zero(nmSlotParts(), kPartSlots * kPartStride * sizeof(int64_t));  // recounted from the new bucket sizes
+    if (H > 0) kFlagsFromBuckets<<<gridFor((H + 3) / 4), kBlock, 0, stream>>>(he, H, bsize, brank, vflags, fattr, nmSlotParts());
 }
 
 // the valence the next simplification starts from, when one follows: the outgoing list
@@ -3216,9 +3271,10 @@ void Pipeline::materializeHalfedges() {
     int4* he = arena.get<int4>("he", (size_t)std::max<int64_t>(H, 1));
     int32_t* bsize = arena.get<int32_t>("bsize", (size_t)std::max<int64_t>(H, 1));
     int32_t* brank = arena.get<int32_t>("brank", (size_t)std::max<int64_t>(H, 1));
+    zero(nmSlotParts(), kPartSlots * kPartStride * sizeof(int64_t));  // recounted from the bucket sizes written here
     if (H > 0)
         kMaterialize<<<gridFor(H), kBlock, 0, stream>>>(arena.ptr<int4>("quads"), H, arena.ptr<int32_t>("twin"), arena.ptr<uint32_t>("bpack"), he, bsize,
-                                                        brank);
+                                                        brank, nmSlotParts());
     heMaterialized = true;
 }
 
@@ -3313,7 +3369,8 @@ void Pipeline::split(int step, int maxIters) {
                                                    arena.ptr<float>("pos"), arena.ptr<float>("nrm"), pid(), w);
         int vsize = 4 * n, hsize = 4 * n, fsize = n;
         kWcUpdate<<<gridFor(hsize), kBlock, 0, stream>>>(w, hsize);
-        if (H > 0) kWcNMFlags<<<gridFor((H + 3) / 4), kBlock, 0, stream>>>(H, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, counters());
+        kWcNMFlags<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(4 * n, arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), w, nmFoundParts());
+        kNMSlotCheck<<<1, 64, 0, stream>>>(nmSlotParts(), nmFoundParts(), counters());
 
         if (step >= 1) {
             kSubdivide<<<gridFor(4 * (int64_t)n), kBlock, 0, stream>>>(nm, w);
