@@ -1193,7 +1193,7 @@ __global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_
 // A vertex whose outgoing halfedges form one closed clockwise cycle (next(twin(h)) from
 // any of them visits all and returns) has a single fan: every fan walk of fanWalk reaches
 // the same minimum.  The others are decided by kDetectFans over their outgoing lists.
-__global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
+__global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
                          uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, const uint8_t* __restrict__ touched, int64_t listsV,
                          const uint8_t* __restrict__ canon, int64_t* __restrict__ nslow) {
     const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1208,8 +1208,17 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t V, const int32_t* 
         for (int k = 1; k <= d; ++k) {
             const int t = e.w;
             if (t < 0) break;
-            cur = nextOf(he, canon, t);
-            e = he[cur];
+            // the next record is loaded at the build layout's next of t alongside t's layout
+            // flag, and again from t's own next only when the flag says it moved
+            const int qn = quadNext(t, 1);
+            int4 en = qn < H ? he[qn] : make_int4(-1, -1, -1, -1);
+            if (canon && !canon[t]) {
+                cur = heN(he, t);
+                en = he[cur];
+            } else {
+                cur = qn;
+            }
+            e = en;
             if (e.x != v) break;
             if (cur == start) {
                 single = k == d;
@@ -3531,7 +3540,7 @@ void Pipeline::detectNonManifoldVertices() {
     } else {
         kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
     }
-    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, V, outdeg, rep, vflags, slow, touched, fromLists ? listsV : 0, canonFlags(), nslow);
+    kFanFast<<<gridFor(V), kBlock, 0, stream>>>(he, H, V, outdeg, rep, vflags, slow, touched, fromLists ? listsV : 0, canonFlags(), nslow);
     int64_t* hc = pinHost();
     MDMC_HIP(hipMemcpyAsync(hc, nslow, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));
