@@ -1424,7 +1424,6 @@ __global__ void kMark3333(const int4* __restrict__ quads, const int32_t* __restr
     }
 }
 
-// keep = not dropped by this simplification nor (vdead, when given) by an earlier one of the chain
 // The chain's vertex compaction in one pass: a vertex is kept unless it is dropped by this
 // step (vtype == dropType) or died in an earlier step of the chain; kept vertices are numbered
 // in order (tile counts, block ranks, look-back), every vertex gets its new address (-1 when
