@@ -38,9 +38,9 @@ constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
 enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_MERGE_SEQ = 6, A_NCOUNTERS = 8 };
-// the slotted non-manifold halfedges the split's working copy marks (ManifoldSplitter.cu:
-// 415-455): every halfedge of a bucket of 4, and of a larger bucket the ones at ranks 1, 2, 3
-// and size - 1
+// the slotted non-manifold halfedges the split's working copy marks (markHalfedgesAsNonManifold,
+// ManifoldSplitter.cu:527-555, reads the four slots of every full vertex-pair bucket): every
+// halfedge of a bucket of 4, and of a larger bucket the ones at ranks 1, 2, 3 and size - 1
 __device__ __forceinline__ bool slottedNM(int n, int r) { return n == 4 || (n > 4 && (r == 1 || r == 2 || r == 3 || r == n - 1)); }
 // A count summed over a whole launch without one hot address: a block adds its total to one of
 // kPartSlots partial sums (by block index, 128 bytes apart), which a later one-wave kernel adds
@@ -1847,10 +1847,10 @@ __global__ void kWcUpdate(WC w, int count) {
     w.he[c] = e;
 }
 
-// markHalfedgesAsNonManifold, ManifoldSplitter.cu:527-555: the four slotted halfedges of every full bucket
-// markNonManifoldHalfedges of the working copy: the copied halfedges that are slotted
-// non-manifold get the NM flag, as does their origin; their count goes to foundParts
-// (kNMSlotCheck compares it with the count of all of them)
+// markHalfedgesAsNonManifold, ManifoldSplitter.cu:527-555: the copied halfedges that are
+// slotted non-manifold get the NM flag, as does their origin; their count goes to foundParts
+// (kNMSlotCheck compares it with the count of all of them: the reference would read the copy
+// index of one outside the copy as an invalid id)
 __global__ void __launch_bounds__(kBlock) kWcNMFlags(int n4, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank, WC w,
                                                      int64_t* foundParts) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
