@@ -180,7 +180,7 @@ mdmc_status mdmc_get_graph_replays(const mdmc_ctx* ctx, int64_t* replays);
  * either way.  enable = 0 sends every merge through the region rebuild (default on). */
 mdmc_status mdmc_set_local_merges(mdmc_ctx* ctx, int32_t enable);
 /* Verification switch for the splitter's merge phase (no reference counterpart).  The merges
- * of ManifoldSplitter::mergeHalfedges (ManifoldSplitter.cu:1103-1263) run in parallel and fall
+ * of ManifoldSplitter's mergeHalfedges (ManifoldSplitter.cu:1021-1067) run in parallel and fall
  * back to a one-thread pass in ascending order only when two of them conflict; enable = 1
  * makes every iteration take that pass, so tests can compare it with the parallel one and
  * with the reference's results (default 0).  The mesh is the same either way. */

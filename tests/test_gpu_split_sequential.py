@@ -1,4 +1,4 @@
-"""The splitter's merge phase (ManifoldSplitter.cu:1103-1263) in its one-thread ascending pass.
+"""The splitter's merge phase (mergeHalfedges, ManifoldSplitter.cu:1021-1067) in its one-thread ascending pass.
 
 The merges run in parallel and fall back to that pass only when two of them conflict, which no
 bench or fixture field triggers.  mdmc_set_sequential_merges(1) makes every split iteration take
