@@ -129,7 +129,9 @@ mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* hip_stream);
 #define MDMC_TILE_SLAB4 7         /* row slabs: a block owns up to 254 cells of 4 rows and marches through
                                      a chunk of slices; counts per row segment, one scan, then the emit */
 #define MDMC_TILE_SLAB2 8         /* the same with 2 rows */
-#define MDMC_TILE_SHAPES 8        /* number of shapes */
+#define MDMC_TILE_2048_QUEUE 9    /* 2048-cell tiles taken from a work queue by persistent blocks (as many
+                                     as the GPU holds at once), the tables loaded once per block */
+#define MDMC_TILE_SHAPES 9        /* number of shapes */
 mdmc_status mdmc_set_tile_shape(mdmc_ctx* ctx, int32_t shape);
 /* The shape the last extraction used; tune_ms (may be NULL, double[MDMC_TILE_SHAPES]) receives
  * the grid-pass times of shapes 1..MDMC_TILE_SHAPES from this context's last tuning (0 where none

@@ -44,7 +44,7 @@ class SplittingStep:
 TILE_AUTO = 0
 TILE_SHAPES = {1: "1024 cells, XCD-chunked", 2: "2048 cells, XCD-chunked", 3: "4096 cells, XCD-chunked",
                4: "1024 cells, launch order", 5: "2048 cells, launch order", 6: "4096 cells, launch order",
-               7: "row slabs of 4 rows, XCD-chunked", 8: "row slabs of 2 rows, XCD-chunked"}
+               7: "row slabs of 4 rows, XCD-chunked", 8: "row slabs of 2 rows, XCD-chunked", 9: "2048 cells, work queue"}
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "CAPACITY", 4: "HASH_FULL", 5: "HIP", 6: "NO_RESULT", 7: "INTERNAL"}
 

@@ -152,7 +152,8 @@ mdmc_status mdmc_set_stream(mdmc_ctx* ctx, void* s) {
 
 static_assert(MDMC_TILE_AUTO == mdmc::GP_SHAPE_AUTO && MDMC_TILE_1024 == mdmc::GP_SHAPE_FIRST &&
                   MDMC_TILE_4096_LINEAR == mdmc::GP_SHAPE_LAST_TILE && MDMC_TILE_SLAB4 == mdmc::GP_SHAPE_SLAB4 &&
-                  MDMC_TILE_SLAB2 == mdmc::GP_SHAPE_SLAB2 && MDMC_TILE_SHAPES == mdmc::GP_SHAPE_LAST,
+                  MDMC_TILE_SLAB2 == mdmc::GP_SHAPE_SLAB2 && MDMC_TILE_2048_QUEUE == mdmc::GP_SHAPE_QUEUE &&
+                  MDMC_TILE_SHAPES == mdmc::GP_SHAPE_LAST,
               "tile shape numbering");
 
 mdmc_status mdmc_set_tile_shape(mdmc_ctx* ctx, int32_t shape) {

@@ -86,7 +86,7 @@ void usage() {
                  "  --no-detect             m_detectNonManifoldVertices = false\n"
                  "  --split-step N          ManifoldSplitter step (0-6, 100 = all)\n"
                  "  --max-split-iters N     bound of the split loop (default 64)\n"
-                 "  --tile-shape S          grid-pass shape 1-8 (default: autotuned)\n"
+                 "  --tile-shape S          grid-pass shape 1-9 (default: autotuned)\n"
                  "  --device D              GPU (default: current)\n"
                  "  --write-grid OUT.bin    write the grid back in the .bin layout\n"
                  "  --quiet                 no statistics on stdout\n");

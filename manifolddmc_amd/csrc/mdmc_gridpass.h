@@ -15,10 +15,12 @@ namespace mdmc {
 //   4..6: the same tiles in launch order (block b -> tile b)
 //   7, 8: row slabs of 4 / 2 rows: a block owns a band of up to 254 cells of its rows and
 //         marches through a chunk of slices (count, scan over row segments, emit)
-enum : int { GP_SHAPE_AUTO = 0, GP_SHAPE_FIRST = 1, GP_SHAPE_LAST_TILE = 6, GP_SHAPE_SLAB4 = 7, GP_SHAPE_SLAB2 = 8, GP_SHAPE_LAST = 8,
-             GP_SHAPE_DEFAULT = 2 };
+enum : int { GP_SHAPE_AUTO = 0, GP_SHAPE_FIRST = 1, GP_SHAPE_LAST_TILE = 6, GP_SHAPE_SLAB4 = 7, GP_SHAPE_SLAB2 = 8, GP_SHAPE_QUEUE = 9,
+             GP_SHAPE_LAST = 9, GP_SHAPE_DEFAULT = 2 };
 int gridPassTile(int shape);
-inline bool gridPassIsSlab(int shape) { return shape >= GP_SHAPE_SLAB4; }
+inline bool gridPassIsSlab(int shape) { return shape == GP_SHAPE_SLAB4 || shape == GP_SHAPE_SLAB2; }
+// 2048-cell tiles taken from a work queue by persistent blocks
+inline bool gridPassIsQueue(int shape) { return shape == GP_SHAPE_QUEUE; }
 inline int gridPassSlabRows(int shape) { return shape == GP_SHAPE_SLAB4 ? 4 : 2; }
 constexpr int SLAB_COLS = 256;            // threads of a slab block, one LDS column each
 constexpr int SLAB_XMAX = SLAB_COLS - 2;  // cells of a band row (plus a halo cell and the x + 1 points)
@@ -45,6 +47,8 @@ struct GridPassArgs {
     int nProblems;
     int surfIter;
     int64_t numTiles;
+    // the work-queue counters of a queue-shape pass (classify, emit; zeroed by its classify launch)
+    int64_t* queue;
     // classify -> emit: per tile its active cells' 32-bit records (polygon table entry, cell in
     // the tile, owned edges, contour count) at t * tile + active index; per tile its vertex /
     // quad / active-cell counts

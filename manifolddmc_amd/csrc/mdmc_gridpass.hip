@@ -374,9 +374,9 @@ struct TileCtx {
 };
 
 template <int TILE>
-__device__ __forceinline__ TileCtx tileCtx(const GridPassArgs& a, int chunked) {
+__device__ __forceinline__ TileCtx tileCtxOf(const GridPassArgs& a, int64_t t) {
     TileCtx c;
-    c.t = tileOfBlock(blockIdx.x, a.numTiles, chunked != 0);
+    c.t = t;
     c.prob = problemOfTile(a.probs, a.nProblems, c.t);
     const ProblemDev& P = a.probs[c.prob];
     c.c0 = (c.t - P.tileBase) * TILE;
@@ -386,6 +386,20 @@ __device__ __forceinline__ TileCtx tileCtx(const GridPassArgs& a, int chunked) {
     c.tb.y0 = (int)(r0 % P.CY);
     c.tb.z0 = (int)(r0 / P.CY);
     return c;
+}
+
+// The tile a block works on: with QUEUE false one per block (its index mapped as tileOfBlock
+// maps it); with QUEUE true a persistent block takes tile after tile from a work queue (one
+// counter bumped by the block's first thread, broadcast through LDS) until none is left.
+// Returns -1 when the block is done.  The caller's previous tile must be finished (a barrier)
+// before it asks for the next one.
+template <bool QUEUE>
+__device__ __forceinline__ int64_t nextTile(const GridPassArgs& a, int slot, int chunked, int iter, int64_t* s_t) {
+    if (!QUEUE) return iter == 0 ? tileOfBlock(blockIdx.x, a.numTiles, chunked != 0) : -1;
+    if (threadIdx.x == 0) *s_t = atomicAdd((unsigned long long*)&a.queue[slot], 1ULL);
+    __syncthreads();
+    const int64_t t = *s_t;
+    return t < a.numTiles ? t : -1;
 }
 
 // coordinates of tile cell l (the tile's first cell at (x0, y0, z0)), with the small divider
@@ -399,7 +413,7 @@ __device__ __forceinline__ void tileCellXYZ(const TileBase& b, int l, const Smal
 }
 
 // ---- classify: active cells of one tile, their records, counts and map entries
-template <int CPT>
+template <int CPT, bool QUEUE>
 __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chunked) {
     constexpr int TILE = GP_THREADS * CPT, RUNS = TILE / 64;
     __shared__ uint32_t s_rec[TILE];
@@ -412,17 +426,21 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
     __shared__ int s_mpre[RUNS + 1];
     __shared__ int s_wave[GP_WAVES];
 
+    __shared__ int64_t s_t;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
-    const TileCtx tc = tileCtx<TILE>(a, chunked);
+    // the two small tables in LDS (read after the first barrier below)
+    s_base[tid] = c_polyBase[tid];
+    for (int i = tid; i < kPolyTabSize; i += GP_THREADS) s_tab[i] = c_polyCmap[i];
+    for (int iter = 0;; ++iter) {
+    const int64_t tile = nextTile<QUEUE>(a, 0, chunked, iter, &s_t);
+    if (tile < 0) break;
+    const TileCtx tc = tileCtxOf<TILE>(a, tile);
     const ProblemDev& P = a.probs[tc.prob];
     const GridDev g = P.g;
     const float iso = P.iso;
     const int CX = P.CX, CY = P.CY;
     const int64_t C = P.cells;
-    // the two small tables in LDS (read after the first barrier below)
-    s_base[tid] = c_polyBase[tid];
-    for (int i = tid; i < kPolyTabSize; i += GP_THREADS) s_tab[i] = c_polyCmap[i];
 
     // MC case of every cell; a ballot per run of 64 cells marks the active ones.  This thread's
     // cells are GP_THREADS apart: their coordinates advance from the first one's, with no
@@ -527,10 +545,12 @@ __global__ void __launch_bounds__(GP_THREADS) kClassify(GridPassArgs a, int chun
             a.tileQ[a.numTiles] = 0;
         }
     }
+    __syncthreads();  // the next tile reuses the block's LDS
+    }
 }
 
 // ---- emit: the tile's dual vertices and the quads its cells own, one active cell per thread
-template <int CPT>
+template <int CPT, bool QUEUE>
 __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked) {
     constexpr int TILE = GP_THREADS * CPT;
     __shared__ uint32_t s_rec[TILE];
@@ -540,15 +560,19 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
     __shared__ uint16_t s_aidx[TILE];  // per tile cell: its active index (0xFFFF: inactive, adjoint map only)
     __shared__ int s_wave[GP_WAVES];
 
+    __shared__ int64_t s_t;
     const int tid = threadIdx.x;
-    const TileCtx tc = tileCtx<TILE>(a, chunked);
+    for (int i = tid; i < kPolyTabSize; i += GP_THREADS) s_tab[i] = c_polyCmap[i];
+    for (int iter = 0;; ++iter) {
+    const int64_t tile = nextTile<QUEUE>(a, 1, chunked, iter, &s_t);
+    if (tile < 0) break;
+    const TileCtx tc = tileCtxOf<TILE>(a, tile);
     const ProblemDev& P = a.probs[tc.prob];
     const int CX = P.CX, CY = P.CY;
     const int64_t C = P.cells;
     const int64_t c0 = tc.c0;
     const int NA = a.tileNA[tc.t];
     const int64_t rb = tc.t * TILE;
-    for (int i = tid; i < kPolyTabSize; i += GP_THREADS) s_tab[i] = c_polyCmap[i];
     if (a.adjMap) {
         for (int l = tid; l < TILE; l += GP_THREADS) s_aidx[l] = 0xFFFF;
         __syncthreads();
@@ -635,6 +659,8 @@ __global__ void __launch_bounds__(GP_THREADS) kEmit(GridPassArgs a, int chunked)
             a.qattr[q] = (uint32_t)(3 * ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)) + axis);
             ++q;
         }
+    }
+    __syncthreads();  // the next tile reuses the block's LDS
     }
 }
 
@@ -1094,7 +1120,7 @@ __global__ void kQuadLists(const int4* __restrict__ quads, const uint32_t* __res
 
 }  // namespace
 
-int gridPassTile(int shape) { return 1024 << ((shape - 1) % 3); }
+int gridPassTile(int shape) { return shape == GP_SHAPE_QUEUE ? 2048 : 1024 << ((shape - 1) % 3); }
 
 void initGridPassTables() {
     static const uint64_t poly[256] = {MDMC_POLYGON_TABLE_DATA};
@@ -1158,18 +1184,39 @@ void initGridPassTables() {
 const char* gridPassShapeName(int shape) {
     static const char* names[] = {"auto", "1024 cells, XCD-chunked", "2048 cells, XCD-chunked", "4096 cells, XCD-chunked",
                                   "1024 cells, launch order", "2048 cells, launch order", "4096 cells, launch order",
-                                  "row slabs of 4 rows, XCD-chunked", "row slabs of 2 rows, XCD-chunked"};
+                                  "row slabs of 4 rows, XCD-chunked", "row slabs of 2 rows, XCD-chunked", "2048 cells, work queue"};
     return (shape >= 0 && shape <= GP_SHAPE_LAST) ? names[shape] : "?";
+}
+
+// zero the two work-queue counters (classify, emit) of a queue-shape pass
+__global__ void kResetQueue(unsigned long long* q) {
+    if (threadIdx.x < 2) q[threadIdx.x] = 0;
+}
+
+// persistent blocks for a queue-shape kernel: as many as the chip holds at once, at most one
+// per tile
+template <class K>
+static unsigned queueBlocks(K kernel, int64_t numTiles) {
+    int dev = 0, cus = 0, perCU = 0;
+    MDMC_HIP(hipGetDevice(&dev));
+    MDMC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MDMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, GP_THREADS, 0));
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(numTiles, (int64_t)std::max(perCU, 1) * std::max(cus, 1)));
 }
 
 void launchGridClassify(int shape, const GridPassArgs& a, hipStream_t s) {
     if (a.numTiles <= 0) return;
     const int chunked = shape <= 3 ? 1 : 0;
     const unsigned nb = (unsigned)a.numTiles;
+    if (gridPassIsQueue(shape)) {
+        kResetQueue<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(a.queue));
+        kClassify<8, true><<<queueBlocks(kClassify<8, true>, a.numTiles), GP_THREADS, 0, s>>>(a, 0);
+        return;
+    }
     switch (gridPassTile(shape)) {
-        case 1024: kClassify<4><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
-        case 2048: kClassify<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
-        default: kClassify<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        case 1024: kClassify<4, false><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        case 2048: kClassify<8, false><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        default: kClassify<16, false><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
     }
 }
 
@@ -1196,10 +1243,14 @@ void launchGridEmit(int shape, const GridPassArgs& a, hipStream_t s) {
     if (a.numTiles <= 0) return;
     const int chunked = shape <= 3 ? 1 : 0;
     const unsigned nb = (unsigned)a.numTiles;
-    switch (gridPassTile(shape)) {
-        case 1024: kEmit<4><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
-        case 2048: kEmit<8><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
-        default: kEmit<16><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+    if (gridPassIsQueue(shape)) {
+        kEmit<8, true><<<queueBlocks(kEmit<8, true>, a.numTiles), GP_THREADS, 0, s>>>(a, 0);
+    } else {
+        switch (gridPassTile(shape)) {
+            case 1024: kEmit<4, false><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+            case 2048: kEmit<8, false><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+            default: kEmit<16, false><<<nb, GP_THREADS, 0, s>>>(a, chunked); break;
+        }
     }
     if (a.V > 0) kVertices<<<(unsigned)((a.V + GP_THREADS - 1) / GP_THREADS), GP_THREADS, 0, s>>>(a, chunked);
 }

@@ -2744,7 +2744,7 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
         // autotuner: on the first extraction of a workload class, time every shape on this
         // very input and keep the fastest for the class.  The XCD-chunked shapes (1-3, 7, 8)
         // move less data out of L2 (about half for the tiles, measured at 512^3); a
-        // launch-order shape must beat them by 3% to be chosen.
+        // launch-order or work-queue shape (4-6, 9) must beat them by 3% to be chosen.
         double best = 1e30;
         for (int sh = GP_SHAPE_FIRST; sh <= GP_SHAPE_LAST; ++sh) {
             gridPass(problems, C, sh, surfIter, recordAdjoint);
@@ -2767,7 +2767,7 @@ void Pipeline::dmc(std::vector<ProblemDev>& problems, bool surfIter, bool record
 // addresses of every device buffer the front of a tile-shape grid pass touches
 std::vector<int64_t> Pipeline::frontBuffers() {
     std::vector<int64_t> b;
-    for (const char* n : {"counters", "grid", "prep_desc", "gp_probs", "gp_rec", "gp_tv", "gp_tq", "gp_tna", "gp_map", "gp_tvo", "gp_tqo",
+    for (const char* n : {"counters", "grid", "prep_desc", "gp_probs", "gp_queue", "gp_rec", "gp_tv", "gp_tq", "gp_tna", "gp_map", "gp_tvo", "gp_tqo",
                           "scan_scratch", "scan_scratch2"})
         b.push_back((int64_t)(intptr_t)arena.ptr<char>(n));
     b.push_back((int64_t)(intptr_t)pinDev());
@@ -2877,6 +2877,7 @@ void Pipeline::gridPassFront(std::vector<ProblemDev>& problems, int64_t C, int s
     a.nProblems = nProblems;
     a.surfIter = surfIter ? 1 : 0;
     a.numTiles = numTiles;
+    a.queue = arena.get<int64_t>("gp_queue", 2);
     a.rec = arena.get<uint32_t>("gp_rec", (size_t)std::max<int64_t>(numTiles * T, 1));
     a.tileV = arena.get<int32_t>("gp_tv", (size_t)numTiles + 1);
     a.tileQ = arena.get<int32_t>("gp_tq", (size_t)numTiles + 1);

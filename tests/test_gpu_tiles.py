@@ -32,7 +32,7 @@ def extract(d, f, iso, flags):
     return d.mesh_numpy(), st.as_dict()
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_every_tile_shape_matches_oracle(shape):
     import manifolddmc_amd as mdmc
     d = mdmc.DualMarchingCubes()
@@ -67,8 +67,9 @@ def test_autotuner_picks_a_shape_once_per_class_and_keeps_the_mesh():
         shape, ms = auto.tile_shape()
         assert shape in mdmc.TILE_SHAPES
         assert all(v > 0 for v in ms.values()), ms
-        # the fastest shape, a launch-order tile shape (4-6) counted 3% slower than its time
-        cost = {k: v * (1.03 if 4 <= k <= 6 else 1.0) for k, v in ms.items()}
+        # the fastest shape, a launch-order or work-queue tile shape (4-6, 9) counted 3% slower
+        # than its time
+        cost = {k: v * (1.03 if 4 <= k <= 6 or k == 9 else 1.0) for k, v in ms.items()}
         assert cost[shape] == min(cost.values())
         g2, _ = extract(auto, f, 0.1, flags)  # same class: no new tuning
         assert auto.tile_shape() == (shape, ms)
@@ -85,7 +86,7 @@ def test_autotuner_picks_a_shape_once_per_class_and_keeps_the_mesh():
         fixed.close()
 
 
-@pytest.mark.parametrize("shape", [1, 3, 5, 7, 8])
+@pytest.mark.parametrize("shape", [1, 3, 5, 7, 8, 9])
 def test_batch_with_fixed_shape_equals_single_extractions(shape):
     import manifolddmc_amd as mdmc
     from manifolddmc_amd import grids
