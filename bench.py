@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--tile-shape", type=int, default=0, help="grid-pass shape 1-8 (0: autotuned)")
+    ap.add_argument("--tile-shape", type=int, default=0, help="grid-pass shape 1-9 (0: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers 5 times (reference for the HBM traffic counters) and exit")
     return ap.parse_args()
@@ -412,6 +412,7 @@ def main():
     out["stages_roofline"] = stage_roofline(s, n, {k: float(np.mean([x[k] for x in stats])) for k in stage_keys})
     out["anomalies"] = s["anomalies"]
     out["split_sequential_merges"] = s["split_sequential_merges"]
+    out["lookback_fallbacks"] = max(x["lookback_fallbacks"] for x in stats)
     d.close()
     if not args.no_batch:
         print("batched", file=sys.stderr, flush=True)

@@ -93,6 +93,10 @@ typedef struct {
     /* split iterations whose merge phase took the one-thread ascending pass (the parallel merge
      * found a schedule conflict, or mdmc_set_sequential_merges forced it); -1 for batches */
     int64_t split_sequential_merges;
+    /* tiles of the single-pass prefix sums and compactions whose wait on the tiles before them
+     * passed its spin limit and that summed their prefix directly instead (exact either way;
+     * 0 normally); -1 in a batch's per-problem statistics, whose call statistics hold the count */
+    int64_t lookback_fallbacks;
 } mdmc_stats;
 
 /* Sizes of the resident result. */

@@ -84,7 +84,8 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_grid", ctypes.c_double), ("ms_dmc", ctypes.c_double),
                 ("ms_halfedge", ctypes.c_double), ("ms_simplify", ctypes.c_double), ("ms_fix_twins", ctypes.c_double),
                 ("ms_split", ctypes.c_double), ("ms_detect", ctypes.c_double),
-                ("ms_grid_pass_kernel", ctypes.c_double), ("split_sequential_merges", ctypes.c_int64)]
+                ("ms_grid_pass_kernel", ctypes.c_double), ("split_sequential_merges", ctypes.c_int64),
+                ("lookback_fallbacks", ctypes.c_int64)]
 
     def as_dict(self):
         out = {}

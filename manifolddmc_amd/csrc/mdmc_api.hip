@@ -365,8 +365,9 @@ mdmc_status runExtraction(mdmc_ctx* ctx, const mdmc_grid* grids, int G, const fl
             for (int k = 0; k < 5; ++k) stats->quads_with_nm[k] = st9[3 + k];
             stats->nm_vertices = (flags & MDMC_DETECT_NM_VERTS) ? st9[8] : -1;
             stats->split_iterations = p.splitIterations;
-            stats->anomalies = p.anomalies() + p.extraAnomalies;
-            stats->split_sequential_merges = p.sequentialMergeIterations();
+            int64_t anomalyCount = 0;
+            p.counterTotals(&anomalyCount, &stats->split_sequential_merges, &stats->lookback_fallbacks);
+            stats->anomalies = anomalyCount + p.extraAnomalies;
             stats->cells = cells;
             float ms = 0.f;
             double* outs[8] = {&stats->ms_total, &stats->ms_grid, &stats->ms_dmc, &stats->ms_halfedge,
@@ -444,6 +445,7 @@ mdmc_status mdmc_batch_stats(const mdmc_ctx* ctx, mdmc_stats* per_problem) {
         st.anomalies = -1;
         st.cells = -1;
         st.split_sequential_merges = -1;
+        st.lookback_fallbacks = -1;
     }
     return MDMC_OK;
 }

@@ -270,7 +270,7 @@ int32_t* Pipeline::stableSortByKey(const int32_t* key, int64_t n, int B, const s
     while ((1 << bits) < B) ++bits;
     for (int bit = 0; bit < bits; ++bit) {
         kSplitFlags<<<gridFor(n), kBlock, 0, stream>>>(a, key, n, bit, flag);
-        scanExclusiveI32(flag, pre, n, zeros, scratch, stream);
+        scanExclusiveI32(flag, pre, n, zeros, scratch, stream, lookbackDiag());
         kSplitScatter<<<gridFor(n), kBlock, 0, stream>>>(a, flag, pre, n, zeros, b);
         std::swap(a, b);
     }

@@ -20,9 +20,10 @@ struct GridDev {
 // --- scans (mdmc_scan.hip): exclusive prefix of n int32 values into int64,
 // returns the total through *total_dev (device int64).  scratch is sized by scanScratchBytes.
 // The result is exact in every case; a tile whose look-back wait passed its spin limit
-// summed its prefix directly and sets *directOut (when given) to 1, a diagnostic only.
+// summed its prefix directly and adds 1 to *directOut (when given; device memory), a
+// diagnostic only.
 size_t scanScratchBytes(int64_t n);
-// total_dev and directOut may point into mapped pinned host memory
+// total_dev may point into mapped pinned host memory
 void scanExclusiveI32(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, void* scratch, hipStream_t s, int64_t* directOut = nullptr);
 // stream compaction in one pass: the i < n with (attr[i] & mask) != 0, ascending, into
 // idx (capacity n); the count through *total_dev; scratch sized by scanScratchBytes(n)

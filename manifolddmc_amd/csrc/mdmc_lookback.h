@@ -23,7 +23,7 @@ __device__ __forceinline__ void lbStore(unsigned long long* p, uint64_t v) {
 // tiles' words and returns (through *s_off, read after a barrier) the number of selected
 // elements before the tile.  A wait past the spin limit does not trust the dispatch order:
 // countBefore(lane) (the selected elements among base .. in steps of 64 from lane) gives the
-// exact prefix instead, and *directOut is set as a diagnostic.
+// exact prefix instead, and *directOut (device memory) counts the tile as a diagnostic.
 template <class CountBefore>
 __device__ __forceinline__ void lbTileOffset(unsigned long long* status, int64_t t, int64_t tileSum, CountBefore countBefore, int64_t* directOut,
                                              long long* s_off) {
@@ -70,7 +70,7 @@ __device__ __forceinline__ void lbTileOffset(unsigned long long* status, int64_t
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
         accum = __shfl(sum, 0, 64);
-        if (lane == 0 && directOut) *directOut = 1;
+        if (lane == 0 && directOut) atomicAdd((unsigned long long*)directOut, 1ULL);
     }
     if (lane == 0) {
         lbStore(&status[t], (FLAG_INC << 62) | ((uint64_t)(accum + tileSum) & VAL_MASK));

@@ -65,8 +65,11 @@ class Pipeline {
     const int32_t* vertexSlots();
 
     void resetCounters();
-    int64_t anomalies();
-    int64_t sequentialMergeIterations();
+    // the counters an extraction reports, in one read-back: anomalies, split iterations that
+    // took the sequential merge pass, look-back tiles that summed their prefix directly
+    void counterTotals(int64_t* anomalyCount, int64_t* mergeSeq, int64_t* lookbackDirect);
+    // device word the single-pass scans and compactions count their direct-sum tiles in
+    int64_t* lookbackDiag();
     // the padding pass's flag word (one of the counters, zeroed by resetCounters)
     int64_t* prepSpecial();
 
@@ -132,7 +135,6 @@ class Pipeline {
     // mapped pinned host words for small read-backs: kernels and copies write them
     // directly, so a read-back is one stream synchronisation (slots 0..kPinSlots-1)
     static constexpr int kPinSlots = 64;
-    static constexpr int kPinMergeTimeout = 8;  // direct-sum diagnostic of the splitter's merge-list scans
     int64_t* pinHost();
     int64_t* pinDev();
     // host -> device copy of a small host array through pinned staging (region 0 or 1, so

@@ -37,7 +37,7 @@ constexpr int kMaxBucket = 16;
 constexpr int kFanGuard = 4096;
 
 // ---------------------------------------------------------------- counters
-enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_MERGE_SEQ = 6, A_NCOUNTERS = 8 };
+enum : int { A_ANOMALY = 0, A_CONFLICT = 1, A_INVALID = 2, A_DIRTY = 3, A_PREP_SPECIAL = 4, A_FALLBACK = 5, A_MERGE_SEQ = 6, A_LB_DIRECT = 7, A_NCOUNTERS = 8 };
 // the slotted non-manifold halfedges the split's working copy marks (markHalfedgesAsNonManifold,
 // ManifoldSplitter.cu:527-555, reads the four slots of every full vertex-pair bucket): every
 // halfedge of a bucket of 4, and of a larger bucket the ones at ranks 1, 2, 3 and size - 1
@@ -2567,11 +2567,10 @@ const int32_t* Pipeline::vertexSlots() { return posSlotted ? arena.ptr<int32_t>(
 int64_t Pipeline::scan32(const int32_t* in, int64_t n, const std::string& outName, int64_t** out) {
     *out = arena.get<int64_t>(outName, (size_t)n + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(n + 1));
-    // total (and the direct-sum diagnostic flag) straight into pinned host memory
+    // total straight into pinned host memory
     int64_t* h = pinHost();
     int64_t* d = pinDev();
-    h[0] = 0;
-    scanExclusiveI32(in, *out, n, d + 1, scratch, stream, d);
+    scanExclusiveI32(in, *out, n, d + 1, scratch, stream, lookbackDiag());
     MDMC_HIP(hipStreamSynchronize(stream));
     return h[1];
 }
@@ -2581,8 +2580,7 @@ int64_t Pipeline::compactNM(int32_t* nm) {
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(F + 1));
     int64_t* h = pinHost();
     int64_t* d = pinDev();
-    h[0] = 0;
-    compactFlagged(arena.ptr<uint32_t>("fattr"), F, (uint32_t)QA_NM, nm, d + 1, scratch, stream, d);
+    compactFlagged(arena.ptr<uint32_t>("fattr"), F, (uint32_t)QA_NM, nm, d + 1, scratch, stream, lookbackDiag());
     MDMC_HIP(hipStreamSynchronize(stream));
     const int64_t n = h[1];
     if (n < 0 || n > F) throw HipError("inconsistent NM-face count (" + std::to_string(n) + ")", -4);
@@ -2624,7 +2622,16 @@ void Pipeline::FillBatch::flush() {
 
 void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t)); }
 
-int64_t Pipeline::anomalies() { return readI64(counters() + A_ANOMALY); }
+int64_t* Pipeline::lookbackDiag() { return counters() + A_LB_DIRECT; }
+
+void Pipeline::counterTotals(int64_t* anomalyCount, int64_t* mergeSeq, int64_t* lookbackDirect) {
+    int64_t* h = pinHost();
+    MDMC_HIP(hipMemcpyAsync(h, counters(), A_NCOUNTERS * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+    MDMC_HIP(hipStreamSynchronize(stream));
+    *anomalyCount = h[A_ANOMALY];
+    *mergeSeq = h[A_MERGE_SEQ];
+    *lookbackDirect = h[A_LB_DIRECT];
+}
 
 int64_t* Pipeline::prepSpecial() { return counters() + A_PREP_SPECIAL; }
 
@@ -2669,11 +2676,11 @@ void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(segs + 1));
     int64_t* h = pinHost();
     int64_t* pd = pinDev();
-    h[0] = h[1] = h[2] = 0;
+    h[1] = h[2] = 0;
     MDMC_HIP(hipEventRecord(kev[0], stream));
     launchSlabCount(shape, a, stream);
-    scanExclusiveI32(a.segV, svo, segs + 1, pd + 1, scratchV, stream, pd);
-    scanExclusiveI32(a.segQ, sqo, segs + 1, pd + 2, scratchQ, stream, pd);
+    scanExclusiveI32(a.segV, svo, segs + 1, pd + 1, scratchV, stream, lookbackDiag());
+    scanExclusiveI32(a.segQ, sqo, segs + 1, pd + 2, scratchQ, stream, lookbackDiag());
     MDMC_HIP(hipEventRecord(kev[1], stream));
     MDMC_HIP(hipStreamSynchronize(stream));
     V = h[1];
@@ -2697,8 +2704,6 @@ void Pipeline::slabPass(std::vector<ProblemDev>& problems, int64_t C, int shape,
     dmcV = V;
     dmcQ = Q;
 }
-int64_t Pipeline::sequentialMergeIterations() { return readI64(counters() + A_MERGE_SEQ); }
-
 int64_t Pipeline::dmcSetup(std::vector<ProblemDev>& problems, bool recordAdjoint) {
     vertHeV = vertHeQ = 0;
     batchVertHe = false;
@@ -2889,13 +2894,13 @@ void Pipeline::gridPassFront(std::vector<ProblemDev>& problems, int64_t C, int s
     a.tileQOff = tqo;
     void* scratchV = arena.get<char>("scan_scratch", scanScratchBytes(numTiles + 1));
     void* scratchQ = arena.get<char>("scan_scratch2", scanScratchBytes(numTiles + 1));
-    // totals straight into pinned host memory: [0] direct-sum diagnostic, [1] V, [2] Q
+    // totals straight into pinned host memory: [1] V, [2] Q
     int64_t* pd = pinDev();
     if (events) MDMC_HIP(hipEventRecord(kev[0], stream));
     launchGridClassify(shape, a, stream);
     if (numTiles > 0) {
-        scanExclusiveI32(a.tileV, tvo, numTiles + 1, pd + 1, scratchV, stream, pd);
-        scanExclusiveI32(a.tileQ, tqo, numTiles + 1, pd + 2, scratchQ, stream, pd);
+        scanExclusiveI32(a.tileV, tvo, numTiles + 1, pd + 1, scratchV, stream, lookbackDiag());
+        scanExclusiveI32(a.tileQ, tqo, numTiles + 1, pd + 2, scratchQ, stream, lookbackDiag());
     } else {
         zero(pd + 1, 2 * sizeof(int64_t));
     }
@@ -2945,7 +2950,7 @@ void Pipeline::buildBuckets(const int32_t* list, int64_t n) {
     int32_t* keys = arena.ptr<int32_t>("bl_key");
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
-    scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
+    scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream, lookbackDiag());
     int2* A = arena.get<int2>("listA", (size_t)std::max<int64_t>(H, 1));
     int32_t* bsize = arena.ptr<int32_t>("bsize");
     int32_t* brank = arena.ptr<int32_t>("brank");
@@ -2992,7 +2997,8 @@ void Pipeline::halfedgeInitialize() {
     // corner vertices' contours: no counting atomics
     zero(cnt + V, sizeof(int32_t));
     int64_t* off = arena.get<int64_t>("out_off", (size_t)V + 1);
-    scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream);
+    scanExclusiveI32(cnt, off, V + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(V + 1)), stream,
+                     lookbackDiag());
     int32_t* out = arena.get<int32_t>("out_list", (size_t)std::max<int64_t>(H, 1));
     int2* ends = arena.get<int2>("out_ends", (size_t)std::max<int64_t>(H, 1));
     launchQuadLists(quads, qattr, Q, V, arena.ptr<int32_t>("gp_vcell"), arena.ptr<uint16_t>("gp_vct"), pid(), probsDev, off, out, ends, face_he, fattr,
@@ -3144,15 +3150,15 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
         fb.add(sq, 0x00, (size_t)tQ * sizeof(unsigned long long));
         if (removedAt) fb.add(removedAt, 0x00, (size_t)V * sizeof(int32_t));
     }
-    // totals straight into pinned host memory: [0] direct-count diagnostic, [1] V, [2] Q
+    // totals straight into pinned host memory: [1] V, [2] Q
     int64_t* h = pinHost();
     int64_t* pd = pinDev();
-    h[0] = h[1] = h[2] = 0;
+    h[1] = h[2] = 0;
     if (V > 0)
         kCompactVertices<RV><<<(unsigned)tV, kLbBlock, 0, stream>>>(vtype, V, dropType, chained ? arena.ptr<uint8_t>("vdead") : nullptr, pid(), vertexSlots(),
-                                                                npid, address, oldV, nvslot, tV, sv, pd + 1, pd);
+                                                                npid, address, oldV, nvslot, tV, sv, pd + 1, lookbackDiag());
     if (Q > 0) {
-        kNumberQuads<RQ><<<(unsigned)tQ, kLbBlock, 0, stream>>>(attr, Q, qnew, tQ, sq, pd + 2, pd);
+        kNumberQuads<RQ><<<(unsigned)tQ, kLbBlock, 0, stream>>>(attr, Q, qnew, tQ, sq, pd + 2, lookbackDiag());
         kCompactQuads<<<gridFor(Q), kBlock, 0, stream>>>(quads, attr, Q, vtarget, address, p3333 ? 1 : 0, nquads, nattr, oldQ, qnew, removedAt, outdeg,
                                                          V, counters());
     }
@@ -3178,7 +3184,8 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     zero(cnt + nV, sizeof(int32_t));
     if (nV > 0) kOutCount<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, removedAt, srcHead, srcNext, outdeg, cnt);
     int64_t* off2 = arena.get<int64_t>("out_off2", (size_t)std::max<int64_t>(nV, dmcV) + 1);
-    scanExclusiveI32(cnt, off2, nV + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(nV + 1)), stream);
+    scanExclusiveI32(cnt, off2, nV + 1, arena.get<int64_t>("scan_total", 1), arena.get<char>("scan_scratch", scanScratchBytes(nV + 1)), stream,
+                     lookbackDiag());
     int32_t* out2 = arena.get<int32_t>("out_list2", 4 * capQ);
     if (nV > 0)
         kOutFill<<<gridFor(nV), kBlock, 0, stream>>>(oldV, nV, off, out, qnew, srcHead, srcNext, off2, out2, nH, counters());
@@ -3315,7 +3322,6 @@ void Pipeline::split(int step, int maxIters) {
             extraAnomalies += 1;
             break;
         }
-        pinHost()[kPinMergeTimeout] = 0;  // direct-sum diagnostic of the merge-list scans
         if (20 * n64 >= (int64_t)INT32_MAX / 2) throw HipError("working copy exceeds the int32 id range", -3);
         const int n = (int)n64;
 
@@ -3425,7 +3431,7 @@ void Pipeline::split(int step, int maxIters) {
 
         if (copyDataBack || step >= 6) {
             // the new vertices, halfedges and faces of the working copy: three prefix sums
-            // and one read-back of their totals ([0] direct-sum diagnostic, [1..3] counts)
+            // and one read-back of their totals ([1..3])
             int32_t* pv = arena.get<int32_t>("cb_pred_v", (size_t)std::max(vsize, 1));
             int32_t* ph = arena.get<int32_t>("cb_pred_h", (size_t)std::max(hsize, 1));
             int32_t* pf = arena.get<int32_t>("cb_pred_f", (size_t)std::max(fsize, 1));
@@ -3437,10 +3443,10 @@ void Pipeline::split(int step, int maxIters) {
             kNewPred<<<gridFor(fsize), kBlock, 0, stream>>>(w.fflags, w.f_c2o, fsize, pf);
             int64_t* hp = pinHost();
             int64_t* dp = pinDev();
-            hp[0] = hp[1] = hp[2] = hp[3] = 0;
-            scanExclusiveI32(pv, av, vsize, dp + 1, arena.get<char>("cb_scan_v", scanScratchBytes(vsize + 1)), stream, dp);
-            scanExclusiveI32(ph, ah, hsize, dp + 2, arena.get<char>("cb_scan_h", scanScratchBytes(hsize + 1)), stream, dp);
-            scanExclusiveI32(pf, af, fsize, dp + 3, arena.get<char>("cb_scan_f", scanScratchBytes(fsize + 1)), stream, dp);
+            hp[1] = hp[2] = hp[3] = 0;
+            scanExclusiveI32(pv, av, vsize, dp + 1, arena.get<char>("cb_scan_v", scanScratchBytes(vsize + 1)), stream, lookbackDiag());
+            scanExclusiveI32(ph, ah, hsize, dp + 2, arena.get<char>("cb_scan_h", scanScratchBytes(hsize + 1)), stream, lookbackDiag());
+            scanExclusiveI32(pf, af, fsize, dp + 3, arena.get<char>("cb_scan_f", scanScratchBytes(fsize + 1)), stream, lookbackDiag());
             MDMC_HIP(hipStreamSynchronize(stream));
             const int64_t addV = hp[1], addH = hp[2], addF = hp[3];
             // vertices
@@ -3510,7 +3516,7 @@ void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* nu
     // offsets only: the total is not needed on the host
     int64_t* roff = arena.get<int64_t>(tag + "_off", (size_t)count + 2);
     scanExclusiveI32(cnt, roff, count + 1, arena.get<int64_t>(tag + "_tot", 1), arena.get<char>(tag + "_scan", scanScratchBytes(count + 1)), stream,
-                     pinDev() + kPinMergeTimeout);
+                     lookbackDiag());
     int32_t* cursor = arena.get<int32_t>(tag + "_cur", (size_t)count + 1);
     zero(cursor, ((size_t)count + 1) * sizeof(int32_t));
     int32_t* idx = arena.get<int32_t>(tag + "_idx", (size_t)R);
@@ -3551,7 +3557,7 @@ void Pipeline::detectNonManifoldVertices() {
     kSlowDegree<<<gridFor(V + 1), kBlock, 0, stream>>>(slow, outdeg, V, deg);
     int64_t* off = arena.get<int64_t>("off", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
-    scanExclusiveI32(deg, off, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream);
+    scanExclusiveI32(deg, off, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream, lookbackDiag());
     int32_t* list = arena.get<int32_t>("list", (size_t)H);
     const bool copyLists = nSlowOff == 0;  // every slow vertex keeps the build's list
     if (copyLists) kCopySlowLists<<<gridFor(V), kBlock, 0, stream>>>(slow, V, arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"), off, list);

@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(kScanBlock) kScan(const int32_t* __restrict__ 
 #pragma unroll
                 for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
                 accum = __shfl(sum, 0, 64);
-                if (lane == 0 && directOut) *directOut = 1;
+                if (lane == 0 && directOut) atomicAdd((unsigned long long*)directOut, 1ULL);
             }
             if (lane == 0) {
                 storeRelaxed(&status[t], (FLAG_INC << 62) | ((uint64_t)(accum + tileSum) & VAL_MASK));
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(kScanBlock) kCompactFlagged(const uint32_t* __
 #pragma unroll
                 for (int d = 32; d > 0; d >>= 1) sum += __shfl_down(sum, d, 64);
                 accum = __shfl(sum, 0, 64);
-                if (lane == 0 && directOut) *directOut = 1;
+                if (lane == 0 && directOut) atomicAdd((unsigned long long*)directOut, 1ULL);
             }
             if (lane == 0) {
                 storeRelaxed(&status[t], (FLAG_INC << 62) | ((uint64_t)(accum + tileSum) & VAL_MASK));

@@ -72,6 +72,7 @@ def test_fullsize_structure_and_statistics(field):
     # undefined happened (out-of-range ids, fan walks past their guard, oversized buckets)
     assert st["nm_edges4"] == 0
     assert st["anomalies"] == 0
+    assert st["lookback_fallbacks"] == 0  # no scan tile waited past its spin limit
     assert st["n_edges"] > 0 and st["n_dmc_quads"] > F
 
 

@@ -43,6 +43,9 @@ def assert_same(vals, origin, delta, iso, flags, refinement=0, split_step=oa.STE
               "nm_vertices", "quads_with_nm", "split_iterations", "anomalies"):
         if gst[k] != o.stats[k]:
             errs.append("stat %s: gpu %s oracle %s" % (k, gst[k], o.stats[k]))
+    # no single-pass scan or compaction tile waited past its spin limit on the tiles before it
+    if gst["lookback_fallbacks"] != 0:
+        errs.append("look-back fallbacks: %d" % gst["lookback_fallbacks"])
     assert not errs, "\n".join(errs)
     return g, gst
 

@@ -95,7 +95,7 @@ def test_batched_workload_equals_single_extractions(batch):
     t, vo, ho, fo = batch["t"], batch["vo"], batch["ho"], batch["fo"]
     tot = batch["tot"]
     assert tot["n_vertices"] == vo[-1] and tot["n_halfedges"] == ho[-1] and tot["n_faces"] == fo[-1]
-    assert tot["anomalies"] == 0 and tot["nm_edges4"] == 0
+    assert tot["anomalies"] == 0 and tot["nm_edges4"] == 0 and tot["lookback_fallbacks"] == 0
     print("batched workload: halfedge invariants over %d halfedges" % int(ho[-1]), flush=True)
     check_halfedges(t, int(vo[-1]), int(fo[-1]))
     one = mdmc.DualMarchingCubes()
@@ -171,7 +171,7 @@ def test_gyroid_1024_structure_statistics_determinism():
         V, H, F = d.info()
         assert st["cells"] == (n + 1) ** 3
         assert H == st["n_halfedges"] and V == st["n_vertices"] and F == st["n_faces"]
-        assert st["anomalies"] == 0 and st["nm_edges4"] == 0 and st["nm_vertices"] == 0
+        assert st["anomalies"] == 0 and st["nm_edges4"] == 0 and st["nm_vertices"] == 0 and st["lookback_fallbacks"] == 0
         assert st["n_dmc_vertices"] > 100_000_000  # beyond the reference's 20 M capacities
         print("gyroid 1024^3: %d vertices, %d faces; halfedge invariants" % (V, F), flush=True)
         check_halfedges(t, V, F)

@@ -29,6 +29,37 @@ def test_options_struct_layout_matches_header():
     assert ctypes.sizeof(mdmc._Grid) == 3 * 4 * 3 + 4 + 8 + 4 + 4  # dims, origin, delta, pad, ptr, flag, tail pad
 
 
+def test_struct_mirrors_match_the_compiled_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset the C compiler gives it in
+    include/mdmc.h, and the structs have the same size (a field added on one side only
+    shifts everything after it)."""
+    import shutil
+    import subprocess
+
+    import pytest
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = [("mdmc_options", mdmc._Options), ("mdmc_grid", mdmc._Grid), ("mdmc_stats", mdmc.Stats)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "mdmc.h"', "int main(void) {"]
+    for cname, py in structs:
+        lines.append('    printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('    printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, py in structs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
+
+
 def test_reference_member_names_map_to_flags():
     # DualMarchingCubes.cu:110-133 defaults; the reference names alias the snake_case options
     d = object.__new__(mdmc.DualMarchingCubes)  # no context: flags() is host logic only
