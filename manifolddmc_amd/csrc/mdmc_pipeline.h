@@ -202,6 +202,8 @@ class Pipeline {
     void buildBuckets(const int32_t* list, int64_t n);
     int32_t* bucketPassValence(const int64_t* off, int64_t nV);
     void refreshHashTable(const uint8_t* dirtyV, int64_t oldH);
+    // a refresh ran whose face-walk check (the invalid-record counter) is not read back yet
+    bool refreshUnchecked = false;
     void checkValid(const char* what);
     void simplifyInit(int32_t** valence, int32_t** count, bool needCount);
     void simplifyCompactAndRebuild(int dropType, bool p3333);

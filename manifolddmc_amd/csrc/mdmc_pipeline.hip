@@ -156,17 +156,6 @@ __global__ void kFaceClear(const int32_t* __restrict__ face_he, int64_t F, int64
 __device__ __forceinline__ int pairEntry(int a, int b) { return a <= b ? b : (int)(0x80000000u | (uint32_t)a); }
 __device__ __forceinline__ int entryOther(int x) { return x & 0x7FFFFFFF; }
 
-// list of the smaller endpoint of h = (e.x, origin of e.z): start, length and h's own entry
-struct ListRef {
-    int lo, len, entry;
-};
-__device__ __forceinline__ ListRef listOf(const int4* __restrict__ he, const int4& e, const int64_t* __restrict__ offM) {
-    const int b = heO(he, e.z);
-    const int m = min(e.x, b);
-    const int64_t lo = offM[m];
-    return ListRef{(int)lo, (int)(offM[m + 1] - lo), pairEntry(e.x, b)};
-}
-
 // HalfedgeMesh's vertex -> halfedge array as collectHalfedgeElements leaves it: the
 // highest halfedge starting at each vertex ("last writer wins" in ascending thread order)
 __global__ void kVertexHalfedge(const int4* __restrict__ quads, int64_t Q, int32_t* __restrict__ vert_he) {
@@ -187,19 +176,19 @@ __global__ void kVertexHalfedgeInit(int32_t* __restrict__ vert_he, int64_t n0, i
     vert_he[i] = i < n0 ? -1 : 0;
 }
 
-// refreshHashTable's first list pass after a split step.  Each halfedge is checked (ids
-// in range, itself on its face's cycle); it is dirty -- its bucket can have changed --
-// when one of its endpoints was touched or it was appended.  A dirty halfedge gets its key
-// key[h] = 4 * face + position in the face's cycle and rank[h] = its rank in m's list.
-// One block checks kDirtyPer * kBlock consecutive halfedges, gathers its dirty ones in LDS
-// (their face walks then run on consecutive lanes) and appends them to the dirty list
-// with one atomic on ctr[A_DIRTY]; the list order is free, since ranks only pick list slots
-// and bucket ranks come from the keys.
+// refreshHashTable's first list pass after a split step.  Each halfedge's ids are checked
+// (in range); it is dirty -- its bucket can have changed -- when it was appended or both its
+// endpoints were touched.  A bucket changes only through a member that the step wrote back,
+// appended or removed, and every such halfedge joins two touched vertices (the working
+// copy's, before and after the step), so a halfedge with an untouched endpoint keeps its
+// bucket, its members and their keys.  One block checks kDirtyPer * kBlock consecutive halfedges,
+// gathers its dirty ones in LDS and appends them to the dirty list with one atomic on
+// ctr[A_DIRTY]; the list order is free, since ranks only pick list slots and bucket ranks
+// come from the keys.
 constexpr int kDirtyPer = 16;
 constexpr int kDirtyBatch = 4;
-__global__ void __launch_bounds__(kBlock) kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t oldH, const int32_t* __restrict__ face_he,
-                                                        int64_t F, int64_t V, const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ degM,
-                                                        int32_t* __restrict__ rank, int32_t* __restrict__ key, int32_t* __restrict__ dlist, int64_t* ctr) {
+__global__ void __launch_bounds__(kBlock) kDirtyRecords(const int4* __restrict__ he, int64_t H, int64_t oldH, int64_t F, int64_t V,
+                                                        const uint8_t* __restrict__ dirtyV, int32_t* __restrict__ dlist, int64_t* ctr) {
     __shared__ int s_list[kBlock * kDirtyPer];
     __shared__ int s_n;
     __shared__ long long s_base;
@@ -227,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) kDirtyRecords(const int4* __restrict__
         for (int j = 0; j < kDirtyBatch; ++j) {
             const int64_t t = b0 + (int64_t)(k0 + j) * kBlock;
             if (t < H && !bad[j]) bad[j] = b[j] < 0 || b[j] >= V;
-            d[j] = t < H && !bad[j] && (t >= oldH || dirtyV[e[j].x] || dirtyV[b[j]]);
+            d[j] = t < H && !bad[j] && (t >= oldH || (dirtyV[e[j].x] && dirtyV[b[j]]));
             nbad += bad[j] ? 1 : 0;
         }
 #pragma unroll
@@ -244,34 +233,51 @@ __global__ void __launch_bounds__(kBlock) kDirtyRecords(const int4* __restrict__
     if (threadIdx.x == 0 && n > 0) s_base = (long long)atomicAdd((unsigned long long*)&ctr[A_DIRTY], (unsigned long long)n);
     __syncthreads();
     const int64_t base = s_base;
-    for (int i = threadIdx.x; i < n; i += kBlock) {
-        const int h = s_list[i];
-        const int4 e = he[h];
-        int k = face_he[e.y], pos = 0;
-        while (pos < 4 && k != h && k >= 0 && k < H) {
-            k = heN(he, k);
-            ++pos;
-        }
-        if (k != h) {
-            ++nbad;  // the refresh stops at the check that follows
-        } else {
-            rank[h] = atomicAdd(&degM[min(e.x, heO(he, e.z))], 1);
-            key[h] = 4 * e.y + pos;
-        }
-        dlist[base + i] = h;
-    }
+    for (int i = threadIdx.x; i < n; i += kBlock) dlist[base + i] = s_list[i];
     if (nbad) atomicAdd((unsigned long long*)&ctr[A_INVALID], (unsigned long long)nbad);
 }
 
-// second list pass, after the scan of the counts: the entry (own entry, key) of each listed
-// halfedge goes to offM[m] + rank[h]
-__global__ void kPlaceEntries(const int4* __restrict__ he, const int32_t* __restrict__ dlist, int64_t n, const int64_t* __restrict__ offM,
-                              const int32_t* __restrict__ rank, const int32_t* __restrict__ key, int2* __restrict__ A) {
+// one thread per listed halfedge h (its ids checked by kDirtyRecords): its list vertex
+// m = the smaller endpoint, its list entry (other endpoint and direction, bucket key
+// 4 * face + position in the face's cycle) and its slot in m's list (an atomic count on
+// degM[m]), written in list order so the two passes after the scan read them contiguously.
+// A halfedge missing from its face's cycle is counted in ctr[A_INVALID] (the host checks it
+// at its next read-back) and still takes a slot, so every later write stays in its list.
+// When the four records 4f .. 4f + 3 of h's face f all keep the build's layout (canon: face
+// f, next = the quad's next) and the face starts among them, the cycle is that quad's and the
+// position is (h - start) mod 4, with no walk.
+__global__ void kDirtyKeys(const int4* __restrict__ he, int64_t H, const int32_t* __restrict__ face_he, const uint8_t* __restrict__ canon,
+                           const int32_t* __restrict__ dlist, int64_t n, int32_t* __restrict__ degM, int32_t* __restrict__ lm, int2* __restrict__ lent,
+                           int32_t* __restrict__ lrank, int64_t* ctr) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int h = dlist[t];
-    const ListRef l = listOf(he, he[h], offM);
-    A[l.lo + rank[h]] = make_int2(l.entry, key[h]);
+    const int4 e = he[h];
+    const int b = heO(he, e.z);
+    int k = face_he[e.y], pos = 0;
+    const int f = e.y;
+    if (canon && (h >> 2) == f && 4 * (int64_t)f + 3 < H && (k >> 2) == f && reinterpret_cast<const uint32_t*>(canon)[f] == 0x01010101u) {
+        pos = (h - k) & 3;
+        k = h;
+    }
+    while (pos < 4 && k != h && k >= 0 && k < H) {
+        k = heN(he, k);
+        ++pos;
+    }
+    if (k != h) atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
+    const int m = min(e.x, b);
+    lm[t] = m;
+    lent[t] = make_int2(pairEntry(e.x, b), k == h ? 4 * e.y + pos : 0x7FFFFFFF);
+    lrank[t] = atomicAdd(&degM[m], 1);
+}
+
+// second list pass, after the scan of the counts: the entry of each listed halfedge goes to
+// offM[m] + its slot
+__global__ void kPlaceEntries(const int32_t* __restrict__ lm, const int2* __restrict__ lent, const int32_t* __restrict__ lrank, int64_t n,
+                              const int64_t* __restrict__ offM, int2* __restrict__ A) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    A[offM[lm[t]] + lrank[t]] = lent[t];
 }
 
 // the four slots of a bucket: entries beyond the fourth land in slot 0 (glm::ivec4::operator[] default)
@@ -284,20 +290,22 @@ __device__ __forceinline__ uint32_t bucketVertexFlag(int n) { return n == 4 ? VF
 
 // One thread per listed halfedge h of a refresh, reading only the short list of its smaller
 // endpoint: the size of its bucket and its rank (position in key order).
-__global__ void kBuckets(const int4* __restrict__ he, const int32_t* __restrict__ dlist, int64_t nList, const int64_t* __restrict__ offM,
-                         const int32_t* __restrict__ keys, const int2* __restrict__ A, int32_t* __restrict__ bsize, int32_t* __restrict__ brank) {
+__global__ void kBuckets(const int32_t* __restrict__ dlist, const int32_t* __restrict__ lm, const int2* __restrict__ lent, int64_t nList,
+                         const int64_t* __restrict__ offM, const int2* __restrict__ A, int32_t* __restrict__ bsize, int32_t* __restrict__ brank) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nList) return;
-    const int h = dlist[t];
-    const ListRef l = listOf(he, he[h], offM);
-    const int o = entryOther(l.entry), key = keys[h];
+    const int m = lm[t];
+    const int2 own = lent[t];
+    const int64_t lo = offM[m], hi = offM[m + 1];
+    const int o = entryOther(own.x), key = own.y;
     int n = 0, rk = 0;
-    for (int i = l.lo; i < l.lo + l.len; ++i) {
+    for (int64_t i = lo; i < hi; ++i) {
         const int2 f = A[i];
         if (entryOther(f.x) != o) continue;
         ++n;
         if (f.y < key) ++rk;
     }
+    const int h = dlist[t];
     bsize[h] = n;
     brank[h] = rk;
 }
@@ -2581,7 +2589,13 @@ int64_t Pipeline::compactNM(int32_t* nm) {
     int64_t* h = pinHost();
     int64_t* d = pinDev();
     compactFlagged(arena.ptr<uint32_t>("fattr"), F, (uint32_t)QA_NM, nm, d + 1, scratch, stream, lookbackDiag());
+    // with the count, the invalid-record counter: a refresh's face walks add to it
+    if (refreshUnchecked) MDMC_HIP(hipMemcpyAsync(h + 2, counters() + A_INVALID, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
     MDMC_HIP(hipStreamSynchronize(stream));
+    if (refreshUnchecked) {
+        refreshUnchecked = false;
+        if (h[2]) throw HipError("inconsistent mesh after splitting (" + std::to_string(h[2]) + " bad ids)", -4);
+    }
     const int64_t n = h[1];
     if (n < 0 || n > F) throw HipError("inconsistent NM-face count (" + std::to_string(n) + ")", -4);
     return n;
@@ -2620,7 +2634,10 @@ void Pipeline::FillBatch::flush() {
     blocks = 0;
 }
 
-void Pipeline::resetCounters() { zero(counters(), A_NCOUNTERS * sizeof(int64_t)); }
+void Pipeline::resetCounters() {
+    zero(counters(), A_NCOUNTERS * sizeof(int64_t));
+    refreshUnchecked = false;
+}
 
 int64_t* Pipeline::lookbackDiag() { return counters() + A_LB_DIRECT; }
 
@@ -2946,8 +2963,6 @@ int32_t* Pipeline::pid() { return nProblems > 1 ? arena.ptr<int32_t>("pid") : nu
 void Pipeline::buildBuckets(const int32_t* list, int64_t n) {
     int4* he = arena.ptr<int4>("he");
     int32_t* degM = arena.ptr<int32_t>("degM");
-    int32_t* rank = arena.ptr<int32_t>("bl_rank");
-    int32_t* keys = arena.ptr<int32_t>("bl_key");
     int64_t* offM = arena.get<int64_t>("offM", (size_t)V + 1);
     void* scratch = arena.get<char>("scan_scratch", scanScratchBytes(V + 1));
     scanExclusiveI32(degM, offM, V + 1, arena.get<int64_t>("scan_total", 1), scratch, stream, lookbackDiag());
@@ -2959,8 +2974,10 @@ void Pipeline::buildBuckets(const int32_t* list, int64_t n) {
     zero(vflags, (size_t)V * sizeof(uint32_t));
     valenceReady = false;
     if (n > 0) {
-        kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, rank, keys, A);
-        kBuckets<<<gridFor(n), kBlock, 0, stream>>>(he, list, n, offM, keys, A, bsize, brank);
+        const int32_t* lm = arena.ptr<int32_t>("bl_m");
+        const int2* lent = arena.ptr<int2>("bl_entry");
+        kPlaceEntries<<<gridFor(n), kBlock, 0, stream>>>(lm, lent, arena.ptr<int32_t>("bl_rank"), n, offM, A);
+        kBuckets<<<gridFor(n), kBlock, 0, stream>>>(list, lm, lent, n, offM, A, bsize, brank);
     }
     zero(nmSlotParts(), kPartSlots * kPartStride * sizeof(int64_t));  // recounted from the new bucket sizes
     if (H > 0) kFlagsFromBuckets<<<gridFor((H + 3) / 4), kBlock, 0, stream>>>(he, H, bsize, brank, vflags, fattr, nmSlotParts());
@@ -3021,15 +3038,13 @@ void Pipeline::checkValid(const char* what) {
 }
 
 // refreshHashTable (HalfedgeMesh.cu:1063-1100) after a split step: only the halfedges
-// whose bucket can have changed (an endpoint in dirtyV, or appended after oldH), which
+// whose bucket can have changed (both endpoints in dirtyV, or appended after oldH), which
 // kDirtyRecords lists, are re-bucketed; the others keep their sizes and ranks (same
 // members, same keys), and the flags are rebuilt from all sizes.
 void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     int4* he = arena.ptr<int4>("he");
     int32_t* face_he = arena.ptr<int32_t>("face_he");
     int32_t* degM = arena.get<int32_t>("degM", (size_t)V + 1);
-    int32_t* rank = arena.get<int32_t>("bl_rank", (size_t)H);
-    int32_t* key = arena.get<int32_t>("bl_key", (size_t)H);
     arena.grow<int32_t>("bsize", (size_t)H, (size_t)std::min(oldH, H), stream);
     arena.grow<int32_t>("brank", (size_t)H, (size_t)std::min(oldH, H), stream);
     int32_t* dlist = arena.get<int32_t>("dirty_list", (size_t)std::max<int64_t>(H, 1));
@@ -3040,7 +3055,7 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     }
     if (F > 0) kFaceClear<<<gridFor((F + 3) / 4), kBlock, 0, stream>>>(face_he, F, H, arena.ptr<uint32_t>("fattr"), counters());
     const int64_t per = (int64_t)kBlock * kDirtyPer;
-    if (H > 0) kDirtyRecords<<<(unsigned)((H + per - 1) / per), kBlock, 0, stream>>>(he, H, oldH, face_he, F, V, dirtyV, degM, rank, key, dlist, counters());
+    if (H > 0) kDirtyRecords<<<(unsigned)((H + per - 1) / per), kBlock, 0, stream>>>(he, H, oldH, F, V, dirtyV, dlist, counters());
     // one read-back: the invalid-record count and the dirty-list length
     int64_t* hp = pinHost();
     MDMC_HIP(hipMemcpyAsync(hp, counters() + A_INVALID, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream));
@@ -3048,7 +3063,13 @@ void Pipeline::refreshHashTable(const uint8_t* dirtyV, int64_t oldH) {
     if (hp[0]) throw HipError("inconsistent mesh after splitting (" + std::to_string(hp[0]) + " bad ids)", -4);
     const int64_t nDirty = hp[1];
     if (nDirty < 0 || nDirty > H) throw HipError("inconsistent mesh after splitting (dirty list of " + std::to_string(nDirty) + ")", -4);
+    if (nDirty > 0) {
+        const size_t nd = (size_t)nDirty;
+        kDirtyKeys<<<gridFor(nDirty), kBlock, 0, stream>>>(he, H, face_he, canonFlags(), dlist, nDirty, degM, arena.get<int32_t>("bl_m", nd), arena.get<int2>("bl_entry", nd),
+                                                          arena.get<int32_t>("bl_rank", nd), counters());
+    }
     buildBuckets(dlist, nDirty);
+    refreshUnchecked = true;  // kDirtyKeys' face-walk check is read back with the next NM-face count
 }
 
 // ---- simplifications
@@ -3507,6 +3528,10 @@ void Pipeline::split(int step, int maxIters) {
         if (!(step >= 6) || !copyDataBack) break;
     }
     if (iter == 0) splitTouched = false;  // nothing changed: the lists describe every vertex
+    if (refreshUnchecked) {
+        refreshUnchecked = false;
+        checkValid("splitting");
+    }
 }
 
 void Pipeline::buildMergeList(const int2* rec, int64_t R, int count, int32_t* num, int32_t* ex, int32_t* ey, const std::string& tag) {
