@@ -611,36 +611,51 @@ __device__ __forceinline__ void spliceChain(int u, int t, int32_t* __restrict__ 
     next[last] = atomicExch(&head[t], u);
 }
 
+// Sparse work whose every item is a chain of dependent loads: a block takes kSparsePer *
+// kBlock consecutive items, calls pred(i) once for each (i < n), gathers the flagged ones in
+// LDS (in no particular order) and returns their number; the block then works through the
+// list on all its lanes, so one block overlaps many chains instead of a few.
+constexpr int kSparsePer = 8;
+template <class Pred>
+__device__ __forceinline__ int gatherSparse(int64_t n, Pred pred, int* s_list, int* s_n) {
+    if (threadIdx.x == 0) *s_n = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t b0 = (int64_t)blockIdx.x * (kSparsePer * kBlock) + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kSparsePer; ++k) {
+        const int64_t i = b0 + (int64_t)k * kBlock;
+        const bool f = i < n && pred(i);
+        const uint64_t m = __ballot(f);
+        int at = 0;
+        if (lane == 0 && m) at = atomicAdd(s_n, __popcll(m));
+        at = __shfl(at, 0, 64);
+        if (f) s_list[at + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)i;
+    }
+    __syncthreads();
+    return *s_n;
+}
+inline unsigned sparseBlocks(int64_t n) { return (unsigned)((n + kSparsePer * kBlock - 1) / (kSparsePer * kBlock)); }
+
 __global__ void __launch_bounds__(kBlock) kSources(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget,
                                                    const int32_t* __restrict__ address, int64_t V, int dropType, uint8_t* __restrict__ touched,
                                                    int32_t* __restrict__ srcHead, int32_t* __restrict__ srcNext, int32_t* __restrict__ outdeg) {
     // the block's dropped vertices gathered in LDS and handled on consecutive lanes
-    __shared__ int s_list[kBlock];
-    __shared__ int s_wave[kBlock / 64];
-    const int64_t uu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool drop = uu < V && vtype[uu] == dropType;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint64_t m = __ballot(drop);
-    if (lane == 0) s_wave[wid] = __popcll(m);
-    __syncthreads();
-    int base = 0, total = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < wid) base += s_wave[w];
-        total += s_wave[w];
-    }
-    if (drop) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)uu;
-    __syncthreads();
-    if ((int)threadIdx.x >= total) return;
-    const int u = s_list[threadIdx.x];
-    const int t = vtarget[u];
-    if (t < 0 || t >= V) return;
-    const int tn = address[t];
-    if (tn < 0) return;  // the relabelled quads then hold -1 and fail validation
-    touched[tn] = 1;
-    spliceChain(u, t, srcHead, srcNext);
-    if (outdeg) {
-        atomicAdd(&outdeg[t], outdeg[u]);
-        outdeg[u] = 0;
+    __shared__ int s_list[kSparsePer * kBlock];
+    __shared__ int s_n;
+    const int total = gatherSparse(V, [&](int64_t i) { return vtype[i] == dropType; }, s_list, &s_n);
+    for (int j = threadIdx.x; j < total; j += kBlock) {
+        const int u = s_list[j];
+        const int t = vtarget[u];
+        if (t < 0 || t >= V) continue;
+        const int tn = address[t];
+        if (tn < 0) continue;  // the relabelled quads then hold -1 and fail validation
+        touched[tn] = 1;
+        spliceChain(u, t, srcHead, srcNext);
+        if (outdeg) {
+            atomicAdd(&outdeg[t], outdeg[u]);
+            outdeg[u] = 0;
+        }
     }
 }
 
@@ -806,35 +821,26 @@ __global__ void __launch_bounds__(kBlock) kLocalMerge(int4* __restrict__ quads, 
                                                       const int32_t* __restrict__ vtarget, int32_t* __restrict__ twin, uint32_t* __restrict__ bpack,
                                                       int32_t* __restrict__ outdeg, int2* __restrict__ zslot, int64_t V, uint8_t* __restrict__ touched,
                                                       int allowLocal, int64_t* ctr) {
-    __shared__ int s_list[kBlock];
-    __shared__ int s_wave[kBlock / 64];
-    const int64_t qq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool merging = false;
-    if (qq < Q) {
-        const uint32_t a = qattr[qq];
-        merging = !(a & QA_DEAD) && (a & QA_REMOVE);
-        if (!(a & QA_DEAD) && !merging) {
-            uint32_t na = QA_COLOR;
-            if (a & QA_NM) na |= QA_NM;
-            if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
-            qattr[qq] = na;
-            fattr[qq] = na;
-        }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint64_t mb = __ballot(merging);
-    if (lane == 0) s_wave[wid] = __popcll(mb);
-    __syncthreads();
-    int base = 0, total = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < wid) base += s_wave[w];
-        total += s_wave[w];
-    }
-    if (merging) s_list[base + __popcll(mb & ((1ULL << lane) - 1ULL))] = (int)qq;
-    __syncthreads();
-    bool fallback = false;
-    if ((int)threadIdx.x < total) {
-        const int q = s_list[threadIdx.x];
+    __shared__ int s_list[kSparsePer * kBlock];
+    __shared__ int s_n;
+    const int total = gatherSparse(
+        Q,
+        [&](int64_t qq) {
+            const uint32_t a = qattr[qq];
+            const bool merging = !(a & QA_DEAD) && (a & QA_REMOVE);
+            if (!(a & QA_DEAD) && !merging) {
+                uint32_t na = QA_COLOR;
+                if (a & QA_NM) na |= QA_NM;
+                if (a & QA_BOUNDARY) na |= QA_BOUNDARY;
+                qattr[qq] = na;
+                fattr[qq] = na;
+            }
+            return merging;
+        },
+        s_list, &s_n);
+    int nfallback = 0;
+    for (int j = threadIdx.x; j < total; j += kBlock) {
+        const int q = s_list[j];
         const int4 v4 = quads[q];
         const int v[4] = {v4.x, v4.y, v4.z, v4.w};
         const int val[4] = {valence[v[0]], valence[v[1]], valence[v[2]], valence[v[3]]};
@@ -885,7 +891,7 @@ __global__ void __launch_bounds__(kBlock) kLocalMerge(int4* __restrict__ quads, 
             if (z >= 0 && z < V) zslot[z] = make_int2(tA, (tZ & ~3) | ((tZ + 1) & 3));
         } else {
             touched[x] = 1;
-            fallback = true;
+            ++nfallback;
             if (z >= 0 && z < V) zslot[z] = make_int2(-1, -1);
         }
         // the removal
@@ -896,8 +902,7 @@ __global__ void __launch_bounds__(kBlock) kLocalMerge(int4* __restrict__ quads, 
         qattr[q] = QA_DEAD;
         fattr[q] = QA_DEAD;
     }
-    const uint64_t fb = __ballot(fallback);
-    if (lane == 0 && fb) atomicAdd((unsigned long long*)&ctr[A_FALLBACK], (unsigned long long)__popcll(fb));
+    if (nfallback) atomicAdd((unsigned long long*)&ctr[A_FALLBACK], (unsigned long long)nfallback);
 }
 
 // valence = out-degree + incoming boundary halfedges (a step with only local merges: no flag
@@ -916,28 +921,10 @@ __global__ void kValenceFromBdin(const int32_t* __restrict__ outdeg, const int32
 // compacting path.
 // (the dropped vertices of a block are gathered in LDS and handled on consecutive lanes: each
 // one's work is a chain of dependent loads, which sparse lanes would spread over every wave)
-__global__ void __launch_bounds__(kBlock) kDropVertices(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V,
-                                                        int dropType, int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off,
-                                                        const int32_t* __restrict__ out, uint8_t* __restrict__ vdead, int32_t* __restrict__ head,
-                                                        int32_t* __restrict__ next, int32_t* __restrict__ outdeg, const int2* __restrict__ zslot,
-                                                        int64_t* ctr) {
-    __shared__ int s_list[kBlock];
-    __shared__ int s_wave[kBlock / 64];
-    const int64_t uu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool drop = uu < V && vtype[uu] == dropType;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint64_t m = __ballot(drop);
-    if (lane == 0) s_wave[wid] = __popcll(m);
-    __syncthreads();
-    int base = 0, total = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < wid) base += s_wave[w];
-        total += s_wave[w];
-    }
-    if (drop) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)uu;
-    __syncthreads();
-    if ((int)threadIdx.x >= total) return;
-    const int u = s_list[threadIdx.x];
+__device__ __forceinline__ void dropVertex(int u, const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V, int dropType,
+                                           int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off, const int32_t* __restrict__ out,
+                                           uint8_t* __restrict__ vdead, int32_t* __restrict__ head, int32_t* __restrict__ next,
+                                           int32_t* __restrict__ outdeg, const int2* __restrict__ zslot, int64_t* ctr) {
     const int t = vtarget[u];
     if (t < 0 || t >= V || vtype[t] == dropType) {
         atomicAdd((unsigned long long*)&ctr[A_INVALID], 1ULL);
@@ -965,6 +952,17 @@ __global__ void __launch_bounds__(kBlock) kDropVertices(const int32_t* __restric
     spliceChain(u, t, head, next);
     atomicAdd(&outdeg[t], outdeg[u]);
     outdeg[u] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) kDropVertices(const int32_t* __restrict__ vtype, const int32_t* __restrict__ vtarget, int64_t V,
+                                                        int dropType, int4* __restrict__ quads, int64_t H, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ out, uint8_t* __restrict__ vdead, int32_t* __restrict__ head,
+                                                        int32_t* __restrict__ next, int32_t* __restrict__ outdeg, const int2* __restrict__ zslot,
+                                                        int64_t* ctr) {
+    __shared__ int s_list[kSparsePer * kBlock];
+    __shared__ int s_n;
+    const int total = gatherSparse(V, [&](int64_t i) { return vtype[i] == dropType; }, s_list, &s_n);
+    for (int j = threadIdx.x; j < total; j += kBlock) dropVertex(s_list[j], vtype, vtarget, V, dropType, quads, H, off, out, vdead, head, next, outdeg, zslot, ctr);
 }
 
 // bucketFromOut over the chain lists: the entries of m's list and of its chain's lists, those
@@ -3107,10 +3105,10 @@ void Pipeline::simplifyDeferredStep(int dropType, bool p3333) {
     if (deferredSteps == 0 && V > 0) kOutdegInit<<<gridFor(V), kBlock, 0, stream>>>(off, V, valence, outdeg, bdin);
     int2* zslot = arena.get<int2>("zslot", nv);
     if (Q > 0)
-        kLocalMerge<<<gridFor(Q), kBlock, 0, stream>>>(quads, qattr, fattr, Q, valence, vtype, vtarget, twin, bpack, outdeg, zslot, V, touched,
+        kLocalMerge<<<sparseBlocks(Q), kBlock, 0, stream>>>(quads, qattr, fattr, Q, valence, vtype, vtarget, twin, bpack, outdeg, zslot, V, touched,
                                                        localMerges ? 1 : 0, counters());
     if (V > 0)
-        kDropVertices<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, quads, H, off, out, vdead, head, next, outdeg, zslot, counters());
+        kDropVertices<<<sparseBlocks(V), kBlock, 0, stream>>>(vtype, vtarget, V, dropType, quads, H, off, out, vdead, head, next, outdeg, zslot, counters());
     // the merges the local update could not take: their region's buckets, flags and boundary
     // counts recomputed
     if (readI64(counters() + A_FALLBACK) > 0) {
@@ -3198,7 +3196,7 @@ void Pipeline::simplifyCompactAndRebuild(int dropType, bool p3333) {
     int32_t* srcNext = arena.get<int32_t>("src_next", (size_t)std::max<int64_t>(V, 1));
     zero(touched, (size_t)nV);
     if (!chained) fillNeg(srcHead, (size_t)V * sizeof(int32_t));
-    if (V > 0) kSources<<<gridFor(V), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext, outdeg);
+    if (V > 0) kSources<<<sparseBlocks(V), kBlock, 0, stream>>>(vtype, vtarget, address, V, dropType, touched, srcHead, srcNext, outdeg);
     const int64_t* off = arena.ptr<int64_t>("out_off");
     const int32_t* out = arena.ptr<int32_t>("out_list");
     int32_t* cnt = arena.get<int32_t>("out_cnt", (size_t)nV + 1);
