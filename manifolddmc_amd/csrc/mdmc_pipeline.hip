@@ -1199,6 +1199,9 @@ __global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_
 // A vertex whose outgoing halfedges form one closed clockwise cycle (next(twin(h)) from
 // any of them visits all and returns) has a single fan: every fan walk of fanWalk reaches
 // the same minimum.  The others are decided by kDetectFans over their outgoing lists.
+// Every face around a vertex the splitter never touched lies outside its working copies, so
+// the records such a walk visits keep the build's layout and need no layout-flag check
+// (touched: the splitter's touched vertices, vertices from listsV on being its additions).
 __global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
                          uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, const uint8_t* __restrict__ touched, int64_t listsV,
                          const uint8_t* __restrict__ canon, int64_t* __restrict__ nslow) {
@@ -1207,6 +1210,7 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, cons
     const int v = (int)vv;
     const int d = outdeg[v];
     bool single = d == 0;  // no outgoing halfedge: no fan, flag cleared
+    const bool mayMove = canon && (!touched || v >= listsV || touched[v]);
     if (d > 0) {
         const int start = rep[v];
         int cur = start;
@@ -1218,7 +1222,7 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, cons
             // flag, and again from t's own next only when the flag says it moved
             const int qn = quadNext(t, 1);
             int4 en = qn < H ? he[qn] : make_int4(-1, -1, -1, -1);
-            if (canon && !canon[t]) {
+            if (mayMove && !canon[t]) {
                 cur = heN(he, t);
                 en = he[cur];
             } else {
