@@ -1202,6 +1202,12 @@ __global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_
 // Every face around a vertex the splitter never touched lies outside its working copies, so
 // the records such a walk visits keep the build's layout and need no layout-flag check
 // (touched: the splitter's touched vertices, vertices from listsV on being its additions).
+// An open fan is decided here too.  When the clockwise walk stops (no twin, or a step to a
+// halfedge of another origin), a counter-clockwise walk (twin of the previous halfedge in
+// the quad) runs from the same start; if it stops too and the two chains hold all d outgoing
+// halfedges, the vertex has one fan.  Every fan walk of fanWalk from any of them then covers
+// both chains: faces are quads, and at a vertex with no bucket of 3 among its edges
+// (VF_WEIRD unset) every twin involved is mutual, so each walk direction undoes the other.
 __global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ rep,
                          uint32_t* __restrict__ vflags, uint8_t* __restrict__ slow, const uint8_t* __restrict__ touched, int64_t listsV,
                          const uint8_t* __restrict__ canon, int64_t* __restrict__ nslow) {
@@ -1215,9 +1221,13 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, cons
         const int start = rep[v];
         int cur = start;
         int4 e = he[start];  // the record of cur: one load gives the next step's origin check and twin
+        int open = 0;        // the outgoing halfedges the clockwise walk visited before it stopped
         for (int k = 1; k <= d; ++k) {
             const int t = e.w;
-            if (t < 0) break;
+            if (t < 0) {
+                open = k;
+                break;
+            }
             // the next record is loaded at the build layout's next of t alongside t's layout
             // flag, and again from t's own next only when the flag says it moved
             const int qn = quadNext(t, 1);
@@ -1229,11 +1239,31 @@ __global__ void kFanFast(const int4* __restrict__ he, int64_t H, int64_t V, cons
                 cur = qn;
             }
             e = en;
-            if (e.x != v) break;
+            if (e.x != v) {
+                open = k;
+                break;
+            }
             if (cur == start) {
                 single = k == d;
                 break;
             }
+        }
+        if (open > 0 && !(vflags[v] & VF_WEIRD)) {
+            int y = start, got = open;
+            bool stopped = false;
+            while (got <= d) {
+                int p;
+                if (!mayMove || canon[y]) p = quadNext(y, 3);
+                else p = heN(he, heN(he, heN(he, y)));
+                const int tp = (p >= 0 && p < H) ? heT(he, p) : -1;
+                if (tp < 0 || tp >= H || heO(he, tp) != v) {
+                    stopped = true;
+                    break;
+                }
+                ++got;
+                y = tp;
+            }
+            single = stopped && got == d;
         }
     }
     slow[v] = single ? 0 : 1;
