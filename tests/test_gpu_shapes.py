@@ -44,3 +44,19 @@ def test_checkerboard_every_cell_ambiguous(flags):
     v *= (1.0 + 0.01 * ((x * 7 + y * 3 + z * 5) % 11)).astype(np.float32)  # break the exact symmetry of the deciders
     g, st = assert_same(v, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags, label="checkerboard")
     assert st["n_faces"] > 0
+
+
+OPEN = [oa.DETECT_NM_VERTS, oa.SURF_ITER | oa.DETECT_NM_VERTS, oa.SURF_ITER | oa.SPLIT_NM | oa.DETECT_NM_VERTS]
+
+
+@pytest.mark.parametrize("flags", OPEN)
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_open_fans_against_oracle(seed, flags):
+    # no padding: the surface ends at the grid's faces, so many vertices have open fans, and
+    # edges of three faces (twins that are not mutual) occur; the fan flags decided by the
+    # walks from one outgoing halfedge must equal the oracle's per-halfedge fan keys
+    rng = np.random.default_rng(700 + seed)
+    v = rng.uniform(-1, 1, size=(14, 13, 12)).astype(np.float32)
+    g, st = assert_same(v, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, flags, label="open fans %d" % seed)
+    print("open fans seed %d flags %d: %d vertices with several fans, %d edges of 3 faces" % (seed, flags, st["nm_vertices"], st["nm_edges3"]))
+    assert st["nm_vertices"] >= 0
