@@ -33,8 +33,10 @@ __device__ __forceinline__ bool plainValue(float v) {
     return (b & 0x7FFFFFFFu) < 0x7E800000u && b != 0x80000000u;
 }
 
+// one output row; its points x = lane, lane + width, ... (a block, or one wave in MODE 0)
 template <int MODE>
-__device__ __forceinline__ void prepareRow(const PrepareDesc* __restrict__ d, int G, int refinement, int pad, int64_t row, int64_t* special) {
+__device__ __forceinline__ void prepareRow(const PrepareDesc* __restrict__ d, int G, int refinement, int pad, int64_t row, int64_t* special,
+                                           int lane, int width) {
     int lo = 0, hi = G;  // d[lo].base <= row
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -54,7 +56,7 @@ __device__ __forceinline__ void prepareRow(const PrepareDesc* __restrict__ d, in
     if (MODE == 0) {
         const float* in = p.in + (int64_t)z * nxy + (int64_t)y * inx;
         bool plain = true;
-        for (int x = threadIdx.x; x < onx; x += kBlock) {
+        for (int x = lane; x < onx; x += width) {
             const float v = in[pad ? clampi(x, 1, onx - 2) - 1 : x];
             plain = plain && plainValue(v);
             out[x] = v;
@@ -78,7 +80,7 @@ __device__ __forceinline__ void prepareRow(const PrepareDesc* __restrict__ d, in
     const float* r10 = p.in + (int64_t)z0 * nxy + (int64_t)y1 * inx;
     const float* r01 = p.in + (int64_t)z1 * nxy + (int64_t)y0 * inx;
     const float* r11 = p.in + (int64_t)z1 * nxy + (int64_t)y1 * inx;
-    for (int x = threadIdx.x; x < onx; x += kBlock) {
+    for (int x = lane; x < onx; x += width) {
         const int px = pad ? clampi(x, 1, onx - 2) - 1 : x;
         int mx = px;
         float dx = 0.f;
@@ -93,15 +95,23 @@ __device__ __forceinline__ void prepareRow(const PrepareDesc* __restrict__ d, in
     }
 }
 
+// MODE 0 copies one row per wave (kBlock / 64 rows per block); the blending modes take a
+// block per row
+constexpr int kPlainRowsPerBlock = kBlock / 64;
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) kPrepareGrids(const PrepareDesc* __restrict__ d, int G, int refinement, int pad, int64_t rows,
                                                         int64_t* special) {
     if (MODE == 1) {
         if (*(volatile int64_t*)special == 0) return;
-        for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) prepareRow<1>(d, G, refinement, pad, row, special);
+        for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) prepareRow<1>(d, G, refinement, pad, row, special, threadIdx.x, kBlock);
         return;
     }
-    prepareRow<MODE>(d, G, refinement, pad, blockIdx.x, special);
+    if (MODE == 0) {
+        const int64_t row = (int64_t)blockIdx.x * kPlainRowsPerBlock + (threadIdx.x >> 6);
+        if (row < rows) prepareRow<0>(d, G, refinement, pad, row, special, threadIdx.x & 63, 64);
+        return;
+    }
+    prepareRow<MODE>(d, G, refinement, pad, blockIdx.x, special, threadIdx.x, kBlock);
 }
 
 __global__ void kTriangles(const int4* __restrict__ he, const int32_t* __restrict__ face_he, int64_t F, uint32_t* __restrict__ idx) {
@@ -178,7 +188,7 @@ void launchPrepareGrids(const PrepareDesc* descDev, int G, int64_t blocks, int r
         return;
     }
     // *special is zero on entry; the exact pass returns at once unless the copy set it
-    kPrepareGrids<0><<<(unsigned)blocks, kBlock, 0, s>>>(descDev, G, refinement, pad, blocks, special);
+    kPrepareGrids<0><<<(unsigned)((blocks + kPlainRowsPerBlock - 1) / kPlainRowsPerBlock), kBlock, 0, s>>>(descDev, G, refinement, pad, blocks, special);
     kPrepareGrids<1><<<(unsigned)std::min<int64_t>(blocks, 2048), kBlock, 0, s>>>(descDev, G, refinement, pad, blocks, special);
 }
 
