@@ -1167,20 +1167,60 @@ __global__ void kOutDegree(const int4* __restrict__ he, int64_t H, int64_t V, in
 
 // out-degree and one outgoing halfedge from the outgoing lists of the last build, for the
 // vertices they still describe: those below listsV that no split step touched (a split
-// step rewrites only halfedges whose origin is in its working copy, before and after);
-// the others are counted by kOutDegreeTouched
-__global__ void kOutFromLists(const int64_t* __restrict__ off, const int32_t* __restrict__ out, int64_t listsV, const uint8_t* __restrict__ touched,
-                              int64_t V, int32_t* __restrict__ outdeg, int32_t* __restrict__ rep) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    if (v < listsV && !(touched && touched[v])) {
-        const int64_t o = off[v];
-        const int d = (int)(off[v + 1] - o);
+// step rewrites only halfedges whose origin is in its working copy, before and after).
+// With the layout flags (canon), a touched vertex counts the entries of its list whose
+// records keep the build's layout (never written back, so still starting at it; the
+// block's touched vertices are gathered in LDS, their list walks run on full waves) and
+// kOutDegreeMoved adds the records the splitter wrote; without them kOutDegreeTouched
+// counts every touched vertex from all records.  Vertices from listsV on are counted there.
+__global__ void __launch_bounds__(kBlock) kOutFromLists(const int64_t* __restrict__ off, const int32_t* __restrict__ out, int64_t listsV,
+                                                        const uint8_t* __restrict__ touched, const uint8_t* __restrict__ canon, int64_t V,
+                                                        int32_t* __restrict__ outdeg, int32_t* __restrict__ rep) {
+    __shared__ int s_list[kSparsePer * kBlock];
+    __shared__ int s_n;
+    const int total = gatherSparse(
+        V,
+        [&](int64_t v) {
+            if (v < listsV && !(touched && touched[v])) {
+                const int64_t o = off[v];
+                const int d = (int)(off[v + 1] - o);
+                outdeg[v] = d;
+                if (d > 0) rep[v] = out[o];
+                return false;
+            }
+            outdeg[v] = 0;
+            return v < listsV && canon != nullptr;
+        },
+        s_list, &s_n);
+    for (int j = threadIdx.x; j < total; j += kBlock) {
+        const int v = s_list[j];
+        int d = 0, r = -1;
+        for (int64_t k = off[v]; k < off[v + 1]; ++k) {
+            const int x = out[k];
+            if (canon[x]) {
+                ++d;
+                r = x;
+            }
+        }
         outdeg[v] = d;
-        if (d > 0) rep[v] = out[o];
-    } else {
-        outdeg[v] = 0;
+        if (d > 0) rep[v] = r;
     }
+}
+
+// the records the splitter wrote back or appended (canon unset), one per lane: each adds to
+// its origin's out-degree, and the first one of a vertex with no other outgoing halfedge
+// becomes its walk start
+__global__ void kOutDegreeMoved(const int4* __restrict__ he, int64_t H, int64_t V, int64_t listsV, const uint8_t* __restrict__ touched,
+                                const uint8_t* __restrict__ canon, int32_t* __restrict__ outdeg, int32_t* __restrict__ rep, int64_t* ctr) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H || canon[h]) return;
+    const int o = heO(he, (int)h);
+    if (o < 0 || o >= V) {
+        anomaly(ctr);
+        return;
+    }
+    if (o >= listsV || (touched && touched[o]))
+        if (atomicAdd(&outdeg[o], 1) == 0) rep[o] = (int32_t)h;
 }
 
 __global__ void kOutDegreeTouched(const int4* __restrict__ he, int64_t H, int64_t V, int64_t listsV, const uint8_t* __restrict__ touched,
@@ -3597,9 +3637,15 @@ void Pipeline::detectNonManifoldVertices() {
     const bool fromLists = listsV >= 0 && listsV <= V;
     const uint8_t* touched = (fromLists && splitTouched) ? arena.ptr<uint8_t>("split_touched") : nullptr;
     if (fromLists) {
-        kOutFromLists<<<gridFor(V), kBlock, 0, stream>>>(arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"), listsV, touched, V, outdeg,
-                                                         rep);
-        if (touched || V > listsV) kOutDegreeTouched<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, listsV, touched, outdeg, rep, counters());
+        const uint8_t* canon = canonFlags();  // kept since the splitter's first write-back
+        if (V > 0)
+            kOutFromLists<<<sparseBlocks(V), kBlock, 0, stream>>>(arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"), listsV, touched, canon,
+                                                                  V, outdeg, rep);
+        if (canon) {
+            if (H > 0) kOutDegreeMoved<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, listsV, touched, canon, outdeg, rep, counters());
+        } else if (touched || V > listsV) {
+            kOutDegreeTouched<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, listsV, touched, outdeg, rep, counters());
+        }
     } else {
         kOutDegree<<<gridFor(H), kBlock, 0, stream>>>(he, H, V, outdeg, rep, counters());
     }
