@@ -1808,39 +1808,39 @@ __device__ void fixBucket(int h, int4* __restrict__ he, int64_t H, const int4* _
 }
 
 // Buckets of four or more are rare (one halfedge in several hundred), and repairing one is
-// a long dependent chain.  Each block of a fixed grid takes a contiguous range of halfedges,
-// gathers the range's rank-0 members of such buckets in LDS (up to 1024 at a time) and
-// repairs them on consecutive lanes, so that the chains run on full waves.
-constexpr int kFixList = 1024;
+// a long dependent chain.  A block takes kFixPer * kBlock consecutive halfedges, reads their
+// bucket sizes four at a time (16-byte loads) and the rank only where the size is four or
+// more, gathers the rank-0 members in LDS and repairs them on consecutive lanes, so that the
+// chains run on full waves.
+constexpr int kFixPer = 16;
 __global__ void __launch_bounds__(kBlock) kFixTwins(int4* __restrict__ he, int64_t H, const int4* __restrict__ quads, const int64_t* __restrict__ off,
                                                     const int32_t* __restrict__ out, const int32_t* __restrict__ bsize, const int32_t* __restrict__ brank,
                                                     const float* __restrict__ pos, const ProblemDev* __restrict__ probs, const int32_t* __restrict__ pid,
                                                     int64_t* ctr) {
-    __shared__ int s_list[kFixList];
+    __shared__ int s_list[kFixPer * kBlock];
     __shared__ int s_n;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int64_t per = (H + gridDim.x - 1) / gridDim.x;
-    const int64_t h0 = (int64_t)blockIdx.x * per, h1 = std::min<int64_t>(H, h0 + per);
-    int64_t cur = h0;
-    while (cur < h1) {
-        if (tid == 0) s_n = 0;
-        __syncthreads();
-        // fill while a whole chunk still fits
-        while (cur < h1 && s_n <= kFixList - kBlock) {
-            const int64_t h = cur + tid;
-            const bool hit = h < h1 && bsize[h] >= 4 && brank[h] == 0;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int64_t g0 = (int64_t)blockIdx.x * (kFixPer / 4 * kBlock) + threadIdx.x;  // groups of four halfedges
+#pragma unroll
+    for (int k = 0; k < kFixPer / 4; ++k) {
+        const int64_t h0 = 4 * (g0 + (int64_t)k * kBlock);
+        int n[4];
+        sizes4(bsize, H, h0, n);  // past H: size 2, never a hit
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool hit = n[j] >= 4 && h0 + j < H && brank[h0 + j] == 0;
             const uint64_t m = __ballot(hit);
-            int base = 0;
-            if (m && lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(&s_n, __popcll(m));
-            base = __shfl(base, __ffsll((unsigned long long)(m ? m : 1ULL)) - 1, 64);
-            if (hit) s_list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)h;
-            cur += kBlock;
-            __syncthreads();
+            int at = 0;
+            if (lane == 0 && m) at = atomicAdd(&s_n, __popcll(m));
+            at = __shfl(at, 0, 64);
+            if (hit) s_list[at + __popcll(m & ((1ULL << lane) - 1ULL))] = (int)(h0 + j);
         }
-        const int n = s_n;
-        for (int k = tid; k < n; k += kBlock) fixBucket(s_list[k], he, H, quads, off, out, pos, probs, pid, ctr);
-        __syncthreads();
     }
+    __syncthreads();
+    const int total = s_n;
+    for (int i = threadIdx.x; i < total; i += kBlock) fixBucket(s_list[i], he, H, quads, off, out, pos, probs, pid, ctr);
 }
 
 // ---------------------------------------------------------------- ManifoldSplitter
@@ -3390,7 +3390,7 @@ void Pipeline::materializeHalfedges() {
 void Pipeline::fixManifoldTwins() {
     materializeHalfedges();
     if (H == 0) return;
-    kFixTwins<<<(unsigned)std::min<int64_t>(gridFor(H), 4096), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("quads"), arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"),
+    kFixTwins<<<(unsigned)((H + kFixPer * kBlock - 1) / (kFixPer * kBlock)), kBlock, 0, stream>>>(arena.ptr<int4>("he"), H, arena.ptr<int4>("quads"), arena.ptr<int64_t>("out_off"), arena.ptr<int32_t>("out_list"),
                                                  arena.ptr<int32_t>("bsize"), arena.ptr<int32_t>("brank"), arena.ptr<float>("pos"), probsDev, pid(),
                                                  counters());
 }
